@@ -1,0 +1,235 @@
+/*
+ * lsgpu.h — C ABI of the MI355X (gfx950) per-timestep hot path of LeachingSim.
+ *
+ * The reference has no plugin or FFI layer: its hot path is the body of the
+ * time loop in src/main.cpp:293-453, made of public methods of
+ * `class Simulation` (src/Simulation.h:29-186).  Each entry point below
+ * replaces one of those methods (or a group that the loop always calls
+ * together) and is cited with the reference lines it stands in for.
+ *
+ * Conventions
+ *   - Plain C types only: pointers, sizes, ints, doubles.  No torch or HIP
+ *     types cross this boundary.
+ *   - Grid (ni, nj, nk) = the reference's internal (i, j, k) = CLI
+ *     (zext, yext, xext) (src/main.cpp:94).  Storage is k-fastest:
+ *     idx = k + nk*(j + nj*i) (src/Domain.cpp:106-117), the same byte order
+ *     as the .raw input (src/MPIRawLoader.h:70).
+ *   - Labels: Air 1, Pore 2, Rock 3, Sulphide 4, Precipitate 5
+ *     (src/compiler_opts.h:69-82).  uint8 or uint16 on upload; any other
+ *     value is rejected (an unknown label would leave an empty pressure row,
+ *     src/Simulation.cpp:683-792).
+ *   - Status: 0 = OK; > 0 = a solve did not converge or the state is
+ *     unphysical (the reference's `bool false`, which triggers the retry
+ *     manager, src/main.cpp:437-452); < 0 = fatal error (the reference's
+ *     thrown std::runtime_error / MPI_Abort).  ls_last_error() explains.
+ *   - Ownership: the context owns all device memory.  Host buffers are owned
+ *     by the caller and copied at the boundary.
+ *   - Threading: one HIP stream per context; calls on one context must be
+ *     serialised by the caller.
+ */
+#ifndef LSGPU_H_
+#define LSGPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSGPU_ABI_VERSION 1
+
+/* Labels (src/compiler_opts.h:69-82). */
+enum ls_label {
+    LS_AIR = 1,
+    LS_PORE = 2,
+    LS_ROCK = 3,
+    LS_SULPHIDE = 4,
+    LS_PRECIPITATE = 5
+};
+
+/* Pressure solver choice.  The reference always uses PETSc GMRES + Jacobi
+ * (src/Simulation.cpp:131-159); the solvers here stop on the same criterion,
+ * ||b - A x||_2 / ||b||_2 <= press_rtol on the reduced SPD pressure system. */
+enum ls_solver {
+    LS_SOLVER_PLAIN64 = 0,  /* FP64 Jacobi-preconditioned CG                     */
+    LS_SOLVER_MIXED = 1,    /* FP32 stencil CG inside FP64 iterative refinement  */
+    LS_SOLVER_LEARNED = 2   /* flexible CG, U-Net preconditioner + warm start,
+                               FP32 inner / FP64 outer, plain PCG fallback       */
+};
+
+/* Pressure boundary mode. */
+enum ls_bc_mode {
+    LS_BC_REFERENCE = 0,    /* Dirichlet data from Air voxels only
+                               (src/Simulation.cpp:683-721)                      */
+    LS_BC_ZFACES = 1        /* additionally: virtual Air planes at i = -1
+                               (p = ramped top pressure) and i = ni (p = 0),
+                               linked to Pore rows by the reference's Pore-Air
+                               conductance (K_air = 1e-9)                        */
+};
+
+/* Warm start of the pressure solve. */
+enum ls_warm_start {
+    LS_WARM_PREVIOUS = 0,   /* previous pressure (src/Simulation.cpp:157,808)    */
+    LS_WARM_RESCALED = 1,   /* previous pressure x (new top / old top pressure)  */
+    LS_WARM_LEARNED = 2     /* U-Net prediction from the previous pressure       */
+};
+
+/* All physics parameters of src/main.cpp:39,69-91 plus solver settings.
+ * ls_default_params() fills the reference's defaults. */
+typedef struct ls_params {
+    double D;                /* --D            (1.0)                 */
+    double dx;               /* --voxel_size   (1.0)                 */
+    double Dpore_fac;        /* --Dpore_fac    (1.0)                 */
+    double kext;             /* --kext         (1.0, inert)          */
+    double kreac;            /* --kreac        (1.0)                 */
+    double theta;            /* --theta        (1.0)                 */
+    double c_sat;            /* --csat         (0.95)                */
+    double evap_flux;        /* --evap_flux    (0.0)                 */
+    double top_pressure;     /* --top_pressure (1e4)                 */
+    double max_cap_grad;     /* --max_cap_grad (5e4)                 */
+    double vg_alpha;         /* 1.0  (src/main.cpp:90)               */
+    double vg_n;             /* 2.0  (src/main.cpp:89)               */
+    double s_res;            /* 0.05 (src/main.cpp:91)               */
+    double porosity;         /* --porosity     (0.1)                 */
+    double solid_density;    /* never assigned in the reference (src/Simulation.h:135); default 1.0 */
+    double time_scalar;      /* --time_scalar  (1.0)                 */
+    double alpha_transfer;   /* --alpha_transfer (1e-5)              */
+    int instant_precip;      /* --instant_precip (0)                 */
+    int bc_mode;             /* enum ls_bc_mode                      */
+    int solver;              /* enum ls_solver                       */
+    int warm_start;          /* enum ls_warm_start                   */
+    double press_rtol;       /* 1e-8                                 */
+    int press_maxit;         /* 10000 (src/Simulation.cpp:156)       */
+    double adr_rtol;         /* 1e-8                                 */
+    int adr_maxit;           /* 10000                                */
+    int mixed_inner_maxit;   /* FP32 inner iterations per refinement (500) */
+    double mixed_inner_rtol; /* FP32 inner residual reduction (1e-4)  */
+} ls_params;
+
+/* Fields that can be uploaded / downloaded.  Element types are fixed:
+ * IMG is uint8 or uint16 (label_bytes of the context), FRAC is float,
+ * everything else double.  KREL, PC, PERM, MU are derived fields, computed
+ * from the state on download (src/Simulation.cpp:187-248,1596-1664). */
+enum ls_field {
+    LS_F_IMG = 0,
+    LS_F_FRAC = 1,
+    LS_F_INVENTORY = 2,   /* precipitate_inventory */
+    LS_F_SATURATION = 3,
+    LS_F_PRESSURE = 4,
+    LS_F_CONC = 5,
+    LS_F_ACID = 6,        /* conc_acid */
+    LS_F_QX = 7,          /* flux_x (i direction) */
+    LS_F_QY = 8,
+    LS_F_QZ = 9,
+    LS_F_KREL = 10,       /* rel_permeability (derived)  */
+    LS_F_PC = 11,         /* cap_pressure (derived)      */
+    LS_F_PERM = 12,       /* permeability (derived)      */
+    LS_F_MU = 13,         /* viscosity (derived)         */
+    LS_F_COUNT = 14
+};
+
+/* Phases of one coupled step, in the order of src/main.cpp:312-432. */
+enum ls_phase_id {
+    LS_PH_PROPERTIES = 0,   /* updateRelativePermeability + updateCapillaryPressure
+                               + initProperties  (src/Simulation.cpp:1634,1596,187) */
+    LS_PH_PRESSURE = 1,     /* setupPressureEqns(t) + solvePressure   (:658, :804)   */
+    LS_PH_FLUX = 2,         /* calculateFlux                           (:866)         */
+    LS_PH_SATURATION = 3,   /* updateSaturation(dt)                    (:1672)        */
+    LS_PH_SURFACE = 4,      /* handleSurfaceEffects                    (:2063)        */
+    LS_PH_TRANSPORT = 5,    /* setupConcentrationEqns + solveConc + solveAcid
+                                                                       (:966,:1243,:2167) */
+    LS_PH_PRECIPITATION = 6,/* handlePrecipitation                     (:2114)        */
+    LS_PH_LEACHING = 7,     /* updateFrac(dt)                          (:315)         */
+    LS_PH_COUNT = 8
+};
+
+/* What one step (or phase) reports back: enough to print the reference's
+ * log lines (src/Simulation.cpp:457,838,1772; src/main.cpp:426-430). */
+typedef struct ls_step_report {
+    int status;
+    int press_iters;          /* total CG iterations (inner iterations for mixed) */
+    int press_outer;          /* refinement / outer iterations                    */
+    int press_fallback;       /* learned path fell back to plain PCG              */
+    int press_unet_applies;   /* U-Net preconditioner applications               */
+    double press_relres;      /* final ||b-Ax||/||b|| (FP64, reduced system)      */
+    double press_bnorm;
+    int adr_iters;
+    double adr_relres;        /* max over the two right-hand sides                */
+    int sat_substeps;
+    double sat_max_divq;
+    int leach_substeps;
+    double leach_max_rate;
+    int leached;              /* voxels fully leached this step                   */
+    int leach_general_path;   /* 1 if the order-exact wavefront path ran          */
+    long long n_floored;      /* pore rows with |diagonal| < 1e-16               */
+    double p_min, p_max, conc_min, conc_max, acid_min, acid_max;
+    double mass_conc, mass_acid, mass_water;   /* FP64 sums over the grid        */
+    double top_pressure_eff;
+    double t_props_ms, t_pressure_ms, t_flux_ms, t_sat_ms, t_surface_ms,
+           t_transport_ms, t_precip_ms, t_leach_ms, t_snapshot_ms, t_total_ms;
+} ls_step_report;
+
+typedef struct ls_ctx ls_ctx;
+
+/* Fill the reference's defaults (src/main.cpp:39,89-91). */
+void ls_default_params(ls_params* p);
+/* ABI check for foreign bindings: returns LSGPU_ABI_VERSION and the sizes of
+ * the structs above as this library was compiled. */
+int ls_abi_info(size_t* sizeof_params, size_t* sizeof_report);
+
+/* Replaces Simulation::setupDomain + setupPETSc (src/Simulation.cpp:81-159):
+ * allocates all device state for an ni x nj x nk grid on the current HIP
+ * device.  label_bytes is 1 (uint8 build) or 2 (uint16 build). */
+int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes,
+              const ls_params* params);
+void ls_destroy(ls_ctx* ctx);
+const char* ls_last_error(const ls_ctx* ctx);
+int ls_get_params(const ls_ctx* ctx, ls_params* out);
+int ls_set_params(ls_ctx* ctx, const ls_params* params);
+
+/* Replaces readRAW + the init* functions' writes (src/Simulation.cpp:68,
+ * 165-310): copy one unpadded host field (ni*nj*nk elements) to the device. */
+int ls_upload(ls_ctx* ctx, int field, const void* host, size_t elem_bytes);
+/* Copy one field back (for VTK output, checkpoints and tests). */
+int ls_download(ls_ctx* ctx, int field, void* host, size_t elem_bytes);
+
+/* Reference initial state (initFields/initFrac/initSaturation/initAcid,
+ * src/Simulation.cpp:165-180,254-310,1573-1590) on the device, from the
+ * uploaded labels: p = conc = acid = q = inventory = 0, frac = 1 on Sulphide,
+ * s = 0.1 on Pore and Sulphide.  Returns the initial sulphide count. */
+int ls_init_state(ls_ctx* ctx, long long* initial_sulphide_count);
+
+/* One coupled step: the body of the retry loop, src/main.cpp:303-432
+ * (saveState, then properties, pressure, flux, saturation, surface effects,
+ * conc + acid transport, precipitation, leaching).  On status > 0 the state
+ * is left as the failed attempt made it; call ls_rollback() and retry with
+ * a smaller dt, as src/main.cpp:437-452 does. */
+int ls_step(ls_ctx* ctx, double t, double dt, ls_step_report* report);
+
+/* One phase of the step on the current state (for tests and the per-method
+ * host mirror).  Fields the report does not cover are left zero. */
+int ls_phase(ls_ctx* ctx, int phase, double t, double dt, ls_step_report* report);
+
+/* Replaces saveState / restoreState (src/Simulation.cpp:2312-2353). */
+int ls_snapshot(ls_ctx* ctx);
+int ls_rollback(ls_ctx* ctx);
+
+/* Test / measurement hooks (no reference counterpart). */
+/* y = A x for the reduced pressure operator assembled by the last pressure
+ * phase (rows outside the solved set return 0).  Host buffers, N doubles. */
+int ls_debug_apply_pressure(ls_ctx* ctx, const double* x, double* y);
+/* Time `reps` launches of the pressure stencil kernel y = A x (precision 64 or
+ * 32) on the context's stream with HIP events; returns ms per launch and the
+ * kernel's algorithmic bytes per launch. */
+int ls_time_stencil(ls_ctx* ctx, int precision, int reps, double* ms_per_launch,
+                    double* bytes_per_launch);
+/* Copy bytes within device memory on the context's stream, timed with HIP
+ * events (a copy kernel of known size, for bandwidth calibration). */
+int ls_time_copy(ls_ctx* ctx, size_t bytes, int reps, double* ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSGPU_H_ */

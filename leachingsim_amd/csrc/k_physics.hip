@@ -1,0 +1,779 @@
+// k_physics.hip — pointwise and stencil kernels of the coupled step other than
+// the linear solvers: initial state, properties, pressure operator assembly
+// and post-solve, Darcy flux, explicit saturation, surface effects,
+// precipitation, derived-field output and step summaries.
+//
+// Each kernel restates one loop of src/Simulation.cpp (cited per kernel) with
+// the same floating-point expressions in the same order.  The library is
+// compiled with -ffp-contract=off so no multiply-add is fused: results round
+// exactly as the reference's scalar C++ does.
+#include <cmath>
+
+#include "ls_context.h"
+
+namespace ls {
+
+// std::max / std::min semantics (NaN handling included): max(a,b) = a<b ? b : a.
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+
+struct Geo {
+    int ni, nj, nk;
+    uint32_t si, sj;  // strides of i and j (k is contiguous)
+};
+
+__device__ __forceinline__ void decode(uint32_t v, const Geo& g, int& i, int& j, int& k) {
+    uint32_t ij = v / (uint32_t)g.nk;
+    k = (int)(v - ij * (uint32_t)g.nk);
+    i = (int)(ij / (uint32_t)g.nj);
+    j = (int)(ij - (uint32_t)i * (uint32_t)g.nj);
+}
+
+// Neighbour f in the reference order +i, -i, +j, -j, +k, -k
+// (src/Simulation.cpp:771-776); false if outside the grid.
+__device__ __forceinline__ bool nbr(const Geo& g, int i, int j, int k, uint32_t v, int f,
+                                    uint32_t& n) {
+    switch (f) {
+        case 0: if (i + 1 >= g.ni) return false; n = v + g.si; return true;
+        case 1: if (i == 0) return false; n = v - g.si; return true;
+        case 2: if (j + 1 >= g.nj) return false; n = v + g.sj; return true;
+        case 3: if (j == 0) return false; n = v - g.sj; return true;
+        case 4: if (k + 1 >= g.nk) return false; n = v + 1; return true;
+        default: if (k == 0) return false; n = v - 1; return true;
+    }
+}
+
+static Geo geo_of(const ls_ctx* c) {
+    Geo g;
+    g.ni = c->ni;
+    g.nj = c->nj;
+    g.nk = c->nk;
+    g.sj = (uint32_t)c->nk;
+    g.si = (uint32_t)c->nk * (uint32_t)c->nj;
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// Properties: krel (src/Simulation.cpp:1634-1664), Pc (:1596-1628),
+// K and mu (:187-248).
+// ---------------------------------------------------------------------------
+struct PropConst {
+    double m, s_res, vg_alpha, vg_n, log_k_rock, log_k_pore;
+};
+
+__device__ __forceinline__ double rel_perm(uint8_t t, double s, const PropConst& pc) {
+    if (t == PORE || t == SUL) {
+        double Se = (s - pc.s_res) / (1.0 - pc.s_res);
+        Se = dmax(0.0, dmin(1.0, Se));
+        if (Se <= 0.0) return 0.0;
+        double term1 = 1.0 - pow(1.0 - pow(Se, 1.0 / pc.m), pc.m);
+        return sqrt(Se) * term1 * term1;
+    }
+    return 0.0;
+}
+
+__device__ __forceinline__ double cap_pressure(uint8_t t, double s, const PropConst& pc) {
+    if (t == PORE || t == SUL) {
+        double Se = (s - pc.s_res) / (1.0 - pc.s_res);
+        Se = dmax(1e-6, dmin(1.0, Se));
+        if (Se >= 1.0) return 0.0;
+        double Se_inv_m = pow(Se, -1.0 / pc.m);
+        return (1.0 / pc.vg_alpha) * pow(Se_inv_m - 1.0, 1.0 / pc.vg_n);
+    }
+    return 0.0;
+}
+
+__device__ __forceinline__ double viscosity(uint8_t t) { return (t == AIR) ? 1.8e-5 : 1.0e-3; }
+
+__device__ __forceinline__ double permeability(uint8_t type, double krel, double inv, float fr,
+                                               const PropConst& pc) {
+    const double pore_permeability = 1.0e-12;
+    const double rock_permeability = 1.0e-18;
+    double K_intrinsic = rock_permeability;
+    if (type == AIR) {
+        K_intrinsic = pore_permeability;
+    } else if (type == PORE) {
+        double p_frac = dmax(0.0, dmin(1.0, inv));
+        double log_k_eff = (p_frac * pc.log_k_rock) + ((1.0 - p_frac) * pc.log_k_pore);
+        K_intrinsic = pow(10.0, log_k_eff);
+    } else if (type == SUL) {
+        double f = fr;
+        double log_k_eff = (f * pc.log_k_rock) + ((1.0 - f) * pc.log_k_pore);
+        K_intrinsic = pow(10.0, log_k_eff);
+    }
+    double K_rel = dmax(krel, 1e-15);
+    return K_intrinsic * K_rel;
+}
+
+static PropConst prop_const(const ls_ctx* c) {
+    PropConst pc;
+    pc.m = 1.0 - (1.0 / c->P.vg_n);
+    pc.s_res = c->P.s_res;
+    pc.vg_alpha = c->P.vg_alpha;
+    pc.vg_n = c->P.vg_n;
+    pc.log_k_rock = c->log_k_rock;
+    pc.log_k_pore = c->log_k_pore;
+    return pc;
+}
+
+__global__ void k_props(const uint8_t* __restrict__ img, const double* __restrict__ s,
+                        const double* __restrict__ inv, const float* __restrict__ frac,
+                        double* __restrict__ perm, double* __restrict__ pcap, uint32_t N,
+                        PropConst pc) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        uint8_t t = img[v];
+        double sv = s[v];
+        double kr = rel_perm(t, sv, pc);
+        pcap[v] = cap_pressure(t, sv, pc);
+        perm[v] = permeability(t, kr, inv[v], frac[v], pc);
+    }
+}
+
+__global__ void k_derived(const uint8_t* __restrict__ img, const double* __restrict__ s,
+                          const double* __restrict__ inv, const float* __restrict__ frac,
+                          double* __restrict__ out, uint32_t N, int field, PropConst pc) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        uint8_t t = img[v];
+        double kr = rel_perm(t, s[v], pc);
+        double r;
+        if (field == LS_F_KREL) r = kr;
+        else if (field == LS_F_PC) r = cap_pressure(t, s[v], pc);
+        else if (field == LS_F_PERM) r = permeability(t, kr, inv[v], frac[v], pc);
+        else r = viscosity(t);
+        out[v] = r;
+    }
+}
+
+int phase_properties(ls_ctx* c) {
+    uint32_t N = (uint32_t)c->N;
+    hipLaunchKernelGGL(k_props, dim3(ew_blocks(N)), dim3(EW_BLOCK), 0, c->stream, c->img, c->s,
+                       c->inv, c->frac, c->perm, c->pc, N, prop_const(c));
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+int derived_field(ls_ctx* c, int field, double* out_dev) {
+    uint32_t N = (uint32_t)c->N;
+    hipLaunchKernelGGL(k_derived, dim3(ew_blocks(N)), dim3(EW_BLOCK), 0, c->stream, c->img, c->s,
+                       c->inv, c->frac, out_dev, N, field, prop_const(c));
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Initial state: initFields / initFrac / initSaturation / initAcid
+// (src/Simulation.cpp:165-180, 254-310, 1573-1590).
+// ---------------------------------------------------------------------------
+__global__ void k_init(const uint8_t* __restrict__ img, float* frac, double* inv, double* s,
+                       double* p, double* conc, double* acid, double* qx, double* qy, double* qz,
+                       uint32_t N, double* partials) {
+    __shared__ double lds[EW_BLOCK / WAVE];
+    double cnt = 0.0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        uint8_t t = img[v];
+        p[v] = 0.0; conc[v] = 0.0; acid[v] = 0.0;
+        qx[v] = 0.0; qy[v] = 0.0; qz[v] = 0.0;
+        inv[v] = 0.0;
+        frac[v] = (t == SUL) ? 1.0f : 0.0f;
+        s[v] = (t == PORE || t == SUL) ? 0.1 : 0.0;
+        cnt += (t == SUL) ? 1.0 : 0.0;
+    }
+    double a[1] = {cnt};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a[0];
+}
+
+template <int OP>
+__global__ void k_finalize_one(const double* __restrict__ partials, int n, double* out) {
+    __shared__ double lds[1024 / WAVE];
+    double acc = red_identity<OP>();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc = red_combine<OP>(acc, partials[i]);
+    double a[1] = {acc};
+    block_reduce<1, OP>(a, lds);
+    if (threadIdx.x == 0) *out = a[0];
+}
+
+int init_state(ls_ctx* c, long long* nsul) {
+    uint32_t N = (uint32_t)c->N;
+    int nb = ew_blocks(N);
+    hipLaunchKernelGGL(k_init, dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img, c->frac, c->inv,
+                       c->s, c->p, c->conc, c->acid, c->qx, c->qy, c->qz, N, c->partials);
+    LS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_finalize_one<RED_SUM>, dim3(1), dim3(1024), 0, c->stream, c->partials,
+                       nb, &c->ss->nsul);
+    LS_LAUNCH_CHECK();
+    LS_HIP(hipMemcpyAsync(c->h_ss, c->ss, sizeof(StepScalars), hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    if (nsul) *nsul = (long long)c->h_ss->nsul;
+    c->have_prev = false;
+    c->press_assembled = false;
+    return phase_properties(c);
+}
+
+// ---------------------------------------------------------------------------
+// Pressure operator (src/Simulation.cpp:658-798), reduced to the solved Pore
+// rows.  Stored per voxel: the three +face conductances as the Pore row
+// computes them, the coupling sum d, the row class and the Dirichlet value.
+// ---------------------------------------------------------------------------
+
+// Harmonic-mean Pore link (src/Simulation.cpp:737-768).
+__device__ __forceinline__ double pore_link(double perm_self, double k_neighbor, double mu_self) {
+    double K_self = dmax(perm_self, 1e-25);
+    double K_neigh = dmax(k_neighbor, 1e-25);
+    double K_avg = (2.0 * K_self * K_neigh) / (K_self + K_neigh);
+    return K_avg / mu_self;
+}
+
+// Conductance of the face between v and n as seen by the Pore side.
+__device__ __forceinline__ double face_cond(uint8_t tv, uint8_t tn, double pv, double pn) {
+    if (tv == PORE) return pore_link(pv, (tn == AIR) ? 1.0e-9 : pn, viscosity(tv));
+    if (tn == PORE) return pore_link(pn, (tv == AIR) ? 1.0e-9 : pv, viscosity(tn));
+    return 0.0;
+}
+
+__global__ void k_press_faces(const uint8_t* __restrict__ img, const double* __restrict__ perm,
+                              const double* __restrict__ p, double* __restrict__ cfx,
+                              double* __restrict__ cfy, double* __restrict__ cfz,
+                              double* __restrict__ cin, double* __restrict__ dsum,
+                              int8_t* __restrict__ cls, double* __restrict__ pD, Geo g, int zfaces,
+                              double ptop) {
+    const int k = blockIdx.x * TK + threadIdx.x;
+    const int j = blockIdx.y * TJ + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nk || j >= g.nj) return;
+    const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
+    const uint8_t tv = img[v];
+    const double pv = perm[v];
+    // +faces, value from the Pore side (identical from both sides when both
+    // are Pore: the harmonic mean is symmetric bit for bit)
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (i + 1 < g.ni) fx = face_cond(tv, img[v + g.si], pv, perm[v + g.si]);
+    else if (zfaces && tv == PORE) fx = pore_link(pv, 1.0e-9, viscosity(tv));  // outlet plane
+    if (j + 1 < g.nj) fy = face_cond(tv, img[v + g.sj], pv, perm[v + g.sj]);
+    if (k + 1 < g.nk) fz = face_cond(tv, img[v + 1], pv, perm[v + 1]);
+    cfx[v] = fx;
+    cfy[v] = fy;
+    cfz[v] = fz;
+    if (i == 0) cin[(uint32_t)j * g.sj + k] = (zfaces && tv == PORE) ? pore_link(pv, 1.0e-9, viscosity(tv)) : 0.0;
+
+    if (tv == PORE) {
+        double diagonal_term = 0.0;
+        for (int f = 0; f < 6; ++f) {
+            uint32_t n;
+            if (!nbr(g, i, j, k, v, f, n)) continue;
+            uint8_t tn = img[n];
+            double K_neighbor = (tn == AIR) ? 1.0e-9 : perm[n];
+            diagonal_term -= pore_link(pv, K_neighbor, viscosity(tv));
+        }
+        if (zfaces && i == 0) diagonal_term -= pore_link(pv, 1.0e-9, viscosity(tv));
+        if (zfaces && i == g.ni - 1) diagonal_term -= pore_link(pv, 1.0e-9, viscosity(tv));
+        dsum[v] = -diagonal_term;
+        int8_t cl = (fabs(diagonal_term) < 1e-16) ? 2 : 1;
+        cls[v] = cl;
+        pD[v] = (cl == 2) ? p[v] : 0.0;  // floored rows: lagged value
+    } else {
+        dsum[v] = 0.0;
+        cls[v] = 0;
+        double val = 0.0;
+        if (tv == AIR) {  // surface Air: any in-grid neighbour is not Air (:696-711)
+            for (int f = 0; f < 6; ++f) {
+                uint32_t n;
+                if (nbr(g, i, j, k, v, f, n) && img[n] != AIR) { val = ptop; break; }
+            }
+        }
+        pD[v] = val;
+    }
+}
+
+// Face conductance of face f of v from the stored +face arrays.
+__device__ __forceinline__ double face_raw(const double* cfx, const double* cfy, const double* cfz,
+                                           uint32_t v, uint32_t n, int f) {
+    switch (f) {
+        case 0: return cfx[v];
+        case 1: return cfx[n];
+        case 2: return cfy[v];
+        case 3: return cfy[n];
+        case 4: return cfz[v];
+        default: return cfz[n];
+    }
+}
+
+__global__ void k_press_rhs(const uint8_t* __restrict__ img, const double* __restrict__ cfx,
+                            const double* __restrict__ cfy, const double* __restrict__ cfz,
+                            const double* __restrict__ cin, const double* __restrict__ dsum,
+                            const int8_t* __restrict__ cls, const double* __restrict__ pD,
+                            const double* __restrict__ p, double* __restrict__ sd,
+                            double* __restrict__ bhat, double* __restrict__ x, Geo g, int zfaces,
+                            double ptop, double evap_dx, double warm_scale) {
+    const int k = blockIdx.x * TK + threadIdx.x;
+    const int j = blockIdx.y * TJ + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nk || j >= g.nj) return;
+    const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
+    if (cls[v] != 1) {
+        sd[v] = 0.0;
+        bhat[v] = 0.0;
+        x[v] = 0.0;
+        return;
+    }
+    double source_term = 0.0;
+    for (int f = 0; f < 6; ++f) {
+        uint32_t n;
+        if (nbr(g, i, j, k, v, f, n) && img[n] == AIR) source_term -= evap_dx;
+    }
+    double rhs = -source_term;
+    for (int f = 0; f < 6; ++f) {
+        uint32_t n;
+        if (!nbr(g, i, j, k, v, f, n)) continue;
+        if (cls[n] == 1) continue;
+        rhs += face_raw(cfx, cfy, cfz, v, n, f) * pD[n];
+    }
+    if (zfaces && i == 0) rhs += cin[(uint32_t)j * g.sj + k] * ptop;
+    double sv = sqrt(dsum[v]);
+    sd[v] = sv;
+    bhat[v] = rhs / sv;
+    x[v] = sv * (p[v] * warm_scale);  // warm start, scaled variables
+}
+
+// Scaled operator faces: c / (sqrt(d_v) sqrt(d_n)) between two solved rows.
+__global__ void k_press_scale(const double* __restrict__ cfx, const double* __restrict__ cfy,
+                              const double* __restrict__ cfz, const int8_t* __restrict__ cls,
+                              const double* __restrict__ sd, double* __restrict__ hx,
+                              double* __restrict__ hy, double* __restrict__ hz,
+                              float* __restrict__ fx, float* __restrict__ fy,
+                              float* __restrict__ fz, Geo g) {
+    const int k = blockIdx.x * TK + threadIdx.x;
+    const int j = blockIdx.y * TJ + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nk || j >= g.nj) return;
+    const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
+    const bool rv = cls[v] == 1;
+    const double sv = sd[v];
+    double a = 0.0, b = 0.0, cc = 0.0;
+    if (rv && i + 1 < g.ni && cls[v + g.si] == 1) a = cfx[v] / (sv * sd[v + g.si]);
+    if (rv && j + 1 < g.nj && cls[v + g.sj] == 1) b = cfy[v] / (sv * sd[v + g.sj]);
+    if (rv && k + 1 < g.nk && cls[v + 1] == 1) cc = cfz[v] / (sv * sd[v + 1]);
+    hx[v] = a; hy[v] = b; hz[v] = cc;
+    fx[v] = (float)a; fy[v] = (float)b; fz[v] = (float)cc;
+}
+
+__global__ void k_count_floored(const int8_t* __restrict__ cls, uint32_t N, double* partials) {
+    __shared__ double lds[EW_BLOCK / WAVE];
+    double cnt = 0.0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        cnt += (cls[v] == 2) ? 1.0 : 0.0;
+    double a[1] = {cnt};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a[0];
+}
+
+static double effective_top_pressure(double top, double t) {
+    const double ramp_duration = 3600.0;  // src/Simulation.cpp:662-670
+    double eff = top;
+    if (t < ramp_duration) eff = top * (t / ramp_duration) + 1e-6;
+    return eff;
+}
+
+int assemble_pressure(ls_ctx* c, double t) {
+    Geo g = geo_of(c);
+    const double ptop = effective_top_pressure(c->P.top_pressure, t);
+    const int zf = c->P.bc_mode == LS_BC_ZFACES;
+    dim3 blk(TK, TJ), grd((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, c->ni);
+    hipLaunchKernelGGL(k_press_faces, grd, blk, 0, c->stream, c->img, c->perm, c->p, c->craw[0],
+                       c->craw[1], c->craw[2], c->cin, c->dsum, c->cls, c->pD, g, zf, ptop);
+    LS_LAUNCH_CHECK();
+    double scale = 1.0;
+    if (c->P.warm_start == LS_WARM_RESCALED && c->have_prev && c->p_top_prev != 0.0)
+        scale = ptop / c->p_top_prev;
+    hipLaunchKernelGGL(k_press_rhs, grd, blk, 0, c->stream, c->img, c->craw[0], c->craw[1],
+                       c->craw[2], c->cin, c->dsum, c->cls, c->pD, c->p, c->sd, c->bhat, c->x64, g,
+                       zf, ptop, c->P.evap_flux * c->P.dx, scale);
+    LS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_press_scale, grd, blk, 0, c->stream, c->craw[0], c->craw[1], c->craw[2],
+                       c->cls, c->sd, c->c64[0], c->c64[1], c->c64[2], c->c32[0], c->c32[1],
+                       c->c32[2], g);
+    LS_LAUNCH_CHECK();
+    uint32_t N = (uint32_t)c->N;
+    int nb = ew_blocks(N);
+    hipLaunchKernelGGL(k_count_floored, dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->cls, N,
+                       c->partials);
+    LS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_finalize_one<RED_SUM>, dim3(1), dim3(1024), 0, c->stream, c->partials,
+                       nb, &c->ss->n_floored);
+    LS_LAUNCH_CHECK();
+    c->press_assembled = true;
+    return 0;
+}
+
+// After the solve: p = x/sqrt(d) on solved rows, the floored rows from their
+// own reference row 1e-16 p_f + sum_n c p_n = source (src/Simulation.cpp:
+// 783-797), the Dirichlet value everywhere else.
+__global__ void k_press_final(const uint8_t* __restrict__ img, const double* __restrict__ cfx,
+                              const double* __restrict__ cfy, const double* __restrict__ cfz,
+                              const double* __restrict__ cin, const int8_t* __restrict__ cls,
+                              const double* __restrict__ pD, const double* __restrict__ sd,
+                              const double* __restrict__ x, double* __restrict__ p, Geo g,
+                              int zfaces, double ptop, double evap_dx) {
+    const int k = blockIdx.x * TK + threadIdx.x;
+    const int j = blockIdx.y * TJ + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nk || j >= g.nj) return;
+    const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
+    const int8_t cl = cls[v];
+    if (cl == 1) {
+        p[v] = x[v] / sd[v];
+    } else if (cl == 2) {
+        double source_term = 0.0, sum = 0.0;
+        for (int f = 0; f < 6; ++f) {
+            uint32_t n;
+            if (!nbr(g, i, j, k, v, f, n)) continue;
+            if (img[n] == AIR) source_term -= evap_dx;
+            double pn = (cls[n] == 1) ? x[n] / sd[n] : pD[n];
+            sum += face_raw(cfx, cfy, cfz, v, n, f) * pn;
+        }
+        if (zfaces && i == 0) sum += cin[(uint32_t)j * g.sj + k] * ptop;
+        p[v] = (source_term - sum) / 1e-16;
+    } else {
+        p[v] = pD[v];
+    }
+}
+
+int finalize_pressure(ls_ctx* c, double ptop) {
+    Geo g = geo_of(c);
+    dim3 blk(TK, TJ), grd((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, c->ni);
+    hipLaunchKernelGGL(k_press_final, grd, blk, 0, c->stream, c->img, c->craw[0], c->craw[1],
+                       c->craw[2], c->cin, c->cls, c->pD, c->sd, c->x64, c->p, g,
+                       c->P.bc_mode == LS_BC_ZFACES, ptop, c->P.evap_flux * c->P.dx);
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Darcy flux (src/Simulation.cpp:866-953); one-sided differences only at the
+// global faces of the single domain.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double grad1(const double* F, uint32_t v, int pos, int ext, uint32_t st,
+                                        double dx) {
+    if (pos == 0) return (F[v + st] - F[v]) / dx;
+    if (pos == ext - 1) return (F[v] - F[v - st]) / dx;
+    return (F[v + st] - F[v - st]) / (2.0 * dx);
+}
+
+__global__ void k_flux(const uint8_t* __restrict__ img, const double* __restrict__ p,
+                       const double* __restrict__ pcap, const double* __restrict__ perm,
+                       double* __restrict__ qx, double* __restrict__ qy, double* __restrict__ qz,
+                       Geo g, double dx, double mcg) {
+    const int k = blockIdx.x * TK + threadIdx.x;
+    const int j = blockIdx.y * TJ + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nk || j >= g.nj) return;
+    const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
+    double gradP_x = grad1(p, v, i, g.ni, g.si, dx);
+    double gradP_y = grad1(p, v, j, g.nj, g.sj, dx);
+    double gradP_z = grad1(p, v, k, g.nk, 1, dx);
+    double gradPc_x = grad1(pcap, v, i, g.ni, g.si, dx);
+    double gradPc_y = grad1(pcap, v, j, g.nj, g.sj, dx);
+    double gradPc_z = grad1(pcap, v, k, g.nk, 1, dx);
+    gradPc_x = dmax(-mcg, dmin(mcg, gradPc_x));
+    gradPc_y = dmax(-mcg, dmin(mcg, gradPc_y));
+    gradPc_z = dmax(-mcg, dmin(mcg, gradPc_z));
+    double transmissivity = perm[v] / viscosity(img[v]);
+    const double rho_g = 1000.0 * 9.81;
+    double liq_grad_x = gradP_x - gradPc_x;
+    double liq_grad_y = gradP_y - gradPc_y;
+    double liq_grad_z = gradP_z - gradPc_z;
+    qx[v] = -transmissivity * (liq_grad_x - rho_g);
+    qy[v] = -transmissivity * liq_grad_y;
+    qz[v] = -transmissivity * liq_grad_z;
+}
+
+int phase_flux(ls_ctx* c) {
+    Geo g = geo_of(c);
+    dim3 blk(TK, TJ), grd((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, c->ni);
+    hipLaunchKernelGGL(k_flux, grd, blk, 0, c->stream, c->img, c->p, c->pc, c->perm, c->qx, c->qy,
+                       c->qz, g, c->P.dx, c->P.max_cap_grad);
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Explicit saturation (src/Simulation.cpp:1672-1856).  The sub-step increment
+// of a voxel is the same in every sub-step (q is fixed), so the N clamped
+// updates run in registers: bit-identical to N full sweeps.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool sat_inactive(uint8_t t) {
+    return t == AIR || t == ROCK || t == SUL || t == PREC;
+}
+
+__device__ __forceinline__ void sat_rates(const uint8_t* img, const double* qx, const double* qy,
+                                          const double* qz, const Geo& g, int i, int j, int k,
+                                          uint32_t v, double dx, double evap, double& div_q,
+                                          double& evap_sink) {
+    double q_face_xp = (i + 1 < g.ni) ? (qx[v] + qx[v + g.si]) / 2.0 : 0.0;
+    double q_face_xm = (i > 0) ? (qx[v] + qx[v - g.si]) / 2.0 : 0.0;
+    double q_face_yp = (j + 1 < g.nj) ? (qy[v] + qy[v + g.sj]) / 2.0 : 0.0;
+    double q_face_ym = (j > 0) ? (qy[v] + qy[v - g.sj]) / 2.0 : 0.0;
+    double q_face_zp = (k + 1 < g.nk) ? (qz[v] + qz[v + 1]) / 2.0 : 0.0;
+    double q_face_zm = (k > 0) ? (qz[v] + qz[v - 1]) / 2.0 : 0.0;
+    div_q = ((q_face_xp - q_face_xm) + (q_face_yp - q_face_ym) + (q_face_zp - q_face_zm)) / dx;
+    evap_sink = 0.0;
+    if (img[v] == PORE) {
+        int air_faces = 0;
+        for (int f = 0; f < 6; ++f) {
+            uint32_t n;
+            if (nbr(g, i, j, k, v, f, n) && img[n] == AIR) air_faces++;
+        }
+        evap_sink = (air_faces * evap) / dx;
+    }
+}
+
+__global__ void k_sat_max(const uint8_t* __restrict__ img, const double* __restrict__ qx,
+                          const double* __restrict__ qy, const double* __restrict__ qz, Geo g,
+                          uint32_t N, double dx, double evap, double* partials) {
+    __shared__ double lds[EW_BLOCK / WAVE];
+    double mx = 0.0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        if (sat_inactive(img[v])) continue;
+        int i, j, k;
+        decode(v, g, i, j, k);
+        double div_q, evap_sink;
+        sat_rates(img, qx, qy, qz, g, i, j, k, v, dx, evap, div_q, evap_sink);
+        double total_rate = fabs(div_q) + evap_sink;
+        if (total_rate > mx) mx = total_rate;
+    }
+    double a[1] = {mx};
+    block_reduce<1, RED_MAX>(a, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a[0];
+}
+
+__global__ void k_sat_finalize(const double* __restrict__ partials, int n, double dt,
+                               double porosity, StepScalars* ss) {
+    __shared__ double lds[1024 / WAVE];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc = fmax(acc, partials[i]);
+    double a[1] = {acc};
+    block_reduce<1, RED_MAX>(a, lds);
+    if (threadIdx.x == 0) {
+        double global_max_div_q = a[0];
+        double max_dsat = 0.1;
+        double dt_safe = dt;
+        if (global_max_div_q > 1e-20) dt_safe = (max_dsat * porosity) / global_max_div_q;
+        double calculated_steps = ceil(dt / dt_safe);
+        int num_steps = 1;
+        if (isinf(calculated_steps) || isnan(calculated_steps)) num_steps = 1000;
+        else if (calculated_steps > 1000) num_steps = 1000;
+        else if (calculated_steps > 1.0) num_steps = (int)calculated_steps;
+        double dt_sub = dt / num_steps;
+        ss->sat_max = global_max_div_q;
+        ss->sat_nsub = num_steps;
+        ss->sat_dt_sub_over_phi = dt_sub / porosity;
+    }
+}
+
+__global__ void k_sat_update(const uint8_t* __restrict__ img, const double* __restrict__ qx,
+                             const double* __restrict__ qy, const double* __restrict__ qz,
+                             double* __restrict__ s, Geo g, double dx, double evap,
+                             const StepScalars* __restrict__ ss) {
+    const int k = blockIdx.x * TK + threadIdx.x;
+    const int j = blockIdx.y * TJ + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nk || j >= g.nj) return;
+    const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
+    if (sat_inactive(img[v])) return;  // inactive voxels keep their value
+    double div_q, evap_sink;
+    sat_rates(img, qx, qy, qz, g, i, j, k, v, dx, evap, div_q, evap_sink);
+    const int nsub = ss->sat_nsub;
+    const double dt_sub_over_phi = ss->sat_dt_sub_over_phi;
+    double sv = s[v];
+    for (int step = 0; step < nsub; ++step) {
+        double new_sat = sv - dt_sub_over_phi * (div_q + evap_sink);
+        sv = dmax(0.0, dmin(1.0, new_sat));
+    }
+    s[v] = sv;
+}
+
+int phase_saturation(ls_ctx* c, double dt, ls_step_report* rep) {
+    Geo g = geo_of(c);
+    uint32_t N = (uint32_t)c->N;
+    int nb = ew_blocks(N);
+    hipLaunchKernelGGL(k_sat_max, dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img, c->qx, c->qy,
+                       c->qz, g, N, c->P.dx, c->P.evap_flux, c->partials);
+    LS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sat_finalize, dim3(1), dim3(1024), 0, c->stream, c->partials, nb, dt,
+                       c->P.porosity, c->ss);
+    LS_LAUNCH_CHECK();
+    dim3 blk(TK, TJ), grd((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, c->ni);
+    hipLaunchKernelGGL(k_sat_update, grd, blk, 0, c->stream, c->img, c->qx, c->qy, c->qz, c->s, g,
+                       c->P.dx, c->P.evap_flux, c->ss);
+    LS_LAUNCH_CHECK();
+    (void)rep;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Surface effects (src/Simulation.cpp:2063-2103): Pore with s > 0 touching
+// Air becomes Precipitate with conc = 1.  Written to a second label buffer so
+// no thread reads a label another thread is changing.
+// ---------------------------------------------------------------------------
+__global__ void k_surface(const uint8_t* __restrict__ img, uint8_t* __restrict__ img_out,
+                          const double* __restrict__ s, double* __restrict__ conc, Geo g) {
+    const int k = blockIdx.x * TK + threadIdx.x;
+    const int j = blockIdx.y * TJ + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k >= g.nk || j >= g.nj) return;
+    const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
+    uint8_t t = img[v];
+    if (t == PORE && s[v] > 0.0) {
+        bool is_surface_voxel = false;
+        for (int f = 0; f < 6; ++f) {
+            uint32_t n;
+            if (nbr(g, i, j, k, v, f, n) && img[n] == AIR) { is_surface_voxel = true; break; }
+        }
+        if (is_surface_voxel) {
+            conc[v] = 1.0;
+            t = PREC;
+        }
+    }
+    img_out[v] = t;
+}
+
+int phase_surface(ls_ctx* c) {
+    Geo g = geo_of(c);
+    dim3 blk(TK, TJ), grd((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, c->ni);
+    hipLaunchKernelGGL(k_surface, grd, blk, 0, c->stream, c->img, c->img2, c->s, c->conc, g);
+    LS_LAUNCH_CHECK();
+    uint8_t* tmp = c->img;
+    c->img = c->img2;
+    c->img2 = tmp;
+    return 0;
+}
+
+// The transport assembly zeroes q at Rock and Precipitate voxels
+// (src/Simulation.cpp:1132-1150); the fluxes written back keep that.
+__global__ void k_zero_flux(const uint8_t* __restrict__ img, double* qx, double* qy, double* qz,
+                            uint32_t N) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        uint8_t t = img[v];
+        if (t == PREC || t == ROCK) {
+            qx[v] = 0.0;
+            qy[v] = 0.0;
+            qz[v] = 0.0;
+        }
+    }
+}
+
+int zero_flux_rock_precip(ls_ctx* c) {
+    uint32_t N = (uint32_t)c->N;
+    hipLaunchKernelGGL(k_zero_flux, dim3(ew_blocks(N)), dim3(EW_BLOCK), 0, c->stream, c->img,
+                       c->qx, c->qy, c->qz, N);
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Precipitation (src/Simulation.cpp:2114-2165).  Pointwise.
+// ---------------------------------------------------------------------------
+__global__ void k_precip(uint8_t* __restrict__ img, double* __restrict__ conc,
+                         double* __restrict__ inv, const double* __restrict__ s, uint32_t N,
+                         double c_sat, int instant, double porosity, double solid_density,
+                         double time_scalar) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        double cv = conc[v];
+        if (img[v] == PORE && cv > c_sat) {
+            double excess_conc = cv - c_sat;
+            if (instant) {
+                img[v] = PREC;
+            } else {
+                double fluid_vol_fraction = porosity * s[v];
+                double precipitated_vol_fraction = (excess_conc * fluid_vol_fraction) / solid_density;
+                double iv = inv[v] + (precipitated_vol_fraction * time_scalar);
+                inv[v] = iv;
+                double critical_limit = 0.90;
+                if (iv > critical_limit) img[v] = PREC;
+            }
+            conc[v] = c_sat;
+        }
+    }
+}
+
+int phase_precipitation(ls_ctx* c) {
+    uint32_t N = (uint32_t)c->N;
+    hipLaunchKernelGGL(k_precip, dim3(ew_blocks(N)), dim3(EW_BLOCK), 0, c->stream, c->img,
+                       c->conc, c->inv, c->s, N, c->P.c_sat, c->P.instant_precip, c->P.porosity,
+                       c->P.solid_density, c->P.time_scalar);
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Step summary: min/max of p, conc, acid and FP64 sums of conc, acid, s (the
+// per-step mass balance).
+// ---------------------------------------------------------------------------
+__global__ void k_summary(const double* __restrict__ p, const double* __restrict__ conc,
+                          const double* __restrict__ acid, const double* __restrict__ s,
+                          uint32_t N, double* partials) {
+    __shared__ double lds[4 * EW_BLOCK / WAVE];
+    double pmn = 1.0 / 0.0, pmx = -1.0 / 0.0, cmn = 1.0 / 0.0, cmx = -1.0 / 0.0;
+    double amn = 1.0 / 0.0, amx = -1.0 / 0.0, sc = 0.0, sa = 0.0, sw = 0.0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        double a = p[v], b = conc[v], d = acid[v];
+        pmn = fmin(pmn, a); pmx = fmax(pmx, a);
+        cmn = fmin(cmn, b); cmx = fmax(cmx, b);
+        amn = fmin(amn, d); amx = fmax(amx, d);
+        sc += b; sa += d; sw += s[v];
+    }
+    double r1[4] = {pmn, cmn, amn, 0.0};
+    block_reduce<4, RED_MIN, RED_MIN, RED_MIN, RED_SUM>(r1, lds);
+    __syncthreads();
+    double r2[3] = {pmx, cmx, amx};
+    block_reduce<3, RED_MAX, RED_MAX, RED_MAX>(r2, lds);
+    __syncthreads();
+    double r3[3] = {sc, sa, sw};
+    block_reduce<3, RED_SUM, RED_SUM, RED_SUM>(r3, lds);
+    if (threadIdx.x == 0) {
+        const int nb = gridDim.x;
+        partials[0 * nb + blockIdx.x] = r1[0];
+        partials[1 * nb + blockIdx.x] = r2[0];
+        partials[2 * nb + blockIdx.x] = r1[1];
+        partials[3 * nb + blockIdx.x] = r2[1];
+        partials[4 * nb + blockIdx.x] = r1[2];
+        partials[5 * nb + blockIdx.x] = r2[2];
+        partials[6 * nb + blockIdx.x] = r3[0];
+        partials[7 * nb + blockIdx.x] = r3[1];
+        partials[8 * nb + blockIdx.x] = r3[2];
+    }
+}
+
+__global__ void k_summary_finalize(const double* __restrict__ partials, int nb, StepScalars* ss) {
+    __shared__ double lds[1024 / WAVE];
+    for (int q = 0; q < 9; ++q) {
+        const int op = (q == 0 || q == 2 || q == 4) ? RED_MIN : (q < 6) ? RED_MAX : RED_SUM;
+        double acc = op == RED_MIN ? 1.0 / 0.0 : op == RED_MAX ? -1.0 / 0.0 : 0.0;
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+            double b = partials[q * nb + i];
+            acc = op == RED_MIN ? fmin(acc, b) : op == RED_MAX ? fmax(acc, b) : acc + b;
+        }
+        double a[1] = {acc};
+        if (op == RED_MIN) block_reduce<1, RED_MIN>(a, lds);
+        else if (op == RED_MAX) block_reduce<1, RED_MAX>(a, lds);
+        else block_reduce<1, RED_SUM>(a, lds);
+        if (threadIdx.x == 0) ss->sums[q] = a[0];
+        __syncthreads();
+    }
+}
+
+int summarize(ls_ctx* c, ls_step_report* rep) {
+    uint32_t N = (uint32_t)c->N;
+    int nb = ew_blocks(N);
+    if (nb * 9 > MAX_PARTIALS * 8) nb = MAX_PARTIALS * 8 / 9;
+    hipLaunchKernelGGL(k_summary, dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->p, c->conc, c->acid,
+                       c->s, N, c->partials);
+    LS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_summary_finalize, dim3(1), dim3(1024), 0, c->stream, c->partials, nb,
+                       c->ss);
+    LS_LAUNCH_CHECK();
+    (void)rep;
+    return 0;
+}
+
+}  // namespace ls

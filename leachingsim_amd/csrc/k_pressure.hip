@@ -1,0 +1,505 @@
+// k_pressure.hip — the Darcy pressure solve (replaces KSPSolve in
+// src/Simulation.cpp:804-855) on the reduced SPD system.
+//
+// The Pore rows with |diagonal| >= 1e-16 form the solved set R.  Jacobi-PCG
+// on A is run as CG on the symmetrically scaled operator
+//     A^ = D^-1/2 A D^-1/2,  A^ x = x - sum_faces c^_f x_n,
+// whose faces c^ are zero wherever either side is not in R.  Vectors are zero
+// outside R, so the operator needs no mask byte: y = A^ x reads x and the
+// three +face arrays and writes y — 2*s_v + 3*s_c bytes per voxel.
+//
+// Kernels per CG iteration: stencil + p.Ap, finalize(alpha), update x, r +
+// r.r, finalize(beta, stop test), direction update.  All scalars stay on the
+// device; every kernel returns at once when the `done` flag is set, so the
+// host launches iterations in batches and polls once per batch.
+#include <cmath>
+
+#include "ls_context.h"
+
+namespace ls {
+
+struct SGeo {
+    int ni, nj, nk;
+    uint32_t si, sj;
+};
+
+static SGeo sgeo(const ls_ctx* c) {
+    SGeo g;
+    g.ni = c->ni; g.nj = c->nj; g.nk = c->nk;
+    g.sj = (uint32_t)c->nk;
+    g.si = (uint32_t)c->nk * (uint32_t)c->nj;
+    return g;
+}
+
+static dim3 stencil_grid(const ls_ctx* c) {
+    return dim3((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, (c->ni + ICHUNK - 1) / ICHUNK);
+}
+
+// y = A^ x, marching ICHUNK planes along i per thread.  Optionally also
+// y += 0 and the block partial of x.y (DOT) and the residual form
+// r = b - A^ x (RES).  MODE 0: y = A^x, dot x.y.   MODE 1: r = b - A^x, p = r,
+// partials (r.r, sum (sd r)^2, sum (sd b)^2).
+template <typename T, int MODE>
+__global__ __launch_bounds__(TK * TJ) void k_stencil(
+    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ cx,
+    const T* __restrict__ cy, const T* __restrict__ cz, SGeo g, double* __restrict__ partials,
+    const int* __restrict__ done, const double* __restrict__ b, T* __restrict__ p2,
+    const double* __restrict__ sd) {
+    if (done != nullptr && *done) return;
+    __shared__ double lds[3 * (TK * TJ) / WAVE];
+    const int nbx = gridDim.x, nby = gridDim.y;
+    const int nb = nbx * nby * gridDim.z;
+    const int hw = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    const int t = xcd_remap(hw, nb);
+    const int bx = t % nbx;
+    const int rest = t / nbx;
+    const int by = rest % nby;
+    const int bz = rest / nby;
+    const int k = bx * TK + threadIdx.x;
+    const int j = by * TJ + threadIdx.y;
+    const int i0 = bz * ICHUNK;
+    const int i1 = min(i0 + ICHUNK, g.ni);
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    if (k < g.nk && j < g.nj) {
+        uint32_t v = (uint32_t)i0 * g.si + (uint32_t)j * g.sj + (uint32_t)k;
+        const bool hj = j > 0, hk = k > 0, pj = j + 1 < g.nj, pk = k + 1 < g.nk;
+        T xm = (i0 > 0) ? x[v - g.si] : T(0);
+        T cxm = (i0 > 0) ? cx[v - g.si] : T(0);
+        T xc = x[v];
+        for (int i = i0; i < i1; ++i, v += g.si) {
+            const T xp = (i + 1 < g.ni) ? x[v + g.si] : T(0);
+            const T cxp = cx[v];
+            const T cyp = cy[v];
+            const T czp = cz[v];
+            const T cym = hj ? cy[v - g.sj] : T(0);
+            const T czm = hk ? cz[v - 1] : T(0);
+            const T xjp = pj ? x[v + g.sj] : T(0);
+            const T xjm = hj ? x[v - g.sj] : T(0);
+            const T xkp = pk ? x[v + 1] : T(0);
+            const T xkm = hk ? x[v - 1] : T(0);
+            const T off = cxp * xp + cxm * xm + cyp * xjp + cym * xjm + czp * xkp + czm * xkm;
+            const T ax = xc - off;
+            if (MODE == 0) {
+                y[v] = ax;
+                acc0 += (double)xc * (double)ax;
+            } else {
+                const double bv = b[v];
+                const double r = bv - (double)ax;
+                y[v] = (T)r;
+                p2[v] = (T)r;
+                const double s = sd[v];
+                acc0 += r * r;
+                acc1 += (s * r) * (s * r);
+                acc2 += (s * bv) * (s * bv);
+            }
+            xm = xc;
+            xc = xp;
+            cxm = cxp;
+        }
+    }
+    if (MODE == 0) {
+        double a[1] = {acc0};
+        block_reduce<1, RED_SUM>(a, lds);
+        if (flat_tid() == 0) partials[t] = a[0];
+    } else {
+        double a[3] = {acc0, acc1, acc2};
+        block_reduce<3, RED_SUM, RED_SUM, RED_SUM>(a, lds);
+        if (flat_tid() == 0) {
+            partials[t] = a[0];
+            partials[nb + t] = a[1];
+            partials[2 * nb + t] = a[2];
+        }
+    }
+}
+
+// alpha = r.r / p.Ap
+__global__ void k_cg_alpha(const double* __restrict__ partials, int n, CgScalars* cs) {
+    if (cs->done) return;
+    __shared__ double lds[1024 / WAVE];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partials[i];
+    double a[1] = {acc};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) {
+        double pAp = a[0];
+        cs->pAp = pAp;
+        if (!(pAp > 0.0) || !isfinite(pAp)) {  // breakdown: not SPD or NaN
+            cs->done = 1;
+            cs->converged = 0;
+            cs->alpha = 0.0;
+        } else {
+            cs->alpha = cs->rr / pAp;
+        }
+    }
+}
+
+// x += alpha p, r -= alpha Ap, partials of r.r and (when sd != null) the
+// unscaled residual norm sum (sd r)^2.
+template <typename T>
+__global__ void k_cg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+                            const T* __restrict__ ap, const double* __restrict__ sd,
+                            const CgScalars* __restrict__ cs, uint32_t N,
+                            double* __restrict__ partials) {
+    if (cs->done) return;
+    __shared__ double lds[2 * EW_BLOCK / WAVE];
+    const double alpha = cs->alpha;
+    const T a = (T)alpha;
+    double rr = 0.0, wrr = 0.0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        const T pv = p[v];
+        const T rv = r[v] - a * ap[v];
+        x[v] = x[v] + a * pv;
+        r[v] = rv;
+        const double rd = (double)rv;
+        rr += rd * rd;
+        if (sd != nullptr) {
+            const double w = sd[v] * rd;
+            wrr += w * w;
+        }
+    }
+    double q[2] = {rr, wrr};
+    block_reduce<2, RED_SUM, RED_SUM>(q, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = q[0];
+        partials[gridDim.x + blockIdx.x] = q[1];
+    }
+}
+
+// beta = r.r(new) / r.r(old); stop test on the unscaled residual (outer FP64
+// solves, use_w = 1) or on the scaled residual (inner FP32 solves).
+__global__ void k_cg_beta(const double* __restrict__ partials, int n, CgScalars* cs, int use_w) {
+    if (cs->done) return;
+    __shared__ double lds[2 * 1024 / WAVE];
+    double a0 = 0.0, a1 = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        a0 += partials[i];
+        a1 += partials[n + i];
+    }
+    double a[2] = {a0, a1};
+    block_reduce<2, RED_SUM, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) {
+        const double rr_new = a[0];
+        const double wrr = a[1];
+        cs->iter += 1;
+        const double test = use_w ? wrr : rr_new;
+        cs->wrr = wrr;
+        cs->relres = use_w ? sqrt(wrr / cs->bnorm2) : sqrt(rr_new / cs->bnorm2);
+        cs->beta = rr_new / cs->rr;
+        cs->rr = rr_new;
+        if (!isfinite(rr_new)) {
+            cs->done = 1;
+            cs->converged = 0;
+        } else if (test <= cs->tol2) {
+            cs->done = 1;
+            cs->converged = 1;
+        } else if (cs->iter >= cs->maxit) {
+            cs->done = 1;
+            cs->converged = 0;
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_cg_direction(T* __restrict__ p, const T* __restrict__ r,
+                               const CgScalars* __restrict__ cs, uint32_t N) {
+    if (cs->done) return;
+    const T beta = (T)cs->beta;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        p[v] = r[v] + beta * p[v];
+}
+
+// After the residual kernel (MODE 1): r.r, ||r||_w^2, ||b||^2.
+__global__ void k_cg_start(const double* __restrict__ partials, int n, CgScalars* cs, double rtol,
+                           int maxit) {
+    __shared__ double lds[3 * 1024 / WAVE];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        a0 += partials[i];
+        a1 += partials[n + i];
+        a2 += partials[2 * n + i];
+    }
+    double a[3] = {a0, a1, a2};
+    block_reduce<3, RED_SUM, RED_SUM, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) {
+        cs->rr = a[0];
+        cs->wrr = a[1];
+        cs->bnorm2 = a[2];
+        cs->tol2 = (rtol * rtol) * a[2];
+        cs->relres = (a[2] > 0.0) ? sqrt(a[1] / a[2]) : 0.0;
+        cs->iter = 0;
+        cs->maxit = maxit;
+        cs->alpha = cs->beta = cs->pAp = 0.0;
+        const bool ok = a[1] <= cs->tol2;
+        cs->done = ok ? 1 : 0;
+        cs->converged = ok ? 1 : 0;
+    }
+}
+
+// FP32 inner solve start: r32 = r64 / scale, p32 = r32, x32 = 0, partial r.r.
+__global__ void k_inner_start(const double* __restrict__ r64, float* __restrict__ r32,
+                              float* __restrict__ p32, float* __restrict__ x32, uint32_t N,
+                              double inv_scale, double* __restrict__ partials) {
+    __shared__ double lds[EW_BLOCK / WAVE];
+    double rr = 0.0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        const float rv = (float)(r64[v] * inv_scale);
+        r32[v] = rv;
+        p32[v] = rv;
+        x32[v] = 0.0f;
+        rr += (double)rv * (double)rv;
+    }
+    double a[1] = {rr};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a[0];
+}
+
+__global__ void k_inner_init(const double* __restrict__ partials, int n, CgScalars* cs,
+                             double rtol, int maxit) {
+    __shared__ double lds[1024 / WAVE];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partials[i];
+    double a[1] = {acc};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) {
+        cs->rr = a[0];
+        cs->bnorm2 = a[0];
+        cs->tol2 = (rtol * rtol) * a[0];
+        cs->wrr = 0.0;
+        cs->relres = 1.0;
+        cs->iter = 0;
+        cs->maxit = maxit;
+        cs->done = (a[0] > 0.0) ? 0 : 1;
+        cs->converged = cs->done;
+    }
+}
+
+// x64 += scale * x32  (the FP64 refinement update)
+__global__ void k_refine_add(double* __restrict__ x64, const float* __restrict__ x32, uint32_t N,
+                             double scale) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        x64[v] += scale * (double)x32[v];
+}
+
+__global__ void k_scale_vec(const double* __restrict__ a, const double* __restrict__ s,
+                            double* __restrict__ out, uint32_t N) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        out[v] = a[v] * s[v];
+}
+
+static int nblocks_stencil(const ls_ctx* c) {
+    dim3 g = stencil_grid(c);
+    return (int)(g.x * g.y * g.z);
+}
+
+// r = b^ - A^ x^ (FP64), p = r; fills cs (rr, wrr, bnorm2, tol2, done).
+static int cg64_residual(ls_ctx* c, double rtol, int maxit) {
+    SGeo g = sgeo(c);
+    dim3 grd = stencil_grid(c), blk(TK, TJ);
+    int nb = nblocks_stencil(c);
+    hipLaunchKernelGGL((k_stencil<double, 1>), grd, blk, 0, c->stream, c->x64, c->r64, c->c64[0],
+                       c->c64[1], c->c64[2], g, c->partials, nullptr, c->bhat, c->p64, c->sd);
+    LS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(1024), 0, c->stream, c->partials, nb, c->cg, rtol,
+                       maxit);
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+template <typename T>
+static int cg_iterations(ls_ctx* c, T* x, T* r, T* p, T* ap, T* const* faces, const double* sdw,
+                         int batch) {
+    SGeo g = sgeo(c);
+    dim3 grd = stencil_grid(c), blk(TK, TJ);
+    int nbs = nblocks_stencil(c);
+    uint32_t N = (uint32_t)c->N;
+    int nbe = ew_blocks(N);
+    for (int b = 0; b < batch; ++b) {
+        hipLaunchKernelGGL((k_stencil<T, 0>), grd, blk, 0, c->stream, p, ap, faces[0], faces[1],
+                           faces[2], g, c->partials, &c->cg->done, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, c->stream, c->partials, nbs, c->cg);
+        hipLaunchKernelGGL(k_cg_update<T>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, x, r, p, ap,
+                           sdw, c->cg, N, c->partials);
+        hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1024), 0, c->stream, c->partials, nbe, c->cg,
+                           sdw != nullptr ? 1 : 0);
+        hipLaunchKernelGGL(k_cg_direction<T>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, p, r, c->cg,
+                           N);
+    }
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+static int poll_cg(ls_ctx* c) {
+    LS_HIP(hipMemcpyAsync(c->h_cg, c->cg, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int batch_size(const ls_ctx* c) {
+    return c->N >= (size_t)64 * 64 * 64 * 8 ? 16 : 32;
+}
+
+// Plain FP64 Jacobi-PCG (as CG on A^).  Returns 0 converged, 1 not.
+static int solve_plain64(ls_ctx* c, ls_step_report* rep) {
+    int st = cg64_residual(c, c->P.press_rtol, c->P.press_maxit);
+    if (st) return st;
+    if ((st = poll_cg(c))) return st;
+    if (c->h_cg->bnorm2 == 0.0) {
+        LS_HIP(hipMemsetAsync(c->x64, 0, c->N * sizeof(double), c->stream));
+        rep->press_relres = 0.0;
+        rep->press_bnorm = 0.0;
+        return 0;
+    }
+    int total = 0, restarts = 0;
+    const int bs = batch_size(c);
+    while (true) {
+        while (!c->h_cg->done) {
+            if ((st = cg_iterations<double>(c, c->x64, c->r64, c->p64, c->ap64, c->c64, c->sd, bs)))
+                return st;
+            if ((st = poll_cg(c))) return st;
+        }
+        total += c->h_cg->iter;
+        const bool conv = c->h_cg->converged;
+        // confirm on the recomputed residual
+        if ((st = cg64_residual(c, c->P.press_rtol, c->P.press_maxit - total))) return st;
+        if ((st = poll_cg(c))) return st;
+        if (c->h_cg->converged || !conv || total >= c->P.press_maxit || restarts > 5) break;
+        ++restarts;
+    }
+    rep->press_iters += total;
+    rep->press_outer += 1;
+    rep->press_relres = c->h_cg->relres;
+    rep->press_bnorm = sqrt(c->h_cg->bnorm2);
+    return c->h_cg->converged ? 0 : 1;
+}
+
+// Mixed precision: FP32 CG on A^ (FP32 faces, FP64 dot products) inside FP64
+// iterative refinement on the FP64 residual.
+static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
+    uint32_t N = (uint32_t)c->N;
+    int nbe = ew_blocks(N);
+    const int bs = batch_size(c);
+    int st;
+    int inner_total = 0;
+    double bnorm = 0.0;
+    for (int outer = 0; outer < 50; ++outer) {
+        if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+        if ((st = poll_cg(c))) return st;
+        bnorm = sqrt(c->h_cg->bnorm2);
+        rep->press_relres = c->h_cg->relres;
+        rep->press_bnorm = bnorm;
+        if (bnorm == 0.0) {
+            LS_HIP(hipMemsetAsync(c->x64, 0, c->N * sizeof(double), c->stream));
+            rep->press_relres = 0.0;
+            return 0;
+        }
+        if (c->h_cg->converged) {
+            rep->press_iters += inner_total;
+            return 0;
+        }
+        if (inner_total >= c->P.press_maxit) break;
+        rep->press_outer += 1;
+        const double scale = sqrt(c->h_cg->rr);
+        hipLaunchKernelGGL(k_inner_start, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->r64, c->r32,
+                           c->p32, c->x32, N, 1.0 / scale, c->partials);
+        hipLaunchKernelGGL(k_inner_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nbe,
+                           c->cg, c->P.mixed_inner_rtol, c->P.mixed_inner_maxit);
+        LS_LAUNCH_CHECK();
+        if ((st = poll_cg(c))) return st;
+        while (!c->h_cg->done) {
+            if ((st = cg_iterations<float>(c, c->x32, c->r32, c->p32, c->ap32, c->c32, nullptr, bs)))
+                return st;
+            if ((st = poll_cg(c))) return st;
+        }
+        inner_total += c->h_cg->iter;
+        hipLaunchKernelGGL(k_refine_add, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->x64, c->x32,
+                           N, scale);
+        LS_LAUNCH_CHECK();
+    }
+    rep->press_iters += inner_total;
+    return 1;
+}
+
+static double effective_top_pressure(double top, double t) {
+    const double ramp_duration = 3600.0;
+    double eff = top;
+    if (t < ramp_duration) eff = top * (t / ramp_duration) + 1e-6;
+    return eff;
+}
+
+int solve_pressure(ls_ctx* c, double t, ls_step_report* rep) {
+    const double ptop = effective_top_pressure(c->P.top_pressure, t);
+    int st = assemble_pressure(c, t);
+    if (st) return st;
+    if (c->P.solver == LS_SOLVER_PLAIN64) st = solve_plain64(c, rep);
+    else if (c->P.solver == LS_SOLVER_MIXED) st = solve_mixed(c, rep);
+    else {
+        c->err = "learned solver not available in this build";
+        return -20;
+    }
+    if (st < 0) return st;
+    int st2 = finalize_pressure(c, ptop);
+    if (st2) return st2;
+    c->p_top_prev = ptop;
+    c->have_prev = true;
+    rep->top_pressure_eff = ptop;
+    LS_HIP(hipMemcpyAsync(c->h_ss, c->ss, sizeof(StepScalars), hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    rep->n_floored = (long long)c->h_ss->n_floored;
+    if (st > 0) c->err = "pressure solve did not converge";
+    return st;
+}
+
+int apply_pressure_unscaled(ls_ctx* c, const double* x_dev, double* y_dev) {
+    if (!c->press_assembled) {
+        c->err = "no pressure system assembled";
+        return -1;
+    }
+    uint32_t N = (uint32_t)c->N;
+    int nbe = ew_blocks(N);
+    SGeo g = sgeo(c);
+    hipLaunchKernelGGL(k_scale_vec, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, x_dev, c->sd, c->p64,
+                       N);
+    hipLaunchKernelGGL((k_stencil<double, 0>), stencil_grid(c), dim3(TK, TJ), 0, c->stream, c->p64,
+                       c->ap64, c->c64[0], c->c64[1], c->c64[2], g, c->partials, nullptr, nullptr,
+                       nullptr, nullptr);
+    hipLaunchKernelGGL(k_scale_vec, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->ap64, c->sd, y_dev,
+                       N);
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) {
+    if (!c->press_assembled) {
+        c->err = "no pressure system assembled";
+        return -1;
+    }
+    SGeo g = sgeo(c);
+    dim3 grd = stencil_grid(c), blk(TK, TJ);
+    hipEvent_t e0 = c->ev[14], e1 = c->ev[15];
+    // warm up once
+    for (int pass = 0; pass < 2; ++pass) {
+        int n = pass == 0 ? 1 : reps;
+        if (pass == 1) LS_HIP(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < n; ++r) {
+            if (precision == 32)
+                hipLaunchKernelGGL((k_stencil<float, 0>), grd, blk, 0, c->stream, c->p32, c->ap32,
+                                   c->c32[0], c->c32[1], c->c32[2], g, c->partials, nullptr,
+                                   nullptr, nullptr, nullptr);
+            else
+                hipLaunchKernelGGL((k_stencil<double, 0>), grd, blk, 0, c->stream, c->p64, c->ap64,
+                                   c->c64[0], c->c64[1], c->c64[2], g, c->partials, nullptr,
+                                   nullptr, nullptr, nullptr);
+        }
+        LS_LAUNCH_CHECK();
+    }
+    LS_HIP(hipEventRecord(e1, c->stream));
+    LS_HIP(hipEventSynchronize(e1));
+    float f = 0.0f;
+    LS_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = (double)f / reps;
+    const double sv = precision == 32 ? 4.0 : 8.0;
+    *bytes = (double)c->N * (2.0 * sv + 3.0 * sv);
+    return 0;
+}
+
+}  // namespace ls

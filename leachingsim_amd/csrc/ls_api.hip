@@ -1,0 +1,498 @@
+// ls_api.hip — the C ABI (include/lsgpu.h): context lifetime, host<->device
+// transfers, the coupled step (src/main.cpp:303-432) and snapshot/rollback.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "ls_context.h"
+
+using namespace ls;
+
+namespace {
+
+template <typename T>
+int dalloc(ls_ctx* c, T** p, size_t n) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T) + 256);
+    if (e != hipSuccess) {
+        c->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        *p = nullptr;
+        return -12;
+    }
+    c->allocations.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+int copy_d2d(ls_ctx* c, void* dst, const void* src, size_t bytes) {
+    LS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+struct Timer {
+    ls_ctx* c;
+    int n = 0;
+    void mark() {
+        if (n < 14) (void)hipEventRecord(c->ev[n++], c->stream);
+    }
+    double between(int a, int b) {
+        float f = 0.0f;
+        if (a < n && b < n) (void)hipEventElapsedTime(&f, c->ev[a], c->ev[b]);
+        return f;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+void ls_default_params(ls_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->D = 1.0;
+    p->dx = 1.0;
+    p->Dpore_fac = 1.0;
+    p->kext = 1.0;
+    p->kreac = 1.0;
+    p->theta = 1.0;
+    p->c_sat = 0.95;
+    p->evap_flux = 0.0;
+    p->top_pressure = 10000.0;
+    p->max_cap_grad = 50000.0;
+    p->vg_alpha = 1.0;
+    p->vg_n = 2.0;
+    p->s_res = 0.05;
+    p->porosity = 0.1;
+    p->solid_density = 1.0;
+    p->time_scalar = 1.0;
+    p->alpha_transfer = 1e-5;
+    p->instant_precip = 0;
+    p->bc_mode = LS_BC_REFERENCE;
+    p->solver = LS_SOLVER_PLAIN64;
+    p->warm_start = LS_WARM_PREVIOUS;
+    p->press_rtol = 1e-8;
+    p->press_maxit = 10000;
+    p->adr_rtol = 1e-8;
+    p->adr_maxit = 10000;
+    p->mixed_inner_maxit = 500;
+    p->mixed_inner_rtol = 1e-4;
+}
+
+int ls_abi_info(size_t* sp, size_t* sr) {
+    if (sp) *sp = sizeof(ls_params);
+    if (sr) *sr = sizeof(ls_step_report);
+    return LSGPU_ABI_VERSION;
+}
+
+static int validate_params(ls_ctx* c, const ls_params* p) {
+    if (!(p->dx > 0.0) || !(p->porosity > 0.0) || !(p->vg_n > 1.0)) {
+        c->err = "invalid parameters: dx, porosity must be > 0 and vg_n > 1";
+        return -3;
+    }
+    if (p->solver < 0 || p->solver > 2 || p->bc_mode < 0 || p->bc_mode > 1) {
+        c->err = "invalid solver or boundary mode";
+        return -3;
+    }
+    return 0;
+}
+
+int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes, const ls_params* params) {
+    if (!out || !params) return -1;
+    *out = nullptr;
+    ls_ctx* c = new (std::nothrow) ls_ctx();
+    if (!c) return -1;
+    if (ni < 2 || nj < 2 || nk < 2 || (label_bytes != 1 && label_bytes != 2) ||
+        (double)ni * nj * nk >= 4294967295.0) {
+        delete c;
+        return -2;
+    }
+    c->ni = ni; c->nj = nj; c->nk = nk;
+    c->N = (size_t)ni * nj * nk;
+    c->label_bytes = label_bytes;
+    if (int st = validate_params(c, params)) { delete c; return st; }
+    c->P = *params;
+    c->log_k_rock = std::log10(1.0e-18);  // src/Simulation.cpp:204-205
+    c->log_k_pore = std::log10(1.0e-12);
+    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        c->err = "no HIP device";
+        *out = c;
+        return -4;
+    }
+    for (int i = 0; i < 16; ++i) (void)hipEventCreate(&c->ev[i]);
+    const size_t N = c->N;
+    int st = 0;
+    st |= dalloc(c, &c->img, N);
+    st |= dalloc(c, &c->img2, N);
+    st |= dalloc(c, &c->frac, N);
+    for (double** f : {&c->inv, &c->s, &c->p, &c->conc, &c->acid, &c->qx, &c->qy, &c->qz})
+        st |= dalloc(c, f, N);
+    st |= dalloc(c, &c->b_img, N);
+    st |= dalloc(c, &c->b_frac, N);
+    for (double** f : {&c->b_inv, &c->b_s, &c->b_p, &c->b_conc, &c->b_acid, &c->b_qx, &c->b_qy,
+                       &c->b_qz})
+        st |= dalloc(c, f, N);
+    st |= dalloc(c, &c->perm, N);
+    st |= dalloc(c, &c->pc, N);
+    for (int a = 0; a < 3; ++a) {
+        st |= dalloc(c, &c->craw[a], N);
+        st |= dalloc(c, &c->c64[a], N);
+        st |= dalloc(c, &c->c32[a], N);
+    }
+    st |= dalloc(c, &c->cin, (size_t)nj * nk);
+    st |= dalloc(c, &c->dsum, N);
+    st |= dalloc(c, &c->cls, N);
+    st |= dalloc(c, &c->pD, N);
+    st |= dalloc(c, &c->sd, N);
+    st |= dalloc(c, &c->bhat, N);
+    for (double** f : {&c->x64, &c->r64, &c->p64, &c->ap64}) st |= dalloc(c, f, N);
+    for (float** f : {&c->x32, &c->r32, &c->p32, &c->ap32}) st |= dalloc(c, f, N);
+    for (int f = 0; f < 2; ++f) {
+        for (int q = 0; q < 8; ++q) st |= dalloc(c, &c->tv[f][q], N);
+        st |= dalloc(c, &c->tb[f], N);
+    }
+    st |= dalloc(c, &c->tinvd, N);
+    st |= dalloc(c, &c->partials, (size_t)MAX_PARTIALS * 8);
+    st |= dalloc(c, &c->cg, 1);
+    st |= dalloc(c, &c->bicg, 1);
+    st |= dalloc(c, &c->ss, 1);
+    if (st) {
+        *out = c;
+        return -12;
+    }
+    if (hipHostMalloc((void**)&c->h_cg, sizeof(CgScalars)) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_bicg, sizeof(BicgScalars)) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_ss, sizeof(StepScalars)) != hipSuccess) {
+        c->err = "hipHostMalloc failed";
+        *out = c;
+        return -12;
+    }
+    // a fresh context holds Rock everywhere and zero fields
+    (void)hipMemsetAsync(c->img, LS_ROCK, N, c->stream);
+    (void)hipMemsetAsync(c->frac, 0, N * sizeof(float), c->stream);
+    for (double* f : {c->inv, c->s, c->p, c->conc, c->acid, c->qx, c->qy, c->qz, c->x64, c->p64})
+        (void)hipMemsetAsync(f, 0, N * sizeof(double), c->stream);
+    (void)hipMemsetAsync(c->p32, 0, N * sizeof(float), c->stream);
+    (void)hipMemsetAsync(c->cg, 0, sizeof(CgScalars), c->stream);
+    (void)hipMemsetAsync(c->bicg, 0, sizeof(BicgScalars), c->stream);
+    (void)hipMemsetAsync(c->ss, 0, sizeof(StepScalars), c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        c->err = "initialisation failed";
+        *out = c;
+        return -10;
+    }
+    *out = c;
+    return 0;
+}
+
+void ls_destroy(ls_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* q : c->allocations) (void)hipFree(q);
+    if (c->h_cg) (void)hipHostFree(c->h_cg);
+    if (c->h_bicg) (void)hipHostFree(c->h_bicg);
+    if (c->h_ss) (void)hipHostFree(c->h_ss);
+    if (c->stream) {
+        for (int i = 0; i < 16; ++i) (void)hipEventDestroy(c->ev[i]);
+        (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+}
+
+const char* ls_last_error(const ls_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int ls_get_params(const ls_ctx* c, ls_params* out) {
+    if (!c || !out) return -1;
+    *out = c->P;
+    return 0;
+}
+
+int ls_set_params(ls_ctx* c, const ls_params* p) {
+    if (!c || !p) return -1;
+    if (int st = validate_params(c, p)) return st;
+    c->P = *p;
+    return 0;
+}
+
+static void* field_ptr(ls_ctx* c, int field, size_t* eb) {
+    *eb = 8;
+    switch (field) {
+        case LS_F_FRAC: *eb = 4; return c->frac;
+        case LS_F_INVENTORY: return c->inv;
+        case LS_F_SATURATION: return c->s;
+        case LS_F_PRESSURE: return c->p;
+        case LS_F_CONC: return c->conc;
+        case LS_F_ACID: return c->acid;
+        case LS_F_QX: return c->qx;
+        case LS_F_QY: return c->qy;
+        case LS_F_QZ: return c->qz;
+        default: return nullptr;
+    }
+}
+
+int ls_upload(ls_ctx* c, int field, const void* host, size_t elem_bytes) {
+    if (!c || !host) return -1;
+    const size_t N = c->N;
+    if (field == LS_F_IMG) {
+        if (elem_bytes != (size_t)c->label_bytes) {
+            c->err = "label element size does not match the context";
+            return -3;
+        }
+        std::vector<uint8_t> lab(N);
+        for (size_t v = 0; v < N; ++v) {
+            unsigned x = elem_bytes == 1 ? ((const uint8_t*)host)[v] : ((const uint16_t*)host)[v];
+            if (x < 1 || x > 5) {
+                c->err = "label outside 1..5 at voxel " + std::to_string(v);
+                return -2;
+            }
+            lab[v] = (uint8_t)x;
+        }
+        LS_HIP(hipMemcpyAsync(c->img, lab.data(), N, hipMemcpyHostToDevice, c->stream));
+        LS_HIP(hipStreamSynchronize(c->stream));
+        c->press_assembled = false;
+        return 0;
+    }
+    size_t eb;
+    void* d = field_ptr(c, field, &eb);
+    if (!d || eb != elem_bytes) {
+        c->err = "field cannot be uploaded or has another element size";
+        return -3;
+    }
+    LS_HIP(hipMemcpyAsync(d, host, N * eb, hipMemcpyHostToDevice, c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ls_download(ls_ctx* c, int field, void* host, size_t elem_bytes) {
+    if (!c || !host) return -1;
+    const size_t N = c->N;
+    if (field == LS_F_IMG) {
+        if (elem_bytes != 1 && elem_bytes != 2) return -3;
+        std::vector<uint8_t> lab(N);
+        LS_HIP(hipMemcpyAsync(lab.data(), c->img, N, hipMemcpyDeviceToHost, c->stream));
+        LS_HIP(hipStreamSynchronize(c->stream));
+        for (size_t v = 0; v < N; ++v) {
+            if (elem_bytes == 1) ((uint8_t*)host)[v] = lab[v];
+            else ((uint16_t*)host)[v] = lab[v];
+        }
+        return 0;
+    }
+    if (field >= LS_F_KREL && field <= LS_F_MU) {
+        if (elem_bytes != 8) return -3;
+        // derived from the current state, as the reference recomputes them
+        // before writing VTK output (src/main.cpp:491-494)
+        if (int st = derived_field(c, field, c->tv[0][0])) return st;
+        LS_HIP(hipMemcpyAsync(host, c->tv[0][0], N * 8, hipMemcpyDeviceToHost, c->stream));
+        LS_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    size_t eb;
+    void* d = field_ptr(c, field, &eb);
+    if (!d || eb != elem_bytes) {
+        c->err = "unknown field or element size";
+        return -3;
+    }
+    LS_HIP(hipMemcpyAsync(host, d, N * eb, hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ls_init_state(ls_ctx* c, long long* nsul) {
+    if (!c) return -1;
+    int st = init_state(c, nsul);
+    if (st) return st;
+    LS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ls_snapshot(ls_ctx* c) {
+    if (!c) return -1;
+    const size_t N = c->N;
+    int st = 0;
+    st |= copy_d2d(c, c->b_img, c->img, N);
+    st |= copy_d2d(c, c->b_frac, c->frac, N * 4);
+    st |= copy_d2d(c, c->b_inv, c->inv, N * 8);
+    st |= copy_d2d(c, c->b_s, c->s, N * 8);
+    st |= copy_d2d(c, c->b_p, c->p, N * 8);
+    st |= copy_d2d(c, c->b_conc, c->conc, N * 8);
+    st |= copy_d2d(c, c->b_acid, c->acid, N * 8);
+    st |= copy_d2d(c, c->b_qx, c->qx, N * 8);
+    st |= copy_d2d(c, c->b_qy, c->qy, N * 8);
+    st |= copy_d2d(c, c->b_qz, c->qz, N * 8);
+    return st;
+}
+
+int ls_rollback(ls_ctx* c) {
+    if (!c) return -1;
+    const size_t N = c->N;
+    int st = 0;
+    st |= copy_d2d(c, c->img, c->b_img, N);
+    st |= copy_d2d(c, c->frac, c->b_frac, N * 4);
+    st |= copy_d2d(c, c->inv, c->b_inv, N * 8);
+    st |= copy_d2d(c, c->s, c->b_s, N * 8);
+    st |= copy_d2d(c, c->p, c->b_p, N * 8);
+    st |= copy_d2d(c, c->conc, c->b_conc, N * 8);
+    st |= copy_d2d(c, c->acid, c->b_acid, N * 8);
+    st |= copy_d2d(c, c->qx, c->b_qx, N * 8);
+    st |= copy_d2d(c, c->qy, c->b_qy, N * 8);
+    st |= copy_d2d(c, c->qz, c->b_qz, N * 8);
+    if (st) return st;
+    // rebuild the derived properties (src/Simulation.cpp:2347-2352)
+    st = phase_properties(c);
+    if (st) return st;
+    LS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static void fill_summary(ls_ctx* c, ls_step_report* r) {
+    const StepScalars& h = *c->h_ss;
+    r->p_min = h.sums[0];
+    r->p_max = h.sums[1];
+    r->conc_min = h.sums[2];
+    r->conc_max = h.sums[3];
+    r->acid_min = h.sums[4];
+    r->acid_max = h.sums[5];
+    r->mass_conc = h.sums[6];
+    r->mass_acid = h.sums[7];
+    r->mass_water = h.sums[8] * c->P.porosity;
+    r->sat_substeps = h.sat_nsub;
+    r->sat_max_divq = h.sat_max;
+}
+
+static int finish(ls_ctx* c, ls_step_report* rep) {
+    int st = summarize(c, rep);
+    if (st) return st;
+    LS_HIP(hipMemcpyAsync(c->h_ss, c->ss, sizeof(StepScalars), hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    fill_summary(c, rep);
+    return 0;
+}
+
+int ls_phase(ls_ctx* c, int phase, double t, double dt, ls_step_report* report) {
+    if (!c) return -1;
+    ls_step_report local;
+    ls_step_report* rep = report ? report : &local;
+    std::memset(rep, 0, sizeof(*rep));
+    int st = 0;
+    switch (phase) {
+        case LS_PH_PROPERTIES: st = phase_properties(c); break;
+        case LS_PH_PRESSURE: st = solve_pressure(c, t, rep); break;
+        case LS_PH_FLUX: st = phase_flux(c); break;
+        case LS_PH_SATURATION: st = phase_saturation(c, dt, rep); break;
+        case LS_PH_SURFACE: st = phase_surface(c); break;
+        case LS_PH_TRANSPORT: st = phase_transport(c, dt, rep); break;
+        case LS_PH_PRECIPITATION: st = phase_precipitation(c); break;
+        case LS_PH_LEACHING: st = phase_leaching(c, dt, rep); break;
+        default: c->err = "unknown phase"; return -1;
+    }
+    if (st < 0) return st;
+    int st2 = finish(c, rep);
+    if (st2) return st2;
+    rep->status = st;
+    return st;
+}
+
+int ls_step(ls_ctx* c, double t, double dt, ls_step_report* report) {
+    if (!c) return -1;
+    ls_step_report local;
+    ls_step_report* rep = report ? report : &local;
+    std::memset(rep, 0, sizeof(*rep));
+    if (!(dt > 0.0)) {
+        c->err = "dt must be > 0";
+        return -3;
+    }
+    Timer tm{c};
+    tm.mark();                                                  // 0
+    int st = ls_snapshot(c);                                    // saveState
+    if (st) return st;
+    tm.mark();                                                  // 1
+    // A. hydrodynamics (one Picard pass: explicit saturation)
+    if ((st = phase_properties(c))) return st;
+    tm.mark();                                                  // 2
+    st = solve_pressure(c, t, rep);
+    if (st < 0) return st;
+    tm.mark();                                                  // 3
+    bool ok = st == 0;
+    if (ok) {
+        if ((st = phase_flux(c))) return st;
+        tm.mark();                                              // 4
+        if ((st = phase_saturation(c, dt, rep))) return st;
+        tm.mark();                                              // 5
+        // B. post-Picard physics
+        if ((st = phase_surface(c))) return st;
+        tm.mark();                                              // 6
+        st = phase_transport(c, dt, rep);
+        if (st < 0) return st;
+        tm.mark();                                              // 7
+        ok = st == 0;
+    }
+    if (ok) {
+        if ((st = phase_precipitation(c))) return st;
+        tm.mark();                                              // 8
+        if ((st = phase_leaching(c, dt, rep))) return st;
+        tm.mark();                                              // 9
+    }
+    if ((st = finish(c, rep))) return st;
+    tm.mark();
+    LS_HIP(hipEventSynchronize(c->ev[tm.n - 1]));
+    rep->t_snapshot_ms = tm.between(0, 1);
+    rep->t_props_ms = tm.between(1, 2);
+    rep->t_pressure_ms = tm.between(2, 3);
+    if (ok) {
+        rep->t_flux_ms = tm.between(3, 4);
+        rep->t_sat_ms = tm.between(4, 5);
+        rep->t_surface_ms = tm.between(5, 6);
+        rep->t_transport_ms = tm.between(6, 7);
+        rep->t_precip_ms = tm.between(7, 8);
+        rep->t_leach_ms = tm.between(8, 9);
+    }
+    rep->t_total_ms = tm.between(0, tm.n - 1);
+    rep->status = ok ? 0 : 1;
+    return rep->status;
+}
+
+int ls_debug_apply_pressure(ls_ctx* c, const double* x, double* y) {
+    if (!c || !x || !y) return -1;
+    const size_t N = c->N;
+    LS_HIP(hipMemcpyAsync(c->tv[0][0], x, N * 8, hipMemcpyHostToDevice, c->stream));
+    int st = apply_pressure_unscaled(c, c->tv[0][0], c->tv[0][1]);
+    if (st) return st;
+    LS_HIP(hipMemcpyAsync(y, c->tv[0][1], N * 8, hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ls_time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) {
+    if (!c || !ms || !bytes || reps < 1) return -1;
+    if (precision != 32 && precision != 64) return -3;
+    return time_stencil(c, precision, reps, ms, bytes);
+}
+
+__global__ void k_copy(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
+         v += (size_t)gridDim.x * blockDim.x)
+        b[v] = a[v];
+}
+
+int ls_time_copy(ls_ctx* c, size_t bytes, int reps, double* ms) {
+    if (!c || !ms || reps < 1) return -1;
+    // copies between two of the transport work vectors (8 N bytes each)
+    size_t cap = c->N * 8;
+    if (bytes > cap) bytes = cap;
+    size_t n4 = bytes / 32;
+    hipEvent_t e0 = c->ev[14], e1 = c->ev[15];
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, (const double4*)c->tv[1][0],
+                       (double4*)c->tv[1][1], n4);
+    LS_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream,
+                           (const double4*)c->tv[1][0], (double4*)c->tv[1][1], n4);
+    LS_LAUNCH_CHECK();
+    LS_HIP(hipEventRecord(e1, c->stream));
+    LS_HIP(hipEventSynchronize(e1));
+    float f = 0.0f;
+    LS_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = (double)f / reps;
+    return 0;
+}
+
+}  // extern "C"

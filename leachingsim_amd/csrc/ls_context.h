@@ -1,0 +1,155 @@
+// ls_context.h — device-resident state of one simulation context and the
+// internal entry points the C ABI (ls_api.hip) dispatches to.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "ls_common.h"
+
+// Device scalars of one CG solve (kept on the device so iterations run
+// without host round trips; the host polls `done` once per batch).
+struct CgScalars {
+    double rr;        // r^T D^-1 r  (scaled residual dot)
+    double pAp;
+    double alpha;
+    double beta;
+    double wrr;       // ||b - A x||^2 of the unscaled system
+    double bnorm2;    // ||b||^2
+    double tol2;      // (rtol * ||b||)^2, or the inner target for FP32 solves
+    double relres;
+    int iter;
+    int maxit;
+    int done;         // 1 = stop (converged or iteration cap)
+    int converged;
+};
+
+// Device scalars of the two-RHS BiCGStab (conc, acid).
+struct BicgScalars {
+    double rho[2], alpha[2], omega[2], beta[2];
+    double r0v[2], ss[2], tt[2], ts[2], rr[2], r0r[2];
+    double bnorm2[2];
+    double tol2[2];
+    int s_conv[2];
+    int done[2];
+    int converged[2];
+    int iter[2];
+    int maxit;
+    int all_done;
+};
+
+struct StepScalars {
+    double sat_max;           // max |div q| + evap over Pore voxels
+    double sat_dt_sub_over_phi;
+    int sat_nsub;
+    double leach_max_rate;
+    int leach_nsub;
+    double leach_dt_sub;
+    int leach_needed;         // some reactive voxel can change this step
+    int leached;
+    double sums[9];           // pmin pmax cmin cmax amin amax  sum c, sum a, sum s
+    double n_floored;
+    double nsul;
+};
+
+struct ls_ctx {
+    int ni = 0, nj = 0, nk = 0;
+    size_t N = 0;
+    int label_bytes = 1;
+    ls_params P{};
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double log_k_rock = 0.0, log_k_pore = 0.0;   // std::log10 on the host
+
+    // --- state (src/Simulation.h:143-158) ---
+    uint8_t* img = nullptr;
+    uint8_t* img2 = nullptr;     // double buffer for label updates
+    float* frac = nullptr;
+    double *inv = nullptr, *s = nullptr, *p = nullptr, *conc = nullptr, *acid = nullptr;
+    double *qx = nullptr, *qy = nullptr, *qz = nullptr;
+    // --- snapshot (src/Simulation.h:71-84) ---
+    uint8_t* b_img = nullptr;
+    float* b_frac = nullptr;
+    double *b_inv = nullptr, *b_s = nullptr, *b_p = nullptr, *b_conc = nullptr,
+           *b_acid = nullptr, *b_qx = nullptr, *b_qy = nullptr, *b_qz = nullptr;
+    // --- properties ---
+    double *perm = nullptr, *pc = nullptr;
+    // --- pressure system ---
+    double* craw[3] = {nullptr, nullptr, nullptr};  // unscaled face conductances
+    double* cin = nullptr;                           // virtual inlet plane faces
+    double* dsum = nullptr;                          // sum of couplings (Pore rows)
+    int8_t* cls = nullptr;                           // 0 Dirichlet, 1 solved, 2 floored
+    double* pD = nullptr;                            // Dirichlet values
+    double* sd = nullptr;                            // sqrt(d) on solved rows, else 0
+    double* bhat = nullptr;                          // D^-1/2 b
+    double* c64[3] = {nullptr, nullptr, nullptr};    // D^-1/2 C D^-1/2 faces, FP64
+    float* c32[3] = {nullptr, nullptr, nullptr};     // same, FP32
+    // CG vectors (scaled variables)
+    double *x64 = nullptr, *r64 = nullptr, *p64 = nullptr, *ap64 = nullptr;
+    float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr;
+    // --- transport (two right-hand sides) ---
+    double* tv[2][8];   // r, r0, p, v, s, t, ph, sh  for conc and acid
+    double* tb[2];      // right-hand sides
+    double* tinvd = nullptr;
+    // --- reductions ---
+    double* partials = nullptr;     // MAX_PARTIALS * 8 values
+    CgScalars* cg = nullptr;
+    BicgScalars* bicg = nullptr;
+    StepScalars* ss = nullptr;
+    // host pinned mirrors
+    CgScalars* h_cg = nullptr;
+    BicgScalars* h_bicg = nullptr;
+    StepScalars* h_ss = nullptr;
+
+    double p_top_prev = 0.0;
+    bool have_prev = false;
+    bool press_assembled = false;
+
+    hipEvent_t ev[16];
+    std::vector<void*> allocations;
+};
+
+// ---- internal entry points (implemented in the kernel files) ----
+namespace ls {
+// k_physics.hip
+int init_state(ls_ctx* c, long long* nsul);
+int phase_properties(ls_ctx* c);
+int assemble_pressure(ls_ctx* c, double t);
+int finalize_pressure(ls_ctx* c, double ptop);
+int phase_flux(ls_ctx* c);
+int phase_saturation(ls_ctx* c, double dt, ls_step_report* rep);
+int phase_surface(ls_ctx* c);
+int phase_precipitation(ls_ctx* c);
+int summarize(ls_ctx* c, ls_step_report* rep);
+int derived_field(ls_ctx* c, int field, double* out_dev);
+int zero_flux_rock_precip(ls_ctx* c);
+// k_pressure.hip
+int solve_pressure(ls_ctx* c, double t, ls_step_report* rep);
+int apply_pressure_unscaled(ls_ctx* c, const double* x_dev, double* y_dev);
+int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes);
+// k_transport.hip
+int phase_transport(ls_ctx* c, double dt, ls_step_report* rep);
+// k_leach.hip
+int phase_leaching(ls_ctx* c, double dt, ls_step_report* rep);
+}  // namespace ls
+
+#define LS_HIP(call)                                                             \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            c->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+            return -10;                                                          \
+        }                                                                        \
+    } while (0)
+
+#define LS_LAUNCH_CHECK()                                                        \
+    do {                                                                         \
+        hipError_t e_ = hipGetLastError();                                       \
+        if (e_ != hipSuccess) {                                                  \
+            c->err = std::string("kernel launch: ") + hipGetErrorString(e_);     \
+            return -11;                                                          \
+        }                                                                        \
+    } while (0)

@@ -1,0 +1,64 @@
+/*
+ * ls_oracle.h — CPU FP64 oracle of the LeachingSim per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker / the timed CPU baseline.  The product path (liblsgpu.so) never
+ * links or calls it.
+ *
+ * The oracle restates src/Simulation.cpp of javalosp/LeachingSim for a single
+ * domain (no MPI ranks), with the semantic decisions recorded in DESIGN.md
+ * (single-domain one-sided differences only at global faces, saturation kept
+ * at inactive voxels, solid_density default, z-face planes, floored pressure
+ * rows handled as lagged Dirichlet nodes and post-solved from their own row).
+ * It uses the same parameter, field, phase and report types as the GPU
+ * library (include/lsgpu.h) so the two can be driven by one test sequence.
+ */
+#ifndef LS_ORACLE_H_
+#define LS_ORACLE_H_
+
+#include "../include/lsgpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lso_ctx lso_ctx;
+
+int lso_create(lso_ctx** out, int ni, int nj, int nk, const ls_params* params);
+void lso_destroy(lso_ctx* ctx);
+const char* lso_last_error(const lso_ctx* ctx);
+int lso_set_params(lso_ctx* ctx, const ls_params* params);
+int lso_upload(lso_ctx* ctx, int field, const void* host, size_t elem_bytes);
+int lso_download(lso_ctx* ctx, int field, void* host, size_t elem_bytes);
+int lso_init_state(lso_ctx* ctx, long long* initial_sulphide_count);
+int lso_step(lso_ctx* ctx, double t, double dt, ls_step_report* report);
+int lso_phase(lso_ctx* ctx, int phase, double t, double dt, ls_step_report* report);
+int lso_snapshot(lso_ctx* ctx);
+int lso_rollback(lso_ctx* ctx);
+void lso_set_threads(int nthreads);
+
+/* The reference's full pressure system (src/Simulation.cpp:658-798) for the
+ * current state, as assembled by MatSetValue/VecSetValue (ADD_VALUES), in
+ * COO form: rows[nnz], cols[nnz], vals[nnz], rhs[N].  Call with rows == NULL
+ * to get nnz only.  Under the z-face boundary mode the virtual-plane links are
+ * included (diagonal and right-hand side). */
+long long lso_pressure_system(lso_ctx* ctx, double t, int* rows, int* cols,
+                              double* vals, double* rhs);
+/* The reference's transport system (src/Simulation.cpp:966-1237) for the
+ * given field (LS_F_CONC or LS_F_ACID), COO form as above.  Like the
+ * reference, this zeroes the flux at Rock and Precipitate voxels. */
+long long lso_transport_system(lso_ctx* ctx, double dt, int field, int* rows,
+                               int* cols, double* vals, double* rhs);
+/* y = A x with the reduced pressure operator of the last pressure phase
+ * (rows outside the solved set return 0). */
+int lso_apply_pressure(lso_ctx* ctx, const double* x, double* y);
+/* Per-voxel class of the last pressure phase: 0 Dirichlet, 1 solved (R),
+ * 2 floored (F). */
+int lso_pressure_classes(lso_ctx* ctx, int8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LS_ORACLE_H_ */
