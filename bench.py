@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Coupled-step throughput of the LeachingSim hot path on one MI355X.
+
+Workload (default): 512^3 uint16 image with pore + three mineral phases
+(Rock 0.30, Sulphide 0.25, Precipitate 0.15, each from its own seeded
+smoothed field, sigma 3), a 4-voxel planar fracture at a seeded orientation
+through the centre, 2-plane Air slabs on both j faces (surface precipitation
+active), constant-pressure inlet/outlet planes on the i (CLI z) faces,
+dt = 0.5 s, pressure converged to 1e-8 relative residual.
+
+A step is one full coupled step (snapshot, properties, pressure, flux,
+saturation, surface effects, conc + acid transport, precipitation,
+leaching) on device-resident state.  The timed region holds `--steps` steps
+after `--warmup` untimed steps, bracketed by device synchronisation.
+
+Prints one JSON line: steps/s of the headline solver, the plain FP64 PCG
+baseline on the same workload, the pressure-stencil roofline measured with
+HIP events during the timed region, and the CPU oracle baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+from leachingsim_amd import geometry as G  # noqa: E402
+from leachingsim_amd import lsgpu as L  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+SOLVER_NAMES = {L.SOLVER_PLAIN64: "plain_fp64_pcg", L.SOLVER_MIXED: "mixed_fp32_fp64ir",
+                L.SOLVER_LEARNED: "learned_fcg_mixed"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def device_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def workload(config: str, size: int | None):
+    if config == "c3":
+        n = size or 512
+        img = G.multiphase((n, n, n), sigma=3.0, seed=1, fracture_width=4.0, air_j_planes=2)
+        return img, 2, 0.5, f"c3: {n}^3 uint16 pore+3 minerals+fracture, Air j-slabs, z-face inlet/outlet, dt 0.5"
+    if config == "c2":
+        img = G.config_geometry("c2")
+        return img, 1, 1.0, "c2: 338x338x283 uint8 sigma 3 porosity 0.30, z-face inlet/outlet, dt 1"
+    if config == "c1":
+        img = G.config_geometry("c1")
+        return img, 1, 1.0, "c1: 64^3 uint8 sigma 2 porosity 0.40, z-face inlet/outlet, dt 1"
+    raise SystemExit(f"unknown config {config}")
+
+
+def run_solver(img, label_bytes, dt, solver, steps, warmup, profile=False, warm=L.WARM_PREVIOUS):
+    P = L.default_params(bc_mode=L.BC_ZFACES, solver=solver, warm_start=warm)
+    sim = L.GpuSimulation(img.shape, P, label_bytes)
+    sim.read_labels(img)
+    sim.init_state()
+    t = 0.0
+    reps = []
+    for n in range(warmup):
+        r = sim.step(t, dt)
+        if r.status != 0:
+            raise RuntimeError(f"warmup step {n} failed")
+        t += dt
+        log(f"  [{SOLVER_NAMES[solver]}] warmup {n}: {r.t_total_ms:.1f} ms, "
+            f"press iters {r.press_iters}")
+    sim.set_profiling(profile)
+    device_sync()
+    t0 = time.perf_counter()
+    for n in range(steps):
+        r = sim.step(t, dt)
+        if r.status != 0:
+            raise RuntimeError(f"timed step {n} failed: {r.as_dict()}")
+        reps.append(r.as_dict())
+        t += dt
+    device_sync()
+    t1 = time.perf_counter()
+    for n, r in enumerate(reps):
+        log(f"  [{SOLVER_NAMES[solver]}] step {warmup + n}: {r['t_total_ms']:.1f} ms "
+            f"(pressure {r['t_pressure_ms']:.1f}, transport {r['t_transport_ms']:.1f}), "
+            f"press iters {r['press_iters']} outer {r['press_outer']} "
+            f"relres {r['press_relres']:.2e}, adr iters {r['adr_iters']}")
+    copy_gbs = None
+    if profile:
+        nb = min(img.size * 8, 1 << 30)
+        ms = sim.time_copy(nb, 20)
+        copy_gbs = 2 * nb / ms / 1e6
+    sim.close()
+    return (t1 - t0), reps, copy_gbs
+
+
+def cpu_baseline(img, label_bytes, dt, crop: int, nsteps: int):
+    """The FP64 oracle (C++/OpenMP) on a bounded sample: a centred crop of
+    the same geometry, `nsteps` coupled steps, scaled to the full grid by
+    voxel count (the CPU's PCG iteration count grows with the grid, so the
+    scaled figure flatters the CPU)."""
+    from oracle.oracle import OracleSimulation, set_threads
+    ncpu = len(os.sched_getaffinity(0))
+    set_threads(ncpu)
+    c = [max(0, (s - crop) // 2) for s in img.shape]
+    sub = np.ascontiguousarray(img[c[0]:c[0] + crop, c[1]:c[1] + crop, c[2]:c[2] + crop])
+    P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_PLAIN64)
+    o = OracleSimulation(sub.shape, P, label_bytes)
+    o.read_labels(sub)
+    o.init_state()
+    t = 0.0
+    times = []
+    for _ in range(nsteps):
+        t0 = time.perf_counter()
+        r = o.step(t, dt)
+        times.append(time.perf_counter() - t0)
+        t += dt
+        if r.status != 0:
+            break
+    sec = float(np.mean(times))
+    scale = sub.size / img.size
+    return {"value": scale / sec, "unit": "steps/s", "cores": ncpu, "kind": "port",
+            "sample": f"oracle FP64 Jacobi-PCG, {nsteps} coupled steps on the centred "
+                      f"{'x'.join(map(str, sub.shape))} crop ({sec:.2f} s/step), scaled "
+                      f"by voxel count to {'x'.join(map(str, img.shape))}"}
+
+
+def stencil_only(img, label_bytes, precision, reps):
+    """Assemble the pressure operator of the initial state, then time `reps`
+    stencil launches and `reps` copies of 1 GiB (HIP events)."""
+    P = L.default_params(bc_mode=L.BC_ZFACES, press_maxit=1)
+    sim = L.GpuSimulation(img.shape, P, label_bytes)
+    sim.read_labels(img)
+    sim.init_state()
+    sim.phase(L.PH_PROPERTIES)
+    sim.phase(L.PH_PRESSURE, 1.0)      # assembles; the 1-iteration solve is irrelevant
+    ms, by = sim.time_stencil(precision, reps)
+    nb = min(img.size * 8, 1 << 30)
+    cms = sim.time_copy(nb, reps)
+    sim.close()
+    print(json.dumps({"kernel": f"k_stencil<{'double' if precision == 64 else 'float'},0>",
+                      "grid": list(img.shape), "reps": reps, "avg_launch_ms": ms,
+                      "bytes_per_launch": by, "achieved_GBs": by / ms / 1e6,
+                      "copy_bytes": 2 * nb, "copy_ms": cms, "copy_GBs": 2 * nb / cms / 1e6}),
+          flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--size", type=int, default=None, help="override the c3 edge length")
+    ap.add_argument("--solver", default="mixed", choices=["plain", "mixed", "learned"])
+    ap.add_argument("--no-baseline", action="store_true", help="skip the plain FP64 run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-crop", type=int, default=96)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--mode", default="steps", choices=["steps", "stencil"],
+                    help="stencil: assemble the pressure operator once and time only the "
+                         "stencil and a copy kernel (for counter collection)")
+    ap.add_argument("--precision", type=int, default=64, choices=[32, 64])
+    ap.add_argument("--reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab decomposition is a design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    img, lb, dt, desc = workload(args.config, args.size)
+    log(f"workload {desc}: generated in {time.perf_counter() - t0:.1f} s; labels "
+        + ", ".join(f"{k}:{int((img == k).sum())}" for k in range(1, 6)))
+    if args.mode == "stencil":
+        stencil_only(img, lb, args.precision, args.reps)
+        return
+    head = {"plain": L.SOLVER_PLAIN64, "mixed": L.SOLVER_MIXED,
+            "learned": L.SOLVER_LEARNED}[args.solver]
+
+    sec, reps, copy_gbs = run_solver(img, lb, dt, head, args.steps, args.warmup, profile=True)
+    value = args.steps / sec
+    base = None
+    if not args.no_baseline and head != L.SOLVER_PLAIN64:
+        bsec, breps, _ = run_solver(img, lb, dt, L.SOLVER_PLAIN64, args.steps, args.warmup)
+        base = {"steps_per_s": args.steps / bsec,
+                "press_iters_per_step": float(np.mean([r["press_iters"] for r in breps]))}
+
+    # roofline of the pressure stencil, from the sampled HIP-event timings
+    prec = 64 if head == L.SOLVER_PLAIN64 else 32
+    key = f"prof_stencil{prec}"
+    n = sum(r[key + "_n"] for r in reps)
+    ms = sum(r[key + "_ms"] for r in reps) / max(n, 1)
+    sv = prec // 8
+    bytes_per_launch = img.size * (2 * sv + 3 * sv)
+    achieved = bytes_per_launch / (ms * 1e-3) / 1e9 if n else 0.0
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(img, lb, dt, min(args.cpu_crop, *img.shape), args.cpu_steps)
+
+    out = {
+        "metric": "steps/s at 512^3, 1e-8 rel residual: learned+mixed vs plain FP64 CG",
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * sec / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head == L.SOLVER_PLAIN64 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "grid": list(img.shape), "solver": SOLVER_NAMES[head],
+                   "pressure_rtol": 1e-8,
+                   "press_iters_per_step": float(np.mean([r["press_iters"] for r in reps])),
+                   "plain_fp64_pcg": base,
+                   "speedup_vs_plain_fp64": (value / base["steps_per_s"]) if base else None},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "kernel": f"k_stencil<{'double' if prec == 64 else 'float'},0>",
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms,
+                     "sampled_launches": n, "measured_copy_GBs": copy_gbs, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -168,6 +168,11 @@ typedef struct ls_step_report {
     double top_pressure_eff;
     double t_props_ms, t_pressure_ms, t_flux_ms, t_sat_ms, t_surface_ms,
            t_transport_ms, t_precip_ms, t_leach_ms, t_snapshot_ms, t_total_ms;
+    /* With profiling on (ls_set_profiling): HIP-event durations of sampled
+     * pressure-stencil launches (the first launch of every CG batch) during
+     * this step, summed, and their counts, for FP64 and FP32 stencils. */
+    double prof_stencil64_ms, prof_stencil32_ms;
+    int prof_stencil64_n, prof_stencil32_n;
 } ls_step_report;
 
 typedef struct ls_ctx ls_ctx;
@@ -216,6 +221,8 @@ int ls_snapshot(ls_ctx* ctx);
 int ls_rollback(ls_ctx* ctx);
 
 /* Test / measurement hooks (no reference counterpart). */
+/* Turn sampled kernel timing inside ls_step/ls_phase on (1) or off (0). */
+int ls_set_profiling(ls_ctx* ctx, int on);
 /* y = A x for the reduced pressure operator assembled by the last pressure
  * phase (rows outside the solved set return 0).  Host buffers, N doubles. */
 int ls_debug_apply_pressure(ls_ctx* ctx, const double* x, double* y);

@@ -314,8 +314,14 @@ static int cg_iterations(ls_ctx* c, T* x, T* r, T* p, T* ap, T* const* faces, co
     uint32_t N = (uint32_t)c->N;
     int nbe = ew_blocks(N);
     for (int b = 0; b < batch; ++b) {
+        const bool sample = c->profiling && b == 0;
+        if (sample) LS_HIP(hipEventRecord(c->ev[12], c->stream));
         hipLaunchKernelGGL((k_stencil<T, 0>), grd, blk, 0, c->stream, p, ap, faces[0], faces[1],
                            faces[2], g, c->partials, &c->cg->done, nullptr, nullptr, nullptr);
+        if (sample) {
+            LS_HIP(hipEventRecord(c->ev[13], c->stream));
+            c->prof_pending = sizeof(T) == 8 ? 64 : 32;
+        }
         hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, c->stream, c->partials, nbs, c->cg);
         hipLaunchKernelGGL(k_cg_update<T>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, x, r, p, ap,
                            sdw, c->cg, N, c->partials);
@@ -331,6 +337,14 @@ static int cg_iterations(ls_ctx* c, T* x, T* r, T* p, T* ap, T* const* faces, co
 static int poll_cg(ls_ctx* c) {
     LS_HIP(hipMemcpyAsync(c->h_cg, c->cg, sizeof(CgScalars), hipMemcpyDeviceToHost, c->stream));
     LS_HIP(hipStreamSynchronize(c->stream));
+    if (c->prof_pending) {
+        float ms = 0.0f;
+        LS_HIP(hipEventElapsedTime(&ms, c->ev[12], c->ev[13]));
+        const int q = c->prof_pending == 64 ? 0 : 1;
+        c->prof_ms[q] += ms;
+        c->prof_n[q] += 1;
+        c->prof_pending = 0;
+    }
     return 0;
 }
 

@@ -43,6 +43,19 @@ struct Timer {
     }
 };
 
+void reset_prof(ls_ctx* c) {
+    c->prof_ms[0] = c->prof_ms[1] = 0.0;
+    c->prof_n[0] = c->prof_n[1] = 0;
+    c->prof_pending = 0;
+}
+
+void report_prof(ls_ctx* c, ls_step_report* r) {
+    r->prof_stencil64_ms = c->prof_ms[0];
+    r->prof_stencil32_ms = c->prof_ms[1];
+    r->prof_stencil64_n = c->prof_n[0];
+    r->prof_stencil32_n = c->prof_n[1];
+}
+
 }  // namespace
 
 extern "C" {
@@ -372,6 +385,7 @@ int ls_phase(ls_ctx* c, int phase, double t, double dt, ls_step_report* report) 
     ls_step_report local;
     ls_step_report* rep = report ? report : &local;
     std::memset(rep, 0, sizeof(*rep));
+    reset_prof(c);
     int st = 0;
     switch (phase) {
         case LS_PH_PROPERTIES: st = phase_properties(c); break;
@@ -387,6 +401,7 @@ int ls_phase(ls_ctx* c, int phase, double t, double dt, ls_step_report* report) 
     if (st < 0) return st;
     int st2 = finish(c, rep);
     if (st2) return st2;
+    report_prof(c, rep);
     rep->status = st;
     return st;
 }
@@ -400,6 +415,7 @@ int ls_step(ls_ctx* c, double t, double dt, ls_step_report* report) {
         c->err = "dt must be > 0";
         return -3;
     }
+    reset_prof(c);
     Timer tm{c};
     tm.mark();                                                  // 0
     int st = ls_snapshot(c);                                    // saveState
@@ -446,8 +462,15 @@ int ls_step(ls_ctx* c, double t, double dt, ls_step_report* report) {
         rep->t_leach_ms = tm.between(8, 9);
     }
     rep->t_total_ms = tm.between(0, tm.n - 1);
+    report_prof(c, rep);
     rep->status = ok ? 0 : 1;
     return rep->status;
+}
+
+int ls_set_profiling(ls_ctx* c, int on) {
+    if (!c) return -1;
+    c->profiling = on ? 1 : 0;
+    return 0;
 }
 
 int ls_debug_apply_pressure(ls_ctx* c, const double* x, double* y) {

@@ -109,6 +109,11 @@ struct ls_ctx {
     bool press_assembled = false;
 
     hipEvent_t ev[16];
+    // sampled stencil timing (profiling on): events 12/13 bracket one launch
+    int profiling = 0;
+    int prof_pending = 0;          // 64 or 32 when an event pair awaits reading
+    double prof_ms[2] = {0.0, 0.0};
+    int prof_n[2] = {0, 0};
     std::vector<void*> allocations;
 };
 

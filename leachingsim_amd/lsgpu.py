@@ -70,6 +70,8 @@ class StepReport(C.Structure):
         ("t_sat_ms", C.c_double), ("t_surface_ms", C.c_double),
         ("t_transport_ms", C.c_double), ("t_precip_ms", C.c_double),
         ("t_leach_ms", C.c_double), ("t_snapshot_ms", C.c_double), ("t_total_ms", C.c_double),
+        ("prof_stencil64_ms", C.c_double), ("prof_stencil32_ms", C.c_double),
+        ("prof_stencil64_n", C.c_int), ("prof_stencil32_n", C.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -124,6 +126,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "ls_debug_apply_pressure": (i, [vp, vp, vp]),
         "ls_time_stencil": (i, [vp, i, i, C.POINTER(d), C.POINTER(d)]),
         "ls_time_copy": (i, [vp, sz, i, C.POINTER(d)]),
+        "ls_set_profiling": (i, [vp, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -288,6 +291,9 @@ class GpuSimulation:
         ms, by = C.c_double(), C.c_double()
         self._check(self.lib.ls_time_stencil(self.h, precision, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def set_profiling(self, on: bool):
+        self._check(self.lib.ls_set_profiling(self.h, 1 if on else 0))
 
     def time_copy(self, nbytes: int, reps: int = 20) -> float:
         ms = C.c_double()
