@@ -103,8 +103,9 @@ typedef struct ls_params {
     int press_maxit;         /* 10000 (src/Simulation.cpp:156)       */
     double adr_rtol;         /* 1e-8                                 */
     int adr_maxit;           /* 10000                                */
-    int mixed_inner_maxit;   /* FP32 inner iterations per refinement (500) */
-    double mixed_inner_rtol; /* FP32 inner residual reduction (1e-4)  */
+    int mixed_inner_maxit;   /* reserved (0)                                       */
+    double mixed_inner_rtol; /* mixed solver: the FP64 residual replaces the FP32 one
+                                whenever the latter has fallen by this factor (0.1) */
 } ls_params;
 
 /* Fields that can be uploaded / downloaded.  Element types are fixed:

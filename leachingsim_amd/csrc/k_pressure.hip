@@ -185,6 +185,7 @@ __global__ void k_cg_beta(const double* __restrict__ partials, int n, CgScalars*
         cs->wrr = wrr;
         cs->relres = use_w ? sqrt(wrr / cs->bnorm2) : sqrt(rr_new / cs->bnorm2);
         cs->beta = rr_new / cs->rr;
+        cs->rr_prev = cs->rr;
         cs->rr = rr_new;
         if (!isfinite(rr_new)) {
             cs->done = 1;
@@ -235,16 +236,17 @@ __global__ void k_cg_start(const double* __restrict__ partials, int n, CgScalars
     }
 }
 
-// FP32 inner solve start: r32 = r64 / scale, p32 = r32, x32 = 0, partial r.r.
+// FP32 residual replacement: r32 = r64 / scale, x32 = 0 (and p32 = r32 on
+// the first replacement of a solve), partial r32.r32.
 __global__ void k_inner_start(const double* __restrict__ r64, float* __restrict__ r32,
                               float* __restrict__ p32, float* __restrict__ x32, uint32_t N,
-                              double inv_scale, double* __restrict__ partials) {
+                              double inv_scale, int reset_p, double* __restrict__ partials) {
     __shared__ double lds[EW_BLOCK / WAVE];
     double rr = 0.0;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
         const float rv = (float)(r64[v] * inv_scale);
         r32[v] = rv;
-        p32[v] = rv;
+        if (reset_p) p32[v] = rv;
         x32[v] = 0.0f;
         rr += (double)rv * (double)rv;
     }
@@ -253,25 +255,36 @@ __global__ void k_inner_start(const double* __restrict__ r64, float* __restrict_
     if (threadIdx.x == 0) partials[blockIdx.x] = a[0];
 }
 
+// Start (first = 1) or continue after a residual replacement (first = 0):
+// r.r of the new FP32 residual; beta = r.r(new) / r.r(before the last
+// iteration) so the search direction carries over; next trigger for the
+// FP64 replacement.
 __global__ void k_inner_init(const double* __restrict__ partials, int n, CgScalars* cs,
-                             double rtol, int maxit) {
+                             double trigger2, int maxit, int first) {
     __shared__ double lds[1024 / WAVE];
     double acc = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partials[i];
     double a[1] = {acc};
     block_reduce<1, RED_SUM>(a, lds);
     if (threadIdx.x == 0) {
+        if (first) {
+            cs->iter = 0;
+            cs->bnorm2 = a[0];
+            cs->beta = 0.0;
+        } else {
+            cs->beta = a[0] / cs->rr_prev;
+        }
         cs->rr = a[0];
-        cs->bnorm2 = a[0];
-        cs->tol2 = (rtol * rtol) * a[0];
+        cs->tol2 = trigger2;
         cs->wrr = 0.0;
         cs->relres = 1.0;
-        cs->iter = 0;
         cs->maxit = maxit;
-        cs->done = (a[0] > 0.0) ? 0 : 1;
-        cs->converged = cs->done;
+        cs->done = (a[0] > 0.0 && cs->iter < maxit) ? 0 : 1;
+        cs->converged = 0;
     }
 }
+
+__global__ void k_set_iter(CgScalars* cs, int iter) { cs->iter = iter; }
 
 // x64 += scale * x32  (the FP64 refinement update)
 __global__ void k_refine_add(double* __restrict__ x64, const float* __restrict__ x32, uint32_t N,
@@ -386,51 +399,89 @@ static int solve_plain64(ls_ctx* c, ls_step_report* rep) {
     return c->h_cg->converged ? 0 : 1;
 }
 
-// Mixed precision: FP32 CG on A^ (FP32 faces, FP64 dot products) inside FP64
-// iterative refinement on the FP64 residual.
+// Mixed precision with reliable updates (Sleijpen & van der Vorst 1996, as
+// used for mixed-precision CG by Clark et al. 2010): one CG run on A^ whose
+// stencil, vectors and direction updates are FP32 (dot products FP64).
+// Whenever the FP32 recurrence residual has fallen by `delta` since the last
+// replacement, or its estimated true norm reaches the tolerance, the FP32
+// solution is added into the FP64 iterate, the FP64 residual b - A x is
+// recomputed with the FP64 operator and replaces the FP32 one.  The search
+// direction is kept, so the Krylov convergence is not restarted.  The stop
+// test is always on the FP64 residual.
 static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
     uint32_t N = (uint32_t)c->N;
     int nbe = ew_blocks(N);
     const int bs = batch_size(c);
+    const double delta = c->P.mixed_inner_rtol > 0 ? c->P.mixed_inner_rtol : 0.1;
     int st;
-    int inner_total = 0;
-    double bnorm = 0.0;
-    for (int outer = 0; outer < 50; ++outer) {
-        if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
-        if ((st = poll_cg(c))) return st;
-        bnorm = sqrt(c->h_cg->bnorm2);
-        rep->press_relres = c->h_cg->relres;
-        rep->press_bnorm = bnorm;
-        if (bnorm == 0.0) {
-            LS_HIP(hipMemsetAsync(c->x64, 0, c->N * sizeof(double), c->stream));
-            rep->press_relres = 0.0;
-            return 0;
-        }
-        if (c->h_cg->converged) {
-            rep->press_iters += inner_total;
-            return 0;
-        }
-        if (inner_total >= c->P.press_maxit) break;
-        rep->press_outer += 1;
-        const double scale = sqrt(c->h_cg->rr);
+    if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+    if ((st = poll_cg(c))) return st;
+    const double bnorm2 = c->h_cg->bnorm2;
+    const double tol2 = c->h_cg->tol2;
+    rep->press_bnorm = sqrt(bnorm2);
+    rep->press_relres = c->h_cg->relres;
+    if (bnorm2 == 0.0) {
+        LS_HIP(hipMemsetAsync(c->x64, 0, c->N * sizeof(double), c->stream));
+        rep->press_relres = 0.0;
+        return 0;
+    }
+    if (c->h_cg->converged) return 0;
+    // fixed power-of-two scale of the FP32 vectors for this solve
+    const double scale = std::ldexp(1.0, std::ilogb(sqrt(c->h_cg->rr)));
+    double rr64 = c->h_cg->rr, wrr = c->h_cg->wrr;
+    double last_wrr = wrr;
+    int stalls = 0;
+    bool first = true;
+    const int maxit = c->P.press_maxit;
+    while (true) {
+        // trigger: FP32 r.r fell by delta^2, or the estimated true norm hits tol
+        const double rr32 = rr64 / (scale * scale);
+        const double w2 = wrr / rr64;
+        double trig = delta * delta * rr32;
+        const double t2 = 0.81 * tol2 / (w2 * scale * scale);
+        if (t2 > trig) trig = t2;
         hipLaunchKernelGGL(k_inner_start, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->r64, c->r32,
-                           c->p32, c->x32, N, 1.0 / scale, c->partials);
+                           c->p32, c->x32, N, 1.0 / scale, first ? 1 : 0, c->partials);
         hipLaunchKernelGGL(k_inner_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nbe,
-                           c->cg, c->P.mixed_inner_rtol, c->P.mixed_inner_maxit);
+                           c->cg, trig, maxit, first ? 1 : 0);
+        if (!first)
+            hipLaunchKernelGGL(k_cg_direction<float>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream,
+                               c->p32, c->r32, c->cg, N);
         LS_LAUNCH_CHECK();
+        first = false;
         if ((st = poll_cg(c))) return st;
         while (!c->h_cg->done) {
             if ((st = cg_iterations<float>(c, c->x32, c->r32, c->p32, c->ap32, c->c32, nullptr, bs)))
                 return st;
             if ((st = poll_cg(c))) return st;
         }
-        inner_total += c->h_cg->iter;
+        const int iters = c->h_cg->iter;
+        // stopped without reaching the trigger or the cap: breakdown / non-finite
+        const bool broke = !c->h_cg->converged && iters < maxit;
         hipLaunchKernelGGL(k_refine_add, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->x64, c->x32,
                            N, scale);
         LS_LAUNCH_CHECK();
+        rep->press_outer += 1;
+        // FP64 residual replacement (keeps the device iteration counter)
+        if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+        if ((st = poll_cg(c))) return st;
+        rr64 = c->h_cg->rr;
+        wrr = c->h_cg->wrr;
+        rep->press_relres = c->h_cg->relres;
+        rep->press_iters = iters;
+        if (c->h_cg->converged) return 0;
+        if (iters >= maxit) return 1;
+        // FP32 stagnation guard: the true residual must keep falling
+        stalls = (wrr > 0.25 * last_wrr) ? stalls + 1 : 0;
+        last_wrr = wrr;
+        if (broke || stalls >= 3) {
+            rep->press_fallback = 1;
+            return solve_plain64(c, rep);   // FP64 PCG from the current iterate
+        }
+        // restore the device iteration counter reset by the residual kernel
+        hipLaunchKernelGGL(k_set_iter, dim3(1), dim3(1), 0, c->stream, c->cg, iters);
+        LS_LAUNCH_CHECK();
     }
-    rep->press_iters += inner_total;
-    return 1;
 }
 
 static double effective_top_pressure(double top, double t) {

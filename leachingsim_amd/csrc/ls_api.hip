@@ -87,8 +87,8 @@ void ls_default_params(ls_params* p) {
     p->press_maxit = 10000;
     p->adr_rtol = 1e-8;
     p->adr_maxit = 10000;
-    p->mixed_inner_maxit = 500;
-    p->mixed_inner_rtol = 1e-4;
+    p->mixed_inner_maxit = 0;
+    p->mixed_inner_rtol = 0.1;
 }
 
 int ls_abi_info(size_t* sp, size_t* sr) {

@@ -20,6 +20,7 @@ struct CgScalars {
     double bnorm2;    // ||b||^2
     double tol2;      // (rtol * ||b||)^2, or the inner target for FP32 solves
     double relres;
+    double rr_prev;   // r.r before the last update (for beta after a residual replacement)
     int iter;
     int maxit;
     int done;         // 1 = stop (converged or iteration cap)

@@ -85,8 +85,8 @@ def default_params(**overrides) -> Params:
                vg_n=2.0, s_res=0.05, porosity=0.1, solid_density=1.0, time_scalar=1.0,
                alpha_transfer=1e-5, instant_precip=0, bc_mode=BC_REFERENCE,
                solver=SOLVER_PLAIN64, warm_start=WARM_PREVIOUS, press_rtol=1e-8,
-               press_maxit=10000, adr_rtol=1e-8, adr_maxit=10000, mixed_inner_maxit=500,
-               mixed_inner_rtol=1e-4)
+               press_maxit=10000, adr_rtol=1e-8, adr_maxit=10000, mixed_inner_maxit=0,
+               mixed_inner_rtol=0.1)
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)
