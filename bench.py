@@ -56,7 +56,15 @@ def device_sync():
 def workload(config: str, size: int | None):
     if config == "c3":
         n = size or 512
-        img = G.multiphase((n, n, n), sigma=3.0, seed=1, fracture_width=4.0, air_j_planes=2)
+        cache = os.environ.get("LS_GEOM_CACHE")
+        path = Path(cache) / f"c3_{n}.npy" if cache else None
+        if path is not None and path.exists():
+            img = np.load(path)
+        else:
+            img = G.multiphase((n, n, n), sigma=3.0, seed=1, fracture_width=4.0, air_j_planes=2)
+            if path is not None:
+                path.parent.mkdir(parents=True, exist_ok=True)
+                np.save(path, img)
         return img, 2, 0.5, f"c3: {n}^3 uint16 pore+3 minerals+fracture, Air j-slabs, z-face inlet/outlet, dt 0.5"
     if config == "c2":
         img = G.config_geometry("c2")

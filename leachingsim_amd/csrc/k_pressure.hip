@@ -112,6 +112,214 @@ __global__ __launch_bounds__(TK * TJ) void k_stencil(
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Vectorised kernels (used when nk is a multiple of the vector width W:
+// 4 floats / 2 doubles = 16 B per lane and array).  Each lane owns W
+// consecutive k values of one (j) row and marches ICH planes along i; k
+// neighbours across lanes come from wave shuffles, the lanes at the wave
+// edges load their single halo value.  Faces are read with non-temporal
+// loads and y is written with non-temporal stores (streamed once).
+// ---------------------------------------------------------------------------
+template <typename T, int W> struct VecOf;
+template <> struct VecOf<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct VecOf<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
+
+template <typename T> struct StencilShape;
+template <> struct StencilShape<float> { static constexpr int W = 4, ICH = 4; };
+template <> struct StencilShape<double> { static constexpr int W = 2, ICH = 8; };
+
+template <typename T>
+__global__ __launch_bounds__(WAVE * TJ) void k_stencil_vec(
+    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ cx,
+    const T* __restrict__ cy, const T* __restrict__ cz, SGeo g, double* __restrict__ partials,
+    const int* __restrict__ done) {
+    constexpr int W = StencilShape<T>::W, ICH = StencilShape<T>::ICH;
+    typedef typename VecOf<T, W>::type V;
+    if (done != nullptr && *done) return;
+    __shared__ double lds[TJ];
+    const int nbx = gridDim.x, nby = gridDim.y;
+    const int nb = nbx * nby * gridDim.z;
+    const int hw = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    const int t = xcd_remap(hw, nb);
+    const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
+    const int lane = threadIdx.x;
+    const int k0 = (bx * WAVE + lane) * W;
+    const int j = by * TJ + threadIdx.y;
+    const int i0 = bz * ICH, i1 = min(i0 + ICH, g.ni);
+    double acc = 0.0;
+    // rows past nj idle as whole waves (threadIdx.y is uniform per wave)
+    if (j < g.nj) {
+        const bool ok = k0 < g.nk;
+        const bool hj = j > 0, pj = j + 1 < g.nj;
+        uint32_t v = (uint32_t)i0 * g.si + (uint32_t)j * g.sj + (ok ? (uint32_t)k0 : 0u);
+        const V zero = V(0);
+        V xm = (ok && i0 > 0) ? *(const V*)(x + v - g.si) : zero;
+        V cxm = (ok && i0 > 0) ? *(const V*)(cx + v - g.si) : zero;
+        V xc = ok ? *(const V*)(x + v) : zero;
+        for (int i = i0; i < i1; ++i, v += g.si) {
+            V xp = zero, cxp = zero, cyp = zero, czp = zero, cym = zero, xjp = zero, xjm = zero;
+            if (ok) {
+                if (i + 1 < g.ni) xp = *(const V*)(x + v + g.si);
+                cxp = __builtin_nontemporal_load((const V*)(cx + v));
+                cyp = __builtin_nontemporal_load((const V*)(cy + v));
+                czp = __builtin_nontemporal_load((const V*)(cz + v));
+                if (hj) {
+                    cym = *(const V*)(cy + v - g.sj);
+                    xjm = *(const V*)(x + v - g.sj);
+                }
+                if (pj) xjp = *(const V*)(x + v + g.sj);
+            }
+            T left = __shfl_up(xc[W - 1], 1, WAVE);
+            T czl = __shfl_up(czp[W - 1], 1, WAVE);
+            T right = __shfl_down(xc[0], 1, WAVE);
+            if (lane == 0) {
+                left = (ok && k0 > 0) ? x[v - 1] : T(0);
+                czl = (ok && k0 > 0) ? cz[v - 1] : T(0);
+            }
+            if (lane == WAVE - 1) right = (ok && k0 + W < g.nk) ? x[v + W] : T(0);
+            V ax;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const T xkm = (w == 0) ? left : xc[w - 1];
+                const T xkp = (w == W - 1) ? right : xc[w + 1];
+                const T czm = (w == 0) ? czl : czp[w - 1];
+                ax[w] = xc[w] - (cxp[w] * xp[w] + cxm[w] * xm[w] + cyp[w] * xjp[w] +
+                                 cym[w] * xjm[w] + czp[w] * xkp + czm * xkm);
+                acc += (double)xc[w] * (double)ax[w];
+            }
+            if (ok) __builtin_nontemporal_store(ax, (V*)(y + v));
+            xm = xc;
+            xc = xp;
+            cxm = cxp;
+        }
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) lds[threadIdx.y] = acc;
+    __syncthreads();
+    if (flat_tid() == 0) {
+        double s = 0.0;
+        for (int q = 0; q < TJ; ++q) s += lds[q];
+        partials[t] = s;
+    }
+}
+
+// Vectorised CG vector updates: U vectors of 16 B per thread, all loads
+// issued before the stores, non-temporal (each vector is streamed once per
+// kernel).  The scalar tail (N % W) is handled by block 0.
+constexpr int VU = 4;
+
+template <typename T>
+__global__ __launch_bounds__(EW_BLOCK) void k_cg_update_vec(
+    T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ ap,
+    const double* __restrict__ sd, const CgScalars* __restrict__ cs, uint32_t N,
+    double* __restrict__ partials) {
+    if (cs->done) return;
+    constexpr int W = StencilShape<T>::W;
+    typedef typename VecOf<T, W>::type V;
+    __shared__ double lds[2 * EW_BLOCK / WAVE];
+    const T a = (T)cs->alpha;
+    const uint32_t nv = N / W;
+    const uint32_t base = blockIdx.x * (uint32_t)(EW_BLOCK * VU) + threadIdx.x;
+    V xv[VU], rv[VU], pv[VU], av[VU];
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+        const uint32_t q = base + u * EW_BLOCK;
+        if (q < nv) {
+            xv[u] = __builtin_nontemporal_load((const V*)x + q);
+            rv[u] = __builtin_nontemporal_load((const V*)r + q);
+            pv[u] = __builtin_nontemporal_load((const V*)p + q);
+            av[u] = __builtin_nontemporal_load((const V*)ap + q);
+        }
+    }
+    double rr = 0.0, wrr = 0.0;
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+        const uint32_t q = base + u * EW_BLOCK;
+        if (q < nv) {
+            const V rn = rv[u] - a * av[u];
+            const V xn = xv[u] + a * pv[u];
+            __builtin_nontemporal_store(xn, (V*)x + q);
+            __builtin_nontemporal_store(rn, (V*)r + q);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const double rd = (double)rn[w];
+                rr += rd * rd;
+                if (sd != nullptr) {
+                    const double s = sd[q * W + w] * rd;
+                    wrr += s * s;
+                }
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < N - nv * W) {  // scalar tail
+        const uint32_t v = nv * W + threadIdx.x;
+        const T rn = r[v] - a * ap[v];
+        x[v] = x[v] + a * p[v];
+        r[v] = rn;
+        const double rd = (double)rn;
+        rr += rd * rd;
+        if (sd != nullptr) { const double s = sd[v] * rd; wrr += s * s; }
+    }
+    double q2[2] = {rr, wrr};
+    block_reduce<2, RED_SUM, RED_SUM>(q2, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = q2[0];
+        partials[gridDim.x + blockIdx.x] = q2[1];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(EW_BLOCK) void k_cg_direction_vec(
+    T* __restrict__ p, const T* __restrict__ r, const CgScalars* __restrict__ cs, uint32_t N) {
+    if (cs->done) return;
+    constexpr int W = StencilShape<T>::W;
+    typedef typename VecOf<T, W>::type V;
+    const T beta = (T)cs->beta;
+    const uint32_t nv = N / W;
+    const uint32_t base = blockIdx.x * (uint32_t)(EW_BLOCK * VU) + threadIdx.x;
+    V pv[VU], rv[VU];
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+        const uint32_t q = base + u * EW_BLOCK;
+        if (q < nv) {
+            pv[u] = __builtin_nontemporal_load((const V*)p + q);
+            rv[u] = __builtin_nontemporal_load((const V*)r + q);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < VU; ++u) {
+        const uint32_t q = base + u * EW_BLOCK;
+        if (q < nv) __builtin_nontemporal_store(rv[u] + beta * pv[u], (V*)p + q);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < N - nv * W) {
+        const uint32_t v = nv * W + threadIdx.x;
+        p[v] = r[v] + beta * p[v];
+    }
+}
+
+template <typename T>
+static int vec_blocks(uint32_t N) {
+    const uint32_t nv = N / StencilShape<T>::W;
+    uint32_t b = (nv + EW_BLOCK * VU - 1) / (EW_BLOCK * VU);
+    return (int)(b < 1 ? 1 : b);
+}
+
+template <typename T>
+static dim3 vec_grid(const ls_ctx* c) {
+    constexpr int W = StencilShape<T>::W, ICH = StencilShape<T>::ICH;
+    return dim3((c->nk / W + WAVE - 1) / WAVE, (c->nj + TJ - 1) / TJ, (c->ni + ICH - 1) / ICH);
+}
+
+// The vectorised path needs nk % W == 0 and its block partials to fit.
+template <typename T>
+static bool use_vec(const ls_ctx* c) {
+    if (c->nk % StencilShape<T>::W != 0) return false;
+    const dim3 g = vec_grid<T>(c);
+    return (size_t)g.x * g.y * g.z <= (size_t)MAX_PARTIALS &&
+           2 * (size_t)vec_blocks<T>((uint32_t)c->N) <= (size_t)MAX_PARTIALS;
+}
+
 // alpha = r.r / p.Ap
 __global__ void k_cg_alpha(const double* __restrict__ partials, int n, CgScalars* cs) {
     if (cs->done) return;
@@ -318,30 +526,53 @@ static int cg64_residual(ls_ctx* c, double rtol, int maxit) {
     return 0;
 }
 
+// One stencil launch y = A^ x with the block partials of x.y; returns the
+// number of partials.
+template <typename T>
+static int launch_stencil(ls_ctx* c, const T* x, T* y, T* const* faces, const int* done) {
+    SGeo g = sgeo(c);
+    if (use_vec<T>(c)) {
+        dim3 grd = vec_grid<T>(c);
+        hipLaunchKernelGGL(k_stencil_vec<T>, grd, dim3(WAVE, TJ), 0, c->stream, x, y, faces[0],
+                           faces[1], faces[2], g, c->partials, done);
+        return (int)(grd.x * grd.y * grd.z);
+    }
+    dim3 grd = stencil_grid(c);
+    hipLaunchKernelGGL((k_stencil<T, 0>), grd, dim3(TK, TJ), 0, c->stream, x, y, faces[0],
+                       faces[1], faces[2], g, c->partials, done, nullptr, nullptr, nullptr);
+    return (int)(grd.x * grd.y * grd.z);
+}
+
 template <typename T>
 static int cg_iterations(ls_ctx* c, T* x, T* r, T* p, T* ap, T* const* faces, const double* sdw,
                          int batch) {
-    SGeo g = sgeo(c);
-    dim3 grd = stencil_grid(c), blk(TK, TJ);
-    int nbs = nblocks_stencil(c);
     uint32_t N = (uint32_t)c->N;
-    int nbe = ew_blocks(N);
+    const bool vec = use_vec<T>(c);
+    const int nbe = vec ? vec_blocks<T>(N) : ew_blocks(N);
     for (int b = 0; b < batch; ++b) {
         const bool sample = c->profiling && b == 0;
         if (sample) LS_HIP(hipEventRecord(c->ev[12], c->stream));
-        hipLaunchKernelGGL((k_stencil<T, 0>), grd, blk, 0, c->stream, p, ap, faces[0], faces[1],
-                           faces[2], g, c->partials, &c->cg->done, nullptr, nullptr, nullptr);
+        const int nbs = launch_stencil<T>(c, p, ap, faces, &c->cg->done);
         if (sample) {
             LS_HIP(hipEventRecord(c->ev[13], c->stream));
             c->prof_pending = sizeof(T) == 8 ? 64 : 32;
         }
         hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, c->stream, c->partials, nbs, c->cg);
-        hipLaunchKernelGGL(k_cg_update<T>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, x, r, p, ap,
-                           sdw, c->cg, N, c->partials);
+        if (vec) {
+            hipLaunchKernelGGL(k_cg_update_vec<T>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, x, r,
+                               p, ap, sdw, c->cg, N, c->partials);
+        } else {
+            hipLaunchKernelGGL(k_cg_update<T>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, x, r, p,
+                               ap, sdw, c->cg, N, c->partials);
+        }
         hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1024), 0, c->stream, c->partials, nbe, c->cg,
                            sdw != nullptr ? 1 : 0);
-        hipLaunchKernelGGL(k_cg_direction<T>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, p, r, c->cg,
-                           N);
+        if (vec)
+            hipLaunchKernelGGL(k_cg_direction_vec<T>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, p,
+                               r, c->cg, N);
+        else
+            hipLaunchKernelGGL(k_cg_direction<T>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, p, r,
+                               c->cg, N);
     }
     LS_LAUNCH_CHECK();
     return 0;
@@ -444,9 +675,14 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
                            c->p32, c->x32, N, 1.0 / scale, first ? 1 : 0, c->partials);
         hipLaunchKernelGGL(k_inner_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nbe,
                            c->cg, trig, maxit, first ? 1 : 0);
-        if (!first)
-            hipLaunchKernelGGL(k_cg_direction<float>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream,
-                               c->p32, c->r32, c->cg, N);
+        if (!first) {
+            if (use_vec<float>(c))
+                hipLaunchKernelGGL(k_cg_direction_vec<float>, dim3(vec_blocks<float>(N)),
+                                   dim3(EW_BLOCK), 0, c->stream, c->p32, c->r32, c->cg, N);
+            else
+                hipLaunchKernelGGL(k_cg_direction<float>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream,
+                                   c->p32, c->r32, c->cg, N);
+        }
         LS_LAUNCH_CHECK();
         first = false;
         if ((st = poll_cg(c))) return st;
@@ -524,9 +760,7 @@ int apply_pressure_unscaled(ls_ctx* c, const double* x_dev, double* y_dev) {
     SGeo g = sgeo(c);
     hipLaunchKernelGGL(k_scale_vec, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, x_dev, c->sd, c->p64,
                        N);
-    hipLaunchKernelGGL((k_stencil<double, 0>), stencil_grid(c), dim3(TK, TJ), 0, c->stream, c->p64,
-                       c->ap64, c->c64[0], c->c64[1], c->c64[2], g, c->partials, nullptr, nullptr,
-                       nullptr, nullptr);
+    launch_stencil<double>(c, c->p64, c->ap64, c->c64, nullptr);
     hipLaunchKernelGGL(k_scale_vec, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->ap64, c->sd, y_dev,
                        N);
     LS_LAUNCH_CHECK();
@@ -538,22 +772,13 @@ int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) 
         c->err = "no pressure system assembled";
         return -1;
     }
-    SGeo g = sgeo(c);
-    dim3 grd = stencil_grid(c), blk(TK, TJ);
     hipEvent_t e0 = c->ev[14], e1 = c->ev[15];
-    // warm up once
     for (int pass = 0; pass < 2; ++pass) {
         int n = pass == 0 ? 1 : reps;
         if (pass == 1) LS_HIP(hipEventRecord(e0, c->stream));
         for (int r = 0; r < n; ++r) {
-            if (precision == 32)
-                hipLaunchKernelGGL((k_stencil<float, 0>), grd, blk, 0, c->stream, c->p32, c->ap32,
-                                   c->c32[0], c->c32[1], c->c32[2], g, c->partials, nullptr,
-                                   nullptr, nullptr, nullptr);
-            else
-                hipLaunchKernelGGL((k_stencil<double, 0>), grd, blk, 0, c->stream, c->p64, c->ap64,
-                                   c->c64[0], c->c64[1], c->c64[2], g, c->partials, nullptr,
-                                   nullptr, nullptr, nullptr);
+            if (precision == 32) launch_stencil<float>(c, c->p32, c->ap32, c->c32, nullptr);
+            else launch_stencil<double>(c, c->p64, c->ap64, c->c64, nullptr);
         }
         LS_LAUNCH_CHECK();
     }

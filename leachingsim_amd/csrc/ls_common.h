@@ -14,7 +14,7 @@ constexpr uint8_t AIR = LS_AIR, PORE = LS_PORE, ROCK = LS_ROCK, SUL = LS_SULPHID
 
 constexpr int WAVE = 64;          // CDNA wavefront width
 constexpr int EW_BLOCK = 256;     // elementwise / reduction block size
-constexpr int MAX_PARTIALS = 32768;
+constexpr int MAX_PARTIALS = 262144;
 
 // Tile of the i-marching stencil kernels: 64 threads along k (one wave, one
 // coalesced 512 B / 256 B row), TJ rows along j, each thread marches ICHUNK
