@@ -227,6 +227,13 @@ int ls_set_profiling(ls_ctx* ctx, int on);
 /* y = A x for the reduced pressure operator assembled by the last pressure
  * phase (rows outside the solved set return 0).  Host buffers, N doubles. */
 int ls_debug_apply_pressure(ls_ctx* ctx, const double* x, double* y);
+/* The scaling of the last assembled pressure system: sqrt(d) (0 off the
+ * solved set), the scaled right-hand side D^-1/2 b and the row classes
+ * (0 Dirichlet, 1 solved, 2 floored).  N elements each. */
+int ls_debug_pressure_system(ls_ctx* ctx, double* sqrt_d, double* bhat, int8_t* cls);
+/* The scaled FP64 +face conductances of the last assembled system (the
+ * operator A^ x = x - sum c^ x_n), 3 x N doubles (i, j, k faces). */
+int ls_debug_pressure_faces(ls_ctx* ctx, double* faces);
 /* Time `reps` launches of the pressure stencil kernel y = A x (precision 64 or
  * 32) on the context's stream with HIP events; returns ms per launch and the
  * kernel's algorithmic bytes per launch. */
@@ -235,6 +242,45 @@ int ls_time_stencil(ls_ctx* ctx, int precision, int reps, double* ms_per_launch,
 /* Copy bytes within device memory on the context's stream, timed with HIP
  * events (a copy kernel of known size, for bandwidth calibration). */
 int ls_time_copy(ls_ctx* ctx, size_t bytes, int reps, double* ms_per_launch);
+
+/* ---- The learned part: a 3-level 3D U-Net (16/32/64 channels, 3x3x3
+ * convolutions, BF16 on MFMA with FP32 accumulation), no reference
+ * counterpart.  Layers and the flat FP32 weight layout are those of
+ * leachingsim_amd/unet_spec.py.  These entry points train it and run it on
+ * host buffers; the pressure solver uses it through ls_set_network(). ---- */
+typedef struct ls_net ls_net;
+
+size_t ls_net_num_weights(void);
+/* A network on an ni x nj x nk grid (multiples of 16) for `batch` samples;
+ * trainable != 0 also allocates gradient and optimiser buffers. */
+int ls_net_create(ls_net** out, int ni, int nj, int nk, int batch, int trainable);
+void ls_net_destroy(ls_net* net);
+const char* ls_net_last_error(const ls_net* net);
+int ls_net_set_weights(ls_net* net, const float* w, size_t count);
+int ls_net_get_weights(ls_net* net, float* w, size_t count);
+/* in: [batch][ni][nj][nk][16] FP32; out: [batch][ni][nj][nk] (channel 0). */
+int ls_net_forward(ls_net* net, const float* in, float* out);
+/* Weight gradient of sum(gout * out) for the input of the last forward. */
+int ls_net_backward(ls_net* net, const float* gout, float* grad_w);
+/* One Adam step on the masked mean squared error of output channel 0. */
+int ls_net_train_step(ls_net* net, const float* in, const float* target, const float* mask,
+                      float lr, double* loss);
+/* One Adam step for an arbitrary loss: backward of the last forward with
+ * gout = d(loss)/d(output channel 0) ([batch][N]). */
+int ls_net_step(ls_net* net, const float* gout, float lr);
+/* Average time of `reps` forward passes on the device-resident input. */
+int ls_net_time_forward(ls_net* net, int reps, double* ms_per_forward);
+
+/* Weights of the learned coarse-level preconditioner used by
+ * LS_SOLVER_LEARNED (flat FP32 layout above).  The network runs on the grid
+ * given by ls_network_grid(): the pressure grid halved (rounded up) and
+ * padded to multiples of 16. */
+int ls_set_network(ls_ctx* ctx, const float* weights, size_t count);
+int ls_network_grid(ls_ctx* ctx, int* dims3);
+/* The network input built from a scaled residual r (N floats, host) by the
+ * learned preconditioner's restriction (for tests and training data):
+ * out = [coarse grid][16] FP32 (values rounded to BF16). */
+int ls_debug_learned_pack(ls_ctx* ctx, const float* r, float* out, size_t out_count);
 
 #ifdef __cplusplus
 }

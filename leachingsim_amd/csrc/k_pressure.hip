@@ -13,8 +13,11 @@
 // device; every kernel returns at once when the `done` flag is set, so the
 // host launches iterations in batches and polls once per batch.
 #include <cmath>
+#include <cstring>
+#include <utility>
 
 #include "ls_context.h"
+#include "ls_unet.h"
 
 namespace ls {
 
@@ -727,16 +730,15 @@ static double effective_top_pressure(double top, double t) {
     return eff;
 }
 
+static int solve_learned(ls_ctx* c, ls_step_report* rep);
+
 int solve_pressure(ls_ctx* c, double t, ls_step_report* rep) {
     const double ptop = effective_top_pressure(c->P.top_pressure, t);
     int st = assemble_pressure(c, t);
     if (st) return st;
     if (c->P.solver == LS_SOLVER_PLAIN64) st = solve_plain64(c, rep);
     else if (c->P.solver == LS_SOLVER_MIXED) st = solve_mixed(c, rep);
-    else {
-        c->err = "learned solver not available in this build";
-        return -20;
-    }
+    else st = solve_learned(c, rep);
     if (st < 0) return st;
     int st2 = finalize_pressure(c, ptop);
     if (st2) return st2;
@@ -790,6 +792,504 @@ int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) 
     const double sv = precision == 32 ? 4.0 : 8.0;
     *bytes = (double)c->N * (2.0 * sv + 3.0 * sv);
     return 0;
+}
+
+
+
+// ===========================================================================
+// Learned solver: flexible CG (Notay 2000, truncation 1) preconditioned by a
+// two-level operator whose coarse level is the U-Net.
+//
+//   M(r) = 2w r - w^2 A^r + g (P e - w A^P e),   e = UNet(R r)
+//
+// i.e. one damped-Jacobi step (weight w, the scaled diagonal is 1), an
+// additive coarse correction P e from the network applied to the residual
+// restricted to the 2x coarser grid, with the Galerkin step length
+// g = (Pe . (r - w A^r)) / (Pe . A^Pe), then one damped-Jacobi post-smoothing
+// step (A^ z is known by linearity, no extra stencil).  R sums the 8 fine
+// children of a coarse voxel; P copies the coarse value to the solved fine
+// children.  Vectors and stencils are FP32 (dot products FP64); the FP64
+// residual replaces the FP32 recurrence as in the mixed solver, and the
+// solver falls back to mixed PCG on breakdown or stagnation.
+// ===========================================================================
+
+constexpr double kJacobiW = 0.6;
+
+struct CGeo {
+    int ni, nj, nk;       // fine grid
+    int ci, cj, ck;       // network (coarse, padded) grid
+    uint32_t si, sj;
+    uint32_t cn;
+};
+
+static CGeo cgeo(const ls_ctx* c) {
+    CGeo g;
+    g.ni = c->ni; g.nj = c->nj; g.nk = c->nk;
+    g.si = (uint32_t)c->nk * (uint32_t)c->nj;
+    g.sj = (uint32_t)c->nk;
+    g.ci = c->net_dims[0]; g.cj = c->net_dims[1]; g.ck = c->net_dims[2];
+    g.cn = (uint32_t)g.ci * g.cj * g.ck;
+    return g;
+}
+
+// Restriction and the static input channels.  One thread per coarse voxel.
+// ch0 (written later, scaled): sum of the 8 children's residual; ch1: fraction
+// of solved children; ch2-4: mean +face conductances; ch5-7: mean -face
+// conductances; ch8-15: zero.
+__global__ void k_lpack(const float* __restrict__ r, const int8_t* __restrict__ cls,
+                        const float* __restrict__ cx, const float* __restrict__ cy,
+                        const float* __restrict__ cz, float* __restrict__ rc,
+                        __bf16* __restrict__ xin, CGeo g, double* __restrict__ partials) {
+    __shared__ double lds[EW_BLOCK / WAVE];
+    double ss = 0.0;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < g.cn; q += gridDim.x * blockDim.x) {
+        const uint32_t kq = q % g.ck, jq = (q / g.ck) % g.cj, iq = q / ((uint32_t)g.ck * g.cj);
+        float rs = 0.f, m = 0.f, f[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int d = 0; d < 2; ++d) {
+                    const int i = 2 * iq + a, j = 2 * jq + b, k = 2 * kq + d;
+                    if (i >= g.ni || j >= g.nj || k >= g.nk) continue;
+                    const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
+                    rs += r[v];
+                    m += cls[v] == 1 ? 1.f : 0.f;
+                    f[0] += cx[v];
+                    f[1] += cy[v];
+                    f[2] += cz[v];
+                    f[3] += i > 0 ? cx[v - g.si] : 0.f;
+                    f[4] += j > 0 ? cy[v - g.sj] : 0.f;
+                    f[5] += k > 0 ? cz[v - 1] : 0.f;
+                }
+        rc[q] = rs;
+        ss += (double)rs * rs;
+        __bf16* o = xin + (size_t)q * 16;
+        o[1] = (__bf16)(m * 0.125f);
+        for (int e = 0; e < 6; ++e) o[2 + e] = (__bf16)(f[e] * 0.125f);
+        for (int e = 8; e < 16; ++e) o[e] = (__bf16)0.f;
+    }
+    double a[1] = {ss};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a[0];
+}
+
+// scale of ch0: 1 / rms of the restricted residual over the coarse grid
+__global__ void k_lpack_scale(const double* __restrict__ partials, int n, uint32_t cn,
+                              FcgScalars* fs) {
+    __shared__ double lds[1024 / WAVE];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partials[i];
+    double a[1] = {acc};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) fs->scale_c = a[0] > 0.0 ? 1.0 / sqrt(a[0] / cn) : 0.0;
+}
+
+__global__ void k_lpack_ch0(const float* __restrict__ rc, __bf16* __restrict__ xin, uint32_t cn,
+                            const FcgScalars* __restrict__ fs) {
+    const float sc = (float)fs->scale_c;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < cn; q += gridDim.x * blockDim.x)
+        xin[(size_t)q * 16] = (__bf16)(rc[q] * sc);
+}
+
+// P: w[v] = e[parent(v)] on solved voxels, 0 elsewhere
+__global__ void k_lprolong(const float* __restrict__ out, const int8_t* __restrict__ cls,
+                           float* __restrict__ w, CGeo g) {
+    const uint32_t N = (uint32_t)g.ni * g.nj * g.nk;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        const uint32_t k = v % g.nk, j = (v / g.nk) % g.nj, i = v / g.si;
+        const uint32_t q = ((i >> 1) * g.cj + (j >> 1)) * g.ck + (k >> 1);
+        w[v] = cls[v] == 1 ? out[(size_t)q * 16] : 0.f;
+    }
+}
+
+// dots (w . (r - om sr)) and (w . sw)
+__global__ void k_ldots(const float* __restrict__ w, const float* __restrict__ r,
+                        const float* __restrict__ sr, const float* __restrict__ sw, float om,
+                        uint32_t N, const FcgScalars* __restrict__ fs, double* __restrict__ partials) {
+    if (fs->done) return;
+    __shared__ double lds[2 * EW_BLOCK / WAVE];
+    double a0 = 0.0, a1 = 0.0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        const double wv = w[v];
+        a0 += wv * (double)(r[v] - om * sr[v]);
+        a1 += wv * (double)sw[v];
+    }
+    double q[2] = {a0, a1};
+    block_reduce<2, RED_SUM, RED_SUM>(q, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = q[0];
+        partials[gridDim.x + blockIdx.x] = q[1];
+    }
+}
+
+__global__ void k_lgamma(const double* __restrict__ partials, int n, FcgScalars* fs) {
+    if (fs->done) return;
+    __shared__ double lds[2 * 1024 / WAVE];
+    double a0 = 0.0, a1 = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        a0 += partials[i];
+        a1 += partials[n + i];
+    }
+    double a[2] = {a0, a1};
+    block_reduce<2, RED_SUM, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) fs->gamma = (a[1] > 0.0 && isfinite(a[0] / a[1])) ? a[0] / a[1] : 0.0;
+}
+
+// z = 2 om r - om^2 sr + g (w - om sw); partial z . qprev
+__global__ void k_lz(const float* __restrict__ r, const float* __restrict__ sr,
+                     const float* __restrict__ w, const float* __restrict__ sw,
+                     const float* __restrict__ qprev, float* __restrict__ z, float om, uint32_t N,
+                     const FcgScalars* __restrict__ fs, double* __restrict__ partials) {
+    if (fs->done) return;
+    __shared__ double lds[EW_BLOCK / WAVE];
+    const float g = (float)fs->gamma;
+    double a0 = 0.0;
+    const bool first = fs->first;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        const float zv = 2.f * om * r[v] - om * om * sr[v] + g * (w[v] - om * sw[v]);
+        z[v] = zv;
+        if (!first) a0 += (double)zv * (double)qprev[v];
+    }
+    double a[1] = {a0};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a[0];
+}
+
+__global__ void k_lbeta(const double* __restrict__ partials, int n, FcgScalars* fs) {
+    if (fs->done) return;
+    __shared__ double lds[1024 / WAVE];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partials[i];
+    double a[1] = {acc};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) fs->beta = (fs->first || !(fs->pq_prev > 0.0)) ? 0.0 : a[0] / fs->pq_prev;
+}
+
+// p = z - beta pprev; partial p . r
+__global__ void k_lp(const float* __restrict__ z, const float* __restrict__ pprev,
+                     float* __restrict__ p, const float* __restrict__ r, uint32_t N,
+                     const FcgScalars* __restrict__ fs, double* __restrict__ partials) {
+    if (fs->done) return;
+    __shared__ double lds[EW_BLOCK / WAVE];
+    const float b = (float)fs->beta;
+    double a0 = 0.0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        const float pv = z[v] - b * pprev[v];
+        p[v] = pv;
+        a0 += (double)pv * (double)r[v];
+    }
+    double a[1] = {a0};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a[0];
+}
+
+__global__ void k_lpr(const double* __restrict__ partials, int n, FcgScalars* fs) {
+    if (fs->done) return;
+    __shared__ double lds[1024 / WAVE];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partials[i];
+    double a[1] = {acc};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) fs->pr = a[0];
+}
+
+// alpha = p.r / p.q  (p.q from the stencil partials)
+__global__ void k_lalpha(const double* __restrict__ partials, int n, FcgScalars* fs, CgScalars* cs) {
+    if (fs->done) return;
+    __shared__ double lds[1024 / WAVE];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partials[i];
+    double a[1] = {acc};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) {
+        const double pq = a[0];
+        fs->pq = pq;
+        if (!(pq > 0.0) || !isfinite(pq) || !isfinite(fs->pr)) {
+            fs->done = 1;
+            fs->broke = 1;
+            cs->done = 1;
+        } else {
+            fs->alpha = fs->pr / pq;
+            cs->alpha = fs->alpha;   // consumed by the shared update kernel
+            cs->done = 0;
+        }
+    }
+}
+
+// after the update: r.r, trigger for the FP64 replacement, bookkeeping
+__global__ void k_lcheck(const double* __restrict__ partials, int n, FcgScalars* fs) {
+    if (fs->done) return;
+    __shared__ double lds[2 * 1024 / WAVE];
+    double a0 = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a0 += partials[i];
+    double a[1] = {a0};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) {
+        const double rr = a[0];
+        fs->iter += 1;
+        fs->rr_prev = fs->rr;
+        fs->rr = rr;
+        fs->pq_prev = fs->pq;
+        fs->first = 0;
+        if (!isfinite(rr)) {
+            fs->done = 1;
+            fs->broke = 1;
+        } else if (rr <= fs->tol2 || fs->iter >= fs->maxit) {
+            fs->done = 1;
+        }
+    }
+}
+
+__global__ void k_lstart(FcgScalars* fs, double rr, double trig, int maxit, int first) {
+    fs->rr = rr;
+    fs->tol2 = trig;
+    fs->maxit = maxit;
+    fs->done = 0;
+    fs->broke = 0;
+    if (first) {
+        fs->iter = 0;
+        fs->first = 1;
+        fs->pq_prev = 0.0;
+    }
+}
+
+int learned_setup(ls_ctx* c) {
+    if (c->net != nullptr) return 0;
+    if (c->net_weights.empty()) {
+        c->err = "learned solver: no network weights (ls_set_network)";
+        return -20;
+    }
+    auto up16 = [](int n) { int h = (n + 1) / 2; return ((h + 15) / 16) * 16; };
+    c->net_dims[0] = up16(c->ni);
+    c->net_dims[1] = up16(c->nj);
+    c->net_dims[2] = up16(c->nk);
+    c->net = new UNet();
+    int st = unet_alloc(c->net, c->net_dims[0], c->net_dims[1], c->net_dims[2], 1, false, c->stream);
+    if (st) {
+        c->err = "learned solver: " + c->net->err;
+        return st;
+    }
+    const size_t cn = (size_t)c->net_dims[0] * c->net_dims[1] * c->net_dims[2];
+    if (hipMalloc(&c->rc, cn * sizeof(float)) != hipSuccess) {
+        c->err = "learned solver: hipMalloc";
+        return -12;
+    }
+    if (c->net_weights.size() != c->net->nw) {
+        c->err = "learned solver: weight count mismatch";
+        return -3;
+    }
+    LS_HIP(hipMemcpyAsync(c->net->w, c->net_weights.data(), c->net->nw * sizeof(float),
+                          hipMemcpyHostToDevice, c->stream));
+    st = unet_convert_weights(c->net);
+    if (st) return st;
+    LS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+void learned_free(ls_ctx* c) {
+    if (c->net) {
+        unet_free(c->net);
+        delete c->net;
+        c->net = nullptr;
+    }
+    if (c->rc) (void)hipFree(c->rc);
+    c->rc = nullptr;
+}
+
+// z = M(r): restriction, U-Net, prolongation, Galerkin step, smoothing
+static int learned_precond(ls_ctx* c, const float* r, float* z, const float* qprev, float* sr,
+                           float* w, float* sw) {
+    const uint32_t N = (uint32_t)c->N;
+    const int nbe = ew_blocks(N);
+    CGeo g = cgeo(c);
+    const float om = (float)kJacobiW;
+    launch_stencil<float>(c, r, sr, c->c32, &c->fcg->done);
+    const int nbc = ew_blocks(g.cn);
+    hipLaunchKernelGGL(k_lpack, dim3(nbc), dim3(EW_BLOCK), 0, c->stream, r, c->cls, c->c32[0],
+                       c->c32[1], c->c32[2], c->rc, c->net->x, g, c->partials);
+    hipLaunchKernelGGL(k_lpack_scale, dim3(1), dim3(1024), 0, c->stream, c->partials, nbc, g.cn,
+                       c->fcg);
+    hipLaunchKernelGGL(k_lpack_ch0, dim3(nbc), dim3(EW_BLOCK), 0, c->stream, c->rc, c->net->x, g.cn,
+                       c->fcg);
+    int st = unet_forward(c->net, 1);
+    if (st) {
+        c->err = c->net->err;
+        return st;
+    }
+    hipLaunchKernelGGL(k_lprolong, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->net->out, c->cls, w, g);
+    launch_stencil<float>(c, w, sw, c->c32, &c->fcg->done);
+    hipLaunchKernelGGL(k_ldots, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, w, r, sr, sw, om, N, c->fcg,
+                       c->partials);
+    hipLaunchKernelGGL(k_lgamma, dim3(1), dim3(1024), 0, c->stream, c->partials, nbe, c->fcg);
+    hipLaunchKernelGGL(k_lz, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, r, sr, w, sw, qprev, z, om, N,
+                       c->fcg, c->partials);
+    hipLaunchKernelGGL(k_lbeta, dim3(1), dim3(1024), 0, c->stream, c->partials, nbe, c->fcg);
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+__global__ void k_lk_reset(FcgScalars* fs) { fs->done = 0; }
+
+__global__ void k_bf16_to_f32(const __bf16* __restrict__ a, float* __restrict__ b, uint32_t n) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+        b[e] = (float)a[e];
+}
+
+int learned_pack_debug(ls_ctx* c, const float* r_dev, float* out_dev) {
+    int st = learned_setup(c);
+    if (st) return st;
+    CGeo g = cgeo(c);
+    const int nbc = ew_blocks(g.cn);
+    hipLaunchKernelGGL(k_lk_reset, dim3(1), dim3(1), 0, c->stream, c->fcg);
+    hipLaunchKernelGGL(k_lpack, dim3(nbc), dim3(EW_BLOCK), 0, c->stream, r_dev, c->cls, c->c32[0],
+                       c->c32[1], c->c32[2], c->rc, c->net->x, g, c->partials);
+    hipLaunchKernelGGL(k_lpack_scale, dim3(1), dim3(1024), 0, c->stream, c->partials, nbc, g.cn,
+                       c->fcg);
+    hipLaunchKernelGGL(k_lpack_ch0, dim3(nbc), dim3(EW_BLOCK), 0, c->stream, c->rc, c->net->x, g.cn,
+                       c->fcg);
+    hipLaunchKernelGGL(k_bf16_to_f32, dim3(ew_blocks(g.cn * 16)), dim3(EW_BLOCK), 0, c->stream,
+                       c->net->x, out_dev, g.cn * 16);
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+static int poll_fcg(ls_ctx* c) {
+    LS_HIP(hipMemcpyAsync(c->h_fcg, c->fcg, sizeof(FcgScalars), hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int solve_learned(ls_ctx* c, ls_step_report* rep) {
+    int st = learned_setup(c);
+    if (st) return st;
+    // after repeated losing probes the network is rested for a few solves
+    if (c->net_skip > 0) {
+        --c->net_skip;
+        rep->press_fallback = 1;
+        return solve_mixed(c, rep);
+    }
+    const uint32_t N = (uint32_t)c->N;
+    const int nbe = ew_blocks(N);
+    // FP32 work vectors borrowed from the (idle) transport buffers
+    float* z = (float*)c->tv[0][0];
+    float* pa = (float*)c->tv[0][1];
+    float* pb = (float*)c->tv[0][2];
+    float* qa = (float*)c->tv[0][3];
+    float* qb = (float*)c->tv[0][4];
+    float* sr = (float*)c->tv[0][5];
+    float* w = (float*)c->tv[0][6];
+    float* sw = (float*)c->tv[0][7];
+    double* xbackup = c->tv[1][0];
+    if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+    if ((st = poll_cg(c))) return st;
+    const double bnorm2 = c->h_cg->bnorm2;
+    const double tol2 = c->h_cg->tol2;
+    rep->press_bnorm = sqrt(bnorm2);
+    rep->press_relres = c->h_cg->relres;
+    if (bnorm2 == 0.0) {
+        LS_HIP(hipMemsetAsync(c->x64, 0, c->N * sizeof(double), c->stream));
+        rep->press_relres = 0.0;
+        return 0;
+    }
+    if (c->h_cg->converged) return 0;
+    LS_HIP(hipMemcpyAsync(xbackup, c->x64, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    const double scale = std::ldexp(1.0, std::ilogb(sqrt(c->h_cg->rr)));
+    const double delta = c->P.mixed_inner_rtol > 0 ? c->P.mixed_inner_rtol : 0.1;
+    double rr64 = c->h_cg->rr, wrr = c->h_cg->wrr, last_wrr = wrr;
+    const double wrr_start = wrr;
+    const int maxit = c->P.press_maxit < 2000 ? c->P.press_maxit : 2000;
+    int stalls = 0, applies = 0;
+    bool first = true;
+    const double rr_start = rr64 / (scale * scale);
+    bool probed = false, lost = false;
+    while (true) {
+        const double rr32 = rr64 / (scale * scale);
+        const double w2 = wrr / rr64;
+        double trig = delta * delta * rr32;
+        const double t2 = 0.81 * tol2 / (w2 * scale * scale);
+        if (t2 > trig) trig = t2;
+        hipLaunchKernelGGL(k_inner_start, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->r64, c->r32,
+                           c->p32, c->x32, N, 1.0 / scale, 0, c->partials);
+        hipLaunchKernelGGL(k_lstart, dim3(1), dim3(1), 0, c->stream, c->fcg, rr32, trig, maxit,
+                           first ? 1 : 0);
+        LS_LAUNCH_CHECK();
+        first = false;
+        if ((st = poll_fcg(c))) return st;
+        while (!c->h_fcg->done) {
+            for (int it = 0; it < 2; ++it) {
+                // z = M(r); beta = z.qprev / pq_prev
+                if ((st = learned_precond(c, c->r32, z, qb, sr, w, sw))) return st;
+                ++applies;
+                hipLaunchKernelGGL(k_lp, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, z, pb, pa, c->r32,
+                                   N, c->fcg, c->partials);
+                hipLaunchKernelGGL(k_lpr, dim3(1), dim3(1024), 0, c->stream, c->partials, nbe, c->fcg);
+                const int nbs = launch_stencil<float>(c, pa, qa, c->c32, &c->fcg->done);
+                hipLaunchKernelGGL(k_lalpha, dim3(1), dim3(1024), 0, c->stream, c->partials, nbs,
+                                   c->fcg, c->cg);
+                if (use_vec<float>(c))
+                    hipLaunchKernelGGL(k_cg_update_vec<float>, dim3(vec_blocks<float>(N)),
+                                       dim3(EW_BLOCK), 0, c->stream, c->x32, c->r32, pa, qa, nullptr,
+                                       c->cg, N, c->partials);
+                else
+                    hipLaunchKernelGGL(k_cg_update<float>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream,
+                                       c->x32, c->r32, pa, qa, nullptr, c->cg, N, c->partials);
+                const int nbu = use_vec<float>(c) ? vec_blocks<float>(N) : nbe;
+                hipLaunchKernelGGL(k_lcheck, dim3(1), dim3(1024), 0, c->stream, c->partials, nbu,
+                                   c->fcg);
+                LS_LAUNCH_CHECK();
+                std::swap(pa, pb);
+                std::swap(qa, qb);
+            }
+            if ((st = poll_fcg(c))) return st;
+            // probe: within its first 32 iterations the network must cut the
+            // residual by 100x, else plain (mixed) PCG takes over
+            if (!probed && c->h_fcg->iter >= 32) {
+                probed = true;
+                if (!(c->h_fcg->rr <= 1e-4 * rr_start)) {
+                    lost = true;
+                    break;
+                }
+            }
+        }
+        const int iters = c->h_fcg->iter;
+        const bool broke = c->h_fcg->broke;
+        rep->press_iters = iters;
+        rep->press_unet_applies = applies;
+        if (lost || broke) break;
+        hipLaunchKernelGGL(k_refine_add, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->x64, c->x32, N,
+                           scale);
+        LS_LAUNCH_CHECK();
+        rep->press_outer += 1;
+        if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+        if ((st = poll_cg(c))) return st;
+        rr64 = c->h_cg->rr;
+        wrr = c->h_cg->wrr;
+        rep->press_relres = c->h_cg->relres;
+        if (c->h_cg->converged) {
+            c->net_failures = 0;
+            return 0;
+        }
+        stalls = (wrr > 0.25 * last_wrr) ? stalls + 1 : 0;
+        last_wrr = wrr;
+        if (stalls >= 2 || iters >= maxit) break;
+    }
+    // fallback: plain (mixed-precision) PCG, from the FCG iterate when it
+    // improved on the start, else from the saved start
+    rep->press_fallback = 1;
+    if (lost || c->h_fcg->broke || !(wrr < wrr_start)) {
+        LS_HIP(hipMemcpyAsync(c->x64, xbackup, c->N * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+        if (++c->net_failures >= 2) {
+            c->net_skip = 8;
+            c->net_failures = 0;
+        }
+    }
+    ls_step_report r2;
+    std::memset(&r2, 0, sizeof(r2));
+    st = solve_mixed(c, &r2);
+    rep->press_iters += r2.press_iters;
+    rep->press_outer += r2.press_outer;
+    rep->press_relres = r2.press_relres;
+    return st;
 }
 
 }  // namespace ls

@@ -168,13 +168,15 @@ int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes, const ls_pa
     st |= dalloc(c, &c->cg, 1);
     st |= dalloc(c, &c->bicg, 1);
     st |= dalloc(c, &c->ss, 1);
+    st |= dalloc(c, &c->fcg, 1);
     if (st) {
         *out = c;
         return -12;
     }
     if (hipHostMalloc((void**)&c->h_cg, sizeof(CgScalars)) != hipSuccess ||
         hipHostMalloc((void**)&c->h_bicg, sizeof(BicgScalars)) != hipSuccess ||
-        hipHostMalloc((void**)&c->h_ss, sizeof(StepScalars)) != hipSuccess) {
+        hipHostMalloc((void**)&c->h_ss, sizeof(StepScalars)) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_fcg, sizeof(FcgScalars)) != hipSuccess) {
         c->err = "hipHostMalloc failed";
         *out = c;
         return -12;
@@ -188,6 +190,7 @@ int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes, const ls_pa
     (void)hipMemsetAsync(c->cg, 0, sizeof(CgScalars), c->stream);
     (void)hipMemsetAsync(c->bicg, 0, sizeof(BicgScalars), c->stream);
     (void)hipMemsetAsync(c->ss, 0, sizeof(StepScalars), c->stream);
+    (void)hipMemsetAsync(c->fcg, 0, sizeof(FcgScalars), c->stream);
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         c->err = "initialisation failed";
         *out = c;
@@ -200,10 +203,12 @@ int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes, const ls_pa
 void ls_destroy(ls_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    learned_free(c);
     for (void* q : c->allocations) (void)hipFree(q);
     if (c->h_cg) (void)hipHostFree(c->h_cg);
     if (c->h_bicg) (void)hipHostFree(c->h_bicg);
     if (c->h_ss) (void)hipHostFree(c->h_ss);
+    if (c->h_fcg) (void)hipHostFree(c->h_fcg);
     if (c->stream) {
         for (int i = 0; i < 16; ++i) (void)hipEventDestroy(c->ev[i]);
         (void)hipStreamDestroy(c->stream);
@@ -465,6 +470,78 @@ int ls_step(ls_ctx* c, double t, double dt, ls_step_report* report) {
     report_prof(c, rep);
     rep->status = ok ? 0 : 1;
     return rep->status;
+}
+
+int ls_set_network(ls_ctx* c, const float* w, size_t count) {
+    if (!c || !w) return -1;
+    learned_free(c);
+    c->net_weights.assign(w, w + count);
+    return 0;
+}
+
+int ls_debug_learned_pack(ls_ctx* c, const float* r_host, float* out_host, size_t out_count) {
+    if (!c || !r_host || !out_host) return -1;
+    if (!c->press_assembled) {
+        c->err = "no pressure system assembled";
+        return -1;
+    }
+    int st = learned_setup(c);
+    if (st) return st;
+    const size_t cn = (size_t)c->net_dims[0] * c->net_dims[1] * c->net_dims[2];
+    if (out_count != cn * 16) {
+        c->err = "output size must be 16 x the coarse network grid";
+        return -3;
+    }
+    float *rd = nullptr, *od = nullptr;
+    if (hipMalloc(&rd, c->N * sizeof(float)) != hipSuccess ||
+        hipMalloc(&od, cn * 16 * sizeof(float)) != hipSuccess) {
+        if (rd) (void)hipFree(rd);
+        c->err = "hipMalloc failed";
+        return -12;
+    }
+    if (hipMemcpyAsync(rd, r_host, c->N * sizeof(float), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        st = -10;
+    if (!st) st = learned_pack_debug(c, rd, od);
+    if (!st && hipMemcpyAsync(out_host, od, cn * 16 * sizeof(float), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        st = -10;
+    if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = -10;
+    (void)hipFree(rd);
+    (void)hipFree(od);
+    return st;
+}
+
+int ls_debug_pressure_system(ls_ctx* c, double* sd, double* bhat, int8_t* cls) {
+    if (!c || !sd || !bhat || !cls) return -1;
+    if (!c->press_assembled) {
+        c->err = "no pressure system assembled";
+        return -1;
+    }
+    LS_HIP(hipMemcpyAsync(sd, c->sd, c->N * 8, hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipMemcpyAsync(bhat, c->bhat, c->N * 8, hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipMemcpyAsync(cls, c->cls, c->N, hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ls_debug_pressure_faces(ls_ctx* c, double* faces) {
+    if (!c || !faces) return -1;
+    if (!c->press_assembled) {
+        c->err = "no pressure system assembled";
+        return -1;
+    }
+    for (int a = 0; a < 3; ++a)
+        LS_HIP(hipMemcpyAsync(faces + a * c->N, c->c64[a], c->N * 8, hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ls_network_grid(ls_ctx* c, int* dims) {
+    if (!c || !dims) return -1;
+    auto up16 = [](int n) { int h = (n + 1) / 2; return ((h + 15) / 16) * 16; };
+    dims[0] = up16(c->ni);
+    dims[1] = up16(c->nj);
+    dims[2] = up16(c->nk);
+    return 0;
 }
 
 int ls_set_profiling(ls_ctx* c, int on) {

@@ -41,6 +41,14 @@ struct BicgScalars {
     int all_done;
 };
 
+// Device scalars of the flexible CG with the learned preconditioner.
+struct FcgScalars {
+    double beta, alpha, gamma, scale_c;
+    double pq, pq_prev, pr, rr, rr_prev;
+    double tol2;
+    int iter, maxit, done, converged, broke, first;
+};
+
 struct StepScalars {
     double sat_max;           // max |div q| + evap over Pore voxels
     double sat_dt_sub_over_phi;
@@ -100,6 +108,15 @@ struct ls_ctx {
     CgScalars* cg = nullptr;
     BicgScalars* bicg = nullptr;
     StepScalars* ss = nullptr;
+    FcgScalars* fcg = nullptr;
+    FcgScalars* h_fcg = nullptr;
+    // learned preconditioner: U-Net on the 2x restricted grid
+    struct UNet* net = nullptr;
+    int net_dims[3] = {0, 0, 0};
+    float* rc = nullptr;            // restricted residual (coarse, FP32)
+    std::vector<float> net_weights;  // host copy, set by ls_set_network
+    int net_failures = 0;            // consecutive probes where the network lost
+    int net_skip = 0;                // solves left that go straight to mixed PCG
     // host pinned mirrors
     CgScalars* h_cg = nullptr;
     BicgScalars* h_bicg = nullptr;
@@ -136,6 +153,9 @@ int zero_flux_rock_precip(ls_ctx* c);
 int solve_pressure(ls_ctx* c, double t, ls_step_report* rep);
 int apply_pressure_unscaled(ls_ctx* c, const double* x_dev, double* y_dev);
 int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes);
+int learned_setup(ls_ctx* c);
+void learned_free(ls_ctx* c);
+int learned_pack_debug(ls_ctx* c, const float* r_dev, float* out_dev);
 // k_transport.hip
 int phase_transport(ls_ctx* c, double dt, ls_step_report* rep);
 // k_leach.hip

@@ -127,6 +127,11 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "ls_time_stencil": (i, [vp, i, i, C.POINTER(d), C.POINTER(d)]),
         "ls_time_copy": (i, [vp, sz, i, C.POINTER(d)]),
         "ls_set_profiling": (i, [vp, i]),
+        "ls_set_network": (i, [vp, C.POINTER(C.c_float), sz]),
+        "ls_network_grid": (i, [vp, C.POINTER(C.c_int)]),
+        "ls_debug_learned_pack": (i, [vp, vp, vp, sz]),
+        "ls_debug_pressure_system": (i, [vp, vp, vp, vp]),
+        "ls_debug_pressure_faces": (i, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -291,6 +296,36 @@ class GpuSimulation:
         ms, by = C.c_double(), C.c_double()
         self._check(self.lib.ls_time_stencil(self.h, precision, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def set_network(self, weights):
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        self._check(self.lib.ls_set_network(self.h, w.ctypes.data_as(C.POINTER(C.c_float)), w.size))
+        self._net_weights = w
+
+    def network_grid(self):
+        d = (C.c_int * 3)()
+        self._check(self.lib.ls_network_grid(self.h, d))
+        return tuple(d)
+
+    def learned_pack(self, r):
+        a = np.ascontiguousarray(r, dtype=np.float32)
+        dims = self.network_grid()
+        out = np.empty(dims + (16,), np.float32)
+        self._check(self.lib.ls_debug_learned_pack(self.h, a.ctypes.data, out.ctypes.data, out.size))
+        return out
+
+    def pressure_system(self):
+        sd = np.empty(self.shape)
+        bh = np.empty(self.shape)
+        cl = np.empty(self.shape, np.int8)
+        self._check(self.lib.ls_debug_pressure_system(self.h, sd.ctypes.data, bh.ctypes.data,
+                                                      cl.ctypes.data))
+        return sd, bh, cl
+
+    def pressure_faces(self):
+        f = np.empty((3,) + self.shape)
+        self._check(self.lib.ls_debug_pressure_faces(self.h, f.ctypes.data))
+        return f
 
     def set_profiling(self, on: bool):
         self._check(self.lib.ls_set_profiling(self.h, 1 if on else 0))

@@ -22,6 +22,14 @@ from leachingsim_amd.unet_spec import LAYERS, unpack_weights
 OFFS = [(di, dj, dk) for di in (-1, 0, 1) for dj in (-1, 0, 1) for dk in (-1, 0, 1)]
 
 
+def bf16_round(x):
+    """Round to the nearest bfloat16 (ties to even), returned as float64."""
+    a = np.asarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
 def shift(x, d):
     """out[v] = x[v + d] with zeros outside (x: (ni, nj, nk, C))."""
     out = np.zeros_like(x)
@@ -68,16 +76,28 @@ def up_bwd(g):
     return g.reshape(s[0] // 2, 2, s[1] // 2, 2, s[2] // 2, 2, s[3]).sum(axis=(1, 3, 5))
 
 
-def forward(x, flat_w):
-    """x: (ni, nj, nk, CIN) float; returns (out (ni,nj,nk), cache)."""
-    P = unpack_weights(np.asarray(flat_w, np.float64))
+def forward(x, flat_w, bf16=False):
+    """x: (ni, nj, nk, CIN) float; returns (out (ni,nj,nk), cache).
+    bf16=True rounds the input, the weights and every stored activation to
+    bfloat16 where the device kernels do (accumulation stays float64)."""
+    R = bf16_round if bf16 else (lambda a: a)
+    flat_w = np.asarray(flat_w, np.float64)
+    if bf16:
+        P0 = unpack_weights(flat_w)
+        parts = []
+        for L in LAYERS:
+            W, b = P0[L["name"]]
+            parts += [R(W).ravel(), b]
+        flat_w = np.concatenate(parts)
+    P = unpack_weights(flat_w)
+    x = R(x)
     cache = {"x": x}
     h = x
     acts = {}
     for L in LAYERS:
         name = L["name"]
         if L["src"] == "pool":
-            h = pool(acts[L["from"]])
+            h = R(pool(acts[L["from"]]))
         elif L["src"] == "cat":
             h = np.concatenate([up(acts[L["deep"]]), acts[L["skip"]]], axis=-1)
         elif L["src"] == "prev":
@@ -87,15 +107,19 @@ def forward(x, flat_w):
         y = conv(h, W, b)
         if L["relu"]:
             y = np.maximum(y, 0.0)
+            y = R(y)
         acts[name] = y
         h = y
     cache["acts"] = acts
     return acts[LAYERS[-1]["name"]][..., 0], cache
 
 
-def backward(gout, flat_w, cache):
+def backward(gout, flat_w, cache, bf16=False):
     """gout: (ni, nj, nk) gradient of the output channel; returns the flat
-    weight gradient (same layout as the weights)."""
+    weight gradient (same layout as the weights).  bf16=True rounds each
+    layer's output gradient to bfloat16 before it enters the convolution
+    gradients, as the device's MFMA operands are."""
+    R = bf16_round if bf16 else (lambda a: a)
     P = unpack_weights(np.asarray(flat_w, np.float64))
     acts = cache["acts"]
     grads = {}
@@ -112,7 +136,8 @@ def backward(gout, flat_w, cache):
         if L["relu"]:
             gy = gy * (acts[name] > 0)
         W, b = P[name]
-        gx, gW, gb = conv_bwd(cache["in_" + name], W, gy)
+        gx, gW, _ = conv_bwd(cache["in_" + name], R(W) if bf16 else W, R(gy))
+        gb = gy.sum(axis=(0, 1, 2))
         grads[name] = (gW, gb)
 
         def acc(key, val):
