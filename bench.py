@@ -185,7 +185,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU slab decomposition is a design, see DESIGN.md)", file=sys.stderr)
+              "(the i-slab decomposition across GPUs is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
     t0 = time.perf_counter()

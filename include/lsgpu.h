@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LSGPU_ABI_VERSION 1
+#define LSGPU_ABI_VERSION 2
 
 /* Labels (src/compiler_opts.h:69-82). */
 enum ls_label {
@@ -71,9 +71,15 @@ enum ls_bc_mode {
 /* Warm start of the pressure solve. */
 enum ls_warm_start {
     LS_WARM_PREVIOUS = 0,   /* previous pressure (src/Simulation.cpp:157,808)    */
-    LS_WARM_RESCALED = 1,   /* previous pressure x (new top / old top pressure)  */
-    LS_WARM_LEARNED = 2     /* U-Net prediction from the previous pressure       */
+    LS_WARM_PROJECTED = 1,  /* Galerkin projection onto the last warm_history
+                               solutions: the A-norm-optimal combination under
+                               the new operator (never worse than the previous
+                               pressure; exact for a fixed operator under the
+                               linear top-pressure ramp)                        */
+    LS_WARM_LEARNED = 2     /* the projection basis plus the U-Net correction of
+                               the projected residual                           */
 };
+#define LS_WARM_MAX 8       /* largest projection basis                          */
 
 /* All physics parameters of src/main.cpp:39,69-91 plus solver settings.
  * ls_default_params() fills the reference's defaults. */
@@ -103,7 +109,9 @@ typedef struct ls_params {
     int press_maxit;         /* 10000 (src/Simulation.cpp:156)       */
     double adr_rtol;         /* 1e-8                                 */
     int adr_maxit;           /* 10000                                */
-    int mixed_inner_maxit;   /* reserved (0)                                       */
+    int warm_history;        /* solutions kept for LS_WARM_PROJECTED / LEARNED (4),
+                                1 .. LS_WARM_MAX (LEARNED uses one slot for the
+                                network correction)                              */
     double mixed_inner_rtol; /* mixed solver: the FP64 residual replaces the FP32 one
                                 whenever the latter has fallen by this factor (0.1) */
 } ls_params;
@@ -174,6 +182,9 @@ typedef struct ls_step_report {
      * this step, summed, and their counts, for FP64 and FP32 stencils. */
     double prof_stencil64_ms, prof_stencil32_ms;
     int prof_stencil64_n, prof_stencil32_n;
+    double press_relres0;     /* ||b-Ax0||/||b|| of the warm start               */
+    int warm_basis;           /* projection vectors kept (0: previous pressure)  */
+    int warm_net;             /* 1 if the network correction was in the basis   */
 } ls_step_report;
 
 typedef struct ls_ctx ls_ctx;

@@ -27,7 +27,8 @@ ARCH = "gfx950"
 
 # -ffp-contract=off: no fused multiply-add, so every expression rounds as the
 # reference's scalar C++ does (bit-exact masks, reproducible fields).
-# -packed-fp32-ops: no packed FP32 instructions (toolchain).
+# -packed-fp32-ops: no packed FP32 instructions (v_pk_*_f32 are kept out of
+# every kernel).
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

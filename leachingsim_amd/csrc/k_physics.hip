@@ -206,6 +206,7 @@ int init_state(ls_ctx* c, long long* nsul) {
     LS_HIP(hipStreamSynchronize(c->stream));
     if (nsul) *nsul = (long long)c->h_ss->nsul;
     c->have_prev = false;
+    c->n_hist = 0;
     c->press_assembled = false;
     return phase_properties(c);
 }
@@ -304,7 +305,7 @@ __global__ void k_press_rhs(const uint8_t* __restrict__ img, const double* __res
                             const int8_t* __restrict__ cls, const double* __restrict__ pD,
                             const double* __restrict__ p, double* __restrict__ sd,
                             double* __restrict__ bhat, double* __restrict__ x, Geo g, int zfaces,
-                            double ptop, double evap_dx, double warm_scale) {
+                            double ptop, double evap_dx) {
     const int k = blockIdx.x * TK + threadIdx.x;
     const int j = blockIdx.y * TJ + threadIdx.y;
     const int i = blockIdx.z;
@@ -332,7 +333,7 @@ __global__ void k_press_rhs(const uint8_t* __restrict__ img, const double* __res
     double sv = sqrt(dsum[v]);
     sd[v] = sv;
     bhat[v] = rhs / sv;
-    x[v] = sv * (p[v] * warm_scale);  // warm start, scaled variables
+    x[v] = sv * p[v];  // warm start from the previous pressure, scaled variables
 }
 
 // Scaled operator faces: c / (sqrt(d_v) sqrt(d_n)) between two solved rows.
@@ -382,12 +383,9 @@ int assemble_pressure(ls_ctx* c, double t) {
     hipLaunchKernelGGL(k_press_faces, grd, blk, 0, c->stream, c->img, c->perm, c->p, c->craw[0],
                        c->craw[1], c->craw[2], c->cin, c->dsum, c->cls, c->pD, g, zf, ptop);
     LS_LAUNCH_CHECK();
-    double scale = 1.0;
-    if (c->P.warm_start == LS_WARM_RESCALED && c->have_prev && c->p_top_prev != 0.0)
-        scale = ptop / c->p_top_prev;
     hipLaunchKernelGGL(k_press_rhs, grd, blk, 0, c->stream, c->img, c->craw[0], c->craw[1],
                        c->craw[2], c->cin, c->dsum, c->cls, c->pD, c->p, c->sd, c->bhat, c->x64, g,
-                       zf, ptop, c->P.evap_flux * c->P.dx, scale);
+                       zf, ptop, c->P.evap_flux * c->P.dx);
     LS_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_press_scale, grd, blk, 0, c->stream, c->craw[0], c->craw[1], c->craw[2],
                        c->cls, c->sd, c->c64[0], c->c64[1], c->c64[2], c->c32[0], c->c32[1],

@@ -600,10 +600,11 @@ static int batch_size(const ls_ctx* c) {
 }
 
 // Plain FP64 Jacobi-PCG (as CG on A^).  Returns 0 converged, 1 not.
-static int solve_plain64(ls_ctx* c, ls_step_report* rep) {
-    int st = cg64_residual(c, c->P.press_rtol, c->P.press_maxit);
+static int solve_plain64(ls_ctx* c, ls_step_report* rep, int maxit) {
+    int st = cg64_residual(c, c->P.press_rtol, maxit);
     if (st) return st;
     if ((st = poll_cg(c))) return st;
+    if (!rep->press_outer) rep->press_relres0 = c->h_cg->relres;
     if (c->h_cg->bnorm2 == 0.0) {
         LS_HIP(hipMemsetAsync(c->x64, 0, c->N * sizeof(double), c->stream));
         rep->press_relres = 0.0;
@@ -621,9 +622,9 @@ static int solve_plain64(ls_ctx* c, ls_step_report* rep) {
         total += c->h_cg->iter;
         const bool conv = c->h_cg->converged;
         // confirm on the recomputed residual
-        if ((st = cg64_residual(c, c->P.press_rtol, c->P.press_maxit - total))) return st;
+        if ((st = cg64_residual(c, c->P.press_rtol, maxit - total))) return st;
         if ((st = poll_cg(c))) return st;
-        if (c->h_cg->converged || !conv || total >= c->P.press_maxit || restarts > 5) break;
+        if (c->h_cg->converged || !conv || total >= maxit || restarts > 5) break;
         ++restarts;
     }
     rep->press_iters += total;
@@ -654,6 +655,7 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
     const double tol2 = c->h_cg->tol2;
     rep->press_bnorm = sqrt(bnorm2);
     rep->press_relres = c->h_cg->relres;
+    if (!rep->press_outer) rep->press_relres0 = c->h_cg->relres;
     if (bnorm2 == 0.0) {
         LS_HIP(hipMemsetAsync(c->x64, 0, c->N * sizeof(double), c->stream));
         rep->press_relres = 0.0;
@@ -666,7 +668,10 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
     double last_wrr = wrr;
     int stalls = 0;
     bool first = true;
-    const int maxit = c->P.press_maxit;
+    // FP32 budget: an error concentrated in the slowest modes (eigenvalues
+    // near the FP32 rounding of the operator) makes the FP32 recurrence
+    // crawl; past this many iterations FP64 PCG finishes from the iterate
+    const int maxit = std::min(c->P.press_maxit, std::max(2000, c->P.press_maxit / 2));
     while (true) {
         // trigger: FP32 r.r fell by delta^2, or the estimated true norm hits tol
         const double rr32 = rr64 / (scale * scale);
@@ -709,13 +714,15 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
         rep->press_relres = c->h_cg->relres;
         rep->press_iters = iters;
         if (c->h_cg->converged) return 0;
-        if (iters >= maxit) return 1;
         // FP32 stagnation guard: the true residual must keep falling
         stalls = (wrr > 0.25 * last_wrr) ? stalls + 1 : 0;
         last_wrr = wrr;
-        if (broke || stalls >= 3) {
+        if (broke || stalls >= 3 || iters >= maxit) {
             rep->press_fallback = 1;
-            return solve_plain64(c, rep);   // FP64 PCG from the current iterate
+            // FP64 PCG from the current iterate, with the remaining budget
+            const int left = c->P.press_maxit - iters;
+            if (left <= 0) return 1;
+            return solve_plain64(c, rep, left);
         }
         // restore the device iteration counter reset by the residual kernel
         hipLaunchKernelGGL(k_set_iter, dim3(1), dim3(1), 0, c->stream, c->cg, iters);
@@ -731,17 +738,21 @@ static double effective_top_pressure(double top, double t) {
 }
 
 static int solve_learned(ls_ctx* c, ls_step_report* rep);
+static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep);
+static int warm_push(ls_ctx* c, double ptop);
 
 int solve_pressure(ls_ctx* c, double t, ls_step_report* rep) {
     const double ptop = effective_top_pressure(c->P.top_pressure, t);
     int st = assemble_pressure(c, t);
     if (st) return st;
-    if (c->P.solver == LS_SOLVER_PLAIN64) st = solve_plain64(c, rep);
+    if ((st = warm_start(c, ptop, rep))) return st;
+    if (c->P.solver == LS_SOLVER_PLAIN64) st = solve_plain64(c, rep, c->P.press_maxit);
     else if (c->P.solver == LS_SOLVER_MIXED) st = solve_mixed(c, rep);
     else st = solve_learned(c, rep);
     if (st < 0) return st;
     int st2 = finalize_pressure(c, ptop);
     if (st2) return st2;
+    if (st == 0 && (st2 = warm_push(c, ptop))) return st2;
     c->p_top_prev = ptop;
     c->have_prev = true;
     rep->top_pressure_eff = ptop;
@@ -1185,6 +1196,7 @@ static int solve_learned(ls_ctx* c, ls_step_report* rep) {
     const double tol2 = c->h_cg->tol2;
     rep->press_bnorm = sqrt(bnorm2);
     rep->press_relres = c->h_cg->relres;
+    if (!rep->press_outer) rep->press_relres0 = c->h_cg->relres;
     if (bnorm2 == 0.0) {
         LS_HIP(hipMemsetAsync(c->x64, 0, c->N * sizeof(double), c->stream));
         rep->press_relres = 0.0;
@@ -1290,6 +1302,290 @@ static int solve_learned(ls_ctx* c, ls_step_report* rep) {
     rep->press_outer += r2.press_outer;
     rep->press_relres = r2.press_relres;
     return st;
+}
+
+
+// ---------------------------------------------------------------------------
+// Warm start by Galerkin projection onto earlier solutions (Fischer 1998,
+// "Projection techniques for iterative solution of Ax = b with successive
+// right-hand sides", CMAME 163).  The last m pressure solutions, mapped to
+// the scaled variables of the new system (v = sqrt(d) p on solved rows), span
+// the space; x0 is the A^-orthogonal projection of the solution onto it,
+// x0 = sum_a (q_a . b^) q_a over an A^-orthonormal basis q.  x0 minimises the
+// A^-norm of the error over the span: never worse than the previous pressure,
+// which is in it, and exact when the operator is unchanged and the right-hand
+// side moves inside the span (the linear top-pressure ramp over a fixed
+// evaporation term needs two vectors).
+//
+// The raw vectors are the newest solution and the differences of successive
+// ones.  They are orthonormalised by classical Gram-Schmidt in the A^ inner
+// product, run twice, with the energy of each orthogonalised vector taken
+// from a fresh stencil application: successive solutions are close to
+// collinear, and a Gram matrix V^T A^ V of them loses the small directions
+// (the top-pressure part is nearly constant, close to the operator's null
+// space) to cancellation.  With the network (LS_WARM_LEARNED) the prolonged
+// U-Net correction of the projected residual joins the basis the same way, so
+// the network can only lower the A^-norm error of x0.
+
+struct WBasis {
+    const double* v[LS_WARM_MAX];
+};
+struct WCoef {
+    double c[LS_WARM_MAX];
+};
+
+// v = sqrt(d) (p - q)  (q == nullptr: v = sqrt(d) p)
+__global__ void k_wscale(const double* __restrict__ p, const double* __restrict__ q,
+                         const double* __restrict__ sd, double* __restrict__ v, uint32_t N) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        v[i] = sd[i] * (q ? p[i] - q[i] : p[i]);
+}
+
+__global__ void k_wfrom32(const float* __restrict__ a, double* __restrict__ b, uint32_t N) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
+
+// block partials of x . y
+__global__ void k_wdot(const double* __restrict__ x, const double* __restrict__ y, uint32_t N,
+                       double* __restrict__ partials) {
+    __shared__ double lds[EW_BLOCK / WAVE];
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        acc += x[i] * y[i];
+    double a[1] = {acc};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a[0];
+}
+
+// out[q] = sum of the nb partials of value q (one block per value, fixed order)
+__global__ void k_wsum(const double* __restrict__ partials, int nb, double* __restrict__ out) {
+    __shared__ double lds[1024 / WAVE];
+    const int q = blockIdx.x;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partials[q * nb + i];
+    double a[1] = {acc};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) out[q] = a[0];
+}
+
+// w -= sum_a C.c[a] B.v[a]
+__global__ void k_wsub(WBasis B, WCoef C, int m, double* __restrict__ w, uint32_t N) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        double s = w[i];
+        for (int a = 0; a < m; ++a) s -= C.c[a] * B.v[a][i];
+        w[i] = s;
+    }
+}
+
+// x (+)= sum_a C.c[a] B.v[a]
+__global__ void k_wcombine(WBasis B, WCoef C, int m, double* __restrict__ x, uint32_t N,
+                           int accumulate) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        double s = accumulate ? x[i] : 0.0;
+        for (int a = 0; a < m; ++a) s += C.c[a] * B.v[a][i];
+        x[i] = s;
+    }
+}
+
+__global__ void k_wscale_inplace(double* __restrict__ a, double* __restrict__ b, double s,
+                                 uint32_t N) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        a[i] *= s;
+        b[i] *= s;
+    }
+}
+
+static int warm_alloc(ls_ctx* c) {
+    if (c->wsum) return 0;
+    if (hipMalloc((void**)&c->wsum, 64 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_wsum, 64 * sizeof(double)) != hipSuccess) {
+        c->err = "warm start: allocation failed";
+        return -12;
+    }
+    c->allocations.push_back(c->wsum);
+    return 0;
+}
+
+// Dots x_a . y for a < n (n <= LS_WARM_MAX), returned on the host.
+static int warm_dots(ls_ctx* c, const double* const* x, int n, const double* y, double* out) {
+    const uint32_t N = (uint32_t)c->N;
+    const int nb = std::min(ew_blocks(N), MAX_PARTIALS / LS_WARM_MAX);
+    for (int a = 0; a < n; ++a)
+        hipLaunchKernelGGL(k_wdot, dim3(nb), dim3(EW_BLOCK), 0, c->stream, x[a], y, N,
+                           c->partials + (size_t)a * nb);
+    hipLaunchKernelGGL(k_wsum, dim3(n), dim3(1024), 0, c->stream, c->partials, nb, c->wsum);
+    LS_LAUNCH_CHECK();
+    LS_HIP(hipMemcpyAsync(c->h_wsum, c->wsum, n * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    for (int a = 0; a < n; ++a) out[a] = c->h_wsum[a];
+    return 0;
+}
+
+struct WarmSet {
+    WBasis q{}, aq{};   // A^-orthonormal basis and A^ q
+    int n = 0;
+};
+
+// Orthonormalise the raw vector in w (A^ w goes to aw) against the set and
+// add it when it keeps a share of its energy above rounding.  Returns 0 and
+// sets *added.
+static int warm_add(ls_ctx* c, WarmSet& S, double* w, double* aw, bool* added) {
+    const uint32_t N = (uint32_t)c->N;
+    const int nbe = ew_blocks(N);
+    *added = false;
+    double e_raw = 0.0, h[LS_WARM_MAX];
+    int st;
+    launch_stencil<double>(c, w, aw, c->c64, nullptr);
+    if ((st = warm_dots(c, (const double* const*)&w, 1, aw, &e_raw))) return st;
+    if (!(e_raw > 0.0) || !std::isfinite(e_raw)) return 0;
+    for (int pass = 0; pass < 2 && S.n > 0; ++pass) {
+        // h_a = w . A^ q_a ; w -= sum h_a q_a
+        if ((st = warm_dots(c, S.aq.v, S.n, w, h))) return st;
+        WCoef C{};
+        for (int a = 0; a < S.n; ++a) C.c[a] = h[a];
+        hipLaunchKernelGGL(k_wsub, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, S.q, C, S.n, w, N);
+        LS_LAUNCH_CHECK();
+    }
+    double e = e_raw;
+    if (S.n > 0) {
+        launch_stencil<double>(c, w, aw, c->c64, nullptr);
+        if ((st = warm_dots(c, (const double* const*)&w, 1, aw, &e))) return st;
+    }
+    // keep directions whose energy is far above the rounding of the
+    // orthogonalisation (relative energy ~ eps^2 times the stencil norm)
+    if (!(e > 1e-20 * e_raw) || !std::isfinite(e)) return 0;
+    hipLaunchKernelGGL(k_wscale_inplace, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, w, aw,
+                       1.0 / std::sqrt(e), N);
+    LS_LAUNCH_CHECK();
+    S.q.v[S.n] = w;
+    S.aq.v[S.n] = aw;
+    ++S.n;
+    *added = true;
+    return 0;
+}
+
+// x64 += sum over q[from..n) of (q . b^) q   (from == 0: x64 = ...)
+static int warm_accumulate(ls_ctx* c, const WarmSet& S, int from) {
+    const uint32_t N = (uint32_t)c->N;
+    double g[LS_WARM_MAX];
+    int st;
+    if ((st = warm_dots(c, S.q.v + from, S.n - from, c->bhat, g))) return st;
+    WBasis B{};
+    WCoef C{};
+    for (int a = from; a < S.n; ++a) {
+        B.v[a - from] = S.q.v[a];
+        C.c[a - from] = g[a - from];
+    }
+    hipLaunchKernelGGL(k_wcombine, dim3(ew_blocks(N)), dim3(EW_BLOCK), 0, c->stream, B, C,
+                       S.n - from, c->x64, N, from > 0 ? 1 : 0);
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+// x64 <- projected warm start (the assembled system is in place and x64
+// holds the previous-pressure start).  Work vectors: the idle transport
+// buffers (basis in tv[0], A^ basis in tv[1]).
+static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
+    rep->warm_basis = 0;
+    rep->warm_net = 0;
+    if (c->P.warm_start == LS_WARM_PREVIOUS || c->n_hist == 0) return 0;
+    int st = warm_alloc(c);
+    if (st) return st;
+    const uint32_t N = (uint32_t)c->N;
+    const int nbe = ew_blocks(N);
+    // Only solutions driven by a comparable top pressure (within 100x) are
+    // used: one solved at a near-zero drive (the ramp starts at 1e-6) is
+    // dominated by terms that do not scale with it (the floored rows' lagged
+    // values from the initial state, evaporation), and scaling it up puts
+    // the start's error into the slowest modes of the operator: fewer
+    // residual, many more CG iterations.
+    int m = 0;
+    while (m < std::min(c->n_hist, c->P.warm_history)) {
+        const int slot = (c->hist_head + LS_WARM_MAX - 1 - m) % LS_WARM_MAX;
+        const double ratio = c->hist_ptop[slot] / ptop;
+        if (!(ratio >= 0.01 && ratio <= 100.0)) break;
+        ++m;
+    }
+    if (m == 0) return 0;
+    WarmSet S;
+    int slot_used = 0;
+    for (int a = 0; a < m; ++a) {   // newest first, then successive differences
+        const int slot = (c->hist_head + LS_WARM_MAX - 1 - a) % LS_WARM_MAX;
+        const int newer = (slot + 1) % LS_WARM_MAX;
+        double* w = c->tv[0][slot_used];
+        double* aw = c->tv[1][slot_used];
+        hipLaunchKernelGGL(k_wscale, dim3(nbe), dim3(EW_BLOCK), 0, c->stream,
+                           a == 0 ? c->hist[slot] : c->hist[newer],
+                           a == 0 ? nullptr : c->hist[slot], c->sd, w, N);
+        LS_LAUNCH_CHECK();
+        bool added = false;
+        if ((st = warm_add(c, S, w, aw, &added))) return st;
+        if (added) ++slot_used;
+    }
+    if (S.n == 0) return 0;
+    if ((st = warm_accumulate(c, S, 0))) return st;
+    rep->warm_basis = S.n;
+    if (c->P.warm_start != LS_WARM_LEARNED) return 0;
+    // network correction of the projected residual, in FP32 like the
+    // preconditioner: r = b^ - A^ x0, e = P U(R r), then e joins the basis
+    if ((st = learned_setup(c))) return st;
+    if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+    if ((st = poll_cg(c))) return st;
+    if (c->h_cg->converged || !(c->h_cg->rr > 0.0)) return 0;
+    const double scale = std::ldexp(1.0, std::ilogb(std::sqrt(c->h_cg->rr)));
+    hipLaunchKernelGGL(k_inner_start, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->r64, c->r32,
+                       c->p32, c->x32, N, 1.0 / scale, 0, c->partials);
+    hipLaunchKernelGGL(k_lk_reset, dim3(1), dim3(1), 0, c->stream, c->fcg);
+    CGeo g = cgeo(c);
+    const int nbc = ew_blocks(g.cn);
+    hipLaunchKernelGGL(k_lpack, dim3(nbc), dim3(EW_BLOCK), 0, c->stream, c->r32, c->cls,
+                       c->c32[0], c->c32[1], c->c32[2], c->rc, c->net->x, g, c->partials);
+    hipLaunchKernelGGL(k_lpack_scale, dim3(1), dim3(1024), 0, c->stream, c->partials, nbc, g.cn,
+                       c->fcg);
+    hipLaunchKernelGGL(k_lpack_ch0, dim3(nbc), dim3(EW_BLOCK), 0, c->stream, c->rc, c->net->x,
+                       g.cn, c->fcg);
+    LS_LAUNCH_CHECK();
+    if ((st = unet_forward(c->net, 1))) {
+        c->err = c->net->err;
+        return st;
+    }
+    hipLaunchKernelGGL(k_lprolong, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->net->out, c->cls,
+                       c->x32, g);
+    double* w = c->tv[0][slot_used];
+    double* aw = c->tv[1][slot_used];
+    hipLaunchKernelGGL(k_wfrom32, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->x32, w, N);
+    LS_LAUNCH_CHECK();
+    bool added = false;
+    if ((st = warm_add(c, S, w, aw, &added))) return st;
+    if (!added) return 0;
+    if ((st = warm_accumulate(c, S, S.n - 1))) return st;
+    rep->warm_basis = S.n;
+    rep->warm_net = 1;
+    return 0;
+}
+
+// After a solve: the new pressure joins the history.
+static int warm_push(ls_ctx* c, double ptop) {
+    if (c->P.warm_start == LS_WARM_PREVIOUS) {
+        c->n_hist = 0;
+        return 0;
+    }
+    const int slot = c->hist_head;
+    if (!c->hist[slot]) {
+        if (hipMalloc((void**)&c->hist[slot], c->N * sizeof(double)) != hipSuccess) {
+            c->err = "out of device memory (pressure history)";
+            return -12;
+        }
+        c->allocations.push_back(c->hist[slot]);
+    }
+    LS_HIP(hipMemcpyAsync(c->hist[slot], c->p, c->N * sizeof(double), hipMemcpyDeviceToDevice,
+                          c->stream));
+    c->hist_ptop[slot] = ptop;
+    c->hist_head = (slot + 1) % LS_WARM_MAX;
+    c->n_hist = std::min(c->n_hist + 1, LS_WARM_MAX);
+    return 0;
 }
 
 }  // namespace ls

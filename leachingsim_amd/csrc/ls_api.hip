@@ -87,7 +87,7 @@ void ls_default_params(ls_params* p) {
     p->press_maxit = 10000;
     p->adr_rtol = 1e-8;
     p->adr_maxit = 10000;
-    p->mixed_inner_maxit = 0;
+    p->warm_history = 4;
     p->mixed_inner_rtol = 0.1;
 }
 
@@ -104,6 +104,13 @@ static int validate_params(ls_ctx* c, const ls_params* p) {
     }
     if (p->solver < 0 || p->solver > 2 || p->bc_mode < 0 || p->bc_mode > 1) {
         c->err = "invalid solver or boundary mode";
+        return -3;
+    }
+    if (p->warm_start < 0 || p->warm_start > 2 ||
+        (p->warm_start != LS_WARM_PREVIOUS &&
+         (p->warm_history < 1 || p->warm_history > LS_WARM_MAX ||
+          (p->warm_start == LS_WARM_LEARNED && p->warm_history > LS_WARM_MAX - 1)))) {
+        c->err = "invalid warm start or warm_history (1..8, 1..7 with the network)";
         return -3;
     }
     return 0;
@@ -209,6 +216,7 @@ void ls_destroy(ls_ctx* c) {
     if (c->h_bicg) (void)hipHostFree(c->h_bicg);
     if (c->h_ss) (void)hipHostFree(c->h_ss);
     if (c->h_fcg) (void)hipHostFree(c->h_fcg);
+    if (c->h_wsum) (void)hipHostFree(c->h_wsum);
     if (c->stream) {
         for (int i = 0; i < 16; ++i) (void)hipEventDestroy(c->ev[i]);
         (void)hipStreamDestroy(c->stream);
@@ -277,6 +285,7 @@ int ls_upload(ls_ctx* c, int field, const void* host, size_t elem_bytes) {
     }
     LS_HIP(hipMemcpyAsync(d, host, N * eb, hipMemcpyHostToDevice, c->stream));
     LS_HIP(hipStreamSynchronize(c->stream));
+    if (field == LS_F_PRESSURE) c->n_hist = 0;   // history no longer matches p
     return 0;
 }
 
@@ -354,6 +363,7 @@ int ls_rollback(ls_ctx* c) {
     st |= copy_d2d(c, c->qy, c->b_qy, N * 8);
     st |= copy_d2d(c, c->qz, c->b_qz, N * 8);
     if (st) return st;
+    c->n_hist = 0;   // the pressure history no longer belongs to this state
     // rebuild the derived properties (src/Simulation.cpp:2347-2352)
     st = phase_properties(c);
     if (st) return st;

@@ -124,6 +124,14 @@ struct ls_ctx {
 
     double p_top_prev = 0.0;
     bool have_prev = false;
+    // warm-start history: the last n_hist pressure solutions (physical p,
+    // FP64), newest at hist[(hist_head + LS_WARM_MAX - 1) % LS_WARM_MAX]
+    double* hist[LS_WARM_MAX] = {};
+    double hist_ptop[LS_WARM_MAX] = {};   // effective top pressure of each
+    int n_hist = 0;
+    int hist_head = 0;
+    double* wsum = nullptr;        // projection dot products (device)
+    double* h_wsum = nullptr;      // pinned mirror
     bool press_assembled = false;
 
     hipEvent_t ev[16];

@@ -21,7 +21,8 @@ AIR, PORE, ROCK, SULPHIDE, PRECIPITATE = 1, 2, 3, 4, 5
 
 SOLVER_PLAIN64, SOLVER_MIXED, SOLVER_LEARNED = 0, 1, 2
 BC_REFERENCE, BC_ZFACES = 0, 1
-WARM_PREVIOUS, WARM_RESCALED, WARM_LEARNED = 0, 1, 2
+WARM_PREVIOUS, WARM_PROJECTED, WARM_LEARNED = 0, 1, 2
+WARM_MAX = 8
 
 (F_IMG, F_FRAC, F_INVENTORY, F_SATURATION, F_PRESSURE, F_CONC, F_ACID, F_QX, F_QY, F_QZ,
  F_KREL, F_PC, F_PERM, F_MU) = range(14)
@@ -44,7 +45,7 @@ class Params(C.Structure):
         ("time_scalar", C.c_double), ("alpha_transfer", C.c_double),
         ("instant_precip", C.c_int), ("bc_mode", C.c_int), ("solver", C.c_int),
         ("warm_start", C.c_int), ("press_rtol", C.c_double), ("press_maxit", C.c_int),
-        ("adr_rtol", C.c_double), ("adr_maxit", C.c_int), ("mixed_inner_maxit", C.c_int),
+        ("adr_rtol", C.c_double), ("adr_maxit", C.c_int), ("warm_history", C.c_int),
         ("mixed_inner_rtol", C.c_double),
     ]
 
@@ -72,6 +73,7 @@ class StepReport(C.Structure):
         ("t_leach_ms", C.c_double), ("t_snapshot_ms", C.c_double), ("t_total_ms", C.c_double),
         ("prof_stencil64_ms", C.c_double), ("prof_stencil32_ms", C.c_double),
         ("prof_stencil64_n", C.c_int), ("prof_stencil32_n", C.c_int),
+        ("press_relres0", C.c_double), ("warm_basis", C.c_int), ("warm_net", C.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -85,7 +87,7 @@ def default_params(**overrides) -> Params:
                vg_n=2.0, s_res=0.05, porosity=0.1, solid_density=1.0, time_scalar=1.0,
                alpha_transfer=1e-5, instant_precip=0, bc_mode=BC_REFERENCE,
                solver=SOLVER_PLAIN64, warm_start=WARM_PREVIOUS, press_rtol=1e-8,
-               press_maxit=10000, adr_rtol=1e-8, adr_maxit=10000, mixed_inner_maxit=0,
+               press_maxit=10000, adr_rtol=1e-8, adr_maxit=10000, warm_history=4,
                mixed_inner_rtol=0.1)
     for k, v in overrides.items():
         if not hasattr(p, k):
@@ -137,6 +139,11 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    sp, sr = C.c_size_t(), C.c_size_t()
+    lib.ls_abi_info(C.byref(sp), C.byref(sr))
+    if (sp.value, sr.value) != (C.sizeof(Params), C.sizeof(StepReport)):
+        raise RuntimeError(f"{p}: struct sizes {sp.value}/{sr.value} differ from this binding's "
+                           f"{C.sizeof(Params)}/{C.sizeof(StepReport)} (rebuild the library)")
     if path is None:
         _lib = lib
     return lib
@@ -146,7 +153,7 @@ def exported_symbols() -> list[str]:
     """Names declared in include/lsgpu.h (for the export check)."""
     import re
     hdr = (PKG.parent / "include" / "lsgpu.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(ls_\w+)\(", hdr, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|size_t|const char\*)\s+(ls_\w+)\(", hdr, re.M)))
 
 
 class LsError(RuntimeError):

@@ -407,12 +407,11 @@ static int phase_pressure(lso_ctx& c, double t, ls_step_report* rep) {
     std::vector<double> pD, b;
     const double ptop = effective_top_pressure(c.P.top_pressure, t);
     assemble_pressure(c, t, pD, b);
-    /* warm start: previous pressure (src/Simulation.cpp:157,808) */
+    /* warm start: previous pressure (src/Simulation.cpp:157,808), whatever
+     * warm_start asks (it changes only the iteration count, not the answer
+     * beyond the tolerance) */
     std::vector<double> x(c.N, 0.0);
-    double scale = 1.0;
-    if (c.P.warm_start == LS_WARM_RESCALED && c.have_prev && c.p_top_prev != 0.0)
-        scale = ptop / c.p_top_prev;
-    for (size_t v = 0; v < c.N; ++v) if (c.cls[v] == 1) x[v] = c.p[v] * scale;
+    for (size_t v = 0; v < c.N; ++v) if (c.cls[v] == 1) x[v] = c.p[v];
     int iters = 0;
     double relres = 0.0, bnorm = 0.0;
     int st = pcg_reduced(c, b, x, c.P.press_rtol, c.P.press_maxit, iters, relres, bnorm);

@@ -7,10 +7,11 @@
  * links or calls it.
  *
  * The oracle restates src/Simulation.cpp of javalosp/LeachingSim for a single
- * domain (no MPI ranks), with the semantic decisions recorded in DESIGN.md
- * (single-domain one-sided differences only at global faces, saturation kept
- * at inactive voxels, solid_density default, z-face planes, floored pressure
- * rows handled as lagged Dirichlet nodes and post-solved from their own row).
+ * domain (no MPI ranks) with these semantic decisions: one-sided differences
+ * only at the global faces, saturation kept at inactive voxels, solid_density
+ * 1.0, optional virtual Air planes on the i faces (LS_BC_ZFACES), floored
+ * pressure rows (|diag| < 1e-16) handled as lagged Dirichlet nodes and
+ * post-solved from their own row, leaching in exact sequential order.
  * It uses the same parameter, field, phase and report types as the GPU
  * library (include/lsgpu.h) so the two can be driven by one test sequence.
  */

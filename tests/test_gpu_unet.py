@@ -64,9 +64,12 @@ def test_unet_backward_matches_reference():
 
 
 def test_unet_gradient_euler_identity():
-    """With zero biases the ReLU U-Net is positively homogeneous of degree 1
-    in each layer's weights (pooling, upsampling and concatenation are
-    linear), so for every layer l: grad_l . w_l = sum(gout * out) exactly.
+    """With zero biases every ReLU conv layer is positively homogeneous, so
+    d/da L(a w_l)|_{a=1} = grad_l . w_l.  Scaling a layer scales everything
+    downstream of it up to the next concatenation, hence:
+      - layers every path runs through (e0a e0b d0a d0b out): grad.w = L;
+      - layers in series on one branch share one value: {e1a e1b d1a d1b}
+        (they scale the whole level-1 branch) and {e2a e2b}.
     Checked on the device's own forward and backward."""
     net, w, x, rng = setup(4, shape=(16, 16, 16), batch=1)
     sizes = S.layer_sizes()
@@ -81,6 +84,11 @@ def test_unet_gradient_euler_identity():
     vals = {L["name"]: float(g[offs[li]:offs[li] + sizes[li][0]] @ w[offs[li]:offs[li] + sizes[li][0]])
             for li, L in enumerate(S.LAYERS)}
     # exact in real arithmetic; the BF16 rounding of activations and of the
-    # gradients fed to the MFMA grows with the depth of the backward pass
-    # (0.1 % at the output layers, a few % at the encoder)
-    assert all(abs(v - Lval) <= 0.10 * abs(Lval) for v in vals.values()), (Lval, vals)
+    # gradients fed to the MFMA adds a few % (the emulated BF16 reference
+    # shows up to 8 % of |L| between members of the level-1 group)
+    for name in ("e0a", "e0b", "d0a", "d0b", "out"):
+        assert abs(vals[name] - Lval) <= 0.10 * abs(Lval), (name, Lval, vals)
+    for group in (("e1a", "e1b", "d1a", "d1b"), ("e2a", "e2b")):
+        v = np.array([vals[n] for n in group])
+        scale = max(abs(v.mean()), abs(Lval))
+        assert v.max() - v.min() <= 0.15 * scale, (group, Lval, vals)

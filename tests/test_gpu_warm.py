@@ -1,0 +1,118 @@
+"""Projected warm start (Galerkin projection onto earlier pressure
+solutions) on the GPU.
+
+Pinned by what the mathematics fixes: with the operator held fixed (the
+properties are not updated between solves) the right-hand side is
+ptop(t) b1 + b0 (top-pressure ramp plus a fixed evaporation source), so from
+the third solve on the solution lies in the span of the two previous ones
+and the projected start must already meet a tight tolerance, while the
+previous-pressure start does not.  Under the full coupled step the warm
+start changes only the iteration count: solutions agree with the
+previous-pressure runs to the solve tolerance."""
+import numpy as np
+import pytest
+
+from leachingsim_amd import lsgpu as L
+from leachingsim_amd import unet_spec as S
+from tests.helpers import relerr, small_mixed_geometry
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (24, 20, 28)
+
+
+def ctx(warm, img, evap=1e-12, **kw):
+    P = L.default_params(bc_mode=L.BC_ZFACES, warm_start=warm, press_rtol=1e-12,
+                         evap_flux=evap, **kw)
+    g = L.GpuSimulation(SHAPE, P)
+    g.read_labels(img)
+    g.init_state()
+    return g
+
+
+# evap 1e-12: the evaporation part is negligible in the right-hand side;
+# evap 1e-7: it dominates the solution (numpy emulation of the same
+# projection with the oracle's operator: warm-start relres 1e-12 / 6e-15
+# against 0.25-0.4 / 1e-5 for the previous pressure)
+@pytest.mark.parametrize("solver,evap,prev_min", [
+    (L.SOLVER_PLAIN64, 1e-12, 1e-2), (L.SOLVER_MIXED, 1e-12, 1e-2),
+    (L.SOLVER_PLAIN64, 1e-7, 1e-6)])
+def test_projection_exact_for_fixed_operator(solver, evap, prev_min):
+    img = small_mixed_geometry(SHAPE, 5)
+    reps = {}
+    for warm in (L.WARM_PREVIOUS, L.WARM_PROJECTED):
+        g = ctx(warm, img, evap, solver=solver)
+        g.phase(L.PH_PROPERTIES)
+        out = []
+        for t in (400.0, 800.0, 1200.0, 2000.0):
+            r = g.phase(L.PH_PRESSURE, t)
+            assert r.status == 0
+            out.append((r.press_relres0, r.warm_basis, r.press_iters))
+        reps[warm] = (out, g.download(L.F_PRESSURE))
+        g.close()
+    prev, proj = reps[L.WARM_PREVIOUS][0], reps[L.WARM_PROJECTED][0]
+    assert proj[0][1] == 0 and proj[1][1] >= 1, proj
+    for n in (2, 3):
+        assert proj[n][1] >= 1, proj
+        assert proj[n][0] <= 1e-9, proj
+        assert prev[n][0] >= prev_min, prev
+        assert proj[n][2] < prev[n][2] or proj[n][2] == 0, (proj, prev)
+    assert relerr(reps[L.WARM_PROJECTED][1], reps[L.WARM_PREVIOUS][1]) <= 1e-7
+
+
+def test_projected_coupled_steps_match_previous():
+    # no Air slabs: pores next to Air precipitate in the first step and leave
+    # a trivial pressure system; the i-face planes drive the flow instead
+    img = small_mixed_geometry(SHAPE, 6, air_slabs=False)
+    res = {}
+    for warm in (L.WARM_PREVIOUS, L.WARM_PROJECTED):
+        g = ctx(warm, img, solver=L.SOLVER_MIXED, warm_history=3)
+        t, iters = 0.0, []
+        for _ in range(6):
+            r = g.step(t, 60.0)
+            assert r.status == 0
+            iters.append(r.press_iters)
+            t += 60.0
+        res[warm] = (g.state(), iters)
+        g.close()
+    assert min(res[L.WARM_PREVIOUS][1][2:]) > 0
+    a, b = res[L.WARM_PREVIOUS][0], res[L.WARM_PROJECTED][0]
+    assert np.array_equal(a[L.F_IMG], b[L.F_IMG])
+    # two solves to rtol 1e-12 differ by up to cond(A) x 1e-12
+    for f in (L.F_PRESSURE, L.F_CONC, L.F_SATURATION):
+        assert relerr(b[f], a[f]) <= 1e-7, f
+    assert sum(res[L.WARM_PROJECTED][1][2:]) < sum(res[L.WARM_PREVIOUS][1][2:])
+
+
+def perturbed_solves(warm, img, weights=None):
+    """Three pressure solves with the saturation (hence the relative
+    permeability, hence the operator) perturbed before the second and third,
+    so that the projection onto earlier solutions is not exact."""
+    g = ctx(warm, img, solver=L.SOLVER_MIXED, warm_history=2)
+    if weights is not None:
+        g.set_network(weights)
+    rng = np.random.default_rng(11)
+    reps = []
+    for n, t in enumerate((400.0, 800.0, 1200.0)):
+        if n:
+            s = g.download(L.F_SATURATION)
+            s = np.clip(s * (1.0 - 0.3 * rng.random(s.shape)), 0.0, 1.0)
+            g.upload(L.F_SATURATION, s)
+        g.phase(L.PH_PROPERTIES)
+        r = g.phase(L.PH_PRESSURE, t)
+        assert r.status == 0
+        reps.append((r.warm_net, r.warm_basis, r.press_relres0, r.press_iters))
+    p = g.download(L.F_PRESSURE)
+    g.close()
+    return reps, p
+
+
+def test_learned_warm_start_adds_network_direction():
+    """The network correction of the projected residual joins the basis, and
+    the solution matches the previous-pressure run to the solve tolerance."""
+    img = small_mixed_geometry(SHAPE, 7, air_slabs=False)
+    reps, p_learned = perturbed_solves(L.WARM_LEARNED, img, S.init_weights(3))
+    assert reps[0][0] == 0                          # nothing to project yet
+    assert all(r[0] == 1 and r[1] >= 2 for r in reps[1:]), reps
+    _, p_prev = perturbed_solves(L.WARM_PREVIOUS, img)
+    assert relerr(p_learned, p_prev) <= 1e-7

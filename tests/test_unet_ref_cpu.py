@@ -48,3 +48,27 @@ def test_bf16_rounding_ties_to_even():
     v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5], np.float32)
     r = U.bf16_round(v)
     assert list(r) == [1.0, 1.0, 1.0 + 2 ** -6, 1.0, -2.5]
+
+
+def test_reference_gradient_homogeneity_groups():
+    """Zero biases: grad_l . w_l = dL/da of scaling layer l.  Layers on every
+    path give L itself; layers in series on one branch share one value."""
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((8, 8, 8, S.CIN))
+    w = S.init_weights(2).astype(np.float64)
+    sizes = S.layer_sizes()
+    offs = np.cumsum([0] + [a + b for a, b in sizes])
+    for li, (nw, nb) in enumerate(sizes):
+        w[offs[li] + nw:offs[li + 1]] = 0.0
+    out, cache = U.forward(x, w)
+    gout = rng.standard_normal(out.shape)
+    Lval = float((out * gout).sum())
+    g = U.backward(gout, w, cache)
+    v = {L["name"]: float(g[offs[li]:offs[li] + sizes[li][0]] @ w[offs[li]:offs[li] + sizes[li][0]])
+         for li, L in enumerate(S.LAYERS)}
+    for name in ("e0a", "e0b", "d0a", "d0b", "out"):
+        assert np.isclose(v[name], Lval, rtol=1e-10), (name, v[name], Lval)
+    for group in (("e1a", "e1b", "d1a", "d1b"), ("e2a", "e2b")):
+        assert np.allclose([v[n] for n in group], v[group[0]], rtol=1e-10), (group, v)
+    # and the branch values are not L (the skip paths are not scaled)
+    assert not np.isclose(v["e2a"], Lval, rtol=1e-3)
