@@ -13,9 +13,20 @@ saturation, surface effects, conc + acid transport, precipitation,
 leaching) on device-resident state.  The timed region holds `--steps` steps
 after `--warmup` untimed steps, bracketed by device synchronisation.
 
-Prints one JSON line: steps/s of the headline solver, the plain FP64 PCG
-baseline on the same workload, the pressure-stencil roofline measured with
-HIP events during the timed region, and the CPU oracle baseline.
+Runs (each on a fresh context, same workload and window):
+  headline  learned+mixed: flexible CG with the U-Net coarse-level
+            preconditioner (probe and mixed-precision PCG fallback), warm
+            start = projection onto earlier solutions plus the network
+            direction (trained weights from leachingsim_amd/weights/);
+  baseline  plain FP64 Jacobi-PCG warm-started from the previous pressure
+            (the reference's algorithm, src/Simulation.cpp:131-159, 808);
+  ablations mixed-precision PCG + projected warm start (no network), and
+            plain FP64 PCG + projected warm start.
+
+Prints one JSON line: steps/s of the headline, the baseline and ablations,
+the pressure-stencil roofline measured with HIP events during the timed
+region (HBM traffic from the committed rocprofv3 counter summary), and the
+CPU oracle baseline.
 """
 from __future__ import annotations
 
@@ -38,6 +49,9 @@ HBM_PEAK_GBS = 8000.0
 
 SOLVER_NAMES = {L.SOLVER_PLAIN64: "plain_fp64_pcg", L.SOLVER_MIXED: "mixed_fp32_fp64ir",
                 L.SOLVER_LEARNED: "learned_fcg_mixed"}
+WARM_NAMES = {L.WARM_PREVIOUS: "previous", L.WARM_PROJECTED: "projected",
+              L.WARM_LEARNED: "projected+network"}
+TRAFFIC_FILE = ROOT / "profiles" / "pressure_stencil_traffic.json"
 
 
 def log(*a):
@@ -53,33 +67,39 @@ def device_sync():
         pass
 
 
-def workload(config: str, size: int | None):
-    if config == "c3":
+def workload(name: str, size: int | None):
+    """(labels, label bytes, dt, description) of a named workload."""
+    if name == "fractured512":
         n = size or 512
         cache = os.environ.get("LS_GEOM_CACHE")
-        path = Path(cache) / f"c3_{n}.npy" if cache else None
+        path = Path(cache) / f"fractured_{n}.npy" if cache else None
         if path is not None and path.exists():
             img = np.load(path)
         else:
-            img = G.multiphase((n, n, n), sigma=3.0, seed=1, fracture_width=4.0, air_j_planes=2)
+            img = G.named_geometry(name, size=n)
             if path is not None:
                 path.parent.mkdir(parents=True, exist_ok=True)
                 np.save(path, img)
-        return img, 2, 0.5, f"c3: {n}^3 uint16 pore+3 minerals+fracture, Air j-slabs, z-face inlet/outlet, dt 0.5"
-    if config == "c2":
-        img = G.config_geometry("c2")
-        return img, 1, 1.0, "c2: 338x338x283 uint8 sigma 3 porosity 0.30, z-face inlet/outlet, dt 1"
-    if config == "c1":
-        img = G.config_geometry("c1")
-        return img, 1, 1.0, "c1: 64^3 uint8 sigma 2 porosity 0.40, z-face inlet/outlet, dt 1"
-    raise SystemExit(f"unknown config {config}")
+        return img, 2, 0.5, (f"{n}^3 uint16, pore + 3 minerals + 4-voxel fracture, Air j-slabs, "
+                             "z-face inlet/outlet, dt 0.5")
+    if name == "smooth338":
+        return (G.named_geometry(name), 1, 1.0,
+                "338x338x283 uint8 sigma 3 porosity 0.30, z-face inlet/outlet, dt 1")
+    if name == "smooth64":
+        return (G.named_geometry(name), 1, 1.0,
+                "64^3 uint8 sigma 2 porosity 0.40, z-face inlet/outlet, dt 1")
+    raise SystemExit(f"unknown workload {name}")
 
 
-def run_solver(img, label_bytes, dt, solver, steps, warmup, profile=False, warm=L.WARM_PREVIOUS):
+def run_solver(img, label_bytes, dt, solver, steps, warmup, profile=False, warm=L.WARM_PREVIOUS,
+               weights=None):
     P = L.default_params(bc_mode=L.BC_ZFACES, solver=solver, warm_start=warm)
     sim = L.GpuSimulation(img.shape, P, label_bytes)
     sim.read_labels(img)
     sim.init_state()
+    if weights is not None:
+        sim.set_network(weights)
+    name = f"{SOLVER_NAMES[solver]}/{WARM_NAMES[warm]}"
     t = 0.0
     reps = []
     for n in range(warmup):
@@ -87,8 +107,7 @@ def run_solver(img, label_bytes, dt, solver, steps, warmup, profile=False, warm=
         if r.status != 0:
             raise RuntimeError(f"warmup step {n} failed")
         t += dt
-        log(f"  [{SOLVER_NAMES[solver]}] warmup {n}: {r.t_total_ms:.1f} ms, "
-            f"press iters {r.press_iters}")
+        log(f"  [{name}] warmup {n}: {r.t_total_ms:.1f} ms, press iters {r.press_iters}")
     sim.set_profiling(profile)
     device_sync()
     t0 = time.perf_counter()
@@ -101,9 +120,11 @@ def run_solver(img, label_bytes, dt, solver, steps, warmup, profile=False, warm=
     device_sync()
     t1 = time.perf_counter()
     for n, r in enumerate(reps):
-        log(f"  [{SOLVER_NAMES[solver]}] step {warmup + n}: {r['t_total_ms']:.1f} ms "
+        log(f"  [{name}] step {warmup + n}: {r['t_total_ms']:.1f} ms "
             f"(pressure {r['t_pressure_ms']:.1f}, transport {r['t_transport_ms']:.1f}), "
             f"press iters {r['press_iters']} outer {r['press_outer']} "
+            f"fallback {r['press_fallback']} unet {r['press_unet_applies']} "
+            f"warm relres {r['press_relres0']:.2e} basis {r['warm_basis']}+{r['warm_net']} "
             f"relres {r['press_relres']:.2e}, adr iters {r['adr_iters']}")
     copy_gbs = None
     if profile:
@@ -158,11 +179,29 @@ def stencil_only(img, label_bytes, precision, reps):
     nb = min(img.size * 8, 1 << 30)
     cms = sim.time_copy(nb, reps)
     sim.close()
-    print(json.dumps({"kernel": f"k_stencil<{'double' if precision == 64 else 'float'},0>",
+    print(json.dumps({"kernel": f"k_stencil_vec<{'double' if precision == 64 else 'float'}>",
                       "grid": list(img.shape), "reps": reps, "avg_launch_ms": ms,
                       "bytes_per_launch": by, "achieved_GBs": by / ms / 1e6,
                       "copy_bytes": 2 * nb, "copy_ms": cms, "copy_GBs": 2 * nb / cms / 1e6}),
           flush=True)
+
+
+def summary(sec, reps, steps):
+    return {"steps_per_s": steps / sec,
+            "press_iters_per_step": [r["press_iters"] for r in reps],
+            "warm_relres": [r["press_relres0"] for r in reps],
+            "final_relres_max": max(r["press_relres"] for r in reps),
+            "fallbacks": sum(r["press_fallback"] for r in reps),
+            "unet_applies": sum(r["press_unet_applies"] for r in reps)}
+
+
+def load_traffic():
+    """HBM bytes per stencil launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE
+    runs summarised in profiles/ (None if absent)."""
+    try:
+        return json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
 
 
 def main():
@@ -170,10 +209,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c3")
-    ap.add_argument("--size", type=int, default=None, help="override the c3 edge length")
-    ap.add_argument("--solver", default="mixed", choices=["plain", "mixed", "learned"])
+    ap.add_argument("--workload", default="fractured512", choices=list(G.WORKLOADS))
+    ap.add_argument("--size", type=int, default=None,
+                    help="edge length of the fractured workload (default 512)")
+    ap.add_argument("--solver", default="learned", choices=["plain", "mixed", "learned"])
+    ap.add_argument("--warm", default=None, choices=["previous", "projected", "learned"],
+                    help="warm start of the headline (default: learned for the learned "
+                         "solver, projected otherwise)")
+    ap.add_argument("--weights", default=None, help="U-Net weights (float32 file)")
     ap.add_argument("--no-baseline", action="store_true", help="skip the plain FP64 run")
+    ap.add_argument("--no-ablation", action="store_true",
+                    help="skip the mixed+projected and plain+projected runs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-crop", type=int, default=96)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -189,7 +235,7 @@ def main():
         sys.exit(2)
 
     t0 = time.perf_counter()
-    img, lb, dt, desc = workload(args.config, args.size)
+    img, lb, dt, desc = workload(args.workload, args.size)
     log(f"workload {desc}: generated in {time.perf_counter() - t0:.1f} s; labels "
         + ", ".join(f"{k}:{int((img == k).sum())}" for k in range(1, 6)))
     if args.mode == "stencil":
@@ -197,23 +243,43 @@ def main():
         return
     head = {"plain": L.SOLVER_PLAIN64, "mixed": L.SOLVER_MIXED,
             "learned": L.SOLVER_LEARNED}[args.solver]
+    warm = {"previous": L.WARM_PREVIOUS, "projected": L.WARM_PROJECTED,
+            "learned": L.WARM_LEARNED, None: None}[args.warm]
+    if warm is None:
+        warm = L.WARM_LEARNED if head == L.SOLVER_LEARNED else L.WARM_PROJECTED
+    weights = None
+    if head == L.SOLVER_LEARNED or warm == L.WARM_LEARNED:
+        from leachingsim_amd.learn import WEIGHTS, load_weights
+        weights = load_weights(args.weights or WEIGHTS)
 
-    sec, reps, copy_gbs = run_solver(img, lb, dt, head, args.steps, args.warmup, profile=True)
+    sec, reps, copy_gbs = run_solver(img, lb, dt, head, args.steps, args.warmup, profile=True,
+                                     warm=warm, weights=weights)
     value = args.steps / sec
     base = None
-    if not args.no_baseline and head != L.SOLVER_PLAIN64:
+    if not args.no_baseline:
         bsec, breps, _ = run_solver(img, lb, dt, L.SOLVER_PLAIN64, args.steps, args.warmup)
-        base = {"steps_per_s": args.steps / bsec,
-                "press_iters_per_step": float(np.mean([r["press_iters"] for r in breps]))}
+        base = summary(bsec, breps, args.steps)
+    ablations = {}
+    if not args.no_ablation:
+        for sv in (L.SOLVER_MIXED, L.SOLVER_PLAIN64):
+            if (sv, L.WARM_PROJECTED) == (head, warm):
+                continue
+            asec, areps, _ = run_solver(img, lb, dt, sv, args.steps, args.warmup,
+                                        warm=L.WARM_PROJECTED)
+            ablations[f"{SOLVER_NAMES[sv]}/projected"] = summary(asec, areps, args.steps)
 
     # roofline of the pressure stencil, from the sampled HIP-event timings
-    prec = 64 if head == L.SOLVER_PLAIN64 else 32
+    prec = 64 if (head == L.SOLVER_PLAIN64 or sum(r["prof_stencil32_n"] for r in reps) == 0) else 32
     key = f"prof_stencil{prec}"
     n = sum(r[key + "_n"] for r in reps)
     ms = sum(r[key + "_ms"] for r in reps) / max(n, 1)
     sv = prec // 8
     bytes_per_launch = img.size * (2 * sv + 3 * sv)
     achieved = bytes_per_launch / (ms * 1e-3) / 1e9 if n else 0.0
+    traffic = None
+    tr = load_traffic()
+    if tr is not None and str(prec) in tr and tr[str(prec)].get("grid") == list(img.shape):
+        traffic = tr[str(prec)]["bytes_per_launch"]
 
     cpu = None
     if not args.no_cpu:
@@ -230,18 +296,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == L.SOLVER_PLAIN64 else "f32+f64",
+        "dtype": "f64" if head == L.SOLVER_PLAIN64 else "f32+f64+bf16",
         "data": "synthetic",
         "config": {"workload": desc, "grid": list(img.shape), "solver": SOLVER_NAMES[head],
-                   "pressure_rtol": 1e-8,
-                   "press_iters_per_step": float(np.mean([r["press_iters"] for r in reps])),
-                   "plain_fp64_pcg": base,
-                   "speedup_vs_plain_fp64": (value / base["steps_per_s"]) if base else None},
+                   "warm_start": WARM_NAMES[warm], "pressure_rtol": 1e-8,
+                   "headline": summary(sec, reps, args.steps),
+                   "plain_fp64_pcg_previous": base,
+                   "speedup_vs_plain_fp64": (value / base["steps_per_s"]) if base else None,
+                   "ablations": ablations},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "kernel": f"k_stencil<{'double' if prec == 64 else 'float'},0>",
+                     "kernel": f"k_stencil_vec<{'double' if prec == 64 else 'float'}>",
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms,
-                     "sampled_launches": n, "measured_copy_GBs": copy_gbs, "traffic": None},
+                     "sampled_launches": n, "measured_copy_GBs": copy_gbs, "traffic": traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

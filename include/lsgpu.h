@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LSGPU_ABI_VERSION 2
+#define LSGPU_ABI_VERSION 3
 
 /* Labels (src/compiler_opts.h:69-82). */
 enum ls_label {
@@ -114,6 +114,10 @@ typedef struct ls_params {
                                 network correction)                              */
     double mixed_inner_rtol; /* mixed solver: the FP64 residual replaces the FP32 one
                                 whenever the latter has fallen by this factor (0.1) */
+    double learned_judge;    /* learned solver: flexible CG must keep at least this
+                                multiple of the last mixed PCG solve's convergence
+                                rate (decades per ms), else mixed PCG takes over
+                                from the saved start (1.0; 0 = never judged)     */
 } ls_params;
 
 /* Fields that can be uploaded / downloaded.  Element types are fixed:

@@ -88,6 +88,7 @@ void ls_default_params(ls_params* p) {
     p->adr_rtol = 1e-8;
     p->adr_maxit = 10000;
     p->warm_history = 4;
+    p->learned_judge = 1.0;
     p->mixed_inner_rtol = 0.1;
 }
 
@@ -111,6 +112,10 @@ static int validate_params(ls_ctx* c, const ls_params* p) {
          (p->warm_history < 1 || p->warm_history > LS_WARM_MAX ||
           (p->warm_start == LS_WARM_LEARNED && p->warm_history > LS_WARM_MAX - 1)))) {
         c->err = "invalid warm start or warm_history (1..8, 1..7 with the network)";
+        return -3;
+    }
+    if (!(p->learned_judge >= 0.0)) {
+        c->err = "learned_judge must be >= 0";
         return -3;
     }
     return 0;

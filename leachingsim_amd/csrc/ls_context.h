@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ls_common.h"
@@ -25,6 +26,8 @@ struct CgScalars {
     int maxit;
     int done;         // 1 = stop (converged or iteration cap)
     int converged;
+    int xlive;        // 1 = this iteration's x += alpha p is still owed (the
+                      // direction kernel applies it)
 };
 
 // Device scalars of the two-RHS BiCGStab (conc, acid).
@@ -117,6 +120,10 @@ struct ls_ctx {
     std::vector<float> net_weights;  // host copy, set by ls_set_network
     int net_failures = 0;            // consecutive probes where the network lost
     int net_skip = 0;                // solves left that go straight to mixed PCG
+    // the judge of the learned solver: the residual curve (ms of wall time,
+    // decades of reduction) of the last converged mixed PCG solve
+    std::vector<std::pair<double, double>> race, race_tmp;
+    bool have_rate = false;
     // host pinned mirrors
     CgScalars* h_cg = nullptr;
     BicgScalars* h_bicg = nullptr;

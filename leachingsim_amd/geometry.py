@@ -131,15 +131,23 @@ def reference_channel(case: int = 1, n: int = 50, dtype=np.uint8) -> np.ndarray:
     return img
 
 
-# Named configurations: (shape (ni, nj, nk), generator kwargs)
-def config_geometry(name: str, seed: int = 1) -> np.ndarray:
-    if name == "c1":    # 64^3 u8, sigma 2, porosity 0.40
+# Named workloads
+WORKLOADS = ("smooth64", "smooth338", "fractured512")
+
+
+def named_geometry(name: str, seed: int = 1, size: int | None = None) -> np.ndarray:
+    """smooth64: 64^3 uint8, sigma 2, porosity 0.40 (the reference's
+    CPU-runnable case); smooth338: 338x338x283 (x, y, z) -> (ni, nj, nk) =
+    (283, 338, 338) uint8, sigma 3, porosity 0.30 (README scale);
+    fractured512: 512^3 (or size^3) uint16 labels, pore + three minerals
+    (0.30 / 0.25 / 0.15), a 4-voxel planar fracture, Air on the j faces."""
+    if name == "smooth64":
         return smoothed_threshold((64, 64, 64), 2.0, 0.40, seed)
-    if name == "c2":    # 338x338x283 (x, y, z) -> (ni, nj, nk) = (283, 338, 338)
+    if name == "smooth338":
         return smoothed_threshold((283, 338, 338), 3.0, 0.30, seed)
-    if name == "c3":    # 512^3 u16, 4 labels + fracture, Air on the j faces
-        return multiphase((512, 512, 512), sigma=3.0, seed=seed, fracture_width=4.0,
-                          air_j_planes=2)
+    if name == "fractured512":
+        n = size or 512
+        return multiphase((n, n, n), sigma=3.0, seed=seed, fracture_width=4.0, air_j_planes=2)
     raise KeyError(name)
 
 

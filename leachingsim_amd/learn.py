@@ -3,8 +3,11 @@
 Data: for seeded geometries (porosity U[0.2, 0.45], smoothing sigma U[1, 4],
 z-face inlet/outlet), the library's own FP64 PCG solves the pressure system
 at a random ramp time to 1e-10; capped solves from p = 0 give iterates p_k.
-Each pair is (scaled residual r_k, scaled error e_k = x* - x_k).  The network
-input is the library's restriction of r_k (ls_debug_learned_pack).
+Each pair is (scaled residual r_k, scaled error e_k = x* - x_k).  Successive
+pressure fields give more pairs: after a coupled step at t, the solution at
+t + dt against the energy-optimal multiple of the one at t (the projected
+warm start), e = x*(t+dt) - c x*(t), r = A^ e.  The network input is the
+library's restriction of r (ls_debug_learned_pack).
 
 Loss: with u = P UNet(R r) (P copies the coarse value to the solved fine
 children), 1 - (u.r)^2 / (|u|_A^2 |e|_A^2): the share of the A-norm error
@@ -91,8 +94,9 @@ def energy_loss_grad(out, r, e, faces, mask):
 
 
 def collect(seed: int, n: int, ks=(30, 80, 200, 400, 800)):
-    """Samples (x [cgrid,16], r, e) and the geometry (faces, mask) of one
-    seeded geometry; r, e are the scaled residual and error of capped solves."""
+    """Samples (x [cgrid,16], r, e, (faces, mask)) of one seeded geometry:
+    scaled residual and error of capped solves, then of the projected warm
+    start after a coupled step (each with the operator it belongs to)."""
     img = training_geometry(seed, n)
     rng = np.random.default_rng(seed)
     t = float(rng.uniform(60.0, 3500.0))
@@ -108,6 +112,7 @@ def collect(seed: int, n: int, ks=(30, 80, 200, 400, 800)):
     faces = sim.pressure_faces()
     R = cl == 1
     xs = sd * pstar
+    geo = (faces.astype(np.float32), R.astype(np.float32))
     samples = []
     for k in ks:
         sim.set_params(L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_PLAIN64,
@@ -120,25 +125,55 @@ def collect(seed: int, n: int, ks=(30, 80, 200, 400, 800)):
         if not np.any(r):
             continue
         x = sim.learned_pack(r.astype(np.float32))
-        samples.append((x, r.astype(np.float32), e.astype(np.float32)))
+        samples.append((x, r.astype(np.float32), e.astype(np.float32), geo))
     sim.close()
-    return samples, (faces.astype(np.float32), R.astype(np.float32))
+    return samples + successive_pairs(img, t, rng)
+
+
+def successive_pairs(img, t, rng, dts=(20.0, 60.0)):
+    """(p_t, p_t+dt) pairs: the error of the projected warm start after a
+    coupled step (the operator changes with the saturation)."""
+    P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_PLAIN64, press_rtol=1e-10)
+    out = []
+    for dt in dts:
+        sim = L.GpuSimulation(img.shape, P)
+        sim.read_labels(img)
+        sim.init_state()
+        sim.set_network(np.zeros(S.n_weights(), np.float32))
+        sim.step(t, dt)
+        p0 = sim.download(L.F_PRESSURE)
+        sim.phase(L.PH_PROPERTIES)
+        sim.phase(L.PH_PRESSURE, t + dt)
+        p1 = sim.download(L.F_PRESSURE)
+        sd, bh, cl = sim.pressure_system()
+        faces = sim.pressure_faces()
+        R = cl == 1
+        xs = np.where(R, sd * p1, 0.0)
+        v = np.where(R, sd * p0, 0.0)
+        av = apply_op(v, faces)
+        b = apply_op(xs, faces)
+        den = float((v * av).sum())
+        c = float((v * b).sum()) / den if den > 0 else 0.0
+        e = np.where(R, xs - c * v, 0.0)
+        r = np.where(R, apply_op(e, faces), 0.0)
+        if np.linalg.norm(r) > 1e-12 * max(np.linalg.norm(b), 1e-300):
+            x = sim.learned_pack(r.astype(np.float32))
+            out.append((x, r.astype(np.float32), e.astype(np.float32),
+                        (faces.astype(np.float32), R.astype(np.float32))))
+        sim.close()
+    return out
 
 
 def train(n_geom=32, n=64, steps=1500, batch=4, lr=1e-3, seed=0, log_every=100, out=WEIGHTS,
           holdout=4):
     from .unet import Net
     t0 = time.time()
-    data, geoms = [], []
+    train_set, test_set = [], []
     for g in range(n_geom + holdout):
-        smp, geo = collect(g, n)
-        geoms.append(geo)
-        data += [(s, g) for s in smp]
-    train_set = [d for d in data if d[1] < n_geom]
-    test_set = [d for d in data if d[1] >= n_geom]
+        (train_set if g < n_geom else test_set).extend(collect(g, n))
     log(f"collected {len(train_set)} training / {len(test_set)} held-out pairs from "
         f"{n_geom}+{holdout} geometries of {n}^3 in {time.time() - t0:.1f} s")
-    cshape = train_set[0][0][0].shape[:3]
+    cshape = train_set[0][0].shape[:3]
     net = Net(cshape, batch=batch, trainable=True, weights=S.init_weights(seed))
     rng = np.random.default_rng(seed)
 
@@ -146,9 +181,8 @@ def train(n_geom=32, n=64, steps=1500, batch=4, lr=1e-3, seed=0, log_every=100, 
         vals = []
         for i in range(0, len(samples) - batch + 1, batch):
             chunk = samples[i:i + batch]
-            outs = net.forward(np.stack([c[0][0] for c in chunk]))
-            for o, ((x, r, e), g) in zip(outs, chunk):
-                faces, mask = geoms[g]
+            outs = net.forward(np.stack([c[0] for c in chunk]))
+            for o, (x, r, e, (faces, mask)) in zip(outs, chunk):
                 vals.append(energy_loss_grad(o, r, e, faces, mask)[0])
         return float(np.mean(vals)) if vals else float("nan")
 
@@ -158,10 +192,9 @@ def train(n_geom=32, n=64, steps=1500, batch=4, lr=1e-3, seed=0, log_every=100, 
     for it in range(steps):
         idx = rng.choice(len(train_set), batch, replace=False)
         chunk = [train_set[i] for i in idx]
-        outs = net.forward(np.stack([c[0][0] for c in chunk]))
+        outs = net.forward(np.stack([c[0] for c in chunk]))
         gs, ls = [], []
-        for o, ((x, r, e), g) in zip(outs, chunk):
-            faces, mask = geoms[g]
+        for o, (x, r, e, (faces, mask)) in zip(outs, chunk):
             l_, g_ = energy_loss_grad(o, r, e, faces, mask)
             ls.append(l_)
             gs.append(g_ / batch)

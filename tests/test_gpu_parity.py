@@ -171,10 +171,10 @@ def test_leaching_fast_path_is_noop():
 
 
 @pytest.mark.parametrize("solver", [L.SOLVER_PLAIN64, L.SOLVER_MIXED])
-def test_step_parity_c1(solver):
-    """Config c1 geometry (64^3, sigma 2, porosity 0.40), z-face boundary,
+def test_step_parity_smooth64(solver):
+    """64^3 geometry (sigma 2, porosity 0.40), z-face boundary,
     Air slabs added so surface effects and transport are active."""
-    img = G.config_geometry("c1")
+    img = G.named_geometry("smooth64")
     img[0:2] = G.AIR
     P = L.default_params(bc_mode=L.BC_ZFACES, solver=solver)
     g, o = both(img.shape, P, img)
@@ -192,7 +192,7 @@ def test_step_parity_c1(solver):
 
 
 def test_rollback_restores_state():
-    img = G.config_geometry("c1")[:32, :32, :32].copy()
+    img = G.named_geometry("smooth64")[:32, :32, :32].copy()
     P = L.default_params(bc_mode=L.BC_ZFACES)
     g = L.GpuSimulation(img.shape, P)
     g.read_labels(img)
