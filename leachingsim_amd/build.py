@@ -21,6 +21,7 @@ HOST = PKG / "host"
 BUILD = PKG / "_build"
 LIB = PKG / "liblsgpu.so"
 LSIM = PKG / "lsim"
+HOSTLIB = PKG / "liblshost.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -74,13 +75,17 @@ def build_oracle() -> Path:
 
 
 def build_host() -> Path | None:
-    """C++ host mirror (Simulation API + lsim driver) linked to liblsgpu.so."""
-    srcs = sorted(HOST.glob("*.cpp")) if HOST.exists() else []
-    if not srcs:
+    """C++ host side: liblshost.so (file formats, CPU-testable) and the lsim
+    driver (Simulation over liblsgpu.so, the reference's command line)."""
+    if not HOST.exists():
         return None
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", str(ROOT / "include"),
-          *map(str, srcs), "-o", str(LSIM), "-L", str(PKG), "-llsgpu",
-          f"-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + str(PKG)])
+    cxx = ["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-I", str(ROOT / "include")]
+    io = [str(HOST / "io.cpp")]
+    tmp = HOSTLIB.with_suffix(".so.tmp")
+    _run([*cxx, "-shared", *io, str(HOST / "capi.cpp"), "-o", str(tmp)])
+    os.replace(tmp, HOSTLIB)
+    _run([*cxx, *io, str(HOST / "simulation.cpp"), str(HOST / "lsim.cpp"), "-o", str(LSIM),
+          "-L", str(PKG), "-llsgpu", "-Wl,-rpath,$ORIGIN"])
     return LSIM
 
 
