@@ -91,8 +91,26 @@ def workload(name: str, size: int | None):
     raise SystemExit(f"unknown workload {name}")
 
 
+COMPARE_FIELDS = (("pressure", L.F_PRESSURE), ("conc", L.F_CONC), ("acid", L.F_ACID),
+                  ("saturation", L.F_SATURATION), ("qx", L.F_QX), ("qy", L.F_QY),
+                  ("qz", L.F_QZ), ("inventory", L.F_INVENTORY))
+
+
+def field_errors(a, b):
+    """Per-field relative L2 difference of run a against the FP64 run b, and
+    whether the phase masks (labels) and the leaching fraction are equal."""
+    out = {}
+    for name, _ in COMPARE_FIELDS:
+        x, y = a[name], b[name]
+        nb = float(np.linalg.norm(y))
+        out[name] = float(np.linalg.norm(x - y)) / nb if nb > 0 else float(np.linalg.norm(x))
+    out["labels_bit_exact"] = bool(np.array_equal(a["img"], b["img"]))
+    out["frac_bit_exact"] = bool(np.array_equal(a["frac"], b["frac"]))
+    return out
+
+
 def run_solver(img, label_bytes, dt, solver, steps, warmup, profile=False, warm=L.WARM_PREVIOUS,
-               weights=None):
+               weights=None, keep_fields=False):
     P = L.default_params(bc_mode=L.BC_ZFACES, solver=solver, warm_start=warm)
     sim = L.GpuSimulation(img.shape, P, label_bytes)
     sim.read_labels(img)
@@ -131,8 +149,13 @@ def run_solver(img, label_bytes, dt, solver, steps, warmup, profile=False, warm=
         nb = min(img.size * 8, 1 << 30)
         ms = sim.time_copy(nb, 20)
         copy_gbs = 2 * nb / ms / 1e6
+    fields = None
+    if keep_fields:
+        fields = {name: sim.download(f) for name, f in COMPARE_FIELDS}
+        fields["img"] = sim.download(L.F_IMG)
+        fields["frac"] = sim.download(L.F_FRAC)
     sim.close()
-    return (t1 - t0), reps, copy_gbs
+    return (t1 - t0), reps, copy_gbs, fields
 
 
 def cpu_baseline(img, label_bytes, dt, crop: int, nsteps: int):
@@ -252,20 +275,26 @@ def main():
         from leachingsim_amd.learn import WEIGHTS, load_weights
         weights = load_weights(args.weights or WEIGHTS)
 
-    sec, reps, copy_gbs = run_solver(img, lb, dt, head, args.steps, args.warmup, profile=True,
-                                     warm=warm, weights=weights)
+    keep = not args.no_baseline
+    sec, reps, copy_gbs, hfields = run_solver(img, lb, dt, head, args.steps, args.warmup,
+                                              profile=True, warm=warm, weights=weights,
+                                              keep_fields=keep)
     value = args.steps / sec
     base = None
+    errors = None
     if not args.no_baseline:
-        bsec, breps, _ = run_solver(img, lb, dt, L.SOLVER_PLAIN64, args.steps, args.warmup)
+        bsec, breps, _, bfields = run_solver(img, lb, dt, L.SOLVER_PLAIN64, args.steps,
+                                             args.warmup, keep_fields=True)
         base = summary(bsec, breps, args.steps)
+        errors = field_errors(hfields, bfields)
+        del hfields, bfields
     ablations = {}
     if not args.no_ablation:
         for sv in (L.SOLVER_MIXED, L.SOLVER_PLAIN64):
             if (sv, L.WARM_PROJECTED) == (head, warm):
                 continue
-            asec, areps, _ = run_solver(img, lb, dt, sv, args.steps, args.warmup,
-                                        warm=L.WARM_PROJECTED)
+            asec, areps, _, _ = run_solver(img, lb, dt, sv, args.steps, args.warmup,
+                                           warm=L.WARM_PROJECTED)
             ablations[f"{SOLVER_NAMES[sv]}/projected"] = summary(asec, areps, args.steps)
 
     # roofline of the pressure stencil, from the sampled HIP-event timings
@@ -303,6 +332,7 @@ def main():
                    "headline": summary(sec, reps, args.steps),
                    "plain_fp64_pcg_previous": base,
                    "speedup_vs_plain_fp64": (value / base["steps_per_s"]) if base else None,
+                   "error_vs_plain_fp64_after_window": errors,
                    "ablations": ablations},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

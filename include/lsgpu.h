@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LSGPU_ABI_VERSION 3
+#define LSGPU_ABI_VERSION 4
 
 /* Labels (src/compiler_opts.h:69-82). */
 enum ls_label {
@@ -118,6 +118,9 @@ typedef struct ls_params {
                                 multiple of the last mixed PCG solve's convergence
                                 rate (decades per ms), else mixed PCG takes over
                                 from the saved start (1.0; 0 = never judged)     */
+    int implicit_sat;        /* --implicit_sat (0): implicit upwind saturation solve
+                                inside 3 Picard passes (src/main.cpp:312-370,
+                                src/Simulation.cpp:1858-2053), solved to adr_rtol */
 } ls_params;
 
 /* Fields that can be uploaded / downloaded.  Element types are fixed:
@@ -189,6 +192,10 @@ typedef struct ls_step_report {
     double press_relres0;     /* ||b-Ax0||/||b|| of the warm start               */
     int warm_basis;           /* projection vectors kept (0: previous pressure)  */
     int warm_net;             /* 1 if the network correction was in the basis   */
+    int sat_iters;            /* implicit saturation: solver iterations (last pass) */
+    int sat_valid;            /* implicit saturation: raw solution within
+                                 [-0.05, 1.05] (the reference's physics_valid)  */
+    double sat_raw_min, sat_raw_max;   /* implicit saturation: before the clamp */
 } ls_step_report;
 
 typedef struct ls_ctx ls_ctx;

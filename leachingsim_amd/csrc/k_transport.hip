@@ -9,6 +9,10 @@
 // and applies them to both right-hand sides at once.  The solver is Jacobi
 // (right) preconditioned BiCGStab run for the two systems in lock step, with
 // per-system scalars on the device.
+//
+// The same solver also runs the implicit saturation system (KIND 1,
+// setupSaturationEqns + solveSaturation, src/Simulation.cpp:1858-2053) with
+// its own matrix-free row builder, as system 0 alone.
 #include <cmath>
 
 #include "ls_context.h"
@@ -22,6 +26,7 @@ struct TGeo {
 
 struct AdrConst {
     double inv_dt, alpha, theta, D, Dpf, dx;
+    double phi, evap;      // implicit saturation: porosity, evaporative flux
 };
 
 __device__ __forceinline__ void tdecode(uint32_t v, const TGeo& g, int& i, int& j, int& k) {
@@ -117,10 +122,57 @@ __device__ __forceinline__ void adr_row(const uint8_t* __restrict__ img, const d
     }
 }
 
+// One row of the implicit saturation matrix and its right-hand side
+// (src/Simulation.cpp:1870-1955): inactive voxels keep their value; a Pore
+// row is phi/dt plus the upwinded mobility advection, the outward face flux
+// being the mean of the two cells' flux and the upwind divisor the
+// assembly-time saturation sf (floored at 1e-6); rhs phi/dt sf - evap sink.
+__device__ __forceinline__ void sat_row(const uint8_t* __restrict__ img, const double* __restrict__ qx,
+                                        const double* __restrict__ qy, const double* __restrict__ qz,
+                                        const double* __restrict__ sf, const TGeo& g, int i, int j,
+                                        int k, uint32_t v, const AdrConst& a, AdrRow& R,
+                                        double& rhs) {
+    const uint8_t tv = img[v];
+    R.type = tv;
+    R.mask = 0;
+    R.explicit_mask = 0;
+    if (tv == AIR || tv == ROCK || tv == SUL || tv == PREC) {
+        R.diag = 1.0;
+        rhs = sf[v];
+        return;
+    }
+    int air_faces = 0;
+    for (int f = 0; f < 6; ++f) {
+        uint32_t n;
+        if (tnbr(g, i, j, k, v, f, n) && img[n] == AIR) ++air_faces;
+    }
+    const double evap_sink = (air_faces * a.evap) / a.dx;
+    double coeff_self = a.phi * a.inv_dt;
+    rhs = (a.phi * a.inv_dt * sf[v]) - evap_sink;
+    for (int f = 0; f < 6; ++f) {      // +i, -i, +j, -j, +k, -k
+        uint32_t n;
+        if (!tnbr(g, i, j, k, v, f, n)) continue;
+        const double* q = f < 2 ? qx : (f < 4 ? qy : qz);
+        const double q_out = (f % 2 == 0) ? (q[v] + q[n]) / 2.0 : -(q[v] + q[n]) / 2.0;
+        if (q_out > 0) {
+            const double v_eff = q_out / fmax(sf[v], 1e-6);
+            coeff_self += v_eff / a.dx;
+        } else {
+            const double v_eff = q_out / fmax(sf[n], 1e-6);
+            R.off[f] = v_eff / a.dx;
+            R.nb[f] = n;
+            R.mask |= 1 << f;
+        }
+    }
+    R.diag = coeff_self;
+}
+
 // Right-hand sides and the Jacobi diagonal; initial residual r = b - A x,
 // r0 = r, p = v = 0.  Partials: |b|^2 and |r|^2 of both systems.
+template <int KIND>
 __global__ void k_adr_start(const uint8_t* __restrict__ img, const double* __restrict__ qx,
                             const double* __restrict__ qy, const double* __restrict__ qz,
+                            const double* __restrict__ sf,
                             const double* __restrict__ x0, const double* __restrict__ x1,
                             double* __restrict__ b0, double* __restrict__ b1,
                             double* __restrict__ invd, double* __restrict__ r_0,
@@ -135,10 +187,18 @@ __global__ void k_adr_start(const uint8_t* __restrict__ img, const double* __res
         int i, j, k;
         tdecode(v, g, i, j, k);
         AdrRow R;
-        adr_row(img, qx, qy, qz, g, i, j, k, v, a, R);
+        double srhs = 0.0;
+        if (KIND == 1) sat_row(img, qx, qy, qz, sf, g, i, j, k, v, a, R, srhs);
+        else adr_row(img, qx, qy, qz, g, i, j, k, v, a, R);
         const double xa = x0[v], xb = x1[v];
         double ba, bbv;
-        if (build_rhs) {
+        if (build_rhs && KIND == 1) {
+            ba = srhs;      // system 1 idles with b = 0
+            bbv = 0.0;
+            b0[v] = ba;
+            b1[v] = bbv;
+            invd[v] = 1.0 / R.diag;
+        } else if (build_rhs) {
             if (R.type == PREC) {
                 ba = xa;
                 bbv = xb;
@@ -259,9 +319,10 @@ __global__ void k_bicg_p(double* __restrict__ p0, double* __restrict__ p1,
 }
 
 // y = A x for both systems; partials of rh.y (MODE 0) or (y.y, y.s) (MODE 1).
-template <int MODE>
+template <int MODE, int KIND>
 __global__ void k_adr_matvec(const uint8_t* __restrict__ img, const double* __restrict__ qx,
                              const double* __restrict__ qy, const double* __restrict__ qz,
+                             const double* __restrict__ sf,
                              const double* __restrict__ x0, const double* __restrict__ x1,
                              double* __restrict__ y0, double* __restrict__ y1,
                              const double* __restrict__ w0, const double* __restrict__ w1,
@@ -275,7 +336,12 @@ __global__ void k_adr_matvec(const uint8_t* __restrict__ img, const double* __re
         int i, j, k;
         tdecode(v, g, i, j, k);
         AdrRow R;
-        adr_row(img, qx, qy, qz, g, i, j, k, v, a, R);
+        if (KIND == 1) {
+            double unused;
+            sat_row(img, qx, qy, qz, sf, g, i, j, k, v, a, R, unused);
+        } else {
+            adr_row(img, qx, qy, qz, g, i, j, k, v, a, R);
+        }
         double ya = R.diag * x0[v], yb = R.diag * x1[v];
         for (int f = 0; f < 6; ++f) {
             if (!(R.mask & (1 << f))) continue;
@@ -475,23 +541,18 @@ __global__ void k_bicg_first_rho(BicgScalars* bs) {
     }
 }
 
-int phase_transport(ls_ctx* c, double dt, ls_step_report* rep) {
-    int st = zero_flux_rock_precip(c);
-    if (st) return st;
+// Lock-step BiCGStab of the two systems x0, x1 (KIND 0: conc, acid; KIND 1:
+// the implicit saturation in x0, x1 idle).  Returns 0 converged, 1 not.
+template <int KIND>
+static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, const AdrConst& a,
+                     int* iters_out, double* relres_out) {
     TGeo g;
     g.ni = c->ni; g.nj = c->nj; g.nk = c->nk;
     g.sj = (uint32_t)c->nk;
     g.si = (uint32_t)c->nk * (uint32_t)c->nj;
-    AdrConst a;
-    a.inv_dt = 1.0 / dt;
-    a.alpha = c->P.alpha_transfer;
-    a.theta = c->P.theta;
-    a.D = c->P.D;
-    a.Dpf = c->P.Dpore_fac;
-    a.dx = c->P.dx;
     const uint32_t N = (uint32_t)c->N;
     const int nb = ew_blocks(N);
-    double* x[2] = {c->conc, c->acid};
+    double* x[2] = {x0_, x1_};
     double *r[2], *rh[2], *pp[2], *vv[2], *s[2], *t[2], *ph[2], *sh[2];
     for (int f = 0; f < 2; ++f) {
         r[f] = c->tv[f][0]; rh[f] = c->tv[f][1]; pp[f] = c->tv[f][2]; vv[f] = c->tv[f][3];
@@ -501,10 +562,10 @@ int phase_transport(ls_ctx* c, double dt, ls_step_report* rep) {
     int restarts = 0;
     bool first = true;
     while (true) {
-        hipLaunchKernelGGL(k_adr_start, dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img, c->qx,
-                           c->qy, c->qz, x[0], x[1], c->tb[0], c->tb[1], c->tinvd, r[0], r[1],
-                           rh[0], rh[1], pp[0], pp[1], vv[0], vv[1], g, N, a, first ? 1 : 0,
-                           c->partials);
+        hipLaunchKernelGGL((k_adr_start<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img,
+                           c->qx, c->qy, c->qz, sf, x[0], x[1], c->tb[0], c->tb[1], c->tinvd,
+                           r[0], r[1], rh[0], rh[1], pp[0], pp[1], vv[0], vv[1], g, N, a,
+                           first ? 1 : 0, c->partials);
         hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nb, c->bicg,
                            c->P.adr_rtol, c->P.adr_maxit, first ? 1 : 0);
         hipLaunchKernelGGL(k_bicg_first_rho, dim3(1), dim3(1), 0, c->stream, c->bicg);
@@ -525,9 +586,9 @@ int phase_transport(ls_ctx* c, double dt, ls_step_report* rep) {
             for (int it = 0; it < batch; ++it) {
                 hipLaunchKernelGGL(k_bicg_p, dim3(nb), dim3(EW_BLOCK), 0, c->stream, pp[0], pp[1],
                                    r[0], r[1], vv[0], vv[1], ph[0], ph[1], c->tinvd, c->bicg, N);
-                hipLaunchKernelGGL(k_adr_matvec<0>, dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img,
-                                   c->qx, c->qy, c->qz, ph[0], ph[1], vv[0], vv[1], rh[0], rh[1],
-                                   g, N, a, c->bicg, c->partials);
+                hipLaunchKernelGGL((k_adr_matvec<0, KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
+                                   c->img, c->qx, c->qy, c->qz, sf, ph[0], ph[1], vv[0], vv[1],
+                                   rh[0], rh[1], g, N, a, c->bicg, c->partials);
                 hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1024), 0, c->stream, c->partials,
                                    nb, c->bicg);
                 hipLaunchKernelGGL(k_bicg_s, dim3(nb), dim3(EW_BLOCK), 0, c->stream, r[0], r[1],
@@ -535,9 +596,9 @@ int phase_transport(ls_ctx* c, double dt, ls_step_report* rep) {
                                    c->partials);
                 hipLaunchKernelGGL(k_bicg_scheck, dim3(1), dim3(1024), 0, c->stream, c->partials,
                                    nb, c->bicg);
-                hipLaunchKernelGGL(k_adr_matvec<1>, dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img,
-                                   c->qx, c->qy, c->qz, sh[0], sh[1], t[0], t[1], s[0], s[1], g,
-                                   N, a, c->bicg, c->partials);
+                hipLaunchKernelGGL((k_adr_matvec<1, KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
+                                   c->img, c->qx, c->qy, c->qz, sf, sh[0], sh[1], t[0], t[1], s[0],
+                                   s[1], g, N, a, c->bicg, c->partials);
                 hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1024), 0, c->stream, c->partials,
                                    nb, c->bicg);
                 hipLaunchKernelGGL(k_bicg_x, dim3(nb), dim3(EW_BLOCK), 0, c->stream, x[0], x[1],
@@ -553,9 +614,10 @@ int phase_transport(ls_ctx* c, double dt, ls_step_report* rep) {
         }
         // confirm on the recomputed residual (the start kernel with the
         // stored right-hand sides); restart systems that fall short
-        hipLaunchKernelGGL(k_adr_start, dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img, c->qx,
-                           c->qy, c->qz, x[0], x[1], c->tb[0], c->tb[1], c->tinvd, r[0], r[1],
-                           rh[0], rh[1], pp[0], pp[1], vv[0], vv[1], g, N, a, 0, c->partials);
+        hipLaunchKernelGGL((k_adr_start<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img,
+                           c->qx, c->qy, c->qz, sf, x[0], x[1], c->tb[0], c->tb[1], c->tinvd,
+                           r[0], r[1], rh[0], rh[1], pp[0], pp[1], vv[0], vv[1], g, N, a, 0,
+                           c->partials);
         hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nb, c->bicg,
                            c->P.adr_rtol, c->P.adr_maxit, 0);
         LS_LAUNCH_CHECK();
@@ -571,17 +633,112 @@ int phase_transport(ls_ctx* c, double dt, ls_step_report* rep) {
         // loop re-enters with the start kernel (rebuild of b skipped)
     }
     const BicgScalars& h = *c->h_bicg;
-    if (rep) {
-        rep->adr_iters = h.iter[0] > h.iter[1] ? h.iter[0] : h.iter[1];
+    if (iters_out) *iters_out = h.iter[0] > h.iter[1] ? h.iter[0] : h.iter[1];
+    if (relres_out) {
         double r0v = h.bnorm2[0] > 0 ? sqrt(h.rr[0] / h.bnorm2[0]) : 0.0;
         double r1v = h.bnorm2[1] > 0 ? sqrt(h.rr[1] / h.bnorm2[1]) : 0.0;
-        rep->adr_relres = r0v > r1v ? r0v : r1v;
+        *relres_out = r0v > r1v ? r0v : r1v;
     }
-    if (!(h.converged[0] && h.converged[1])) {
-        c->err = "transport solve did not converge";
-        return 1;
+    return (h.converged[0] && h.converged[1]) ? 0 : 1;
+}
+
+static AdrConst adr_const(const ls_ctx* c, double dt) {
+    AdrConst a;
+    a.inv_dt = 1.0 / dt;
+    a.alpha = c->P.alpha_transfer;
+    a.theta = c->P.theta;
+    a.D = c->P.D;
+    a.Dpf = c->P.Dpore_fac;
+    a.dx = c->P.dx;
+    a.phi = c->P.porosity;
+    a.evap = c->P.evap_flux;
+    return a;
+}
+
+int phase_transport(ls_ctx* c, double dt, ls_step_report* rep) {
+    int st = zero_flux_rock_precip(c);
+    if (st) return st;
+    int iters = 0;
+    double relres = 0.0;
+    st = bicg_pair<0>(c, c->conc, c->acid, nullptr, adr_const(c, dt), &iters, &relres);
+    if (rep) {
+        rep->adr_iters = iters;
+        rep->adr_relres = relres;
     }
-    return 0;
+    if (st) c->err = "transport solve did not converge";
+    return st;
+}
+
+// Raw min / max of the saturation (block partials), then the clamp to
+// [0, 1] when the raw solution is physical.
+__global__ void k_sat_range(const double* __restrict__ s, uint32_t N, double* __restrict__ partials) {
+    __shared__ double lds[2 * EW_BLOCK / WAVE];
+    double lo = 1e300, hi = -1e300;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        lo = fmin(lo, s[v]);
+        hi = fmax(hi, s[v]);
+    }
+    double a[2] = {lo, hi};
+    block_reduce<2, RED_MIN, RED_MAX>(a, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = a[0];
+        partials[gridDim.x + blockIdx.x] = a[1];
+    }
+}
+
+__global__ void k_sat_range_final(const double* __restrict__ partials, int n, double* out) {
+    __shared__ double lds[2 * 1024 / WAVE];
+    double lo = 1e300, hi = -1e300;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        lo = fmin(lo, partials[i]);
+        hi = fmax(hi, partials[n + i]);
+    }
+    double a[2] = {lo, hi};
+    block_reduce<2, RED_MIN, RED_MAX>(a, lds);
+    if (threadIdx.x == 0) {
+        out[0] = a[0];
+        out[1] = a[1];
+    }
+}
+
+__global__ void k_sat_clamp(double* __restrict__ s, uint32_t N) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        s[v] = fmax(0.0, fmin(1.0, s[v]));
+}
+
+int phase_saturation_implicit(ls_ctx* c, double dt, ls_step_report* rep) {
+    const uint32_t N = (uint32_t)c->N;
+    const int nb = ew_blocks(N);
+    // the assembly-time saturation is frozen in the (idle) FP64 CG residual
+    // vector; the second system idles on the zeroed FP64 Ap vector
+    double* sf = c->r64;
+    double* idle = c->ap64;
+    LS_HIP(hipMemcpyAsync(sf, c->s, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    LS_HIP(hipMemsetAsync(idle, 0, c->N * sizeof(double), c->stream));
+    int iters = 0;
+    double relres = 0.0;
+    const int st = bicg_pair<1>(c, c->s, idle, sf, adr_const(c, dt), &iters, &relres);
+    hipLaunchKernelGGL(k_sat_range, dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->s, N, c->partials);
+    hipLaunchKernelGGL(k_sat_range_final, dim3(1), dim3(1024), 0, c->stream, c->partials, nb,
+                       &c->ss->sat_lo);
+    LS_LAUNCH_CHECK();
+    LS_HIP(hipMemcpyAsync(c->h_ss, c->ss, sizeof(StepScalars), hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    const double lo = c->h_ss->sat_lo, hi = c->h_ss->sat_hi;
+    const bool valid = !(hi > 1.05 || lo < -0.05);
+    if (valid) {
+        hipLaunchKernelGGL(k_sat_clamp, dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->s, N);
+        LS_LAUNCH_CHECK();
+    }
+    c->sat_valid = valid;
+    if (rep) {
+        rep->sat_iters = iters;
+        rep->sat_valid = valid ? 1 : 0;
+        rep->sat_raw_min = lo;
+        rep->sat_raw_max = hi;
+    }
+    if (st) c->err = "implicit saturation solve did not converge";
+    return st;
 }
 
 }  // namespace ls

@@ -89,6 +89,7 @@ void ls_default_params(ls_params* p) {
     p->adr_maxit = 10000;
     p->warm_history = 4;
     p->learned_judge = 1.0;
+    p->implicit_sat = 0;
     p->mixed_inner_rtol = 0.1;
 }
 
@@ -411,7 +412,10 @@ int ls_phase(ls_ctx* c, int phase, double t, double dt, ls_step_report* report) 
         case LS_PH_PROPERTIES: st = phase_properties(c); break;
         case LS_PH_PRESSURE: st = solve_pressure(c, t, rep); break;
         case LS_PH_FLUX: st = phase_flux(c); break;
-        case LS_PH_SATURATION: st = phase_saturation(c, dt, rep); break;
+        case LS_PH_SATURATION:
+            st = c->P.implicit_sat ? phase_saturation_implicit(c, dt, rep)
+                                   : phase_saturation(c, dt, rep);
+            break;
         case LS_PH_SURFACE: st = phase_surface(c); break;
         case LS_PH_TRANSPORT: st = phase_transport(c, dt, rep); break;
         case LS_PH_PRECIPITATION: st = phase_precipitation(c); break;
@@ -441,18 +445,37 @@ int ls_step(ls_ctx* c, double t, double dt, ls_step_report* report) {
     int st = ls_snapshot(c);                                    // saveState
     if (st) return st;
     tm.mark();                                                  // 1
-    // A. hydrodynamics (one Picard pass: explicit saturation)
-    if ((st = phase_properties(c))) return st;
-    tm.mark();                                                  // 2
-    st = solve_pressure(c, t, rep);
-    if (st < 0) return st;
-    tm.mark();                                                  // 3
-    bool ok = st == 0;
-    if (ok) {
+    // A. hydrodynamics: one Picard pass with the explicit saturation, three
+    // with the implicit one (src/main.cpp:312-370); the phase marks are those
+    // of the last pass, earlier passes count into t_props_ms
+    bool ok = true;
+    const int passes = c->P.implicit_sat ? 3 : 1;
+    for (int pass = 0; pass < passes && ok; ++pass) {
+        if (pass) tm.n = 1;
+        if ((st = phase_properties(c))) return st;
+        tm.mark();                                              // 2
+        st = solve_pressure(c, t, rep);
+        if (st < 0) return st;
+        tm.mark();                                              // 3
+        ok = st == 0;
+        if (!ok) break;
         if ((st = phase_flux(c))) return st;
         tm.mark();                                              // 4
-        if ((st = phase_saturation(c, dt, rep))) return st;
+        if (c->P.implicit_sat) {
+            st = phase_saturation_implicit(c, dt, rep);
+            if (st < 0) return st;
+            ok = st == 0;
+        } else if ((st = phase_saturation(c, dt, rep))) {
+            return st;
+        }
         tm.mark();                                              // 5
+    }
+    // an unphysical implicit saturation after the last pass forces a retry
+    if (ok && c->P.implicit_sat && !c->sat_valid) {
+        c->err = "implicit saturation: unphysical state after the Picard passes";
+        ok = false;
+    }
+    if (ok) {
         // B. post-Picard physics
         if ((st = phase_surface(c))) return st;
         tm.mark();                                              // 6

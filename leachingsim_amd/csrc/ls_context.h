@@ -64,6 +64,7 @@ struct StepScalars {
     double sums[9];           // pmin pmax cmin cmax amin amax  sum c, sum a, sum s
     double n_floored;
     double nsul;
+    double sat_lo, sat_hi;    // raw range of an implicit saturation solve
 };
 
 struct ls_ctx {
@@ -131,6 +132,7 @@ struct ls_ctx {
 
     double p_top_prev = 0.0;
     bool have_prev = false;
+    bool sat_valid = true;     // physics check of the last implicit saturation solve
     // warm-start history: the last n_hist pressure solutions (physical p,
     // FP64), newest at hist[(hist_head + LS_WARM_MAX - 1) % LS_WARM_MAX]
     double* hist[LS_WARM_MAX] = {};
@@ -159,6 +161,8 @@ int assemble_pressure(ls_ctx* c, double t);
 int finalize_pressure(ls_ctx* c, double ptop);
 int phase_flux(ls_ctx* c);
 int phase_saturation(ls_ctx* c, double dt, ls_step_report* rep);
+// k_transport.hip
+int phase_saturation_implicit(ls_ctx* c, double dt, ls_step_report* rep);
 int phase_surface(ls_ctx* c);
 int phase_precipitation(ls_ctx* c);
 int summarize(ls_ctx* c, ls_step_report* rep);

@@ -148,15 +148,15 @@ int main(int argc, char** argv) {
         o.add("porosity", "0.1", "Material porosity");
         o.add("top_pressure", "10000.0", "Applied pressure head at the top boundary (Pa)");
         o.add("max_cap_grad", "50000.0", "Maximum capillary pressure gradient (Pa/m)");
-        o.add("implicit_sat", "false", "Implicit saturation solver (not available on this path)");
+        o.add("implicit_sat", "false", "Toggle: True = Implicit upwind solve in 3 Picard passes, False = Explicit Sub-stepping");
         o.add("restart_step", "0", "Step to restart from (0 = fresh start)");
         o.add("chk_freq", "1000", "Number of steps between saving binary checkpoints");
         o.add("debug", "false", "Enable detailed runtime debugging");
         o.add("use_out_time", "false", "Output based on physical time instead of iterations");
         o.add("out_time", "3600.0", "Physical time interval between VTK outputs (seconds)");
         o.add("time_scalar", "1.0", "Physical time accelerator for solid mass transfer");
-        o.add("enable_microporosity", "false", "Acid diffusion through the rock matrix (not available on this path)");
-        o.add("rock_D_fac", "1e-4", "Diffusion multiplier inside the rock matrix");
+        o.add("enable_microporosity", "false", "Acid diffusion through the rock matrix (read only by code the reference has commented out: no effect)");
+        o.add("rock_D_fac", "1e-4", "Diffusion multiplier inside the rock matrix (no effect, as above)");
         o.add("alpha_transfer", "1e-5", "Mass transfer coefficient for acid entering the rock matrix");
         // options of this build
         o.add("label_bytes", "1", "Bytes per label in the RAW file: 1 (uint8) or 2 (uint16)");
@@ -170,10 +170,7 @@ int main(int argc, char** argv) {
             o.usage(std::cout);
             return 0;
         }
-        if (o.flag("implicit_sat"))
-            throw std::runtime_error("--implicit_sat true: the implicit saturation solve is not available on this path");
-        if (o.flag("enable_microporosity"))
-            throw std::runtime_error("--enable_microporosity true is not available on this path");
+        (void)o.flag("enable_microporosity");   // validated; inert (see its help text)
 
         ls_params p;
         ls_default_params(&p);
@@ -191,6 +188,7 @@ int main(int argc, char** argv) {
         p.porosity = o.num("porosity");
         p.time_scalar = o.num("time_scalar");
         p.alpha_transfer = o.num("alpha_transfer");
+        p.implicit_sat = o.flag("implicit_sat") ? 1 : 0;
         const std::string bc = o.str("bc"), sv = o.str("solver"), wm = o.str("warm");
         p.bc_mode = bc == "zfaces" ? LS_BC_ZFACES : LS_BC_REFERENCE;
         p.solver = sv == "plain" ? LS_SOLVER_PLAIN64 : sv == "learned" ? LS_SOLVER_LEARNED : LS_SOLVER_MIXED;
@@ -267,6 +265,7 @@ int main(int argc, char** argv) {
                   << "  Time step (dt)                   : " << dt_user << " s\n"
                   << "  Time discretisation (Theta)      : " << p.theta << "\n"
                   << "  Precipitation mode               : " << (p.instant_precip ? "True (Instant)" : "False (Gradual)") << "\n"
+                  << "  Saturation Solver                : " << (p.implicit_sat ? "Implicit (upwind, 3 Picard passes)" : "Explicit (Sub-stepping)") << "\n"
                   << "  Pressure solver / warm start     : " << sv << " / " << wm << " (boundary " << bc << ")\n"
                   << "  Output Mode                      : " << (use_out_time ? "Physical time based (seconds)" : "Time steps based (timesteps)") << "\n"
                   << "==================================================\n" << std::endl;

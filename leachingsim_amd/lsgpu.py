@@ -47,6 +47,7 @@ class Params(C.Structure):
         ("warm_start", C.c_int), ("press_rtol", C.c_double), ("press_maxit", C.c_int),
         ("adr_rtol", C.c_double), ("adr_maxit", C.c_int), ("warm_history", C.c_int),
         ("mixed_inner_rtol", C.c_double), ("learned_judge", C.c_double),
+        ("implicit_sat", C.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -74,6 +75,8 @@ class StepReport(C.Structure):
         ("prof_stencil64_ms", C.c_double), ("prof_stencil32_ms", C.c_double),
         ("prof_stencil64_n", C.c_int), ("prof_stencil32_n", C.c_int),
         ("press_relres0", C.c_double), ("warm_basis", C.c_int), ("warm_net", C.c_int),
+        ("sat_iters", C.c_int), ("sat_valid", C.c_int), ("sat_raw_min", C.c_double),
+        ("sat_raw_max", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -88,7 +91,7 @@ def default_params(**overrides) -> Params:
                alpha_transfer=1e-5, instant_precip=0, bc_mode=BC_REFERENCE,
                solver=SOLVER_PLAIN64, warm_start=WARM_PREVIOUS, press_rtol=1e-8,
                press_maxit=10000, adr_rtol=1e-8, adr_maxit=10000, warm_history=4,
-               mixed_inner_rtol=0.1, learned_judge=1.0)
+               mixed_inner_rtol=0.1, learned_judge=1.0, implicit_sat=0)
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)

@@ -129,6 +129,7 @@ struct lso_ctx {
     std::vector<double> d;            /* sum of couplings of a Pore row     */
     double p_top_prev = 0.0;
     bool have_prev = false;
+    bool sat_valid = true;      /* physics_valid of the last implicit saturation */
 
     size_t idx(int i, int j, int k) const {
         return (size_t)k + (size_t)nk * ((size_t)j + (size_t)nj * (size_t)i);
@@ -742,6 +743,90 @@ static int bicgstab(const Csr& A, const std::vector<double>& b, std::vector<doub
     return 1;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Implicit saturation: setupSaturationEqns + solveSaturation               */
+/* (src/Simulation.cpp:1858-2053).  Inactive voxels keep their value; Pore  */
+/* rows: phi/dt s_new + upwinded mobility advection = phi/dt s - evap sink, */
+/* the outward face flux being the mean of the two cells' flux and the      */
+/* upwind divisor the assembly-time saturation (floored at 1e-6).           */
+/* ------------------------------------------------------------------------ */
+
+static void assemble_saturation(lso_ctx& c, double dt, std::vector<int>& R, std::vector<int>& C,
+                                std::vector<double>& V, std::vector<double>& b) {
+    R.clear();
+    C.clear();
+    V.clear();
+    b.assign(c.N, 0.0);
+    const double inv_dt = 1.0 / dt;
+    const double dx = c.P.dx;
+    auto add = [&](size_t r, size_t col, double val) {
+        R.push_back((int)r);
+        C.push_back((int)col);
+        V.push_back(val);
+    };
+    for (int kk = 0; kk < c.nk; ++kk)
+        for (int jj = 0; jj < c.nj; ++jj)
+            for (int ii = 0; ii < c.ni; ++ii) {
+                const size_t v = c.idx(ii, jj, kk);
+                const uint8_t t = c.img[v];
+                if (t == AIR || t == ROCK || t == SUL || t == PREC) {
+                    add(v, v, 1.0);
+                    b[v] += c.s[v];
+                    continue;
+                }
+                int air_faces = 0;
+                for (int f = 0; f < 6; ++f) {
+                    size_t n;
+                    if (c.nb(ii, jj, kk, f, n) && c.img[n] == AIR) ++air_faces;
+                }
+                const double evap_sink = (air_faces * c.P.evap_flux) / dx;
+                double coeff_self = c.P.porosity * inv_dt;
+                const double explicit_rhs = (c.P.porosity * inv_dt * c.s[v]) - evap_sink;
+                for (int f = 0; f < 6; ++f) {      /* xp, xm, yp, ym, zp, zm */
+                    size_t n;
+                    if (!c.nb(ii, jj, kk, f, n)) continue;
+                    const std::vector<double>& q = f < 2 ? c.qx : (f < 4 ? c.qy : c.qz);
+                    const double q_out = (f % 2 == 0) ? (q[v] + q[n]) / 2.0 : -(q[v] + q[n]) / 2.0;
+                    if (q_out > 0) {
+                        const double v_eff = q_out / std::max(c.s[v], 1e-6);
+                        coeff_self += v_eff / dx;
+                    } else {
+                        const double v_eff = q_out / std::max(c.s[n], 1e-6);
+                        add(v, n, v_eff / dx);
+                    }
+                }
+                add(v, v, coeff_self);
+                b[v] += explicit_rhs;
+            }
+}
+
+static int phase_saturation_implicit(lso_ctx& c, double dt, ls_step_report* rep) {
+    std::vector<int> R, C;
+    std::vector<double> V, b;
+    assemble_saturation(c, dt, R, C, V, b);
+    Csr A;
+    A.build((int)c.N, R, C, V);
+    int it = 0;
+    double rr = 0.0;
+    const int st = bicgstab(A, b, c.s, c.P.adr_rtol, c.P.adr_maxit, it, rr);  /* warm start: s */
+    double lo = 1e300, hi = -1e300;
+    for (size_t v = 0; v < c.N; ++v) {
+        lo = std::min(lo, c.s[v]);
+        hi = std::max(hi, c.s[v]);
+    }
+    const bool valid = !(hi > 1.05 || lo < -0.05);
+    if (valid)
+        for (size_t v = 0; v < c.N; ++v) c.s[v] = std::max(0.0, std::min(1.0, c.s[v]));
+    c.sat_valid = valid;
+    if (rep) {
+        rep->sat_iters = it;
+        rep->sat_valid = valid ? 1 : 0;
+        rep->sat_raw_min = lo;
+        rep->sat_raw_max = hi;
+    }
+    return st;
+}
+
 static int phase_transport(lso_ctx& c, double dt, ls_step_report* rep) {
     std::vector<int> R, C;
     std::vector<double> V, b;
@@ -1019,7 +1104,10 @@ int lso_phase(lso_ctx* c, int phase, double t, double dt, ls_step_report* rep) {
         case LS_PH_PROPERTIES: phase_properties(*c); return 0;
         case LS_PH_PRESSURE: return phase_pressure(*c, t, rep);
         case LS_PH_FLUX: phase_flux(*c); return 0;
-        case LS_PH_SATURATION: phase_saturation(*c, dt, rep); return 0;
+        case LS_PH_SATURATION:
+            if (c->P.implicit_sat) return phase_saturation_implicit(*c, dt, rep);
+            phase_saturation(*c, dt, rep);
+            return 0;
         case LS_PH_SURFACE: phase_surface(*c); return 0;
         case LS_PH_TRANSPORT: return phase_transport(*c, dt, rep);
         case LS_PH_PRECIPITATION: phase_precipitation(*c); return 0;
@@ -1031,11 +1119,23 @@ int lso_phase(lso_ctx* c, int phase, double t, double dt, ls_step_report* rep) {
 int lso_step(lso_ctx* c, double t, double dt, ls_step_report* rep) {
     if (rep) std::memset(rep, 0, sizeof(*rep));
     snapshot(*c);
-    phase_properties(*c);
-    int st = phase_pressure(*c, t, rep);
-    if (st == 0) {
+    int st = 0;
+    /* hydrodynamics: 1 Picard pass, or 3 with the implicit saturation
+     * (src/main.cpp:312-370); an unphysical final pass forces a retry */
+    const int passes = c->P.implicit_sat ? 3 : 1;
+    for (int pass = 0; pass < passes && st == 0; ++pass) {
+        phase_properties(*c);
+        st = phase_pressure(*c, t, rep);
+        if (st != 0) break;
         phase_flux(*c);
-        phase_saturation(*c, dt, rep);
+        if (c->P.implicit_sat) st = phase_saturation_implicit(*c, dt, rep);
+        else phase_saturation(*c, dt, rep);
+    }
+    if (st == 0 && c->P.implicit_sat && !c->sat_valid) {
+        c->err = "implicit saturation: unphysical state after the Picard passes";
+        st = 1;
+    }
+    if (st == 0) {
         phase_surface(*c);
         st = phase_transport(*c, dt, rep);
     }
@@ -1092,6 +1192,18 @@ long long lso_pressure_system(lso_ctx* c, double t, int* rows, int* cols, double
         if (rhs) rhs[v] = source_term;
     }
     return e;
+}
+
+long long lso_saturation_system(lso_ctx* c, double dt, int* rows, int* cols, double* vals,
+                                double* rhs) {
+    std::vector<int> R, C;
+    std::vector<double> V, b;
+    assemble_saturation(*c, dt, R, C, V, b);
+    if (rows) {
+        for (size_t e = 0; e < R.size(); ++e) { rows[e] = R[e]; cols[e] = C[e]; vals[e] = V[e]; }
+        std::memcpy(rhs, b.data(), c->N * 8);
+    }
+    return (long long)R.size();
 }
 
 long long lso_transport_system(lso_ctx* c, double dt, int field, int* rows, int* cols,

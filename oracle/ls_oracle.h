@@ -49,6 +49,10 @@ long long lso_pressure_system(lso_ctx* ctx, double t, int* rows, int* cols,
 /* The reference's transport system (src/Simulation.cpp:966-1237) for the
  * given field (LS_F_CONC or LS_F_ACID), COO form as above.  Like the
  * reference, this zeroes the flux at Rock and Precipitate voxels. */
+/* the implicit saturation system of the current state (COO + rhs), as
+ * assembled by src/Simulation.cpp:1858-1964 */
+long long lso_saturation_system(lso_ctx* ctx, double dt, int* rows, int* cols, double* vals,
+                                double* rhs);
 long long lso_transport_system(lso_ctx* ctx, double dt, int field, int* rows,
                                int* cols, double* vals, double* rhs);
 /* y = A x with the reduced pressure operator of the last pressure phase

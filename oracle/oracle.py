@@ -52,6 +52,7 @@ def load() -> C.CDLL:
         "lso_set_threads": (None, [i]),
         "lso_pressure_system": (ll, [vp, d, vp, vp, vp, vp]),
         "lso_transport_system": (ll, [vp, d, i, vp, vp, vp, vp]),
+        "lso_saturation_system": (ll, [vp, d, vp, vp, vp, vp]),
         "lso_apply_pressure": (i, [vp, vp, vp]),
         "lso_pressure_classes": (i, [vp, vp]),
     }
@@ -160,6 +161,14 @@ class OracleSimulation:
         v = np.empty(nnz, np.float64); b = np.empty(self.N, np.float64)
         self.lib.lso_pressure_system(self.h, t, r.ctypes.data, c.ctypes.data, v.ctypes.data,
                                      b.ctypes.data)
+        return r, c, v, b
+
+    def saturation_system(self, dt):
+        nnz = self.lib.lso_saturation_system(self.h, dt, None, None, None, None)
+        r = np.empty(nnz, np.int32); c = np.empty(nnz, np.int32)
+        v = np.empty(nnz, np.float64); b = np.empty(self.N, np.float64)
+        self.lib.lso_saturation_system(self.h, dt, r.ctypes.data, c.ctypes.data, v.ctypes.data,
+                                       b.ctypes.data)
         return r, c, v, b
 
     def transport_system(self, dt, field):
