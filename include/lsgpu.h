@@ -114,10 +114,12 @@ typedef struct ls_params {
                                 network correction)                              */
     double mixed_inner_rtol; /* mixed solver: the FP64 residual replaces the FP32 one
                                 whenever the latter has fallen by this factor (0.1) */
-    double learned_judge;    /* learned solver: flexible CG must keep at least this
-                                multiple of the last mixed PCG solve's convergence
-                                rate (decades per ms), else mixed PCG takes over
-                                from the saved start (1.0; 0 = never judged)     */
+    double learned_race;    /* learned solver: a race against the residual curve
+                                of the last mixed PCG solve; after the same wall
+                                time flexible CG must have reduced the residual by
+                                at least this multiple of the decades mixed PCG
+                                had, else mixed PCG takes over from the saved
+                                start (1.0; 0 = no race)                         */
     int implicit_sat;        /* --implicit_sat (0): implicit upwind saturation solve
                                 inside 3 Picard passes (src/main.cpp:312-370,
                                 src/Simulation.cpp:1858-2053), solved to adr_rtol */

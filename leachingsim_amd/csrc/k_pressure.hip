@@ -708,7 +708,7 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
     double last_wrr = wrr;
     int stalls = 0;
     bool first = true;
-    // residual curve of this solve (the learned solver's judge races it)
+    // residual curve of this solve (the learned solver races against it)
     const auto t_start = std::chrono::steady_clock::now();
     const double rr_start32 = rr64 / (scale * scale);
     c->race_tmp.clear();
@@ -1227,11 +1227,11 @@ static int poll_fcg(ls_ctx* c) {
 }
 
 // Mixed-precision PCG inside the learned solver (a converged solve leaves
-// its residual curve for the judge).
-static int mixed_judged(ls_ctx* c, ls_step_report* rep) {
+// its residual curve for the learned solver's race).
+static int mixed_recorded(ls_ctx* c, ls_step_report* rep) {
     ls_step_report r2;
     std::memset(&r2, 0, sizeof(r2));
-    const int st = solve_mixed(c, &r2);   // records the judge's curve
+    const int st = solve_mixed(c, &r2);   // records the race curve
     if (!rep->press_outer && !rep->press_iters) {
         rep->press_relres0 = r2.press_relres0;
         rep->press_bnorm = r2.press_bnorm;
@@ -1258,14 +1258,14 @@ static double race_decades(const std::vector<std::pair<double, double>>& r, doub
 static int solve_learned(ls_ctx* c, ls_step_report* rep) {
     int st = learned_setup(c);
     if (st) return st;
-    // The plain method is the judge: the network runs only against a known
+    // The plain method sets the bar: the network runs only against a known
     // convergence rate of mixed PCG on this context (the first solve sets
     // it), and after repeated losses it is rested for a few solves.
-    const bool judged = c->P.learned_judge > 0.0;
-    if ((judged && !c->have_rate) || c->net_skip > 0) {
+    const bool raced = c->P.learned_race > 0.0;
+    if ((raced && !c->have_rate) || c->net_skip > 0) {
         if (c->net_skip > 0) --c->net_skip;
         rep->press_fallback = 1;
-        return mixed_judged(c, rep);
+        return mixed_recorded(c, rep);
     }
     const uint32_t N = (uint32_t)c->N;
     const int nbe = ew_blocks(N);
@@ -1343,14 +1343,14 @@ static int solve_learned(ls_ctx* c, ls_step_report* rep) {
                 std::swap(qa, qb);
             }
             if ((st = poll_fcg(c))) return st;
-            // judge: a race against the last mixed PCG solve's residual
+            // race against the last mixed PCG solve's residual
             // curve; after the same wall time the network must have reduced
-            // the residual by at least learned_judge x as many decades
-            if (judged && c->h_fcg->iter >= 16 && !c->h_fcg->done) {
+            // the residual by at least learned_race x as many decades
+            if (raced && c->h_fcg->iter >= 16 && !c->h_fcg->done) {
                 const double ms = std::chrono::duration<double, std::milli>(
                                       std::chrono::steady_clock::now() - t_fcg).count();
                 const double dec = 0.5 * std::log10(rr_start / c->h_fcg->rr);
-                if (!(dec >= c->P.learned_judge * race_decades(c->race, ms))) {
+                if (!(dec >= c->P.learned_race * race_decades(c->race, ms))) {
                     lost = true;
                     break;
                 }
@@ -1393,7 +1393,7 @@ static int solve_learned(ls_ctx* c, ls_step_report* rep) {
     rep->press_iters = 0;
     const int outer_fcg = rep->press_outer;
     rep->press_outer = 0;
-    st = mixed_judged(c, rep);
+    st = mixed_recorded(c, rep);
     rep->press_iters += iters_fcg;
     rep->press_outer += outer_fcg;
     return st;

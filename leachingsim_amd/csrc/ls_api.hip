@@ -88,7 +88,7 @@ void ls_default_params(ls_params* p) {
     p->adr_rtol = 1e-8;
     p->adr_maxit = 10000;
     p->warm_history = 4;
-    p->learned_judge = 1.0;
+    p->learned_race = 1.0;
     p->implicit_sat = 0;
     p->mixed_inner_rtol = 0.1;
 }
@@ -115,8 +115,8 @@ static int validate_params(ls_ctx* c, const ls_params* p) {
         c->err = "invalid warm start or warm_history (1..8, 1..7 with the network)";
         return -3;
     }
-    if (!(p->learned_judge >= 0.0)) {
-        c->err = "learned_judge must be >= 0";
+    if (!(p->learned_race >= 0.0)) {
+        c->err = "learned_race must be >= 0";
         return -3;
     }
     return 0;

@@ -121,7 +121,7 @@ struct ls_ctx {
     std::vector<float> net_weights;  // host copy, set by ls_set_network
     int net_failures = 0;            // consecutive probes where the network lost
     int net_skip = 0;                // solves left that go straight to mixed PCG
-    // the judge of the learned solver: the residual curve (ms of wall time,
+    // the bar of the learned solver: the residual curve (ms of wall time,
     // decades of reduction) of the last converged mixed PCG solve
     std::vector<std::pair<double, double>> race, race_tmp;
     bool have_rate = false;

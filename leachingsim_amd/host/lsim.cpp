@@ -3,7 +3,7 @@
 // Same options, defaults and loop as src/main.cpp of javalosp/LeachingSim
 // (option table :39; setup :69-148; initialisation or restart :151-199; the
 // adaptive step loop with retry manager, output cadence, .pvd tracking,
-// checkpoints and dt ramp :293-541).  Options of this build (GPU solver
+// checkpoints and dt ramp :293-541).  Options of the GPU path (solver
 // choice, boundary mode, label width) are listed under --help.  Unknown
 // options (the reference passes PETSc options through) are ignored with a
 // notice.
@@ -158,7 +158,7 @@ int main(int argc, char** argv) {
         o.add("enable_microporosity", "false", "Acid diffusion through the rock matrix (read only by code the reference has commented out: no effect)");
         o.add("rock_D_fac", "1e-4", "Diffusion multiplier inside the rock matrix (no effect, as above)");
         o.add("alpha_transfer", "1e-5", "Mass transfer coefficient for acid entering the rock matrix");
-        // options of this build
+        // options of the GPU path
         o.add("label_bytes", "1", "Bytes per label in the RAW file: 1 (uint8) or 2 (uint16)");
         o.add("bc", "reference", "Pressure boundary: reference (Air voxels) | zfaces (inlet/outlet planes)");
         o.add("solver", "mixed", "Pressure solver: plain | mixed | learned");

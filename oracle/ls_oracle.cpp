@@ -1,8 +1,8 @@
 /*
  * ls_oracle.cpp — CPU FP64 oracle of the LeachingSim per-timestep hot path.
  *
- * TEST INFRASTRUCTURE ONLY (see ls_oracle.h): the checker for the HIP path and
- * the timed CPU baseline.  Never linked into the product.
+ * TEST INFRASTRUCTURE ONLY (see ls_oracle.h): the values the HIP path is
+ * checked against, and the timed CPU baseline.  Never linked into the product.
  *
  * Every function restates the reference loop it cites, for one domain
  * (global faces only, no ghost planes).  Loops run over the flat k-fastest

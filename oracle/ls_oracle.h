@@ -1,10 +1,10 @@
 /*
  * ls_oracle.h — CPU FP64 oracle of the LeachingSim per-timestep hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the
- * checker / the timed CPU baseline.  The product path (liblsgpu.so) never
- * links or calls it.
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, the smoke entry point and the
+ * cpu_baseline leg of bench.py may load this library, and only as the values
+ * the GPU path is compared against, or as the timed CPU baseline.  The
+ * product path (liblsgpu.so) never links or calls it.
  *
  * The oracle restates src/Simulation.cpp of javalosp/LeachingSim for a single
  * domain (no MPI ranks) with these semantic decisions: one-sided differences

@@ -1,8 +1,9 @@
 """ctypes binding of the CPU FP64 oracle (liblsoracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker and the timed CPU baseline.  The
-product path (leachingsim_amd) never imports this module.
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke entry point and the
+cpu_baseline leg of bench.py, as the values the GPU path is compared against
+and the timed CPU baseline.  The product path (leachingsim_amd) never imports
+this module.
 
 ``OracleSimulation`` exposes the same methods as
 ``leachingsim_amd.lsgpu.GpuSimulation`` so one test sequence drives both.

@@ -1,7 +1,7 @@
 """CPU reference of the 3-level 3D U-Net (forward and backward), float64.
 
-TEST INFRASTRUCTURE ONLY: the checker for the BF16/MFMA kernels in
-leachingsim_amd/csrc/k_unet.hip (imported by tests/ only).  It has no
+TEST INFRASTRUCTURE ONLY: the values the BF16/MFMA kernels of
+leachingsim_amd/csrc/k_unet.hip are checked against (imported by tests/ only).  It has no
 counterpart in the reference (the learned preconditioner is new); it
 restates the network defined by leachingsim_amd/unet_spec.py, and its
 gradients are pinned by central finite differences in the CPU tests.

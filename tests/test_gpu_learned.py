@@ -71,12 +71,12 @@ def test_learned_pack_matches_restriction():
 @pytest.mark.parametrize("seed", [1, 2])
 def test_learned_solver_reaches_tolerance_and_matches_plain(seed):
     """Flexible CG with the network preconditioner run to convergence (the
-    judge switched off), against plain FP64 PCG."""
+    race switched off), against plain FP64 PCG."""
     img = G.smoothed_threshold((32, 32, 48), 2.0, 0.4, seed)
     sol = {}
     for solver in (L.SOLVER_PLAIN64, L.SOLVER_LEARNED):
         P = L.default_params(bc_mode=L.BC_ZFACES, solver=solver, press_rtol=1e-12,
-                             learned_judge=0.0)
+                             learned_race=0.0)
         g = L.GpuSimulation(img.shape, P)
         g.read_labels(img)
         g.init_state()
@@ -93,8 +93,8 @@ def test_learned_solver_reaches_tolerance_and_matches_plain(seed):
     assert relerr(sol[L.SOLVER_LEARNED], sol[L.SOLVER_PLAIN64]) <= 1e-7
 
 
-def test_judge_hands_over_to_mixed_pcg():
-    """With the judge on: the first solve of a context is mixed PCG (it sets
+def test_race_hands_over_to_mixed_pcg():
+    """With the race on: the first solve of a context is mixed PCG (it sets
     the rate); a later solve runs flexible CG, and the result meets the
     tolerance whether the network keeps up or mixed PCG takes over."""
     img = G.smoothed_threshold((32, 32, 48), 2.0, 0.4, 3)
@@ -127,7 +127,7 @@ def test_trained_network_gain():
     on geometries outside its training seeds: (1) its own objective, the
     share of the A-norm error of CG iterates and projected warm starts that
     its Galerkin-scaled coarse correction removes, against the untrained
-    initialisation; (2) flexible CG with it (judge off) against plain mixed
+    initialisation; (2) flexible CG with it (race off) against plain mixed
     PCG, iterations and wall time (recorded; the network is slower here)."""
     import time
     from leachingsim_amd import learn
@@ -151,7 +151,7 @@ def test_trained_network_gain():
     img = learn.training_geometry(903, 96)
     out = {}
     for solver in (L.SOLVER_MIXED, L.SOLVER_LEARNED):
-        P = L.default_params(bc_mode=L.BC_ZFACES, solver=solver, learned_judge=0.0)
+        P = L.default_params(bc_mode=L.BC_ZFACES, solver=solver, learned_race=0.0)
         g = L.GpuSimulation(img.shape, P)
         g.read_labels(img)
         g.init_state()
