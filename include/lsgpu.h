@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LSGPU_ABI_VERSION 4
+#define LSGPU_ABI_VERSION 5
 
 /* Labels (src/compiler_opts.h:69-82). */
 enum ls_label {
@@ -53,7 +53,8 @@ enum ls_label {
  * ||b - A x||_2 / ||b||_2 <= press_rtol on the reduced SPD pressure system. */
 enum ls_solver {
     LS_SOLVER_PLAIN64 = 0,  /* FP64 Jacobi-preconditioned CG                     */
-    LS_SOLVER_MIXED = 1,    /* FP32 stencil CG inside FP64 iterative refinement  */
+    LS_SOLVER_MIXED = 1,    /* FP32 CG (red-black Schur complement, mixed_schur)
+                               inside FP64 reliable updates                     */
     LS_SOLVER_LEARNED = 2   /* flexible CG, U-Net preconditioner + warm start,
                                FP32 inner / FP64 outer, plain PCG fallback       */
 };
@@ -123,6 +124,10 @@ typedef struct ls_params {
     int implicit_sat;        /* --implicit_sat (0): implicit upwind saturation solve
                                 inside 3 Picard passes (src/main.cpp:312-370,
                                 src/Simulation.cpp:1858-2053), solved to adr_rtol */
+    int mixed_schur;         /* mixed solver (1): FP32 inner CG on the red-black
+                                Schur complement of the scaled operator (about
+                                half the iterations of Jacobi-PCG); 0: inner CG
+                                on the operator itself                          */
 } ls_params;
 
 /* Fields that can be uploaded / downloaded.  Element types are fixed:

@@ -220,6 +220,115 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Red-black Schur complement.  The 7-point graph is bipartite, so with the
+// colour c = (i + j + k) & 1 (black 0, red 1) the scaled operator
+// A^ = I - N has no coupling inside a colour, and eliminating the red
+// unknowns leaves S = I - N_br N_rb on the black ones.  Since the spectrum of
+// N is symmetric, cond(S) ~ cond(A^) / 4 and CG on S needs about half the
+// iterations; red values follow from x_r = b_r + N_rb x_b.  One pass kernel,
+// with n = sum over the 6 faces of c^ xn[neighbour]:
+//   MODE 1 (gather)      y = red ? n : 0                      (w = N_rb p)
+//   MODE 2 (apply)       y = black ? xc - n : 0, dot xc.y     (q = S p)
+//   MODE 3 (rhs)         y = black ? xc + n : 0, dot y.y      (b_b + N_br b_r)
+//   MODE 4 (reconstruct) y = red ? xc + n : xn               (x_r = r_r + N_rb x_b)
+// Vectorised like k_stencil_vec: 4 floats of one k-row per lane, marching
+// ICH planes along i.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(WAVE * StencilShape<float>::TJV) void k_rb_vec(
+    const float* __restrict__ xn, const float* __restrict__ xc, float* __restrict__ y,
+    const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
+    SGeo g, double* __restrict__ partials, const int* __restrict__ done) {
+    constexpr int W = 4, ICH = StencilShape<float>::ICH, TJV = StencilShape<float>::TJV;
+    typedef VecOf<float, 4>::type V;
+    if (done != nullptr && *done) return;
+    __shared__ double lds[TJV];
+    const int nbx = gridDim.x, nby = gridDim.y;
+    const int nb = nbx * nby * gridDim.z;
+    const int hw = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    const int t = xcd_remap(hw, nb);
+    const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
+    const int lane = threadIdx.x;
+    const int k0 = (bx * WAVE + lane) * W;
+    const int j = by * TJV + threadIdx.y;
+    const int i0 = bz * ICH, i1 = min(i0 + ICH, g.ni);
+    double acc = 0.0;
+    if (j < g.nj) {
+        const bool ok = k0 < g.nk;
+        const bool hj = j > 0, pj = j + 1 < g.nj;
+        uint32_t v = (uint32_t)i0 * g.si + (uint32_t)j * g.sj + (ok ? (uint32_t)k0 : 0u);
+        const V zero = V(0);
+        V xm = (ok && i0 > 0) ? *(const V*)(xn + v - g.si) : zero;
+        V cxm = (ok && i0 > 0) ? *(const V*)(cx + v - g.si) : zero;
+        V xcur = ok ? *(const V*)(xn + v) : zero;
+        auto plane = [&](int i) {
+            V xp = zero, cxp = zero, cyp = zero, czp = zero, cym = zero, xjp = zero, xjm = zero;
+            V cen = zero;
+            if (ok) {
+                if (i + 1 < g.ni) xp = *(const V*)(xn + v + g.si);
+                cxp = *(const V*)(cx + v);
+                cyp = *(const V*)(cy + v);
+                czp = *(const V*)(cz + v);
+                if (hj) {
+                    cym = *(const V*)(cy + v - g.sj);
+                    xjm = *(const V*)(xn + v - g.sj);
+                }
+                if (pj) xjp = *(const V*)(xn + v + g.sj);
+                if (MODE != 1) cen = *(const V*)(xc + v);
+            }
+            float left = __shfl_up(xcur[W - 1], 1, WAVE);
+            float czl = __shfl_up(czp[W - 1], 1, WAVE);
+            float right = __shfl_down(xcur[0], 1, WAVE);
+            if (lane == 0) {
+                left = (ok && k0 > 0) ? xn[v - 1] : 0.0f;
+                czl = (ok && k0 > 0) ? cz[v - 1] : 0.0f;
+            }
+            if (lane == WAVE - 1) right = (ok && k0 + W < g.nk) ? xn[v + W] : 0.0f;
+            const int par = (i + j + k0) & 1;      // colour of the lane's first element
+            V out;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const float xkm = (w == 0) ? left : xcur[w - 1];
+                const float xkp = (w == W - 1) ? right : xcur[w + 1];
+                const float czm = (w == 0) ? czl : czp[w - 1];
+                const float n = cxp[w] * xp[w] + cxm[w] * xm[w] + cyp[w] * xjp[w] +
+                                cym[w] * xjm[w] + czp[w] * xkp + czm * xkm;
+                const bool red = ((par + w) & 1) != 0;
+                float o;
+                if (MODE == 1) o = red ? n : 0.0f;
+                else if (MODE == 2) o = red ? 0.0f : cen[w] - n;
+                else if (MODE == 3) o = red ? 0.0f : cen[w] + n;
+                else o = red ? cen[w] + n : xcur[w];
+                out[w] = o;
+                if (MODE == 2) acc += (double)cen[w] * (double)o;
+                if (MODE == 3) acc += (double)o * (double)o;
+            }
+            if (ok) __builtin_nontemporal_store(out, (V*)(y + v));
+            xm = xcur;
+            xcur = xp;
+            cxm = cxp;
+            v += g.si;
+        };
+        if (i1 - i0 == ICH) {
+#pragma unroll
+            for (int q = 0; q < ICH; ++q) plane(i0 + q);
+        } else {
+            for (int i = i0; i < i1; ++i) plane(i);
+        }
+    }
+    if (MODE == 2 || MODE == 3) {
+        acc = wave_sum(acc);
+        if (lane == 0) lds[threadIdx.y] = acc;
+        __syncthreads();
+        if (flat_tid() == 0) {
+            double s = 0.0;
+            for (int q = 0; q < TJV; ++q) s += lds[q];
+            partials[t] = s;
+        }
+    }
+}
+
 // Vectorised CG vector updates: U vectors of 16 B per thread, all loads
 // issued before the stores, non-temporal (each vector is streamed once per
 // kernel).  The scalar tail (N % W) is handled by block 0.
@@ -787,6 +896,163 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
     }
 }
 
+// Launch of one red-black pass over the FP32 faces; returns the number of
+// block partials (MODE 2 and 3 write them).
+template <int MODE>
+static int launch_rb(ls_ctx* c, const float* xn, const float* xc, float* y, const int* done) {
+    const dim3 grd = vec_grid<float>(c);
+    hipLaunchKernelGGL(k_rb_vec<MODE>, grd, dim3(WAVE, StencilShape<float>::TJV), 0, c->stream,
+                       xn, xc, y, c->c32[0], c->c32[1], c->c32[2], sgeo(c), c->partials, done);
+    return (int)(grd.x * grd.y * grd.z);
+}
+
+__global__ void k_set_tol2(CgScalars* cs, double tol2) { cs->tol2 = tol2; }
+
+// CG iterations on the Schur complement S = I - N_br N_rb (black unknowns;
+// red entries of p, rs, q and e stay 0).  e += alpha p rides in the
+// direction kernel, as in cg_iterations.
+static int cg_iterations_rb(ls_ctx* c, float* e, float* rs, float* p, float* q, float* w, int batch) {
+    const uint32_t N = (uint32_t)c->N;
+    const int nbe = vec_blocks<float>(N);
+    for (int b = 0; b < batch; ++b) {
+        const bool sample = c->profiling && b == 0;
+        if (sample) LS_HIP(hipEventRecord(c->ev[12], c->stream));
+        launch_rb<1>(c, p, nullptr, w, &c->cg->done);                 // w = N_rb p
+        if (sample) {
+            LS_HIP(hipEventRecord(c->ev[13], c->stream));
+            c->prof_pending = 32;
+        }
+        const int nbs = launch_rb<2>(c, w, p, q, &c->cg->done);      // q = S p, p.q
+        hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, c->stream, c->partials, nbs, c->cg);
+        hipLaunchKernelGGL((k_cg_update_vec<float, false>), dim3(nbe), dim3(EW_BLOCK), 0,
+                           c->stream, e, rs, p, q, nullptr, c->cg, N, c->partials);
+        hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1024), 0, c->stream, c->partials, nbe, c->cg, 0);
+        hipLaunchKernelGGL(k_cg_direction_vec<float>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, p,
+                           rs, e, c->cg, N);
+    }
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+// Mixed precision with reliable updates, the FP32 inner iteration being CG
+// on the red-black Schur complement.  Each replacement cycle starts from the
+// FP32 copy r of the FP64 residual: the black right-hand side is
+// r_b + N_br r_r, CG on S runs until the Schur residual has fallen by delta
+// (or its norm reaches the tolerance), then the red correction is
+// e_r = r_r + N_rb e_b, the full correction is added to the FP64 iterate and
+// the FP64 residual recomputed.  Needs the vectorised layout (nk % 4 == 0).
+static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep) {
+    const uint32_t N = (uint32_t)c->N;
+    const int nbe = ew_blocks(N);
+    const int bs = batch_size(c);
+    const double delta = c->P.mixed_inner_rtol > 0 ? c->P.mixed_inner_rtol : 0.1;
+    float* rs = (float*)c->tv[0][0];    // Schur residual (FP32 work vectors borrowed from
+    float* w = (float*)c->tv[0][1];     // the idle transport buffers)
+    int st;
+    if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+    if ((st = poll_cg(c))) return st;
+    const double bnorm2 = c->h_cg->bnorm2;
+    const double tol2 = c->h_cg->tol2;
+    rep->press_bnorm = sqrt(bnorm2);
+    rep->press_relres = c->h_cg->relres;
+    if (!rep->press_outer) rep->press_relres0 = c->h_cg->relres;
+    if (bnorm2 == 0.0) {
+        LS_HIP(hipMemsetAsync(c->x64, 0, c->N * sizeof(double), c->stream));
+        rep->press_relres = 0.0;
+        return 0;
+    }
+    if (c->h_cg->converged) return 0;
+    const double scale = std::ldexp(1.0, std::ilogb(sqrt(c->h_cg->rr)));
+    double rr64 = c->h_cg->rr, wrr = c->h_cg->wrr;
+    double last_wrr = wrr;
+    int stalls = 0;
+    bool first = true;
+    const auto t_start = std::chrono::steady_clock::now();
+    double rr_start32 = 0.0;
+    c->race_tmp.clear();
+    auto record = [&](double rr32) {
+        if (!(rr32 > 0.0) || !(rr_start32 > 0.0)) return;
+        const double ms = std::chrono::duration<double, std::milli>(
+                              std::chrono::steady_clock::now() - t_start).count();
+        c->race_tmp.emplace_back(ms, 0.5 * std::log10(rr_start32 / rr32));
+    };
+    const int maxit = std::min(c->P.press_maxit, std::max(2000, 3 * (c->P.press_maxit / 4)));
+    while (true) {
+        // r (FP32 copy of the FP64 residual), e = 0; Schur right-hand side
+        hipLaunchKernelGGL(k_inner_start, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->r64, c->r32,
+                           c->p32, c->x32, N, 1.0 / scale, 0, c->partials);
+        const int nbs = launch_rb<3>(c, c->r32, c->r32, rs, nullptr);
+        hipLaunchKernelGGL(k_inner_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nbs,
+                           c->cg, 0.0, maxit, first ? 1 : 0);
+        LS_LAUNCH_CHECK();
+        if (first) {
+            LS_HIP(hipMemcpyAsync(c->p32, rs, c->N * sizeof(float), hipMemcpyDeviceToDevice,
+                                  c->stream));
+        } else {   // xlive = 0 here: only the direction carries over
+            hipLaunchKernelGGL(k_cg_direction_vec<float>, dim3(vec_blocks<float>(N)),
+                               dim3(EW_BLOCK), 0, c->stream, c->p32, rs, c->x32, c->cg, N);
+            LS_LAUNCH_CHECK();
+        }
+        if ((st = poll_cg(c))) return st;
+        // trigger: the Schur residual fell by delta, or its estimated
+        // unscaled norm reaches the tolerance (the Schur residual is the
+        // full residual once the red values are reconstructed)
+        const double rrs = c->h_cg->rr;
+        if (first) rr_start32 = rrs;
+        const double w2 = wrr / rr64;
+        double trig = delta * delta * rrs;
+        const double t2 = 0.81 * tol2 / (w2 * scale * scale);
+        if (t2 > trig) trig = t2;
+        hipLaunchKernelGGL(k_set_tol2, dim3(1), dim3(1), 0, c->stream, c->cg, trig);
+        LS_LAUNCH_CHECK();
+        first = false;
+        if (!(rrs <= trig)) {
+            while (!c->h_cg->done) {
+                if ((st = cg_iterations_rb(c, c->x32, rs, c->p32, c->ap32, w, bs))) return st;
+                if ((st = poll_cg(c))) return st;
+                record(c->h_cg->rr);
+            }
+        }
+        const int iters = c->h_cg->iter;
+        const bool broke = !c->h_cg->converged && iters < maxit && !(rrs <= trig);
+        // red reconstruction e_r = r_r + N_rb e_b into w, then x64 += scale w
+        launch_rb<4>(c, c->x32, c->r32, w, nullptr);
+        hipLaunchKernelGGL(k_refine_add, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->x64, w, N,
+                           scale);
+        LS_LAUNCH_CHECK();
+        rep->press_outer += 1;
+        if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+        if ((st = poll_cg(c))) return st;
+        rr64 = c->h_cg->rr;
+        wrr = c->h_cg->wrr;
+        rep->press_relres = c->h_cg->relres;
+        rep->press_iters = iters;
+        record(rr64 / (scale * scale));
+        if (c->h_cg->converged) {
+            c->race.swap(c->race_tmp);
+            c->have_rate = true;
+            return 0;
+        }
+        stalls = (wrr > 0.25 * last_wrr) ? stalls + 1 : 0;
+        last_wrr = wrr;
+        if (broke || stalls >= 3 || iters >= maxit) {
+            rep->press_fallback = 1;
+            const int left = c->P.press_maxit - iters;
+            if (left <= 0) return 1;
+            return solve_plain64(c, rep, left);
+        }
+        hipLaunchKernelGGL(k_set_iter, dim3(1), dim3(1), 0, c->stream, c->cg, iters);
+        LS_LAUNCH_CHECK();
+    }
+}
+
+// The mixed solver of the context: red-black Schur inner CG when enabled and
+// the vectorised layout applies, else inner CG on the operator.
+static int solve_mixed_any(ls_ctx* c, ls_step_report* rep) {
+    if (c->P.mixed_schur && use_vec<float>(c)) return solve_mixed_rb(c, rep);
+    return solve_mixed(c, rep);
+}
+
 static double effective_top_pressure(double top, double t) {
     const double ramp_duration = 3600.0;
     double eff = top;
@@ -804,7 +1070,7 @@ int solve_pressure(ls_ctx* c, double t, ls_step_report* rep) {
     if (st) return st;
     if ((st = warm_start(c, ptop, rep))) return st;
     if (c->P.solver == LS_SOLVER_PLAIN64) st = solve_plain64(c, rep, c->P.press_maxit);
-    else if (c->P.solver == LS_SOLVER_MIXED) st = solve_mixed(c, rep);
+    else if (c->P.solver == LS_SOLVER_MIXED) st = solve_mixed_any(c, rep);
     else st = solve_learned(c, rep);
     if (st < 0) return st;
     int st2 = finalize_pressure(c, ptop);
@@ -1231,7 +1497,7 @@ static int poll_fcg(ls_ctx* c) {
 static int mixed_recorded(ls_ctx* c, ls_step_report* rep) {
     ls_step_report r2;
     std::memset(&r2, 0, sizeof(r2));
-    const int st = solve_mixed(c, &r2);   // records the race curve
+    const int st = solve_mixed_any(c, &r2);   // records the race curve
     if (!rep->press_outer && !rep->press_iters) {
         rep->press_relres0 = r2.press_relres0;
         rep->press_bnorm = r2.press_bnorm;
