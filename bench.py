@@ -189,9 +189,14 @@ def cpu_baseline(img, label_bytes, dt, crop: int, nsteps: int):
                       f"by voxel count to {'x'.join(map(str, img.shape))}"}
 
 
-def stencil_only(img, label_bytes, precision, reps):
+KERNELS = {"stencil64": (64, "k_stencil_vec<double>"), "stencil32": (32, "k_stencil_vec<float>"),
+           "rb_gather": (1, "k_rb_vec<1>"), "rb_apply": (2, "k_rb_vec<2>")}
+
+
+def stencil_only(img, label_bytes, kernel, reps):
     """Assemble the pressure operator of the initial state, then time `reps`
-    stencil launches and `reps` copies of 1 GiB (HIP events)."""
+    launches of one pressure kernel and `reps` copies of 1 GiB (HIP events)."""
+    precision, kname = KERNELS[kernel]
     P = L.default_params(bc_mode=L.BC_ZFACES, press_maxit=1)
     sim = L.GpuSimulation(img.shape, P, label_bytes)
     sim.read_labels(img)
@@ -202,7 +207,7 @@ def stencil_only(img, label_bytes, precision, reps):
     nb = min(img.size * 8, 1 << 30)
     cms = sim.time_copy(nb, reps)
     sim.close()
-    print(json.dumps({"kernel": f"k_stencil_vec<{'double' if precision == 64 else 'float'}>",
+    print(json.dumps({"kernel": kname,
                       "grid": list(img.shape), "reps": reps, "avg_launch_ms": ms,
                       "bytes_per_launch": by, "achieved_GBs": by / ms / 1e6,
                       "copy_bytes": 2 * nb, "copy_ms": cms, "copy_GBs": 2 * nb / cms / 1e6}),
@@ -249,7 +254,8 @@ def main():
     ap.add_argument("--mode", default="steps", choices=["steps", "stencil"],
                     help="stencil: assemble the pressure operator once and time only the "
                          "stencil and a copy kernel (for counter collection)")
-    ap.add_argument("--precision", type=int, default=64, choices=[32, 64])
+    ap.add_argument("--kernel", default="rb_gather", choices=list(KERNELS),
+                    help="kernel timed in --mode stencil")
     ap.add_argument("--reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -262,7 +268,7 @@ def main():
     log(f"workload {desc}: generated in {time.perf_counter() - t0:.1f} s; labels "
         + ", ".join(f"{k}:{int((img == k).sum())}" for k in range(1, 6)))
     if args.mode == "stencil":
-        stencil_only(img, lb, args.precision, args.reps)
+        stencil_only(img, lb, args.kernel, args.reps)
         return
     head = {"plain": L.SOLVER_PLAIN64, "mixed": L.SOLVER_MIXED,
             "learned": L.SOLVER_LEARNED}[args.solver]
@@ -297,8 +303,15 @@ def main():
                                            warm=L.WARM_PROJECTED)
             ablations[f"{SOLVER_NAMES[sv]}/projected"] = summary(asec, areps, args.steps)
 
-    # roofline of the pressure stencil, from the sampled HIP-event timings
+    # roofline of the sampled pressure kernel (HIP events in the timed
+    # region): the FP32 red-black gather pass when the mixed solver runs on
+    # the Schur complement, else the FP32 or FP64 stencil; all three read
+    # x and 3 faces and write y (20 B / 40 B per voxel)
     prec = 64 if (head == L.SOLVER_PLAIN64 or sum(r["prof_stencil32_n"] for r in reps) == 0) else 32
+    if prec == 64:
+        kernel = "stencil64"
+    else:
+        kernel = "rb_gather" if img.shape[2] % 4 == 0 else "stencil32"
     key = f"prof_stencil{prec}"
     n = sum(r[key + "_n"] for r in reps)
     ms = sum(r[key + "_ms"] for r in reps) / max(n, 1)
@@ -307,8 +320,8 @@ def main():
     achieved = bytes_per_launch / (ms * 1e-3) / 1e9 if n else 0.0
     traffic = None
     tr = load_traffic()
-    if tr is not None and str(prec) in tr and tr[str(prec)].get("grid") == list(img.shape):
-        traffic = tr[str(prec)]["bytes_per_launch"]
+    if tr is not None and kernel in tr and tr[kernel].get("grid") == list(img.shape):
+        traffic = tr[kernel]["bytes_per_launch"]
 
     cpu = None
     if not args.no_cpu:
@@ -336,7 +349,7 @@ def main():
                    "ablations": ablations},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "kernel": f"k_stencil_vec<{'double' if prec == 64 else 'float'}>",
+                     "kernel": KERNELS[kernel][1],
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms,
                      "sampled_launches": n, "measured_copy_GBs": copy_gbs, "traffic": traffic},
         "cpu_baseline": cpu,

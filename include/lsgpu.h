@@ -263,9 +263,13 @@ int ls_debug_pressure_system(ls_ctx* ctx, double* sqrt_d, double* bhat, int8_t* 
 /* The scaled FP64 +face conductances of the last assembled system (the
  * operator A^ x = x - sum c^ x_n), 3 x N doubles (i, j, k faces). */
 int ls_debug_pressure_faces(ls_ctx* ctx, double* faces);
-/* Time `reps` launches of the pressure stencil kernel y = A x (precision 64 or
- * 32) on the context's stream with HIP events; returns ms per launch and the
- * kernel's algorithmic bytes per launch. */
+/* Time `reps` launches of a pressure kernel on the context's stream with HIP
+ * events: the stencil y = A x (precision 64 or 32), or one of the FP32
+ * red-black Schur passes (LS_TIME_RB_GATHER: w = N_rb p; LS_TIME_RB_APPLY:
+ * q = p - N_br w with its dot); returns ms per launch and the kernel's
+ * algorithmic bytes per launch. */
+#define LS_TIME_RB_GATHER 1
+#define LS_TIME_RB_APPLY 2
 int ls_time_stencil(ls_ctx* ctx, int precision, int reps, double* ms_per_launch,
                     double* bytes_per_launch);
 /* Copy bytes within device memory on the context's stream, timed with HIP

@@ -1108,12 +1108,18 @@ int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) 
         c->err = "no pressure system assembled";
         return -1;
     }
+    if ((precision == LS_TIME_RB_GATHER || precision == LS_TIME_RB_APPLY) && !use_vec<float>(c)) {
+        c->err = "the red-black passes need nk % 4 == 0";
+        return -3;
+    }
     hipEvent_t e0 = c->ev[14], e1 = c->ev[15];
     for (int pass = 0; pass < 2; ++pass) {
         int n = pass == 0 ? 1 : reps;
         if (pass == 1) LS_HIP(hipEventRecord(e0, c->stream));
         for (int r = 0; r < n; ++r) {
-            if (precision == 32) launch_stencil<float>(c, c->p32, c->ap32, c->c32, nullptr);
+            if (precision == LS_TIME_RB_GATHER) launch_rb<1>(c, c->p32, nullptr, c->ap32, nullptr);
+            else if (precision == LS_TIME_RB_APPLY) launch_rb<2>(c, c->r32, c->p32, c->ap32, nullptr);
+            else if (precision == 32) launch_stencil<float>(c, c->p32, c->ap32, c->c32, nullptr);
             else launch_stencil<double>(c, c->p64, c->ap64, c->c64, nullptr);
         }
         LS_LAUNCH_CHECK();
@@ -1123,8 +1129,14 @@ int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) 
     float f = 0.0f;
     LS_HIP(hipEventElapsedTime(&f, e0, e1));
     *ms = (double)f / reps;
-    const double sv = precision == 32 ? 4.0 : 8.0;
-    *bytes = (double)c->N * (2.0 * sv + 3.0 * sv);
+    if (precision == LS_TIME_RB_GATHER) {
+        *bytes = (double)c->N * 20.0;      // p, 3 faces, w
+    } else if (precision == LS_TIME_RB_APPLY) {
+        *bytes = (double)c->N * 24.0;      // w, 3 faces, p, q
+    } else {
+        const double sv = precision == 32 ? 4.0 : 8.0;
+        *bytes = (double)c->N * (2.0 * sv + 3.0 * sv);
+    }
     return 0;
 }
 

@@ -602,7 +602,11 @@ int ls_debug_apply_pressure(ls_ctx* c, const double* x, double* y) {
 
 int ls_time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) {
     if (!c || !ms || !bytes || reps < 1) return -1;
-    if (precision != 32 && precision != 64) return -3;
+    if (precision != 32 && precision != 64 && precision != LS_TIME_RB_GATHER &&
+        precision != LS_TIME_RB_APPLY) {
+        c->err = "ls_time_stencil: precision must be 64, 32, LS_TIME_RB_GATHER or LS_TIME_RB_APPLY";
+        return -3;
+    }
     return time_stencil(c, precision, reps, ms, bytes);
 }
 
