@@ -190,7 +190,8 @@ def cpu_baseline(img, label_bytes, dt, crop: int, nsteps: int):
 
 
 KERNELS = {"stencil64": (64, "k_stencil_vec<double>"), "stencil32": (32, "k_stencil_vec<float>"),
-           "rb_gather": (1, "k_rb_vec<1>"), "rb_apply": (2, "k_rb_vec<2>")}
+           "rb_gather": (1, "k_rb_vec<1>"), "rb_apply": (2, "k_rb_vec<2>"),
+           "schur": (3, "k_schur_apply<16>")}
 
 
 def stencil_only(img, label_bytes, kernel, reps):
@@ -254,7 +255,7 @@ def main():
     ap.add_argument("--mode", default="steps", choices=["steps", "stencil"],
                     help="stencil: assemble the pressure operator once and time only the "
                          "stencil and a copy kernel (for counter collection)")
-    ap.add_argument("--kernel", default="rb_gather", choices=list(KERNELS),
+    ap.add_argument("--kernel", default="schur", choices=list(KERNELS),
                     help="kernel timed in --mode stencil")
     ap.add_argument("--reps", type=int, default=20)
     args = ap.parse_args()
@@ -304,19 +305,20 @@ def main():
             ablations[f"{SOLVER_NAMES[sv]}/projected"] = summary(asec, areps, args.steps)
 
     # roofline of the sampled pressure kernel (HIP events in the timed
-    # region): the FP32 red-black gather pass when the mixed solver runs on
-    # the Schur complement, else the FP32 or FP64 stencil; all three read
-    # x and 3 faces and write y (20 B / 40 B per voxel)
-    prec = 64 if (head == L.SOLVER_PLAIN64 or sum(r["prof_stencil32_n"] for r in reps) == 0) else 32
-    if prec == 64:
-        kernel = "stencil64"
+    # region): the fused Schur apply on compacted vectors when the mixed
+    # solver runs on the Schur complement (3 faces + half-length p and q:
+    # 16 B per voxel), its gather pass in the two-pass form, else the FP32
+    # or FP64 stencil (x, 3 faces, y: 20 B / 40 B per voxel)
+    if sum(r["prof_schur_n"] for r in reps) > 0:
+        key = "prof_schur"
+        bytes_per_launch = reps[-1]["prof_schur_bytes"]
+        kernel = "schur" if bytes_per_launch == img.size * 16 else "rb_gather"
+    elif head == L.SOLVER_PLAIN64 or sum(r["prof_stencil32_n"] for r in reps) == 0:
+        key, kernel, bytes_per_launch = "prof_stencil64", "stencil64", img.size * 40
     else:
-        kernel = "rb_gather" if img.shape[2] % 4 == 0 else "stencil32"
-    key = f"prof_stencil{prec}"
+        key, kernel, bytes_per_launch = "prof_stencil32", "stencil32", img.size * 20
     n = sum(r[key + "_n"] for r in reps)
     ms = sum(r[key + "_ms"] for r in reps) / max(n, 1)
-    sv = prec // 8
-    bytes_per_launch = img.size * (2 * sv + 3 * sv)
     achieved = bytes_per_launch / (ms * 1e-3) / 1e9 if n else 0.0
     traffic = None
     tr = load_traffic()

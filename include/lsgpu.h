@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LSGPU_ABI_VERSION 5
+#define LSGPU_ABI_VERSION 6
 
 /* Labels (src/compiler_opts.h:69-82). */
 enum ls_label {
@@ -126,8 +126,11 @@ typedef struct ls_params {
                                 src/Simulation.cpp:1858-2053), solved to adr_rtol */
     int mixed_schur;         /* mixed solver (1): FP32 inner CG on the red-black
                                 Schur complement of the scaled operator (about
-                                half the iterations of Jacobi-PCG); 0: inner CG
-                                on the operator itself                          */
+                                half the iterations of Jacobi-PCG), on vectors
+                                compacted to the black voxels with one fused
+                                pass per operator apply when nk % 8 == 0; 2: the
+                                same on full-length vectors in two passes; both
+                                need nk % 4 == 0; 0: inner CG on the operator  */
 } ls_params;
 
 /* Fields that can be uploaded / downloaded.  Element types are fixed:
@@ -203,6 +206,12 @@ typedef struct ls_step_report {
     int sat_valid;            /* implicit saturation: raw solution within
                                  [-0.05, 1.05] (the reference's physics_valid)  */
     double sat_raw_min, sat_raw_max;   /* implicit saturation: before the clamp */
+    /* With profiling on: the same for the FP32 Schur complement apply of
+     * the mixed solver (the fused kernel, or the gather pass of the two-pass
+     * form) and its algorithmic HBM bytes per launch. */
+    double prof_schur_ms;
+    int prof_schur_n;
+    double prof_schur_bytes;
 } ls_step_report;
 
 typedef struct ls_ctx ls_ctx;
@@ -263,13 +272,20 @@ int ls_debug_pressure_system(ls_ctx* ctx, double* sqrt_d, double* bhat, int8_t* 
 /* The scaled FP64 +face conductances of the last assembled system (the
  * operator A^ x = x - sum c^ x_n), 3 x N doubles (i, j, k faces). */
 int ls_debug_pressure_faces(ls_ctx* ctx, double* faces);
+/* q = S p with S = I - N_br N_rb, the red-black Schur complement of the
+ * last assembled FP32 scaled operator, on host vectors compacted to the black
+ * voxels (N/2 floats: black voxel v at index v >> 1), with *dot = p.q.
+ * form 1: the fused kernel (nk % 8 == 0); form 2: the gather and apply passes
+ * on full-length vectors (nk % 4 == 0). */
+int ls_debug_schur_apply(ls_ctx* ctx, int form, const float* p, float* q, double* dot);
 /* Time `reps` launches of a pressure kernel on the context's stream with HIP
  * events: the stencil y = A x (precision 64 or 32), or one of the FP32
  * red-black Schur passes (LS_TIME_RB_GATHER: w = N_rb p; LS_TIME_RB_APPLY:
- * q = p - N_br w with its dot); returns ms per launch and the kernel's
+ * q = p - N_br w with its dot; LS_TIME_SCHUR: both fused); returns ms per launch and the kernel's
  * algorithmic bytes per launch. */
 #define LS_TIME_RB_GATHER 1
 #define LS_TIME_RB_APPLY 2
+#define LS_TIME_SCHUR 3   /* fused q = S p on compacted vectors (nk % 8 == 0) */
 int ls_time_stencil(ls_ctx* ctx, int precision, int reps, double* ms_per_launch,
                     double* bytes_per_launch);
 /* Copy bytes within device memory on the context's stream, timed with HIP

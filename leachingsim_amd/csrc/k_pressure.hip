@@ -232,6 +232,8 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
 //   MODE 2 (apply)       y = black ? xc - n : 0, dot xc.y     (q = S p)
 //   MODE 3 (rhs)         y = black ? xc + n : 0, dot y.y      (b_b + N_br b_r)
 //   MODE 4 (reconstruct) y = red ? xc + n : xn               (x_r = r_r + N_rb x_b)
+//   MODE 5 (rhs, packed) as MODE 3, y written as the compacted black vector
+//                        (layout below k_schur_apply's header)
 // Vectorised like k_stencil_vec: 4 floats of one k-row per lane, marching
 // ICH planes along i.
 // ---------------------------------------------------------------------------
@@ -298,13 +300,21 @@ __global__ __launch_bounds__(WAVE * StencilShape<float>::TJV) void k_rb_vec(
                 float o;
                 if (MODE == 1) o = red ? n : 0.0f;
                 else if (MODE == 2) o = red ? 0.0f : cen[w] - n;
-                else if (MODE == 3) o = red ? 0.0f : cen[w] + n;
+                else if (MODE == 3 || MODE == 5) o = red ? 0.0f : cen[w] + n;
                 else o = red ? cen[w] + n : xcur[w];
                 out[w] = o;
                 if (MODE == 2) acc += (double)cen[w] * (double)o;
-                if (MODE == 3) acc += (double)o * (double)o;
+                if (MODE == 3 || MODE == 5) acc += (double)o * (double)o;
             }
-            if (ok) __builtin_nontemporal_store(out, (V*)(y + v));
+            if (MODE == 5) {
+                // the lane's two black values, black first element at par
+                typedef float F2 __attribute__((ext_vector_type(2)));
+                const int pb = (i + j) & 1;
+                const F2 o2 = pb ? F2{out[1], out[3]} : F2{out[0], out[2]};
+                if (ok) *(F2*)(y + (v >> 1)) = o2;
+            } else if (ok) {
+                __builtin_nontemporal_store(out, (V*)(y + v));
+            }
             xm = xcur;
             xcur = xp;
             cxm = cxp;
@@ -317,7 +327,7 @@ __global__ __launch_bounds__(WAVE * StencilShape<float>::TJV) void k_rb_vec(
             for (int i = i0; i < i1; ++i) plane(i);
         }
     }
-    if (MODE == 2 || MODE == 3) {
+    if (MODE == 2 || MODE == 3 || MODE == 5) {
         acc = wave_sum(acc);
         if (lane == 0) lds[threadIdx.y] = acc;
         __syncthreads();
@@ -327,6 +337,234 @@ __global__ __launch_bounds__(WAVE * StencilShape<float>::TJV) void k_rb_vec(
             partials[t] = s;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// The Schur complement on colour-compacted vectors.
+//
+// With nk even, voxel v = (i nj + j) nk + k of either colour is stored at
+// v >> 1 of its colour's half-length array: row (i, j) holds its nk / 2
+// black voxels k = par + 2m (par = (i + j) & 1) at m = 0 .. nk/2 - 1, and its
+// red voxels k = 1 - par + 2m likewise.  The i and j neighbours of a voxel
+// (other colour, other row parity) then sit at the same m; the k neighbours
+// of red m at black m - par and m + 1 - par, those of black m at red
+// m - 1 + par and m + par.
+//
+// k_schur_apply: q = S p = p - N_br (N_rb p) on compacted black vectors in
+// one pass.  A block owns TJ rows of full k length and marches ICH planes
+// along i; wave (y, z) covers 256 compacted values (1024 B) of row
+// j0 - 1 + z, the first and last row being halo rows that only compute w.
+// At plane i each wave computes w = N_rb p at the red voxels of its row,
+// keeps its own row's w of planes i - 2 .. i in registers and publishes
+// plane i in LDS (three rotating buffers, one barrier per plane); q of plane
+// i - 1 then takes w of the own row (planes i - 2, i), of rows j +- 1 (LDS)
+// and of the k neighbours (shuffles; LDS at the wave edges).  A face serves
+// w at plane i and q at the black end one plane later from registers, so
+// HBM sees p, the three face arrays and q once: 16 B per voxel of the grid
+// (the two-pass form on full-length vectors moves 44 B).  Each product is
+// summed in the order of k_rb_vec (+i, -i, +j, -j, +k, -k).
+// ---------------------------------------------------------------------------
+struct RbGeo {
+    int ni, nj, nh;            // nh = nk / 2
+    uint32_t si, sj, hsi, hsj; // full and compacted strides
+};
+
+static RbGeo rbgeo(const ls_ctx* c) {
+    RbGeo g;
+    g.ni = c->ni; g.nj = c->nj; g.nh = c->nk / 2;
+    g.sj = (uint32_t)c->nk;
+    g.si = (uint32_t)c->nk * (uint32_t)c->nj;
+    g.hsj = (uint32_t)g.nh;
+    g.hsi = (uint32_t)g.nh * (uint32_t)c->nj;
+    return g;
+}
+
+#ifndef LS_SCHUR_ROWS
+#define LS_SCHUR_ROWS 14       // TJ: rows per block (plus 2 halo rows)
+#endif
+#ifndef LS_SCHUR_ICH
+#define LS_SCHUR_ICH 16        // planes per block
+#endif
+#ifndef LS_SCHUR_THREADS
+#define LS_SCHUR_THREADS 1024  // launch bound: 64 x WR x (TJ + 2) at most
+#endif
+constexpr int SCHUR_ICH = LS_SCHUR_ICH;
+
+typedef float F4 __attribute__((ext_vector_type(4)));
+
+// Colour split of 8 consecutive values (a0, a1) of one row whose first
+// element has colour par (0 black): b[x] = element 2x + par, r[x] = element
+// 2x + 1 - par.
+__device__ __forceinline__ void csplit(F4 a0, F4 a1, int par, F4& b, F4& r) {
+    const F4 e = F4{a0[0], a0[2], a1[0], a1[2]};
+    const F4 o = F4{a0[1], a0[3], a1[1], a1[3]};
+    b = par ? o : e;
+    r = par ? e : o;
+}
+
+template <int ICH>
+__global__ __launch_bounds__(LS_SCHUR_THREADS) void k_schur_apply(
+    const float* __restrict__ p, float* __restrict__ q, const float* __restrict__ cx,
+    const float* __restrict__ cy, const float* __restrict__ cz, RbGeo g,
+    double* __restrict__ partials, const int* __restrict__ done) {
+    if (done != nullptr && *done) return;
+    extern __shared__ float wsh[];                   // [3][NS][rowlen]
+    __shared__ double red[LS_SCHUR_THREADS / WAVE];
+    const int lane = threadIdx.x, wr = threadIdx.y, s = threadIdx.z;
+    const int NS = blockDim.z, TJ = NS - 2;
+    const int rowlen = blockDim.y * (WAVE * 4);
+    const int bstride = NS * rowlen;
+    const int nbj = gridDim.x;
+    const int t = xcd_remap(blockIdx.x + nbj * blockIdx.y, nbj * gridDim.y);
+    const int j = (t % nbj) * TJ - 1 + s;
+    const int i0 = (t / nbj) * ICH, i1 = min(i0 + ICH, g.ni);
+    const int mloc = (wr * WAVE + lane) * 4;
+    const bool ok = j >= 0 && j < g.nj && mloc < g.nh;
+    const bool halo = s == 0 || s == NS - 1;
+    const bool hj = j > 0, pj = j + 1 < g.nj;
+    const bool hl = mloc > 0, hr = mloc + 4 < g.nh;
+    const uint32_t vrow = ok ? (uint32_t)j * g.sj + 2u * (uint32_t)mloc : 0u;
+    const uint32_t hrow = ok ? (uint32_t)j * g.hsj + (uint32_t)mloc : 0u;
+    float* const lrow = wsh + s * rowlen + mloc;
+    const F4 zero = F4(0.0f);
+    F4 Pm = zero, Pc = zero;                 // p of planes i - 1, i (own row)
+    F4 Wmm = zero, Wm = zero;                // w of planes i - 2, i - 1 (own row)
+    F4 cxB1 = zero, cxR1 = zero, cxR2 = zero;    // cx: plane i - 1 (black, red), i - 2 (red)
+    F4 cyB1 = zero, cymB1 = zero, czB1 = zero, czR1 = zero;   // plane i - 1
+    float czL1 = 0.0f;
+    double acc = 0.0;
+    if (ok && i0 >= 2) Pc = *(const F4*)(p + (uint32_t)(i0 - 2) * g.hsi + hrow);
+#pragma unroll
+    for (int st = 0; st < ICH + 3; ++st) {
+        const int i = i0 - 2 + st;
+        if (i > i1) break;
+        const bool inp = ok && i >= 0 && i < g.ni;
+        const int par = (i + j) & 1;
+        const uint32_t v = (uint32_t)i * g.si + vrow;
+        const uint32_t h = (uint32_t)i * g.hsi + hrow;
+        // h and v wrap for the planes i < 0 (never dereferenced there);
+        // plane i + 1 is indexed from its own unsigned offset
+        const uint32_t hn = h + g.hsi;
+        F4 Pn = zero;
+        if (ok && i + 1 >= 0 && i + 1 < g.ni) Pn = *(const F4*)(p + hn);
+        F4 cx0 = zero, cx1 = zero;
+        if (inp) {
+            cx0 = *(const F4*)(cx + v);
+            cx1 = *(const F4*)(cx + v + 4);
+        }
+        F4 cxB, cxR;
+        csplit(cx0, cx1, par, cxB, cxR);
+        F4 cyB = zero, cymB = zero, czB = zero, czR = zero;
+        float czL = 0.0f;
+        if (st >= 1) {
+            F4 cy0 = zero, cy1 = zero, cz0 = zero, cz1 = zero, cm0 = zero, cm1 = zero;
+            F4 Pjm = zero, Pjp = zero;
+            float pL = 0.0f, pR = 0.0f, czl = 0.0f;
+            if (inp) {
+                cy0 = *(const F4*)(cy + v);
+                cy1 = *(const F4*)(cy + v + 4);
+                cz0 = *(const F4*)(cz + v);
+                cz1 = *(const F4*)(cz + v + 4);
+                if (hj) {
+                    cm0 = *(const F4*)(cy + v - g.sj);
+                    cm1 = *(const F4*)(cy + v - g.sj + 4);
+                    Pjm = *(const F4*)(p + h - g.hsj);
+                }
+                if (pj) Pjp = *(const F4*)(p + h + g.hsj);
+                if (lane == 0 && hl) {
+                    pL = p[h - 1];
+                    czl = cz[v - 1];
+                }
+                if (lane == WAVE - 1 && hr) pR = p[h + 4];
+            }
+            const float sl = __shfl_up(Pc[3], 1, WAVE);
+            const float sr = __shfl_down(Pc[0], 1, WAVE);
+            const float sz = __shfl_up(cz1[3], 1, WAVE);
+            if (lane != 0) { pL = sl; czl = sz; }
+            if (lane != WAVE - 1) pR = sr;
+            czL = czl;
+            F4 cyR, cymR;
+            csplit(cy0, cy1, par, cyB, cyR);
+            csplit(cm0, cm1, par, cymB, cymR);
+            csplit(cz0, cz1, par, czB, czR);
+            // w at the red voxels of plane i
+            F4 Wc = zero;
+            if (i >= 0 && i < g.ni) {
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    const float pkp = par ? Pc[x] : (x < 3 ? Pc[x + 1] : pR);
+                    const float pkm = par ? (x > 0 ? Pc[x - 1] : pL) : Pc[x];
+                    const float czm = par ? (x > 0 ? czB[x - 1] : czL) : czB[x];
+                    Wc[x] = cxR[x] * Pn[x] + cxB1[x] * Pm[x] + cyR[x] * Pjp[x] +
+                            cymR[x] * Pjm[x] + czR[x] * pkp + czm * pkm;
+                }
+            }
+            if (st >= 2) *(F4*)(lrow + (st % 3) * bstride) = Wc;
+            if (st >= 3) {
+                __syncthreads();
+                // q at the black voxels of plane i - 1 (parity 1 - par)
+                const float* lb = lrow + ((st - 1) % 3) * bstride;
+                const float sl2 = __shfl_up(Wm[3], 1, WAVE);
+                const float sr2 = __shfl_down(Wm[0], 1, WAVE);
+                if (!halo) {
+                    const F4 Wjm = *(const F4*)(lb - rowlen);
+                    const F4 Wjp = *(const F4*)(lb + rowlen);
+                    float wL = sl2, wR = sr2;
+                    if (lane == 0) wL = hl ? lb[-1] : 0.0f;
+                    if (lane == WAVE - 1) wR = hr ? lb[4] : 0.0f;
+                    const int pb = 1 - par;
+                    F4 Q;
+#pragma unroll
+                    for (int x = 0; x < 4; ++x) {
+                        const float wkp = pb ? (x < 3 ? Wm[x + 1] : wR) : Wm[x];
+                        const float wkm = pb ? Wm[x] : (x > 0 ? Wm[x - 1] : wL);
+                        const float czm = pb ? czR1[x] : (x > 0 ? czR1[x - 1] : czL1);
+                        const float n = cxB1[x] * Wc[x] + cxR2[x] * Wmm[x] + cyB1[x] * Wjp[x] +
+                                        cymB1[x] * Wjm[x] + czB1[x] * wkp + czm * wkm;
+                        Q[x] = Pm[x] - n;
+                        acc += (double)Pm[x] * (double)Q[x];
+                    }
+                    if (ok) __builtin_nontemporal_store(Q, (F4*)(q + h - g.hsi));
+                }
+            }
+            Wmm = Wm;
+            Wm = Wc;
+        }
+        Pm = Pc;
+        Pc = Pn;
+        cxR2 = cxR1;
+        cxR1 = cxR;
+        cxB1 = cxB;
+        cyB1 = cyB;
+        cymB1 = cymB;
+        czB1 = czB;
+        czR1 = czR;
+        czL1 = czL;
+    }
+    acc = wave_sum(acc);
+    const int wid = threadIdx.y + blockDim.y * threadIdx.z;
+    const int nw = blockDim.y * blockDim.z;
+    if (lane == 0) red[wid] = acc;
+    __syncthreads();
+    if (wid == 0 && lane == 0) {
+        double a = 0.0;
+        for (int w = 0; w < nw; ++w) a += red[w];
+        partials[t] = a;
+    }
+}
+
+// Compacted black vector -> full-length vector with zero red entries.
+__global__ __launch_bounds__(EW_BLOCK) void k_rb_unpack(const float* __restrict__ hc,
+                                                       float* __restrict__ y, SGeo g,
+                                                       uint32_t N) {
+    const uint32_t v = 4u * (blockIdx.x * (uint32_t)blockDim.x + threadIdx.x);
+    if (v >= N) return;
+    const uint32_t row = v / g.sj;                        // i nj + j
+    const int par = (int)((row / (uint32_t)g.nj + row % (uint32_t)g.nj) & 1u);
+    typedef float F2 __attribute__((ext_vector_type(2)));
+    const F2 e = *(const F2*)(hc + (v >> 1));
+    const F4 o = par ? F4{0.0f, e[0], 0.0f, e[1]} : F4{e[0], 0.0f, e[1], 0.0f};
+    *(F4*)(y + v) = o;
 }
 
 // Vectorised CG vector updates: U vectors of 16 B per thread, all loads
@@ -736,7 +974,7 @@ static int poll_cg(ls_ctx* c) {
     if (c->prof_pending) {
         float ms = 0.0f;
         LS_HIP(hipEventElapsedTime(&ms, c->ev[12], c->ev[13]));
-        const int q = c->prof_pending == 64 ? 0 : 1;
+        const int q = c->prof_pending == 64 ? 0 : (c->prof_pending == 32 ? 1 : 2);
         c->prof_ms[q] += ms;
         c->prof_n[q] += 1;
         c->prof_pending = 0;
@@ -906,29 +1144,89 @@ static int launch_rb(ls_ctx* c, const float* xn, const float* xc, float* y, cons
     return (int)(grd.x * grd.y * grd.z);
 }
 
+// Fused Schur apply on compacted vectors: block shape 64 x WR x (TJ + 2)
+// with WR waves per row (256 compacted values each); needs nk % 8 == 0 so a
+// lane's 4 compacted values are 16 B aligned.
+struct SchurShape {
+    int wr, tj;
+    dim3 grid, block;
+    size_t lds;
+};
+
+static bool schur_shape(const ls_ctx* c, SchurShape* sh) {
+    if (c->nk % 8 != 0) return false;
+    const int nh = c->nk / 2;
+    const int wr = (nh + 4 * WAVE - 1) / (4 * WAVE);
+    const int ns_max = (LS_SCHUR_THREADS / WAVE) / wr;
+    const int tj = std::min(LS_SCHUR_ROWS, ns_max - 2);
+    if (tj < 1) return false;
+    sh->wr = wr;
+    sh->tj = tj;
+    sh->grid = dim3((c->nj + tj - 1) / tj, (c->ni + SCHUR_ICH - 1) / SCHUR_ICH, 1);
+    sh->block = dim3(WAVE, wr, tj + 2);
+    sh->lds = (size_t)3 * (tj + 2) * wr * 4 * WAVE * sizeof(float);
+    return (size_t)sh->grid.x * sh->grid.y <= (size_t)MAX_PARTIALS &&
+           2 * (size_t)vec_blocks<float>((uint32_t)(c->N / 2)) <= (size_t)MAX_PARTIALS;
+}
+
+// The mixed solver's Schur form: 0 none, 1 fused on compacted vectors,
+// 2 two passes on full-length vectors.
+static int schur_form(const ls_ctx* c) {
+    if (!c->P.mixed_schur || !use_vec<float>(c)) return 0;
+    SchurShape sh;
+    if (c->P.mixed_schur == 1 && schur_shape(c, &sh)) return 1;
+    return 2;
+}
+
+// q = S p on compacted black vectors; returns the number of block partials
+// of p.q.
+static int launch_schur(ls_ctx* c, const float* p, float* q, const int* done) {
+    SchurShape sh;
+    schur_shape(c, &sh);
+    hipLaunchKernelGGL(k_schur_apply<SCHUR_ICH>, sh.grid, sh.block, sh.lds, c->stream, p, q,
+                       c->c32[0], c->c32[1], c->c32[2], rbgeo(c), c->partials, done);
+    return (int)(sh.grid.x * sh.grid.y);
+}
+
+static void launch_unpack(ls_ctx* c, const float* hc, float* y) {
+    const uint32_t N = (uint32_t)c->N;
+    const uint32_t nt = N / 4;
+    hipLaunchKernelGGL(k_rb_unpack, dim3((nt + EW_BLOCK - 1) / EW_BLOCK), dim3(EW_BLOCK), 0,
+                       c->stream, hc, y, sgeo(c), N);
+}
+
 __global__ void k_set_tol2(CgScalars* cs, double tol2) { cs->tol2 = tol2; }
 
-// CG iterations on the Schur complement S = I - N_br N_rb (black unknowns;
-// red entries of p, rs, q and e stay 0).  e += alpha p rides in the
+// CG iterations on the Schur complement S = I - N_br N_rb.  fused: the
+// vectors are compacted black vectors of N/2 values and q = S p is one
+// k_schur_apply; else they are full-length with zero red entries and S p
+// takes the gather (into w) and apply passes.  e += alpha p rides in the
 // direction kernel, as in cg_iterations.
-static int cg_iterations_rb(ls_ctx* c, float* e, float* rs, float* p, float* q, float* w, int batch) {
-    const uint32_t N = (uint32_t)c->N;
-    const int nbe = vec_blocks<float>(N);
+static int cg_iterations_rb(ls_ctx* c, bool fused, float* e, float* rs, float* p, float* q,
+                            float* w, int batch) {
+    const uint32_t Nv = fused ? (uint32_t)(c->N / 2) : (uint32_t)c->N;
+    const int nbe = vec_blocks<float>(Nv);
     for (int b = 0; b < batch; ++b) {
         const bool sample = c->profiling && b == 0;
         if (sample) LS_HIP(hipEventRecord(c->ev[12], c->stream));
-        launch_rb<1>(c, p, nullptr, w, &c->cg->done);                 // w = N_rb p
+        int nbs;
+        if (fused) {
+            nbs = launch_schur(c, p, q, &c->cg->done);               // q = S p, p.q
+        } else {
+            launch_rb<1>(c, p, nullptr, w, &c->cg->done);             // w = N_rb p
+        }
         if (sample) {
             LS_HIP(hipEventRecord(c->ev[13], c->stream));
-            c->prof_pending = 32;
+            c->prof_pending = 16;
+            c->prof_schur_bytes = (double)c->N * (fused ? 16.0 : 20.0);
         }
-        const int nbs = launch_rb<2>(c, w, p, q, &c->cg->done);      // q = S p, p.q
+        if (!fused) nbs = launch_rb<2>(c, w, p, q, &c->cg->done);    // q = S p, p.q
         hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, c->stream, c->partials, nbs, c->cg);
         hipLaunchKernelGGL((k_cg_update_vec<float, false>), dim3(nbe), dim3(EW_BLOCK), 0,
-                           c->stream, e, rs, p, q, nullptr, c->cg, N, c->partials);
+                           c->stream, e, rs, p, q, nullptr, c->cg, Nv, c->partials);
         hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1024), 0, c->stream, c->partials, nbe, c->cg, 0);
         hipLaunchKernelGGL(k_cg_direction_vec<float>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, p,
-                           rs, e, c->cg, N);
+                           rs, e, c->cg, Nv);
     }
     LS_LAUNCH_CHECK();
     return 0;
@@ -940,9 +1238,11 @@ static int cg_iterations_rb(ls_ctx* c, float* e, float* rs, float* p, float* q, 
 // r_b + N_br r_r, CG on S runs until the Schur residual has fallen by delta
 // (or its norm reaches the tolerance), then the red correction is
 // e_r = r_r + N_rb e_b, the full correction is added to the FP64 iterate and
-// the FP64 residual recomputed.  Needs the vectorised layout (nk % 4 == 0).
-static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep) {
+// the FP64 residual recomputed.  Needs the vectorised layout (nk % 4 == 0);
+// fused (nk % 8 == 0): the inner vectors are compacted (k_schur_apply).
+static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep, bool fused) {
     const uint32_t N = (uint32_t)c->N;
+    const size_t Nv = fused ? c->N / 2 : c->N;
     const int nbe = ew_blocks(N);
     const int bs = batch_size(c);
     const double delta = c->P.mixed_inner_rtol > 0 ? c->P.mixed_inner_rtol : 0.1;
@@ -981,16 +1281,18 @@ static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep) {
         // r (FP32 copy of the FP64 residual), e = 0; Schur right-hand side
         hipLaunchKernelGGL(k_inner_start, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->r64, c->r32,
                            c->p32, c->x32, N, 1.0 / scale, 0, c->partials);
-        const int nbs = launch_rb<3>(c, c->r32, c->r32, rs, nullptr);
+        const int nbs = fused ? launch_rb<5>(c, c->r32, c->r32, rs, nullptr)
+                              : launch_rb<3>(c, c->r32, c->r32, rs, nullptr);
         hipLaunchKernelGGL(k_inner_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nbs,
                            c->cg, 0.0, maxit, first ? 1 : 0);
         LS_LAUNCH_CHECK();
         if (first) {
-            LS_HIP(hipMemcpyAsync(c->p32, rs, c->N * sizeof(float), hipMemcpyDeviceToDevice,
+            LS_HIP(hipMemcpyAsync(c->p32, rs, Nv * sizeof(float), hipMemcpyDeviceToDevice,
                                   c->stream));
         } else {   // xlive = 0 here: only the direction carries over
-            hipLaunchKernelGGL(k_cg_direction_vec<float>, dim3(vec_blocks<float>(N)),
-                               dim3(EW_BLOCK), 0, c->stream, c->p32, rs, c->x32, c->cg, N);
+            hipLaunchKernelGGL(k_cg_direction_vec<float>, dim3(vec_blocks<float>((uint32_t)Nv)),
+                               dim3(EW_BLOCK), 0, c->stream, c->p32, rs, c->x32, c->cg,
+                               (uint32_t)Nv);
             LS_LAUNCH_CHECK();
         }
         if ((st = poll_cg(c))) return st;
@@ -1008,7 +1310,8 @@ static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep) {
         first = false;
         if (!(rrs <= trig)) {
             while (!c->h_cg->done) {
-                if ((st = cg_iterations_rb(c, c->x32, rs, c->p32, c->ap32, w, bs))) return st;
+                if ((st = cg_iterations_rb(c, fused, c->x32, rs, c->p32, c->ap32, w, bs)))
+                    return st;
                 if ((st = poll_cg(c))) return st;
                 record(c->h_cg->rr);
             }
@@ -1016,7 +1319,13 @@ static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep) {
         const int iters = c->h_cg->iter;
         const bool broke = !c->h_cg->converged && iters < maxit && !(rrs <= trig);
         // red reconstruction e_r = r_r + N_rb e_b into w, then x64 += scale w
-        launch_rb<4>(c, c->x32, c->r32, w, nullptr);
+        // (compacted e: expanded into ap32 first, q being dead here)
+        if (fused) {
+            launch_unpack(c, c->x32, c->ap32);
+            launch_rb<4>(c, c->ap32, c->r32, w, nullptr);
+        } else {
+            launch_rb<4>(c, c->x32, c->r32, w, nullptr);
+        }
         hipLaunchKernelGGL(k_refine_add, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->x64, w, N,
                            scale);
         LS_LAUNCH_CHECK();
@@ -1049,7 +1358,8 @@ static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep) {
 // The mixed solver of the context: red-black Schur inner CG when enabled and
 // the vectorised layout applies, else inner CG on the operator.
 static int solve_mixed_any(ls_ctx* c, ls_step_report* rep) {
-    if (c->P.mixed_schur && use_vec<float>(c)) return solve_mixed_rb(c, rep);
+    const int form = schur_form(c);
+    if (form) return solve_mixed_rb(c, rep, form == 1);
     return solve_mixed(c, rep);
 }
 
@@ -1103,6 +1413,58 @@ int apply_pressure_unscaled(ls_ctx* c, const double* x_dev, double* y_dev) {
     return 0;
 }
 
+int debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot) {
+    if (!c->press_assembled) {
+        c->err = "no pressure system assembled";
+        return -1;
+    }
+    SchurShape sh;
+    if (form == 1 && !schur_shape(c, &sh)) {
+        c->err = "the fused Schur apply needs nk % 8 == 0 and nk <= 2560";
+        return -3;
+    }
+    if (!use_vec<float>(c)) {
+        c->err = "the red-black passes need nk % 4 == 0";
+        return -3;
+    }
+    const size_t N = c->N, Nh = N / 2;
+    float* pc = (float*)c->tv[0][0];         // 2N floats per transport buffer
+    float* qc = pc + Nh;
+    LS_HIP(hipMemcpyAsync(pc, p, Nh * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    int nbs;
+    std::vector<float> full;
+    if (form == 1) {
+        nbs = launch_schur(c, pc, qc, nullptr);
+        LS_LAUNCH_CHECK();
+        LS_HIP(hipMemcpyAsync(q, qc, Nh * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    } else {
+        float* pf = (float*)c->tv[0][1];
+        float* wf = pf + N;
+        launch_unpack(c, pc, pf);
+        launch_rb<1>(c, pf, nullptr, wf, nullptr);
+        nbs = launch_rb<2>(c, wf, pf, c->ap32, nullptr);
+        LS_LAUNCH_CHECK();
+        full.resize(N);
+        LS_HIP(hipMemcpyAsync(full.data(), c->ap32, N * sizeof(float), hipMemcpyDeviceToHost,
+                              c->stream));
+    }
+    std::vector<double> part(nbs);
+    LS_HIP(hipMemcpyAsync(part.data(), c->partials, nbs * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    if (form == 2) {
+        const size_t nk = (size_t)c->nk, nj = (size_t)c->nj;
+        for (size_t v = 0; v < N; ++v) {
+            const size_t row = v / nk;
+            if (((row / nj + row % nj + v % nk) & 1) == 0) q[v >> 1] = full[v];
+        }
+    }
+    double a = 0.0;
+    for (double x : part) a += x;
+    *dot = a;
+    return 0;
+}
+
 int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) {
     if (!c->press_assembled) {
         c->err = "no pressure system assembled";
@@ -1112,6 +1474,11 @@ int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) 
         c->err = "the red-black passes need nk % 4 == 0";
         return -3;
     }
+    SchurShape sh;
+    if (precision == LS_TIME_SCHUR && !schur_shape(c, &sh)) {
+        c->err = "the fused Schur apply needs nk % 8 == 0 and nk <= 2560";
+        return -3;
+    }
     hipEvent_t e0 = c->ev[14], e1 = c->ev[15];
     for (int pass = 0; pass < 2; ++pass) {
         int n = pass == 0 ? 1 : reps;
@@ -1119,6 +1486,7 @@ int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) 
         for (int r = 0; r < n; ++r) {
             if (precision == LS_TIME_RB_GATHER) launch_rb<1>(c, c->p32, nullptr, c->ap32, nullptr);
             else if (precision == LS_TIME_RB_APPLY) launch_rb<2>(c, c->r32, c->p32, c->ap32, nullptr);
+            else if (precision == LS_TIME_SCHUR) launch_schur(c, c->p32, c->ap32, nullptr);
             else if (precision == 32) launch_stencil<float>(c, c->p32, c->ap32, c->c32, nullptr);
             else launch_stencil<double>(c, c->p64, c->ap64, c->c64, nullptr);
         }
@@ -1133,6 +1501,8 @@ int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) 
         *bytes = (double)c->N * 20.0;      // p, 3 faces, w
     } else if (precision == LS_TIME_RB_APPLY) {
         *bytes = (double)c->N * 24.0;      // w, 3 faces, p, q
+    } else if (precision == LS_TIME_SCHUR) {
+        *bytes = (double)c->N * 16.0;      // 3 faces; p, q compacted (half length)
     } else {
         const double sv = precision == 32 ? 4.0 : 8.0;
         *bytes = (double)c->N * (2.0 * sv + 3.0 * sv);

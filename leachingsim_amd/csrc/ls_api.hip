@@ -44,8 +44,8 @@ struct Timer {
 };
 
 void reset_prof(ls_ctx* c) {
-    c->prof_ms[0] = c->prof_ms[1] = 0.0;
-    c->prof_n[0] = c->prof_n[1] = 0;
+    c->prof_ms[0] = c->prof_ms[1] = c->prof_ms[2] = 0.0;
+    c->prof_n[0] = c->prof_n[1] = c->prof_n[2] = 0;
     c->prof_pending = 0;
 }
 
@@ -54,6 +54,9 @@ void report_prof(ls_ctx* c, ls_step_report* r) {
     r->prof_stencil32_ms = c->prof_ms[1];
     r->prof_stencil64_n = c->prof_n[0];
     r->prof_stencil32_n = c->prof_n[1];
+    r->prof_schur_ms = c->prof_ms[2];
+    r->prof_schur_n = c->prof_n[2];
+    r->prof_schur_bytes = c->prof_schur_bytes;
 }
 
 }  // namespace
@@ -600,11 +603,21 @@ int ls_debug_apply_pressure(ls_ctx* c, const double* x, double* y) {
     return 0;
 }
 
+int ls_debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot) {
+    if (!c || !p || !q || !dot) return -1;
+    if (form != 1 && form != 2) {
+        c->err = "ls_debug_schur_apply: form must be 1 (fused) or 2 (two passes)";
+        return -3;
+    }
+    return debug_schur_apply(c, form, p, q, dot);
+}
+
 int ls_time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) {
     if (!c || !ms || !bytes || reps < 1) return -1;
     if (precision != 32 && precision != 64 && precision != LS_TIME_RB_GATHER &&
-        precision != LS_TIME_RB_APPLY) {
-        c->err = "ls_time_stencil: precision must be 64, 32, LS_TIME_RB_GATHER or LS_TIME_RB_APPLY";
+        precision != LS_TIME_RB_APPLY && precision != LS_TIME_SCHUR) {
+        c->err = "ls_time_stencil: precision must be 64, 32, LS_TIME_RB_GATHER, LS_TIME_RB_APPLY "
+                 "or LS_TIME_SCHUR";
         return -3;
     }
     return time_stencil(c, precision, reps, ms, bytes);

@@ -146,9 +146,12 @@ struct ls_ctx {
     hipEvent_t ev[16];
     // sampled stencil timing (profiling on): events 12/13 bracket one launch
     int profiling = 0;
-    int prof_pending = 0;          // 64 or 32 when an event pair awaits reading
-    double prof_ms[2] = {0.0, 0.0};
-    int prof_n[2] = {0, 0};
+    // prof_pending: 64 / 32 (FP64 / FP32 stencil) or 16 (Schur apply) when an
+    // event pair awaits reading; slots 0, 1, 2 accumulate them
+    int prof_pending = 0;
+    double prof_ms[3] = {0.0, 0.0, 0.0};
+    int prof_n[3] = {0, 0, 0};
+    double prof_schur_bytes = 0.0; // algorithmic bytes of the sampled Schur kernel
     std::vector<void*> allocations;
 };
 
@@ -172,6 +175,7 @@ int zero_flux_rock_precip(ls_ctx* c);
 int solve_pressure(ls_ctx* c, double t, ls_step_report* rep);
 int apply_pressure_unscaled(ls_ctx* c, const double* x_dev, double* y_dev);
 int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes);
+int debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot);
 int learned_setup(ls_ctx* c);
 void learned_free(ls_ctx* c);
 int learned_pack_debug(ls_ctx* c, const float* r_dev, float* out_dev);

@@ -76,7 +76,8 @@ class StepReport(C.Structure):
         ("prof_stencil64_n", C.c_int), ("prof_stencil32_n", C.c_int),
         ("press_relres0", C.c_double), ("warm_basis", C.c_int), ("warm_net", C.c_int),
         ("sat_iters", C.c_int), ("sat_valid", C.c_int), ("sat_raw_min", C.c_double),
-        ("sat_raw_max", C.c_double),
+        ("sat_raw_max", C.c_double), ("prof_schur_ms", C.c_double), ("prof_schur_n", C.c_int),
+        ("prof_schur_bytes", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -138,6 +139,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "ls_debug_learned_pack": (i, [vp, vp, vp, sz]),
         "ls_debug_pressure_system": (i, [vp, vp, vp, vp]),
         "ls_debug_pressure_faces": (i, [vp, vp]),
+        "ls_debug_schur_apply": (i, [vp, i, vp, vp, C.POINTER(d)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -337,6 +339,17 @@ class GpuSimulation:
         f = np.empty((3,) + self.shape)
         self._check(self.lib.ls_debug_pressure_faces(self.h, f.ctypes.data))
         return f
+
+    def schur_apply(self, p_black, form: int = 1):
+        """q = S p on a vector compacted to the black voxels (N/2 float32,
+        black voxel v at v >> 1) with the dot p.q; form 1 fused, 2 two-pass."""
+        a = np.ascontiguousarray(p_black, dtype=np.float32)
+        assert a.size * 2 == int(np.prod(self.shape))
+        q = np.empty_like(a)
+        dot = C.c_double()
+        self._check(self.lib.ls_debug_schur_apply(self.h, form, a.ctypes.data, q.ctypes.data,
+                                                  C.byref(dot)))
+        return q, dot.value
 
     def set_profiling(self, on: bool):
         self._check(self.lib.ls_set_profiling(self.h, 1 if on else 0))

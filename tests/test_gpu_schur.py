@@ -3,7 +3,12 @@ plain FP64 PCG (to the solve tolerance) in about half the iterations of the
 inner CG on the operator itself.  The factor is fixed by the mathematics:
 the scaled operator is I - N with N 2-cyclic (the 7-point graph is
 bipartite), so S = I - N_br N_rb has cond(S) ~ cond(A^) / 4; a numpy
-emulation on the oracle's operator of a 96^3 workload gave 1016 -> 508."""
+emulation on the oracle's operator of a 96^3 workload gave 1016 -> 508.
+
+The operator apply q = S p exists twice: fused on vectors compacted to the
+black voxels (nk % 8 == 0) and as gather + apply passes on full-length
+vectors.  Both sum the same products in the same order, so they agree bit
+for bit; numpy (float64, from the FP32 faces) checks the mathematics."""
 import numpy as np
 import pytest
 
@@ -26,23 +31,78 @@ def solve(img, t, **kw):
     return rep, p
 
 
+def schur_numpy(faces32, p_black, shape):
+    """S p = p - N_br N_rb p in float64 on the compacted black vector."""
+    ni, nj, nk = shape
+    ii, jj, kk = np.meshgrid(np.arange(ni), np.arange(nj), np.arange(nk), indexing="ij")
+    black = ((ii + jj + kk) % 2 == 0).reshape(-1)
+    v = np.arange(ni * nj * nk)
+    cx, cy, cz = (f.astype(np.float64) for f in faces32)
+
+    def nmul(x):
+        X = x.reshape(shape)
+        y = np.zeros(shape)
+        y[:-1] += cx[:-1] * X[1:]
+        y[1:] += cx[:-1] * X[:-1]
+        y[:, :-1] += cy[:, :-1] * X[:, 1:]
+        y[:, 1:] += cy[:, :-1] * X[:, :-1]
+        y[:, :, :-1] += cz[:, :, :-1] * X[:, :, 1:]
+        y[:, :, 1:] += cz[:, :, :-1] * X[:, :, :-1]
+        return y.reshape(-1)
+
+    pf = np.zeros(v.size)
+    pf[black] = p_black[v[black] >> 1]
+    w = nmul(pf)
+    w[black] = 0.0
+    qf = pf - nmul(w)
+    return qf[black][np.argsort(v[black] >> 1)], float(pf[black] @ qf[black])
+
+
+@pytest.mark.parametrize("shape", [(24, 20, 40), (19, 33, 16), (5, 7, 1040), (34, 30, 512)])
+def test_fused_schur_apply(shape):
+    """Ragged tiles: nj not a multiple of the block's rows, ni not a
+    multiple of its planes, rows spanning 3 waves (nk = 1040)."""
+    img = small_mixed_geometry(shape, 3)
+    P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED)
+    g = L.GpuSimulation(shape, P)
+    g.read_labels(img)
+    g.init_state()
+    g.phase(L.PH_PROPERTIES)
+    g.phase(L.PH_PRESSURE, 1500.0)
+    p = np.random.default_rng(sum(shape)).standard_normal(img.size // 2).astype(np.float32)
+    print("shape", shape, flush=True)
+    q1, d1 = g.schur_apply(p, 1)
+    q2, d2 = g.schur_apply(p, 2)
+    faces = g.pressure_faces().astype(np.float32)
+    g.close()
+    assert np.array_equal(q1, q2)
+    assert abs(d1 - d2) <= 1e-10 * abs(d2)
+    qr, dr = schur_numpy(faces, p, shape)
+    assert np.abs(q1 - qr).max() <= 1e-5 * np.abs(qr).max()
+    assert abs(d1 - dr) <= 1e-5 * abs(dr)
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_schur_matches_plain_in_fewer_iterations(seed):
-    img = G.smoothed_threshold((40, 36, 48), 2.0, 0.4, seed)     # nk % 4 == 0: vector path
+    img = G.smoothed_threshold((40, 36, 48), 2.0, 0.4, seed)     # nk % 8 == 0: fused form
     r64, p64 = solve(img, 1500.0, solver=L.SOLVER_PLAIN64)
     rs, ps = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=1)
+    r2, p2 = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=2)
     rm, pm = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=0)
-    for r in (r64, rs, rm):
+    for r in (r64, rs, r2, rm):
         assert r.status == 0 and r.press_relres <= 1e-12
-    assert relerr(ps, p64) <= 1e-7 and relerr(pm, p64) <= 1e-7
+    assert relerr(ps, p64) <= 1e-7 and relerr(pm, p64) <= 1e-7 and relerr(p2, p64) <= 1e-7
     print(f"iterations: plain FP64 {r64.press_iters}, mixed {rm.press_iters}, "
-          f"mixed Schur {rs.press_iters}")
+          f"mixed Schur fused {rs.press_iters}, two-pass {r2.press_iters}")
     assert rs.press_iters <= 0.7 * rm.press_iters
+    # same operator bit for bit; only the dot products' summation order differs
+    assert abs(rs.press_iters - r2.press_iters) <= max(3, 0.03 * r2.press_iters)
 
 
-def test_schur_coupled_steps_match_oracle():
+@pytest.mark.parametrize("nk", [28, 32])     # two-pass form, fused form
+def test_schur_coupled_steps_match_oracle(nk):
     from tests.helpers import both
-    img = small_mixed_geometry((24, 20, 28), 9)
+    img = small_mixed_geometry((24, 20, nk), 9)
     P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED, mixed_schur=1,
                          press_rtol=1e-12)
     g, o = both(img.shape, P, img)
