@@ -128,7 +128,8 @@ typedef struct ls_params {
                                 Schur complement of the scaled operator (about
                                 half the iterations of Jacobi-PCG), on vectors
                                 compacted to the black voxels with one fused
-                                pass per operator apply when nk % 8 == 0; 2: the
+                                pass per operator apply when nk % 8 == 0 and
+                                nk <= 1024 (else as 2); 2: the
                                 same on full-length vectors in two passes; both
                                 need nk % 4 == 0; 0: inner CG on the operator  */
 } ls_params;
@@ -275,7 +276,7 @@ int ls_debug_pressure_faces(ls_ctx* ctx, double* faces);
 /* q = S p with S = I - N_br N_rb, the red-black Schur complement of the
  * last assembled FP32 scaled operator, on host vectors compacted to the black
  * voxels (N/2 floats: black voxel v at index v >> 1), with *dot = p.q.
- * form 1: the fused kernel (nk % 8 == 0); form 2: the gather and apply passes
+ * form 1: the fused kernel (nk % 8 == 0, nk <= 1024); form 2: the gather and apply passes
  * on full-length vectors (nk % 4 == 0). */
 int ls_debug_schur_apply(ls_ctx* ctx, int form, const float* p, float* q, double* dot);
 /* Time `reps` launches of a pressure kernel on the context's stream with HIP

@@ -380,15 +380,17 @@ static RbGeo rbgeo(const ls_ctx* c) {
 }
 
 #ifndef LS_SCHUR_ROWS
-#define LS_SCHUR_ROWS 14       // TJ: rows per block (plus 2 halo rows)
+#define LS_SCHUR_ROWS 6        // TJ: rows per block (plus 2 halo rows)
 #endif
 #ifndef LS_SCHUR_ICH
 #define LS_SCHUR_ICH 16        // planes per block
 #endif
 #ifndef LS_SCHUR_THREADS
-#define LS_SCHUR_THREADS 1024  // launch bound: 64 x WR x (TJ + 2) at most
+#define LS_SCHUR_THREADS 512   // launch bound: 64 x WR x (TJ + 2) at most
 #endif
 constexpr int SCHUR_ICH = LS_SCHUR_ICH;
+// Widest row: a block holds WR waves per row and at least 3 rows.
+constexpr int SCHUR_MAX_NK = 8 * WAVE * ((LS_SCHUR_THREADS / WAVE) / 3);
 
 typedef float F4 __attribute__((ext_vector_type(4)));
 
@@ -1153,7 +1155,7 @@ struct SchurShape {
     size_t lds;
 };
 
-static bool schur_shape(const ls_ctx* c, SchurShape* sh) {
+static bool schur_shape(ls_ctx* c, SchurShape* sh) {
     if (c->nk % 8 != 0) return false;
     const int nh = c->nk / 2;
     const int wr = (nh + 4 * WAVE - 1) / (4 * WAVE);
@@ -1162,16 +1164,16 @@ static bool schur_shape(const ls_ctx* c, SchurShape* sh) {
     if (tj < 1) return false;
     sh->wr = wr;
     sh->tj = tj;
-    sh->grid = dim3((c->nj + tj - 1) / tj, (c->ni + SCHUR_ICH - 1) / SCHUR_ICH, 1);
     sh->block = dim3(WAVE, wr, tj + 2);
     sh->lds = (size_t)3 * (tj + 2) * wr * 4 * WAVE * sizeof(float);
+    sh->grid = dim3((c->nj + tj - 1) / tj, (c->ni + SCHUR_ICH - 1) / SCHUR_ICH, 1);
     return (size_t)sh->grid.x * sh->grid.y <= (size_t)MAX_PARTIALS &&
            2 * (size_t)vec_blocks<float>((uint32_t)(c->N / 2)) <= (size_t)MAX_PARTIALS;
 }
 
 // The mixed solver's Schur form: 0 none, 1 fused on compacted vectors,
 // 2 two passes on full-length vectors.
-static int schur_form(const ls_ctx* c) {
+static int schur_form(ls_ctx* c) {
     if (!c->P.mixed_schur || !use_vec<float>(c)) return 0;
     SchurShape sh;
     if (c->P.mixed_schur == 1 && schur_shape(c, &sh)) return 1;
@@ -1420,7 +1422,7 @@ int debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot
     }
     SchurShape sh;
     if (form == 1 && !schur_shape(c, &sh)) {
-        c->err = "the fused Schur apply needs nk % 8 == 0 and nk <= 2560";
+        c->err = "the fused Schur apply needs nk % 8 == 0 and nk <= " + std::to_string(SCHUR_MAX_NK);
         return -3;
     }
     if (!use_vec<float>(c)) {
@@ -1476,7 +1478,7 @@ int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) 
     }
     SchurShape sh;
     if (precision == LS_TIME_SCHUR && !schur_shape(c, &sh)) {
-        c->err = "the fused Schur apply needs nk % 8 == 0 and nk <= 2560";
+        c->err = "the fused Schur apply needs nk % 8 == 0 and nk <= " + std::to_string(SCHUR_MAX_NK);
         return -3;
     }
     hipEvent_t e0 = c->ev[14], e1 = c->ev[15];

@@ -58,10 +58,12 @@ def schur_numpy(faces32, p_black, shape):
     return qf[black][np.argsort(v[black] >> 1)], float(pf[black] @ qf[black])
 
 
-@pytest.mark.parametrize("shape", [(24, 20, 40), (19, 33, 16), (5, 7, 1040), (34, 30, 512)])
+@pytest.mark.parametrize("shape", [(24, 20, 40), (19, 33, 16), (5, 7, 1024), (34, 30, 512),
+                                   (61, 700, 16), (97, 1300, 8)])
 def test_fused_schur_apply(shape):
     """Ragged tiles: nj not a multiple of the block's rows, ni not a
-    multiple of its planes, rows spanning 3 waves (nk = 1040)."""
+    multiple of its planes, rows spanning 2 waves (nk = 1024), grids of
+    thousands of blocks."""
     img = small_mixed_geometry(shape, 3)
     P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED)
     g = L.GpuSimulation(shape, P)
@@ -80,6 +82,26 @@ def test_fused_schur_apply(shape):
     qr, dr = schur_numpy(faces, p, shape)
     assert np.abs(q1 - qr).max() <= 1e-5 * np.abs(qr).max()
     assert abs(d1 - dr) <= 1e-5 * abs(dr)
+
+
+def test_fused_schur_rejects_wide_rows():
+    """Rows wider than the fused kernel's block (nk > 1024): the debug
+    entry refuses form 1, the solver runs the two-pass form."""
+    shape = (4, 4, 1040)
+    img = small_mixed_geometry(shape, 3, air_slabs=False)
+    P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED)
+    g = L.GpuSimulation(shape, P)
+    g.read_labels(img)
+    g.init_state()
+    g.phase(L.PH_PROPERTIES)
+    rep = g.phase(L.PH_PRESSURE, 1500.0)
+    assert rep.status == 0
+    p = np.ones(img.size // 2, np.float32)
+    with pytest.raises(L.LsError, match="nk <= 1024"):
+        g.schur_apply(p, 1)
+    q2, _ = g.schur_apply(p, 2)
+    assert np.all(np.isfinite(q2))
+    g.close()
 
 
 @pytest.mark.parametrize("seed", [1, 2])
