@@ -14,6 +14,7 @@
 // setupSaturationEqns + solveSaturation, src/Simulation.cpp:1858-2053) with
 // its own matrix-free row builder, as system 0 alone.
 #include <cmath>
+#include <type_traits>
 
 #include "ls_context.h"
 
@@ -371,6 +372,179 @@ __global__ void k_adr_matvec(const uint8_t* __restrict__ img, const double* __re
     }
 }
 
+// ---------------------------------------------------------------------------
+// Stored rows.  The matrix is fixed for a whole solve, so its rows are built
+// once (the row builder above, scalar) into 7 arrays of N doubles: the
+// diagonal and the six face coefficients in face order (+i, -i, +j, -j, +k,
+// -k), 0 where a face is not linked.  The matvec then streams them: a
+// vectorised i-march like the pressure stencil (2 doubles of one k-row per
+// lane, x[i-1], x[i], x[i+1] in registers, k neighbours by shuffles),
+// 7 x 8 B of rows + 2 x (x, y, w) = 104 B per voxel, against ~73 B and a
+// scalar, branchy row rebuild per matvec.  Terms are added in the row
+// builder's order, so y is the same as the matrix-free matvec's.
+// ---------------------------------------------------------------------------
+template <int KIND>
+__global__ __launch_bounds__(EW_BLOCK) void k_adr_rows(
+    const uint8_t* __restrict__ img, const double* __restrict__ qx,
+    const double* __restrict__ qy, const double* __restrict__ qz,
+    const double* __restrict__ sf, TGeo g, uint32_t N, AdrConst a,
+    double* __restrict__ rows) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        int i, j, k;
+        tdecode(v, g, i, j, k);
+        AdrRow R;
+        if (KIND == 1) {
+            double unused;
+            sat_row(img, qx, qy, qz, sf, g, i, j, k, v, a, R, unused);
+        } else {
+            adr_row(img, qx, qy, qz, g, i, j, k, v, a, R);
+        }
+        rows[v] = R.diag;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            rows[(size_t)(f + 1) * N + v] = (R.mask & (1 << f)) ? R.off[f] : 0.0;
+    }
+}
+
+constexpr int ADR_TJ = 4, ADR_ICH = 8;
+
+static dim3 adr_vec_grid(const ls_ctx* c) {
+    return dim3((c->nk / 2 + WAVE - 1) / WAVE, (c->nj + ADR_TJ - 1) / ADR_TJ,
+                (c->ni + ADR_ICH - 1) / ADR_ICH);
+}
+
+// y = A x for both systems from the stored rows; partials of rh.y (MODE 0)
+// or (y.y, y.s) (MODE 1), as k_adr_matvec.
+template <int MODE>
+__global__ __launch_bounds__(WAVE * ADR_TJ) void k_adr_matvec_rows(
+    const double* __restrict__ rows, const double* __restrict__ x0,
+    const double* __restrict__ x1, double* __restrict__ y0, double* __restrict__ y1,
+    const double* __restrict__ w0, const double* __restrict__ w1, TGeo g, uint32_t N,
+    const BicgScalars* __restrict__ bs, double* __restrict__ partials) {
+    if (bs->all_done) return;
+    typedef double D2 __attribute__((ext_vector_type(2)));
+    __shared__ double lds[4 * ADR_TJ];
+    const bool a0 = !bs->done[0], a1 = !bs->done[1];
+    const int nbx = gridDim.x, nby = gridDim.y;
+    const int nb = nbx * nby * gridDim.z;
+    const int hw = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    const int t = xcd_remap(hw, nb);
+    const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
+    const int lane = threadIdx.x;
+    const int k0 = (bx * WAVE + lane) * 2;
+    const int j = by * ADR_TJ + threadIdx.y;
+    const int i0 = bz * ADR_ICH, i1 = min(i0 + ADR_ICH, g.ni);
+    const bool ok = j < g.nj && k0 < g.nk;
+    const bool hj = j > 0, pj = j + 1 < g.nj;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    const D2 zero = D2(0.0);
+    uint32_t v = ok ? (uint32_t)i0 * g.si + (uint32_t)j * g.sj + (uint32_t)k0 : 0u;
+    D2 am = zero, bm = zero, ac = zero, bc = zero;
+    if (ok && i0 > 0) {
+        am = *(const D2*)(x0 + v - g.si);
+        bm = *(const D2*)(x1 + v - g.si);
+    }
+    if (ok) {
+        ac = *(const D2*)(x0 + v);
+        bc = *(const D2*)(x1 + v);
+    }
+    for (int i = i0; i < i1; ++i, v += g.si) {
+        D2 ap = zero, bp = zero, ajm = zero, bjm = zero, ajp = zero, bjp = zero;
+        D2 r[7];
+#pragma unroll
+        for (int f = 0; f < 7; ++f) r[f] = zero;
+        double la = 0.0, lb = 0.0, ra = 0.0, rb = 0.0;
+        if (ok) {
+            if (i + 1 < g.ni) {
+                ap = *(const D2*)(x0 + v + g.si);
+                bp = *(const D2*)(x1 + v + g.si);
+            }
+            if (hj) {
+                ajm = *(const D2*)(x0 + v - g.sj);
+                bjm = *(const D2*)(x1 + v - g.sj);
+            }
+            if (pj) {
+                ajp = *(const D2*)(x0 + v + g.sj);
+                bjp = *(const D2*)(x1 + v + g.sj);
+            }
+#pragma unroll
+            for (int f = 0; f < 7; ++f) r[f] = *(const D2*)(rows + (size_t)f * N + v);
+            if (lane == 0 && k0 > 0) {
+                la = x0[v - 1];
+                lb = x1[v - 1];
+            }
+            if (lane == WAVE - 1 && k0 + 2 < g.nk) {
+                ra = x0[v + 2];
+                rb = x1[v + 2];
+            }
+        }
+        const double sla = __shfl_up(ac[1], 1, WAVE), slb = __shfl_up(bc[1], 1, WAVE);
+        const double sra = __shfl_down(ac[0], 1, WAVE), srb = __shfl_down(bc[0], 1, WAVE);
+        if (lane != 0) { la = sla; lb = slb; }
+        if (lane != WAVE - 1) { ra = sra; rb = srb; }
+        D2 ya, yb;
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            const double akp = w == 0 ? ac[1] : ra, akm = w == 0 ? la : ac[0];
+            const double bkp = w == 0 ? bc[1] : rb, bkm = w == 0 ? lb : bc[0];
+            double u = r[0][w] * ac[w];
+            u += r[1][w] * ap[w];
+            u += r[2][w] * am[w];
+            u += r[3][w] * ajp[w];
+            u += r[4][w] * ajm[w];
+            u += r[5][w] * akp;
+            u += r[6][w] * akm;
+            ya[w] = u;
+            double z = r[0][w] * bc[w];
+            z += r[1][w] * bp[w];
+            z += r[2][w] * bm[w];
+            z += r[3][w] * bjp[w];
+            z += r[4][w] * bjm[w];
+            z += r[5][w] * bkp;
+            z += r[6][w] * bkm;
+            yb[w] = z;
+        }
+        if (ok) {
+            if (a0) *(D2*)(y0 + v) = ya;
+            if (a1) *(D2*)(y1 + v) = yb;
+            const D2 wa = *(const D2*)(w0 + v), wb = *(const D2*)(w1 + v);
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                if (MODE == 0) {
+                    s0 += wa[w] * ya[w];
+                    s1 += wb[w] * yb[w];
+                } else {
+                    s0 += ya[w] * ya[w];
+                    s1 += yb[w] * yb[w];
+                    s2 += ya[w] * wa[w];
+                    s3 += yb[w] * wb[w];
+                }
+            }
+        }
+        am = ac;
+        bm = bc;
+        ac = ap;
+        bc = bp;
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    s3 = wave_sum(s3);
+    if (lane == 0) {
+        lds[4 * threadIdx.y + 0] = s0;
+        lds[4 * threadIdx.y + 1] = s1;
+        lds[4 * threadIdx.y + 2] = s2;
+        lds[4 * threadIdx.y + 3] = s3;
+    }
+    __syncthreads();
+    if (flat_tid() == 0) {
+        double q[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int y = 0; y < ADR_TJ; ++y)
+            for (int u = 0; u < 4; ++u) q[u] += lds[4 * y + u];
+        for (int u = 0; u < 4; ++u) partials[(size_t)u * nb + t] = q[u];
+    }
+}
+
 __global__ void k_bicg_alpha(const double* __restrict__ partials, int n, BicgScalars* bs) {
     if (bs->all_done) return;
     __shared__ double lds[1024 / WAVE];
@@ -561,6 +735,35 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     const int batch = 4;
     int restarts = 0;
     bool first = true;
+    // stored rows (nk even, 7 N doubles allocated on first use)
+    const dim3 vg = adr_vec_grid(c);
+    const int nbm = (int)(vg.x * vg.y * vg.z);
+    bool stored = c->nk % 2 == 0 && 4 * (size_t)nbm <= 8 * (size_t)MAX_PARTIALS;
+    if (stored && c->arows == nullptr) {
+        if (hipMalloc((void**)&c->arows, 7 * c->N * sizeof(double)) == hipSuccess)
+            c->allocations.push_back(c->arows);
+        else
+            c->arows = nullptr;
+    }
+    stored = stored && c->arows != nullptr;
+    if (stored) {
+        hipLaunchKernelGGL((k_adr_rows<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img,
+                           c->qx, c->qy, c->qz, sf, g, N, a, c->arows);
+        LS_LAUNCH_CHECK();
+    }
+    auto matvec = [&](auto MC, const double* xa, const double* xb, double* ya, double* yb,
+                      const double* wa, const double* wb) {
+        constexpr int M = decltype(MC)::value;
+        if (stored) {
+            hipLaunchKernelGGL((k_adr_matvec_rows<M>), vg, dim3(WAVE, ADR_TJ), 0, c->stream,
+                               c->arows, xa, xb, ya, yb, wa, wb, g, N, c->bicg, c->partials);
+        } else {
+            hipLaunchKernelGGL((k_adr_matvec<M, KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
+                               c->img, c->qx, c->qy, c->qz, sf, xa, xb, ya, yb, wa, wb, g, N, a,
+                               c->bicg, c->partials);
+        }
+        return stored ? nbm : nb;
+    };
     while (true) {
         hipLaunchKernelGGL((k_adr_start<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img,
                            c->qx, c->qy, c->qz, sf, x[0], x[1], c->tb[0], c->tb[1], c->tinvd,
@@ -586,21 +789,19 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             for (int it = 0; it < batch; ++it) {
                 hipLaunchKernelGGL(k_bicg_p, dim3(nb), dim3(EW_BLOCK), 0, c->stream, pp[0], pp[1],
                                    r[0], r[1], vv[0], vv[1], ph[0], ph[1], c->tinvd, c->bicg, N);
-                hipLaunchKernelGGL((k_adr_matvec<0, KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
-                                   c->img, c->qx, c->qy, c->qz, sf, ph[0], ph[1], vv[0], vv[1],
-                                   rh[0], rh[1], g, N, a, c->bicg, c->partials);
+                const int n0 = matvec(std::integral_constant<int, 0>{}, ph[0], ph[1], vv[0],
+                                      vv[1], rh[0], rh[1]);
                 hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1024), 0, c->stream, c->partials,
-                                   nb, c->bicg);
+                                   n0, c->bicg);
                 hipLaunchKernelGGL(k_bicg_s, dim3(nb), dim3(EW_BLOCK), 0, c->stream, r[0], r[1],
                                    vv[0], vv[1], s[0], s[1], sh[0], sh[1], c->tinvd, c->bicg, N,
                                    c->partials);
                 hipLaunchKernelGGL(k_bicg_scheck, dim3(1), dim3(1024), 0, c->stream, c->partials,
                                    nb, c->bicg);
-                hipLaunchKernelGGL((k_adr_matvec<1, KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
-                                   c->img, c->qx, c->qy, c->qz, sf, sh[0], sh[1], t[0], t[1], s[0],
-                                   s[1], g, N, a, c->bicg, c->partials);
+                const int n1 = matvec(std::integral_constant<int, 1>{}, sh[0], sh[1], t[0], t[1],
+                                      s[0], s[1]);
                 hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1024), 0, c->stream, c->partials,
-                                   nb, c->bicg);
+                                   n1, c->bicg);
                 hipLaunchKernelGGL(k_bicg_x, dim3(nb), dim3(EW_BLOCK), 0, c->stream, x[0], x[1],
                                    r[0], r[1], ph[0], ph[1], sh[0], sh[1], s[0], s[1], t[0], t[1],
                                    rh[0], rh[1], c->bicg, N, c->partials);

@@ -107,6 +107,7 @@ struct ls_ctx {
     double* tv[2][8];   // r, r0, p, v, s, t, ph, sh  for conc and acid
     double* tb[2];      // right-hand sides
     double* tinvd = nullptr;
+    double* arows = nullptr;   // stored transport rows, 7 x N (allocated on first use)
     // --- reductions ---
     double* partials = nullptr;     // MAX_PARTIALS * 8 values
     CgScalars* cg = nullptr;
