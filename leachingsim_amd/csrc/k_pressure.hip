@@ -389,53 +389,67 @@ static RbGeo rbgeo(const ls_ctx* c) {
 #define LS_SCHUR_THREADS 512   // launch bound: 64 x WR x (TJ + 2) at most
 #endif
 constexpr int SCHUR_ICH = LS_SCHUR_ICH;
-// Widest row: a block holds WR waves per row and at least 3 rows.
-constexpr int SCHUR_MAX_NK = 8 * WAVE * ((LS_SCHUR_THREADS / WAVE) / 3);
 
 typedef float F4 __attribute__((ext_vector_type(4)));
+template <int N> struct FVec { typedef float type __attribute__((ext_vector_type(N))); };
 
-// Colour split of 8 consecutive values (a0, a1) of one row whose first
-// element has colour par (0 black): b[x] = element 2x + par, r[x] = element
-// 2x + 1 - par.
-__device__ __forceinline__ void csplit(F4 a0, F4 a1, int par, F4& b, F4& r) {
-    const F4 e = F4{a0[0], a0[2], a1[0], a1[2]};
-    const F4 o = F4{a0[1], a0[3], a1[1], a1[3]};
+#ifndef LS_SCHUR_VW
+#define LS_SCHUR_VW 4          // compacted values per lane (2 or 4)
+#endif
+constexpr int SCHUR_VW = LS_SCHUR_VW;
+// Widest row: a block holds WR waves per row and at least 3 rows.
+constexpr int SCHUR_MAX_NK = 2 * SCHUR_VW * WAVE * ((LS_SCHUR_THREADS / WAVE) / 3);
+
+// Colour split of the 2 VW consecutive values a of one row whose first
+// element has colour par (0 black): b[x] = a[2x + par], r[x] = a[2x + 1 - par].
+template <int VW>
+__device__ __forceinline__ void csplit(const typename FVec<2 * VW>::type& a, int par,
+                                       typename FVec<VW>::type& b, typename FVec<VW>::type& r) {
+    typename FVec<VW>::type e, o;
+#pragma unroll
+    for (int x = 0; x < VW; ++x) {
+        e[x] = a[2 * x];
+        o[x] = a[2 * x + 1];
+    }
     b = par ? o : e;
     r = par ? e : o;
 }
 
-template <int ICH>
+template <int ICH, int VW>
 __global__ __launch_bounds__(LS_SCHUR_THREADS) void k_schur_apply(
     const float* __restrict__ p, float* __restrict__ q, const float* __restrict__ cx,
     const float* __restrict__ cy, const float* __restrict__ cz, RbGeo g,
     double* __restrict__ partials, const int* __restrict__ done) {
+    typedef typename FVec<VW>::type V;          // compacted values of one lane
+    typedef typename FVec<2 * VW>::type VF;     // the lane's full-row values (faces)
     if (done != nullptr && *done) return;
     extern __shared__ float wsh[];                   // [3][NS][rowlen]
     __shared__ double red[LS_SCHUR_THREADS / WAVE];
     const int lane = threadIdx.x, wr = threadIdx.y, s = threadIdx.z;
     const int NS = blockDim.z, TJ = NS - 2;
-    const int rowlen = blockDim.y * (WAVE * 4);
+    const int rowlen = blockDim.y * (WAVE * VW);
     const int bstride = NS * rowlen;
     const int nbj = gridDim.x;
     const int t = xcd_remap(blockIdx.x + nbj * blockIdx.y, nbj * gridDim.y);
     const int j = (t % nbj) * TJ - 1 + s;
     const int i0 = (t / nbj) * ICH, i1 = min(i0 + ICH, g.ni);
-    const int mloc = (wr * WAVE + lane) * 4;
+    const int mloc = (wr * WAVE + lane) * VW;
     const bool ok = j >= 0 && j < g.nj && mloc < g.nh;
     const bool halo = s == 0 || s == NS - 1;
     const bool hj = j > 0, pj = j + 1 < g.nj;
-    const bool hl = mloc > 0, hr = mloc + 4 < g.nh;
+    const bool hl = mloc > 0, hr = mloc + VW < g.nh;
     const uint32_t vrow = ok ? (uint32_t)j * g.sj + 2u * (uint32_t)mloc : 0u;
     const uint32_t hrow = ok ? (uint32_t)j * g.hsj + (uint32_t)mloc : 0u;
     float* const lrow = wsh + s * rowlen + mloc;
-    const F4 zero = F4(0.0f);
-    F4 Pm = zero, Pc = zero;                 // p of planes i - 1, i (own row)
-    F4 Wmm = zero, Wm = zero;                // w of planes i - 2, i - 1 (own row)
-    F4 cxB1 = zero, cxR1 = zero, cxR2 = zero;    // cx: plane i - 1 (black, red), i - 2 (red)
-    F4 cyB1 = zero, cymB1 = zero, czB1 = zero, czR1 = zero;   // plane i - 1
+    const V zero = V(0.0f);
+    const VF zf = VF(0.0f);
+    V Pm = zero, Pc = zero;                 // p of planes i - 1, i (own row)
+    V Wmm = zero, Wm = zero;                // w of planes i - 2, i - 1 (own row)
+    V cxB1 = zero, cxR1 = zero, cxR2 = zero;    // cx: plane i - 1 (black, red), i - 2 (red)
+    V cyB1 = zero, cymB1 = zero, czB1 = zero, czR1 = zero;   // plane i - 1
     float czL1 = 0.0f;
     double acc = 0.0;
-    if (ok && i0 >= 2) Pc = *(const F4*)(p + (uint32_t)(i0 - 2) * g.hsi + hrow);
+    if (ok && i0 >= 2) Pc = *(const V*)(p + (uint32_t)(i0 - 2) * g.hsi + hrow);
 #pragma unroll
     for (int st = 0; st < ICH + 3; ++st) {
         const int i = i0 - 2 + st;
@@ -447,78 +461,72 @@ __global__ __launch_bounds__(LS_SCHUR_THREADS) void k_schur_apply(
         // h and v wrap for the planes i < 0 (never dereferenced there);
         // plane i + 1 is indexed from its own unsigned offset
         const uint32_t hn = h + g.hsi;
-        F4 Pn = zero;
-        if (ok && i + 1 >= 0 && i + 1 < g.ni) Pn = *(const F4*)(p + hn);
-        F4 cx0 = zero, cx1 = zero;
-        if (inp) {
-            cx0 = *(const F4*)(cx + v);
-            cx1 = *(const F4*)(cx + v + 4);
-        }
-        F4 cxB, cxR;
-        csplit(cx0, cx1, par, cxB, cxR);
-        F4 cyB = zero, cymB = zero, czB = zero, czR = zero;
+        V Pn = zero;
+        if (ok && i + 1 >= 0 && i + 1 < g.ni) Pn = *(const V*)(p + hn);
+        VF cxv = zf;
+        if (inp) cxv = *(const VF*)(cx + v);
+        V cxB, cxR;
+        csplit<VW>(cxv, par, cxB, cxR);
+        V cyB = zero, cymB = zero, czB = zero, czR = zero;
         float czL = 0.0f;
         if (st >= 1) {
-            F4 cy0 = zero, cy1 = zero, cz0 = zero, cz1 = zero, cm0 = zero, cm1 = zero;
-            F4 Pjm = zero, Pjp = zero;
+            VF cyv = zf, czv = zf, cmv = zf;
+            V Pjm = zero, Pjp = zero;
             float pL = 0.0f, pR = 0.0f, czl = 0.0f;
             if (inp) {
-                cy0 = *(const F4*)(cy + v);
-                cy1 = *(const F4*)(cy + v + 4);
-                cz0 = *(const F4*)(cz + v);
-                cz1 = *(const F4*)(cz + v + 4);
+                cyv = *(const VF*)(cy + v);
+                czv = *(const VF*)(cz + v);
                 if (hj) {
-                    cm0 = *(const F4*)(cy + v - g.sj);
-                    cm1 = *(const F4*)(cy + v - g.sj + 4);
-                    Pjm = *(const F4*)(p + h - g.hsj);
+                    cmv = *(const VF*)(cy + v - g.sj);
+                    Pjm = *(const V*)(p + h - g.hsj);
                 }
-                if (pj) Pjp = *(const F4*)(p + h + g.hsj);
+                if (pj) Pjp = *(const V*)(p + h + g.hsj);
                 if (lane == 0 && hl) {
                     pL = p[h - 1];
                     czl = cz[v - 1];
                 }
-                if (lane == WAVE - 1 && hr) pR = p[h + 4];
+                if (lane == WAVE - 1 && hr) pR = p[h + VW];
             }
-            const float sl = __shfl_up(Pc[3], 1, WAVE);
+            const float sl = __shfl_up(Pc[VW - 1], 1, WAVE);
             const float sr = __shfl_down(Pc[0], 1, WAVE);
-            const float sz = __shfl_up(cz1[3], 1, WAVE);
+            const float sz = __shfl_up(czv[2 * VW - 1], 1, WAVE);
             if (lane != 0) { pL = sl; czl = sz; }
             if (lane != WAVE - 1) pR = sr;
             czL = czl;
-            F4 cyR, cymR;
-            csplit(cy0, cy1, par, cyB, cyR);
-            csplit(cm0, cm1, par, cymB, cymR);
-            csplit(cz0, cz1, par, czB, czR);
+            V cyR, cymR;
+            csplit<VW>(cyv, par, cyB, cyR);
+            csplit<VW>(cmv, par, cymB, cymR);
+            csplit<VW>(czv, par, czB, czR);
             // w at the red voxels of plane i
-            F4 Wc = zero;
+            V Wc = zero;
             if (i >= 0 && i < g.ni) {
 #pragma unroll
-                for (int x = 0; x < 4; ++x) {
-                    const float pkp = par ? Pc[x] : (x < 3 ? Pc[x + 1] : pR);
+                for (int x = 0; x < VW; ++x) {
+                    const float pkp = par ? Pc[x] : (x < VW - 1 ? Pc[x + 1] : pR);
                     const float pkm = par ? (x > 0 ? Pc[x - 1] : pL) : Pc[x];
                     const float czm = par ? (x > 0 ? czB[x - 1] : czL) : czB[x];
                     Wc[x] = cxR[x] * Pn[x] + cxB1[x] * Pm[x] + cyR[x] * Pjp[x] +
                             cymR[x] * Pjm[x] + czR[x] * pkp + czm * pkm;
                 }
             }
-            if (st >= 2) *(F4*)(lrow + (st % 3) * bstride) = Wc;
+            if (st >= 2) *(V*)(lrow + (st % 3) * bstride) = Wc;
             if (st >= 3) {
                 __syncthreads();
                 // q at the black voxels of plane i - 1 (parity 1 - par)
                 const float* lb = lrow + ((st - 1) % 3) * bstride;
-                const float sl2 = __shfl_up(Wm[3], 1, WAVE);
+                const float sl2 = __shfl_up(Wm[VW - 1], 1, WAVE);
                 const float sr2 = __shfl_down(Wm[0], 1, WAVE);
                 if (!halo) {
-                    const F4 Wjm = *(const F4*)(lb - rowlen);
-                    const F4 Wjp = *(const F4*)(lb + rowlen);
+                    const V Wjm = *(const V*)(lb - rowlen);
+                    const V Wjp = *(const V*)(lb + rowlen);
                     float wL = sl2, wR = sr2;
                     if (lane == 0) wL = hl ? lb[-1] : 0.0f;
-                    if (lane == WAVE - 1) wR = hr ? lb[4] : 0.0f;
+                    if (lane == WAVE - 1) wR = hr ? lb[VW] : 0.0f;
                     const int pb = 1 - par;
-                    F4 Q;
+                    V Q;
 #pragma unroll
-                    for (int x = 0; x < 4; ++x) {
-                        const float wkp = pb ? (x < 3 ? Wm[x + 1] : wR) : Wm[x];
+                    for (int x = 0; x < VW; ++x) {
+                        const float wkp = pb ? (x < VW - 1 ? Wm[x + 1] : wR) : Wm[x];
                         const float wkm = pb ? Wm[x] : (x > 0 ? Wm[x - 1] : wL);
                         const float czm = pb ? czR1[x] : (x > 0 ? czR1[x - 1] : czL1);
                         const float n = cxB1[x] * Wc[x] + cxR2[x] * Wmm[x] + cyB1[x] * Wjp[x] +
@@ -526,7 +534,7 @@ __global__ __launch_bounds__(LS_SCHUR_THREADS) void k_schur_apply(
                         Q[x] = Pm[x] - n;
                         acc += (double)Pm[x] * (double)Q[x];
                     }
-                    if (ok) __builtin_nontemporal_store(Q, (F4*)(q + h - g.hsi));
+                    if (ok) __builtin_nontemporal_store(Q, (V*)(q + h - g.hsi));
                 }
             }
             Wmm = Wm;
@@ -1156,16 +1164,16 @@ struct SchurShape {
 };
 
 static bool schur_shape(ls_ctx* c, SchurShape* sh) {
-    if (c->nk % 8 != 0) return false;
+    if (c->nk % (2 * SCHUR_VW) != 0) return false;
     const int nh = c->nk / 2;
-    const int wr = (nh + 4 * WAVE - 1) / (4 * WAVE);
+    const int wr = (nh + SCHUR_VW * WAVE - 1) / (SCHUR_VW * WAVE);
     const int ns_max = (LS_SCHUR_THREADS / WAVE) / wr;
     const int tj = std::min(LS_SCHUR_ROWS, ns_max - 2);
     if (tj < 1) return false;
     sh->wr = wr;
     sh->tj = tj;
     sh->block = dim3(WAVE, wr, tj + 2);
-    sh->lds = (size_t)3 * (tj + 2) * wr * 4 * WAVE * sizeof(float);
+    sh->lds = (size_t)3 * (tj + 2) * wr * SCHUR_VW * WAVE * sizeof(float);
     sh->grid = dim3((c->nj + tj - 1) / tj, (c->ni + SCHUR_ICH - 1) / SCHUR_ICH, 1);
     return (size_t)sh->grid.x * sh->grid.y <= (size_t)MAX_PARTIALS &&
            2 * (size_t)vec_blocks<float>((uint32_t)(c->N / 2)) <= (size_t)MAX_PARTIALS;
@@ -1185,7 +1193,7 @@ static int schur_form(ls_ctx* c) {
 static int launch_schur(ls_ctx* c, const float* p, float* q, const int* done) {
     SchurShape sh;
     schur_shape(c, &sh);
-    hipLaunchKernelGGL(k_schur_apply<SCHUR_ICH>, sh.grid, sh.block, sh.lds, c->stream, p, q,
+    hipLaunchKernelGGL((k_schur_apply<SCHUR_ICH, SCHUR_VW>), sh.grid, sh.block, sh.lds, c->stream, p, q,
                        c->c32[0], c->c32[1], c->c32[2], rbgeo(c), c->partials, done);
     return (int)(sh.grid.x * sh.grid.y);
 }
@@ -1422,7 +1430,7 @@ int debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot
     }
     SchurShape sh;
     if (form == 1 && !schur_shape(c, &sh)) {
-        c->err = "the fused Schur apply needs nk % 8 == 0 and nk <= " + std::to_string(SCHUR_MAX_NK);
+        c->err = "the fused Schur apply needs nk % " + std::to_string(2 * SCHUR_VW) + " == 0 and nk <= " + std::to_string(SCHUR_MAX_NK);
         return -3;
     }
     if (!use_vec<float>(c)) {
@@ -1478,7 +1486,7 @@ int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) 
     }
     SchurShape sh;
     if (precision == LS_TIME_SCHUR && !schur_shape(c, &sh)) {
-        c->err = "the fused Schur apply needs nk % 8 == 0 and nk <= " + std::to_string(SCHUR_MAX_NK);
+        c->err = "the fused Schur apply needs nk % " + std::to_string(2 * SCHUR_VW) + " == 0 and nk <= " + std::to_string(SCHUR_MAX_NK);
         return -3;
     }
     hipEvent_t e0 = c->ev[14], e1 = c->ev[15];
