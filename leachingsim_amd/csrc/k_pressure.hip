@@ -2003,8 +2003,11 @@ static int solve_learned(ls_ctx* c, ls_step_report* rep) {
             if ((st = poll_fcg(c))) return st;
             // race against the last mixed PCG solve's residual
             // curve; after the same wall time the network must have reduced
-            // the residual by at least learned_race x as many decades
-            if (raced && c->h_fcg->iter >= 16 && !c->h_fcg->done) {
+            // the residual by at least learned_race x as many decades.  The
+            // check repeats at every poll from the 6th iteration on (a
+            // preconditioner that stalls after a fast start is caught
+            // later; one that is behind is stopped after ~6 applies)
+            if (raced && c->h_fcg->iter >= 6 && !c->h_fcg->done) {
                 const double ms = std::chrono::duration<double, std::milli>(
                                       std::chrono::steady_clock::now() - t_fcg).count();
                 const double dec = 0.5 * std::log10(rr_start / c->h_fcg->rr);

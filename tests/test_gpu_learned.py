@@ -95,8 +95,9 @@ def test_learned_solver_reaches_tolerance_and_matches_plain(seed):
 
 def test_race_hands_over_to_mixed_pcg():
     """With the race on: the first solve of a context is mixed PCG (it sets
-    the rate); a later solve runs flexible CG, and the result meets the
-    tolerance whether the network keeps up or mixed PCG takes over."""
+    the rate); a later solve runs flexible CG for at least the 6 iterations
+    before the first race check, and the result meets the tolerance whether
+    the network keeps up or mixed PCG takes over."""
     img = G.smoothed_threshold((32, 32, 48), 2.0, 0.4, 3)
     P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_LEARNED, press_rtol=1e-12)
     g = L.GpuSimulation(img.shape, P)
@@ -109,7 +110,9 @@ def test_race_hands_over_to_mixed_pcg():
     g.upload(L.F_PRESSURE, np.zeros(img.shape))          # a full solve again
     r2 = g.phase(L.PH_PRESSURE, 2500.0)
     assert r2.status == 0 and r2.press_relres <= 1e-12
-    assert r2.press_unet_applies >= 16
+    # flexible CG ran; the race is first checked after 6 iterations and
+    # polled every 2, so a network that keeps up applies more
+    assert r2.press_unet_applies >= 6
     p = g.download(L.F_PRESSURE)
     g.close()
     Q = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_PLAIN64, press_rtol=1e-12)
