@@ -383,22 +383,48 @@ __global__ void k_adr_matvec(const uint8_t* __restrict__ img, const double* __re
 // scalar, branchy row rebuild per matvec.  Terms are added in the row
 // builder's order, so y is the same as the matrix-free matvec's.
 // ---------------------------------------------------------------------------
+// Also builds the right-hand sides and the Jacobi inverse diagonal from the
+// current x0, x1, as k_adr_start does with build_rhs.
 template <int KIND>
 __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows(
     const uint8_t* __restrict__ img, const double* __restrict__ qx,
     const double* __restrict__ qy, const double* __restrict__ qz,
     const double* __restrict__ sf, TGeo g, uint32_t N, AdrConst a,
-    double* __restrict__ rows) {
+    double* __restrict__ rows, const double* __restrict__ x0, const double* __restrict__ x1,
+    double* __restrict__ b0, double* __restrict__ b1, double* __restrict__ invd) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
         int i, j, k;
         tdecode(v, g, i, j, k);
         AdrRow R;
+        double srhs = 0.0;
         if (KIND == 1) {
-            double unused;
-            sat_row(img, qx, qy, qz, sf, g, i, j, k, v, a, R, unused);
+            sat_row(img, qx, qy, qz, sf, g, i, j, k, v, a, R, srhs);
         } else {
             adr_row(img, qx, qy, qz, g, i, j, k, v, a, R);
         }
+        const double xa = x0[v], xb = x1[v];
+        double ba, bbv;
+        if (KIND == 1) {
+            ba = srhs;      // system 1 idles with b = 0
+            bbv = 0.0;
+        } else if (R.type == PREC) {
+            ba = xa;
+            bbv = xb;
+        } else {
+            ba = xa * a.inv_dt;
+            bbv = xb * a.inv_dt;
+            for (int f = 0; f < 6; ++f) {
+                if (!(R.explicit_mask & (1 << f))) continue;
+                const uint32_t n = R.nb[f];
+                double e0 = -(R.cs[f] * xa + R.cn[f] * x0[n]);
+                double e1 = -(R.cs[f] * xb + R.cn[f] * x1[n]);
+                ba += (1.0 - a.theta) * e0;
+                bbv += (1.0 - a.theta) * e1;
+            }
+        }
+        b0[v] = ba;
+        b1[v] = bbv;
+        invd[v] = 1.0 / R.diag;
         rows[v] = R.diag;
 #pragma unroll
         for (int f = 0; f < 6; ++f)
@@ -413,18 +439,24 @@ static dim3 adr_vec_grid(const ls_ctx* c) {
                 (c->ni + ADR_ICH - 1) / ADR_ICH);
 }
 
-// y = A x for both systems from the stored rows; partials of rh.y (MODE 0)
-// or (y.y, y.s) (MODE 1), as k_adr_matvec.
-template <int MODE>
+// y = A x from the stored rows for the systems in SYS (bit 0: system 0,
+// bit 1: system 1; the host drops a system once it has converged); partials
+// of rh.y (MODE 0) or (y.y, y.s) (MODE 1), as k_adr_matvec.  MODE 2 is the
+// solver start: y = b - A x (w = b) into r, and also rh = r, p = v = 0 and
+// the partials (b.b, r.r) of k_adr_start.
+template <int MODE, int SYS>
 __global__ __launch_bounds__(WAVE * ADR_TJ) void k_adr_matvec_rows(
     const double* __restrict__ rows, const double* __restrict__ x0,
     const double* __restrict__ x1, double* __restrict__ y0, double* __restrict__ y1,
     const double* __restrict__ w0, const double* __restrict__ w1, TGeo g, uint32_t N,
-    const BicgScalars* __restrict__ bs, double* __restrict__ partials) {
-    if (bs->all_done) return;
+    const BicgScalars* __restrict__ bs, double* __restrict__ partials,
+    double* __restrict__ rh0, double* __restrict__ rh1, double* __restrict__ p0,
+    double* __restrict__ p1, double* __restrict__ v0, double* __restrict__ v1) {
+    constexpr bool S0 = (SYS & 1) != 0, S1 = (SYS & 2) != 0;
+    if (MODE != 2 && bs->all_done) return;
     typedef double D2 __attribute__((ext_vector_type(2)));
     __shared__ double lds[4 * ADR_TJ];
-    const bool a0 = !bs->done[0], a1 = !bs->done[1];
+    const bool a0 = MODE == 2 || !bs->done[0], a1 = MODE == 2 || !bs->done[1];
     const int nbx = gridDim.x, nby = gridDim.y;
     const int nb = nbx * nby * gridDim.z;
     const int hw = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
@@ -441,12 +473,12 @@ __global__ __launch_bounds__(WAVE * ADR_TJ) void k_adr_matvec_rows(
     uint32_t v = ok ? (uint32_t)i0 * g.si + (uint32_t)j * g.sj + (uint32_t)k0 : 0u;
     D2 am = zero, bm = zero, ac = zero, bc = zero;
     if (ok && i0 > 0) {
-        am = *(const D2*)(x0 + v - g.si);
-        bm = *(const D2*)(x1 + v - g.si);
+        if (S0) am = *(const D2*)(x0 + v - g.si);
+        if (S1) bm = *(const D2*)(x1 + v - g.si);
     }
     if (ok) {
-        ac = *(const D2*)(x0 + v);
-        bc = *(const D2*)(x1 + v);
+        if (S0) ac = *(const D2*)(x0 + v);
+        if (S1) bc = *(const D2*)(x1 + v);
     }
     for (int i = i0; i < i1; ++i, v += g.si) {
         D2 ap = zero, bp = zero, ajm = zero, bjm = zero, ajp = zero, bjp = zero;
@@ -456,68 +488,105 @@ __global__ __launch_bounds__(WAVE * ADR_TJ) void k_adr_matvec_rows(
         double la = 0.0, lb = 0.0, ra = 0.0, rb = 0.0;
         if (ok) {
             if (i + 1 < g.ni) {
-                ap = *(const D2*)(x0 + v + g.si);
-                bp = *(const D2*)(x1 + v + g.si);
+                if (S0) ap = *(const D2*)(x0 + v + g.si);
+                if (S1) bp = *(const D2*)(x1 + v + g.si);
             }
             if (hj) {
-                ajm = *(const D2*)(x0 + v - g.sj);
-                bjm = *(const D2*)(x1 + v - g.sj);
+                if (S0) ajm = *(const D2*)(x0 + v - g.sj);
+                if (S1) bjm = *(const D2*)(x1 + v - g.sj);
             }
             if (pj) {
-                ajp = *(const D2*)(x0 + v + g.sj);
-                bjp = *(const D2*)(x1 + v + g.sj);
+                if (S0) ajp = *(const D2*)(x0 + v + g.sj);
+                if (S1) bjp = *(const D2*)(x1 + v + g.sj);
             }
 #pragma unroll
             for (int f = 0; f < 7; ++f) r[f] = *(const D2*)(rows + (size_t)f * N + v);
             if (lane == 0 && k0 > 0) {
-                la = x0[v - 1];
-                lb = x1[v - 1];
+                if (S0) la = x0[v - 1];
+                if (S1) lb = x1[v - 1];
             }
             if (lane == WAVE - 1 && k0 + 2 < g.nk) {
-                ra = x0[v + 2];
-                rb = x1[v + 2];
+                if (S0) ra = x0[v + 2];
+                if (S1) rb = x1[v + 2];
             }
         }
-        const double sla = __shfl_up(ac[1], 1, WAVE), slb = __shfl_up(bc[1], 1, WAVE);
-        const double sra = __shfl_down(ac[0], 1, WAVE), srb = __shfl_down(bc[0], 1, WAVE);
+        double sla = 0.0, slb = 0.0, sra = 0.0, srb = 0.0;
+        if (S0) {
+            sla = __shfl_up(ac[1], 1, WAVE);
+            sra = __shfl_down(ac[0], 1, WAVE);
+        }
+        if (S1) {
+            slb = __shfl_up(bc[1], 1, WAVE);
+            srb = __shfl_down(bc[0], 1, WAVE);
+        }
         if (lane != 0) { la = sla; lb = slb; }
         if (lane != WAVE - 1) { ra = sra; rb = srb; }
-        D2 ya, yb;
+        D2 ya = zero, yb = zero;
 #pragma unroll
         for (int w = 0; w < 2; ++w) {
-            const double akp = w == 0 ? ac[1] : ra, akm = w == 0 ? la : ac[0];
-            const double bkp = w == 0 ? bc[1] : rb, bkm = w == 0 ? lb : bc[0];
-            double u = r[0][w] * ac[w];
-            u += r[1][w] * ap[w];
-            u += r[2][w] * am[w];
-            u += r[3][w] * ajp[w];
-            u += r[4][w] * ajm[w];
-            u += r[5][w] * akp;
-            u += r[6][w] * akm;
-            ya[w] = u;
-            double z = r[0][w] * bc[w];
-            z += r[1][w] * bp[w];
-            z += r[2][w] * bm[w];
-            z += r[3][w] * bjp[w];
-            z += r[4][w] * bjm[w];
-            z += r[5][w] * bkp;
-            z += r[6][w] * bkm;
-            yb[w] = z;
+            if (S0) {
+                const double akp = w == 0 ? ac[1] : ra, akm = w == 0 ? la : ac[0];
+                double u = r[0][w] * ac[w];
+                u += r[1][w] * ap[w];
+                u += r[2][w] * am[w];
+                u += r[3][w] * ajp[w];
+                u += r[4][w] * ajm[w];
+                u += r[5][w] * akp;
+                u += r[6][w] * akm;
+                ya[w] = u;
+            }
+            if (S1) {
+                const double bkp = w == 0 ? bc[1] : rb, bkm = w == 0 ? lb : bc[0];
+                double z = r[0][w] * bc[w];
+                z += r[1][w] * bp[w];
+                z += r[2][w] * bm[w];
+                z += r[3][w] * bjp[w];
+                z += r[4][w] * bjm[w];
+                z += r[5][w] * bkp;
+                z += r[6][w] * bkm;
+                yb[w] = z;
+            }
         }
         if (ok) {
-            if (a0) *(D2*)(y0 + v) = ya;
-            if (a1) *(D2*)(y1 + v) = yb;
-            const D2 wa = *(const D2*)(w0 + v), wb = *(const D2*)(w1 + v);
+            D2 wa = zero, wb = zero;
+            if (S0) wa = *(const D2*)(w0 + v);
+            if (S1) wb = *(const D2*)(w1 + v);
+            if (MODE == 2) {
+                // r = b - A x, rh = r, p = v = 0
+                const D2 ra2 = wa - ya, rb2 = wb - yb;
+                if (S0) {
+                    *(D2*)(y0 + v) = ra2;
+                    *(D2*)(rh0 + v) = ra2;
+                    *(D2*)(p0 + v) = zero;
+                    *(D2*)(v0 + v) = zero;
+                }
+                if (S1) {
+                    *(D2*)(y1 + v) = rb2;
+                    *(D2*)(rh1 + v) = rb2;
+                    *(D2*)(p1 + v) = zero;
+                    *(D2*)(v1 + v) = zero;
+                }
 #pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                if (MODE == 0) {
-                    s0 += wa[w] * ya[w];
-                    s1 += wb[w] * yb[w];
-                } else {
-                    s0 += ya[w] * ya[w];
-                    s1 += yb[w] * yb[w];
-                    s2 += ya[w] * wa[w];
-                    s3 += yb[w] * wb[w];
+                for (int w = 0; w < 2; ++w) {
+                    s0 += wa[w] * wa[w];
+                    s1 += wb[w] * wb[w];
+                    s2 += ra2[w] * ra2[w];
+                    s3 += rb2[w] * rb2[w];
+                }
+            } else {
+                if (S0 && a0) *(D2*)(y0 + v) = ya;
+                if (S1 && a1) *(D2*)(y1 + v) = yb;
+#pragma unroll
+                for (int w = 0; w < 2; ++w) {
+                    if (MODE == 0) {
+                        s0 += wa[w] * ya[w];
+                        s1 += wb[w] * yb[w];
+                    } else {
+                        s0 += ya[w] * ya[w];
+                        s1 += yb[w] * yb[w];
+                        s2 += ya[w] * wa[w];
+                        s3 += yb[w] * wb[w];
+                    }
                 }
             }
         }
@@ -746,30 +815,56 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             c->arows = nullptr;
     }
     stored = stored && c->arows != nullptr;
-    if (stored) {
+    if (stored) {       // rows, right-hand sides and D^-1 once per solve
         hipLaunchKernelGGL((k_adr_rows<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img,
-                           c->qx, c->qy, c->qz, sf, g, N, a, c->arows);
+                           c->qx, c->qy, c->qz, sf, g, N, a, c->arows, x[0], x[1], c->tb[0],
+                           c->tb[1], c->tinvd);
         LS_LAUNCH_CHECK();
     }
+    // systems still iterating, as the host last saw them (a system that
+    // converges inside a batch is masked on the device)
+    int sys = 3;
+    auto launch_rows = [&](auto MC, auto SC, const double* xa, const double* xb, double* ya,
+                           double* yb, const double* wa, const double* wb) {
+        constexpr int M = decltype(MC)::value, S = decltype(SC)::value;
+        hipLaunchKernelGGL((k_adr_matvec_rows<M, S>), vg, dim3(WAVE, ADR_TJ), 0, c->stream,
+                           c->arows, xa, xb, ya, yb, wa, wb, g, N, c->bicg, c->partials, rh[0],
+                           rh[1], pp[0], pp[1], vv[0], vv[1]);
+    };
     auto matvec = [&](auto MC, const double* xa, const double* xb, double* ya, double* yb,
                       const double* wa, const double* wb) {
-        constexpr int M = decltype(MC)::value;
         if (stored) {
-            hipLaunchKernelGGL((k_adr_matvec_rows<M>), vg, dim3(WAVE, ADR_TJ), 0, c->stream,
-                               c->arows, xa, xb, ya, yb, wa, wb, g, N, c->bicg, c->partials);
+            if (sys == 1)
+                launch_rows(MC, std::integral_constant<int, 1>{}, xa, xb, ya, yb, wa, wb);
+            else if (sys == 2)
+                launch_rows(MC, std::integral_constant<int, 2>{}, xa, xb, ya, yb, wa, wb);
+            else
+                launch_rows(MC, std::integral_constant<int, 3>{}, xa, xb, ya, yb, wa, wb);
         } else {
+            constexpr int M = decltype(MC)::value;
             hipLaunchKernelGGL((k_adr_matvec<M, KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
                                c->img, c->qx, c->qy, c->qz, sf, xa, xb, ya, yb, wa, wb, g, N, a,
                                c->bicg, c->partials);
         }
         return stored ? nbm : nb;
     };
-    while (true) {
+    // r = b - A x, rh = r, p = v = 0, partials (b.b, r.r)
+    auto start = [&](bool build) {
+        if (stored) {
+            sys = 3;
+            launch_rows(std::integral_constant<int, 2>{}, std::integral_constant<int, 3>{}, x[0],
+                        x[1], r[0], r[1], c->tb[0], c->tb[1]);
+            return nbm;
+        }
         hipLaunchKernelGGL((k_adr_start<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img,
                            c->qx, c->qy, c->qz, sf, x[0], x[1], c->tb[0], c->tb[1], c->tinvd,
                            r[0], r[1], rh[0], rh[1], pp[0], pp[1], vv[0], vv[1], g, N, a,
-                           first ? 1 : 0, c->partials);
-        hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nb, c->bicg,
+                           build ? 1 : 0, c->partials);
+        return nb;
+    };
+    while (true) {
+        const int ns = start(first);
+        hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(1024), 0, c->stream, c->partials, ns, c->bicg,
                            c->P.adr_rtol, c->P.adr_maxit, first ? 1 : 0);
         hipLaunchKernelGGL(k_bicg_first_rho, dim3(1), dim3(1), 0, c->stream, c->bicg);
         LS_LAUNCH_CHECK();
@@ -786,6 +881,8 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         }
         first = false;
         while (!c->h_bicg->all_done) {
+            sys = (c->h_bicg->done[0] ? 0 : 1) | (c->h_bicg->done[1] ? 0 : 2);
+            if (sys == 0) sys = 3;
             for (int it = 0; it < batch; ++it) {
                 hipLaunchKernelGGL(k_bicg_p, dim3(nb), dim3(EW_BLOCK), 0, c->stream, pp[0], pp[1],
                                    r[0], r[1], vv[0], vv[1], ph[0], ph[1], c->tinvd, c->bicg, N);
@@ -815,11 +912,8 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         }
         // confirm on the recomputed residual (the start kernel with the
         // stored right-hand sides); restart systems that fall short
-        hipLaunchKernelGGL((k_adr_start<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img,
-                           c->qx, c->qy, c->qz, sf, x[0], x[1], c->tb[0], c->tb[1], c->tinvd,
-                           r[0], r[1], rh[0], rh[1], pp[0], pp[1], vv[0], vv[1], g, N, a, 0,
-                           c->partials);
-        hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nb, c->bicg,
+        const int nc = start(false);
+        hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nc, c->bicg,
                            c->P.adr_rtol, c->P.adr_maxit, 0);
         LS_LAUNCH_CHECK();
         LS_HIP(hipMemcpyAsync(c->h_bicg, c->bicg, sizeof(BicgScalars), hipMemcpyDeviceToHost,
