@@ -114,14 +114,18 @@ def test_surface_precipitation_bit_exact(rng):
 
 
 @pytest.mark.parametrize("theta", [1.0, 0.5])
-def test_transport_parity(rng, theta):
-    shape = (11, 10, 12)
+@pytest.mark.parametrize("nk,acid_zero", [(12, False), (12, True), (13, False)])
+def test_transport_parity(rng, theta, nk, acid_zero):
+    """nk even: rows stored once per solve, vectorised matvec; acid = 0: the
+    acid system converges at the start and the matvec runs conc alone;
+    nk odd: rows rebuilt in every matvec."""
+    shape = (11, 10, nk)
     img = small_mixed_geometry(shape, 13)
     P = L.default_params(adr_rtol=1e-13, theta=theta)
     g, o = both(shape, P, img)
     q = [rng.uniform(-0.3, 0.3, size=shape) for _ in range(3)]
     conc = rng.uniform(0.0, 1.0, size=shape)
-    acid = rng.uniform(0.0, 1.0, size=shape)
+    acid = np.zeros(shape) if acid_zero else rng.uniform(0.0, 1.0, size=shape)
     for x in (g, o):
         for f, a in zip((L.F_QX, L.F_QY, L.F_QZ), q):
             x.upload(f, a)
@@ -131,7 +135,10 @@ def test_transport_parity(rng, theta):
     ro = o.phase(L.PH_TRANSPORT, 0.0, 0.7)
     assert rg.status == 0 and ro.status == 0
     assert relerr(g.download(L.F_CONC), o.download(L.F_CONC)) <= 1e-11
-    assert relerr(g.download(L.F_ACID), o.download(L.F_ACID)) <= 1e-11
+    if acid_zero:
+        assert np.array_equal(g.download(L.F_ACID), np.zeros(shape))
+    else:
+        assert relerr(g.download(L.F_ACID), o.download(L.F_ACID)) <= 1e-11
     for f in (L.F_QX, L.F_QY, L.F_QZ):   # zeroed at Rock/Precipitate
         assert np.array_equal(g.download(f), o.download(f))
 
@@ -189,6 +196,26 @@ def test_step_parity_smooth64(solver):
         assert relerr(g.download(L.F_PRESSURE), o.download(L.F_PRESSURE)) <= 1e-6
         assert relerr(g.download(L.F_CONC), o.download(L.F_CONC)) <= 1e-7
         t += dt
+
+
+@pytest.mark.parametrize("shape", [(21, 18, 27), (20, 18, 30)])
+def test_step_parity_scalar_paths(shape):
+    """nk odd: scalar stencil, FP32 CG on the operator itself, transport rows
+    rebuilt per matvec; nk = 30 (even, not a multiple of 4): scalar stencil,
+    stored transport rows."""
+    img = small_mixed_geometry(shape, 21)
+    P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED,
+                         warm_start=L.WARM_PROJECTED, press_rtol=1e-12, adr_rtol=1e-12)
+    g, o = both(shape, P, img)
+    t = 0.0
+    for n in range(4):
+        rg = g.step(t, 30.0)
+        ro = o.step(t, 30.0)
+        assert rg.status == ro.status == 0, (n, rg.status, ro.status)
+        assert np.array_equal(g.download(L.F_IMG), o.download(L.F_IMG)), n
+        t += 30.0
+    assert relerr(g.download(L.F_PRESSURE), o.download(L.F_PRESSURE)) <= 1e-7
+    assert relerr(g.download(L.F_CONC), o.download(L.F_CONC)) <= 1e-7
 
 
 def test_rollback_restores_state():
