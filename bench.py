@@ -15,18 +15,22 @@ after `--warmup` untimed steps, bracketed by device synchronisation.
 
 Runs (each on a fresh context, same workload and window):
   headline  learned+mixed: flexible CG with the U-Net coarse-level
-            preconditioner (probe and mixed-precision PCG fallback), warm
-            start = projection onto earlier solutions plus the network
-            direction (trained weights from leachingsim_amd/weights/);
+            preconditioner, raced against the recorded curve of
+            mixed-precision PCG (FP32 CG on the red-black Schur complement
+            inside FP64 reliable updates), which takes over when the network
+            falls behind; warm start = projection onto earlier solutions
+            plus the network direction (trained weights from
+            leachingsim_amd/weights/);
   baseline  plain FP64 Jacobi-PCG warm-started from the previous pressure
             (the reference's algorithm, src/Simulation.cpp:131-159, 808);
   ablations mixed-precision PCG + projected warm start (no network), and
             plain FP64 PCG + projected warm start.
 
 Prints one JSON line: steps/s of the headline, the baseline and ablations,
-the pressure-stencil roofline measured with HIP events during the timed
-region (HBM traffic from the committed rocprofv3 counter summary), and the
-CPU oracle baseline.
+the roofline of the dominant pressure kernel measured with HIP events
+during the timed region (HBM traffic from the committed rocprofv3 counter
+summary), the plain FP32 / FP64 stencil y = A x timed alone on the same
+operator, and the CPU oracle baseline.
 """
 from __future__ import annotations
 
@@ -194,6 +198,33 @@ KERNELS = {"stencil64": (64, "k_stencil_vec<double>"), "stencil32": (32, "k_sten
            "schur": (3, "k_schur_apply<16>")}
 
 
+def stencil_rooflines(img, label_bytes, reps=20):
+    """The plain 7-point stencil y = A x alone, FP32 and FP64 (20 / 40 B per
+    voxel), on the pressure operator of the initial state: HIP-event time
+    per launch, achieved GB/s against the 8 TB/s peak, and the HBM traffic
+    of the committed counter runs."""
+    P = L.default_params(bc_mode=L.BC_ZFACES, press_maxit=1)
+    sim = L.GpuSimulation(img.shape, P, label_bytes)
+    sim.read_labels(img)
+    sim.init_state()
+    sim.phase(L.PH_PROPERTIES)
+    sim.phase(L.PH_PRESSURE, 1.0)      # assembles; the 1-iteration solve is irrelevant
+    tr = load_traffic() or {}
+    out = {}
+    for kernel in ("stencil32", "stencil64"):
+        precision, kname = KERNELS[kernel]
+        ms, by = sim.time_stencil(precision, reps)
+        achieved = by / ms / 1e6
+        t = tr.get(kernel, {})
+        out[kname] = {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": achieved / HBM_PEAK_GBS, "bytes_per_launch": by, "avg_launch_ms": ms,
+                      "launches": reps,
+                      "traffic": t.get("bytes_per_launch") if t.get("grid") == list(img.shape)
+                      else None}
+    sim.close()
+    return out
+
+
 def stencil_only(img, label_bytes, kernel, reps):
     """Assemble the pressure operator of the initial state, then time `reps`
     launches of one pressure kernel and `reps` copies of 1 GiB (HIP events)."""
@@ -325,6 +356,8 @@ def main():
     if tr is not None and kernel in tr and tr[kernel].get("grid") == list(img.shape):
         traffic = tr[kernel]["bytes_per_launch"]
 
+    stencils = stencil_rooflines(img, lb)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(img, lb, dt, min(args.cpu_crop, *img.shape), args.cpu_steps)
@@ -354,6 +387,7 @@ def main():
                      "kernel": KERNELS[kernel][1],
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms,
                      "sampled_launches": n, "measured_copy_GBs": copy_gbs, "traffic": traffic},
+        "stencil_roofline": stencils,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
