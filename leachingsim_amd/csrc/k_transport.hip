@@ -809,10 +809,12 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     const int nbm = (int)(vg.x * vg.y * vg.z);
     bool stored = c->nk % 2 == 0 && 4 * (size_t)nbm <= 8 * (size_t)MAX_PARTIALS;
     if (stored && c->arows == nullptr) {
-        if (hipMalloc((void**)&c->arows, 7 * c->N * sizeof(double)) == hipSuccess)
+        if (hipMalloc((void**)&c->arows, 7 * c->N * sizeof(double)) == hipSuccess) {
             c->allocations.push_back(c->arows);
-        else
+        } else {            // out of memory: rebuild rows per matvec instead
             c->arows = nullptr;
+            (void)hipGetLastError();   // the failed allocation must not fail the next check
+        }
     }
     stored = stored && c->arows != nullptr;
     if (stored) {       // rows, right-hand sides and D^-1 once per solve
