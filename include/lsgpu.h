@@ -225,7 +225,8 @@ int ls_abi_info(size_t* sizeof_params, size_t* sizeof_report);
 
 /* Replaces Simulation::setupDomain + setupPETSc (src/Simulation.cpp:81-159):
  * allocates all device state for an ni x nj x nk grid on the current HIP
- * device.  label_bytes is 1 (uint8 build) or 2 (uint16 build). */
+ * device (each extent >= 2, ni nj nk < 2^32; else -2).  label_bytes is 1
+ * (uint8 build) or 2 (uint16 build). */
 int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes,
               const ls_params* params);
 void ls_destroy(ls_ctx* ctx);

@@ -59,12 +59,14 @@ def schur_numpy(faces32, p_black, shape):
 
 
 @pytest.mark.parametrize("shape", [(24, 20, 40), (19, 33, 16), (5, 7, 1024), (34, 30, 512),
-                                   (61, 700, 16), (97, 1300, 8)])
+                                   (61, 700, 16), (97, 1300, 8),
+                                   (2, 2, 8), (2, 2, 16), (2, 7, 8), (17, 2, 24)])
 def test_fused_schur_apply(shape):
     """Ragged tiles: nj not a multiple of the block's rows, ni not a
     multiple of its planes, rows spanning 2 waves (nk = 1024), grids of
-    thousands of blocks."""
-    img = small_mixed_geometry(shape, 3)
+    thousands of blocks; two planes or two rows (the smallest grids
+    ls_create takes), one lane's worth of k."""
+    img = small_mixed_geometry(shape, 3, air_slabs=shape[0] > 4)
     P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED)
     g = L.GpuSimulation(shape, P)
     g.read_labels(img)
