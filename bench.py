@@ -195,7 +195,7 @@ def cpu_baseline(img, label_bytes, dt, crop: int, nsteps: int):
 
 KERNELS = {"stencil64": (64, "k_stencil_vec<double>"), "stencil32": (32, "k_stencil_vec<float>"),
            "rb_gather": (1, "k_rb_vec<1>"), "rb_apply": (2, "k_rb_vec<2>"),
-           "schur": (3, "k_schur_apply<16>")}
+           "schur": (3, "k_schur_apply<16, 4>")}
 
 
 def stencil_rooflines(img, label_bytes, reps=20):
