@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LSGPU_ABI_VERSION 6
+#define LSGPU_ABI_VERSION 7
 
 /* Labels (src/compiler_opts.h:69-82). */
 enum ls_label {
@@ -132,6 +132,11 @@ typedef struct ls_params {
                                 nk <= 1024 (else as 2); 2: the
                                 same on full-length vectors in two passes; both
                                 need nk % 4 == 0; 0: inner CG on the operator  */
+    int adr_schur;           /* transport and implicit saturation (1): BiCGStab on
+                                the red-black Schur complement of the stored rows
+                                (black voxels, about half the iterations), the
+                                red values recovered after convergence; needs
+                                nk even; 0: BiCGStab on the full system         */
 } ls_params;
 
 /* Fields that can be uploaded / downloaded.  Element types are fixed:

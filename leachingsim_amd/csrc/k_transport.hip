@@ -614,6 +614,259 @@ __global__ __launch_bounds__(WAVE * ADR_TJ) void k_adr_matvec_rows(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Red-black Schur complement of the stored rows.  The 7-point graph is
+// bipartite (colour (i + j + k) & 1, black 0), so with the rows split by
+// colour, A = [D_b A_br; A_rb D_r] with diagonal D_b, D_r, and
+//   S x_b = b_b - A_br D_r^-1 b_r,   S = D_b - A_br D_r^-1 A_rb,
+//   x_r   = D_r^-1 (b_r - A_rb x_b).
+// BiCGStab with Jacobi on diag(S) runs on the black unknowns: the
+// spectral radius of the Jacobi iteration matrix squares, so about half
+// the iterations, on half-length vectors.  After the red recovery the full
+// residual is (b_b - A_bb x_b - A_br x_r, 0) = (r_S, 0), so the solver stops
+// on ||r_S|| <= rtol ||b|| with the full system's b.
+//
+// Layout (nk even): voxel v of colour c is stored at v >> 1 of its colour's
+// half-length arrays; the pair (2m, 2m + 1) holds one voxel of each colour.
+// Rows: 7 coefficients (diagonal, then faces +i, -i, +j, -j, +k, -k) x 2
+// colours x N/2 doubles.
+// ---------------------------------------------------------------------------
+struct SPair {
+    uint32_t vb, vr;   // full indices of the pair's black and red voxel
+    int i, j, kb, kr;
+};
+
+__device__ __forceinline__ SPair spair(uint32_t m, const TGeo& g) {
+    SPair s;
+    const uint32_t v0 = 2u * m;
+    const uint32_t row = v0 / (uint32_t)g.nk;
+    s.i = (int)(row / (uint32_t)g.nj);
+    s.j = (int)(row - (uint32_t)s.i * (uint32_t)g.nj);
+    const int k0 = (int)(v0 - row * (uint32_t)g.nk);
+    const int par = (s.i + s.j) & 1;          // k0 is even
+    s.vb = v0 + (uint32_t)par;
+    s.vr = v0 + 1u - (uint32_t)par;
+    s.kb = k0 + par;
+    s.kr = k0 + 1 - par;
+    return s;
+}
+
+// neighbour f (0 +i, 1 -i, 2 +j, 3 -j, 4 +k, 5 -k) of voxel v at (i, j, k)
+__device__ __forceinline__ bool snbr(const TGeo& g, int i, int j, int k, uint32_t v, int f,
+                                     uint32_t& n) {
+    return tnbr(g, i, j, k, v, f, n);
+}
+
+__device__ __forceinline__ double srow(const double* __restrict__ rows, uint32_t Nh, int f,
+                                       int colour, uint32_t m) {
+    return rows[((size_t)f * 2 + colour) * Nh + m];
+}
+
+// The stored rows, colour-split, with b and D^-1 (full layout) as k_adr_rows.
+template <int KIND>
+__global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
+    const uint8_t* __restrict__ img, const double* __restrict__ qx,
+    const double* __restrict__ qy, const double* __restrict__ qz,
+    const double* __restrict__ sf, TGeo g, uint32_t N, AdrConst a,
+    double* __restrict__ rows, const double* __restrict__ x0, const double* __restrict__ x1,
+    double* __restrict__ b0, double* __restrict__ b1, double* __restrict__ invd) {
+    const uint32_t Nh = N / 2;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        int i, j, k;
+        tdecode(v, g, i, j, k);
+        AdrRow R;
+        double srhs = 0.0;
+        if (KIND == 1) {
+            sat_row(img, qx, qy, qz, sf, g, i, j, k, v, a, R, srhs);
+        } else {
+            adr_row(img, qx, qy, qz, g, i, j, k, v, a, R);
+        }
+        const double xa = x0[v], xb = x1[v];
+        double ba, bbv;
+        if (KIND == 1) {
+            ba = srhs;
+            bbv = 0.0;
+        } else if (R.type == PREC) {
+            ba = xa;
+            bbv = xb;
+        } else {
+            ba = xa * a.inv_dt;
+            bbv = xb * a.inv_dt;
+            for (int f = 0; f < 6; ++f) {
+                if (!(R.explicit_mask & (1 << f))) continue;
+                const uint32_t n = R.nb[f];
+                double e0 = -(R.cs[f] * xa + R.cn[f] * x0[n]);
+                double e1 = -(R.cs[f] * xb + R.cn[f] * x1[n]);
+                ba += (1.0 - a.theta) * e0;
+                bbv += (1.0 - a.theta) * e1;
+            }
+        }
+        b0[v] = ba;
+        b1[v] = bbv;
+        invd[v] = 1.0 / R.diag;
+        const int colour = (i + j + k) & 1;
+        const uint32_t m = v >> 1;
+        rows[((size_t)0 * 2 + colour) * Nh + m] = R.diag;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            rows[((size_t)(f + 1) * 2 + colour) * Nh + m] = (R.mask & (1 << f)) ? R.off[f] : 0.0;
+    }
+}
+
+// diag(S)^-1, the Schur right-hand sides and the black part of x.
+__global__ __launch_bounds__(EW_BLOCK) void k_tschur_prep(
+    const double* __restrict__ rows, TGeo g, uint32_t Nh, const double* __restrict__ b0,
+    const double* __restrict__ b1, const double* __restrict__ x0, const double* __restrict__ x1,
+    double* __restrict__ invdS, double* __restrict__ bS0, double* __restrict__ bS1,
+    double* __restrict__ xb0, double* __restrict__ xb1) {
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < Nh; m += gridDim.x * blockDim.x) {
+        const SPair P = spair(m, g);
+        double dS = srow(rows, Nh, 0, 0, m);
+        double r0 = b0[P.vb], r1 = b1[P.vb];
+        for (int f = 0; f < 6; ++f) {
+            uint32_t n;
+            if (!snbr(g, P.i, P.j, P.kb, P.vb, f, n)) continue;
+            const double o = srow(rows, Nh, f + 1, 0, m);
+            const uint32_t mn = n >> 1;
+            const double dn = srow(rows, Nh, 0, 1, mn);
+            const double back = srow(rows, Nh, (f ^ 1) + 1, 1, mn);   // the neighbour's face toward v
+            dS -= o * back / dn;
+            r0 -= o * (b0[n] / dn);
+            r1 -= o * (b1[n] / dn);
+        }
+        invdS[m] = 1.0 / dS;
+        bS0[m] = r0;
+        bS1[m] = r1;
+        xb0[m] = x0[P.vb];
+        xb1[m] = x1[P.vb];
+    }
+}
+
+// w = D_r^-1 A_rb p at the red voxels (compacted), for the systems in SYS.
+template <int SYS>
+__global__ __launch_bounds__(EW_BLOCK) void k_tschur_red(
+    const double* __restrict__ rows, TGeo g, uint32_t Nh, const double* __restrict__ p0,
+    const double* __restrict__ p1, double* __restrict__ w0, double* __restrict__ w1,
+    const BicgScalars* __restrict__ bs, int start) {
+    if (!start && bs->all_done) return;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < Nh; m += gridDim.x * blockDim.x) {
+        const SPair P = spair(m, g);
+        double a0 = 0.0, a1 = 0.0;
+        for (int f = 0; f < 6; ++f) {
+            uint32_t n;
+            if (!snbr(g, P.i, P.j, P.kr, P.vr, f, n)) continue;
+            const double o = srow(rows, Nh, f + 1, 1, m);
+            if (SYS & 1) a0 += o * p0[n >> 1];
+            if (SYS & 2) a1 += o * p1[n >> 1];
+        }
+        const double d = srow(rows, Nh, 0, 1, m);
+        if (SYS & 1) w0[m] = a0 / d;
+        if (SYS & 2) w1[m] = a1 / d;
+    }
+}
+
+// y = S x = D_b x - A_br w at the black voxels; partials as
+// k_adr_matvec_rows: rh.y (MODE 0), (y.y, y.s) (MODE 1); MODE 2 is the
+// start r = bS - S x, rh = r, p = v = 0 with partials (b.b over the whole
+// grid, r.r).
+template <int MODE, int SYS>
+__global__ __launch_bounds__(EW_BLOCK) void k_tschur_black(
+    const double* __restrict__ rows, TGeo g, uint32_t Nh, const double* __restrict__ x0,
+    const double* __restrict__ x1, const double* __restrict__ wr0, const double* __restrict__ wr1,
+    double* __restrict__ y0, double* __restrict__ y1, const double* __restrict__ w0,
+    const double* __restrict__ w1, const BicgScalars* __restrict__ bs,
+    double* __restrict__ partials, double* __restrict__ rh0, double* __restrict__ rh1,
+    double* __restrict__ p0, double* __restrict__ p1, double* __restrict__ v0,
+    double* __restrict__ v1, const double* __restrict__ bf0, const double* __restrict__ bf1) {
+    constexpr bool S0 = (SYS & 1) != 0, S1 = (SYS & 2) != 0;
+    if (MODE != 2 && bs->all_done) return;
+    __shared__ double lds[4 * EW_BLOCK / WAVE];
+    const bool a0 = MODE == 2 || !bs->done[0], a1 = MODE == 2 || !bs->done[1];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < Nh; m += gridDim.x * blockDim.x) {
+        const SPair P = spair(m, g);
+        const double d = srow(rows, Nh, 0, 0, m);
+        double ya = S0 ? d * x0[m] : 0.0, yb = S1 ? d * x1[m] : 0.0;
+        for (int f = 0; f < 6; ++f) {
+            uint32_t n;
+            if (!snbr(g, P.i, P.j, P.kb, P.vb, f, n)) continue;
+            const double o = srow(rows, Nh, f + 1, 0, m);
+            if (S0) ya -= o * wr0[n >> 1];
+            if (S1) yb -= o * wr1[n >> 1];
+        }
+        if (MODE == 2) {
+            const double ra = S0 ? w0[m] - ya : 0.0, rb = S1 ? w1[m] - yb : 0.0;
+            if (S0) {
+                y0[m] = ra;
+                rh0[m] = ra;
+                p0[m] = 0.0;
+                v0[m] = 0.0;
+            }
+            if (S1) {
+                y1[m] = rb;
+                rh1[m] = rb;
+                p1[m] = 0.0;
+                v1[m] = 0.0;
+            }
+            // ||b|| of the full system: the pair (2m, 2m + 1)
+            const double ba = bf0[2 * m], bb = bf0[2 * m + 1];
+            const double bc = bf1[2 * m], bd = bf1[2 * m + 1];
+            s0 += ba * ba + bb * bb;
+            s1 += bc * bc + bd * bd;
+            s2 += ra * ra;
+            s3 += rb * rb;
+        } else {
+            if (S0 && a0) y0[m] = ya;
+            if (S1 && a1) y1[m] = yb;
+            if (MODE == 0) {
+                if (S0) s0 += w0[m] * ya;
+                if (S1) s1 += w1[m] * yb;
+            } else {
+                if (S0) {
+                    s0 += ya * ya;
+                    s2 += ya * w0[m];
+                }
+                if (S1) {
+                    s1 += yb * yb;
+                    s3 += yb * w1[m];
+                }
+            }
+        }
+    }
+    double q[4] = {s0, s1, s2, s3};
+    block_reduce<4, RED_SUM, RED_SUM, RED_SUM, RED_SUM>(q, lds);
+    if (threadIdx.x == 0) {
+        const int nb = gridDim.x;
+        partials[blockIdx.x] = q[0];
+        partials[nb + blockIdx.x] = q[1];
+        partials[2 * nb + blockIdx.x] = q[2];
+        partials[3 * nb + blockIdx.x] = q[3];
+    }
+}
+
+// x_b from the solver, x_r = D_r^-1 (b_r - A_rb x_b), into the full fields.
+__global__ __launch_bounds__(EW_BLOCK) void k_tschur_recover(
+    const double* __restrict__ rows, TGeo g, uint32_t Nh, const double* __restrict__ xb0,
+    const double* __restrict__ xb1, const double* __restrict__ b0, const double* __restrict__ b1,
+    double* __restrict__ x0, double* __restrict__ x1) {
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < Nh; m += gridDim.x * blockDim.x) {
+        const SPair P = spair(m, g);
+        double a0 = b0[P.vr], a1 = b1[P.vr];
+        for (int f = 0; f < 6; ++f) {
+            uint32_t n;
+            if (!snbr(g, P.i, P.j, P.kr, P.vr, f, n)) continue;
+            const double o = srow(rows, Nh, f + 1, 1, m);
+            a0 -= o * xb0[n >> 1];
+            a1 -= o * xb1[n >> 1];
+        }
+        const double d = srow(rows, Nh, 0, 1, m);
+        x0[P.vb] = xb0[m];
+        x1[P.vb] = xb1[m];
+        x0[P.vr] = a0 / d;
+        x1[P.vr] = a1 / d;
+    }
+}
+
 __global__ void k_bicg_alpha(const double* __restrict__ partials, int n, BicgScalars* bs) {
     if (bs->all_done) return;
     __shared__ double lds[1024 / WAVE];
@@ -817,12 +1070,38 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         }
     }
     stored = stored && c->arows != nullptr;
-    if (stored) {       // rows, right-hand sides and D^-1 once per solve
+    // the Schur form: the solver runs on the black unknowns (N/2 each); its
+    // extra half-length arrays use the second halves of work vectors whose
+    // first halves hold the half-length BiCGStab vectors
+    const bool tschur = stored && c->P.adr_schur != 0;
+    const uint32_t Nh = N / 2;
+    const int nbh = ew_blocks(Nh);
+    double *xs[2] = {nullptr, nullptr}, *bS[2] = {nullptr, nullptr}, *wr[2] = {nullptr, nullptr};
+    double* invdS = nullptr;
+    if (tschur) {
+        for (int f = 0; f < 2; ++f) {
+            xs[f] = c->tv[f][0] + Nh;
+            bS[f] = c->tv[f][1] + Nh;
+            wr[f] = c->tv[f][2] + Nh;
+        }
+        invdS = c->tv[0][3] + Nh;
+        hipLaunchKernelGGL((k_adr_rows_split<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
+                           c->img, c->qx, c->qy, c->qz, sf, g, N, a, c->arows, x[0], x[1],
+                           c->tb[0], c->tb[1], c->tinvd);
+        hipLaunchKernelGGL(k_tschur_prep, dim3(nbh), dim3(EW_BLOCK), 0, c->stream, c->arows, g, Nh,
+                           c->tb[0], c->tb[1], x[0], x[1], invdS, bS[0], bS[1], xs[0], xs[1]);
+        LS_LAUNCH_CHECK();
+    } else if (stored) {       // rows, right-hand sides and D^-1 once per solve
         hipLaunchKernelGGL((k_adr_rows<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img,
                            c->qx, c->qy, c->qz, sf, g, N, a, c->arows, x[0], x[1], c->tb[0],
                            c->tb[1], c->tinvd);
         LS_LAUNCH_CHECK();
     }
+    // the solver's unknowns, length and Jacobi inverse diagonal
+    double* xv[2] = {tschur ? xs[0] : x[0], tschur ? xs[1] : x[1]};
+    const uint32_t Nv = tschur ? Nh : N;
+    const int nbv = tschur ? nbh : nb;
+    const double* invd_v = tschur ? invdS : c->tinvd;
     // systems still iterating, as the host last saw them (a system that
     // converges inside a batch is masked on the device)
     int sys = 3;
@@ -833,8 +1112,27 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
                            c->arows, xa, xb, ya, yb, wa, wb, g, N, c->bicg, c->partials, rh[0],
                            rh[1], pp[0], pp[1], vv[0], vv[1]);
     };
+    auto launch_schur = [&](auto MC, auto SC, const double* xa, const double* xb, double* ya,
+                            double* yb, const double* wa, const double* wb) {
+        constexpr int M = decltype(MC)::value, S = decltype(SC)::value;
+        hipLaunchKernelGGL((k_tschur_red<S>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream, c->arows,
+                           g, Nh, xa, xb, wr[0], wr[1], c->bicg, M == 2 ? 1 : 0);
+        hipLaunchKernelGGL((k_tschur_black<M, S>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream,
+                           c->arows, g, Nh, xa, xb, wr[0], wr[1], ya, yb, wa, wb, c->bicg,
+                           c->partials, rh[0], rh[1], pp[0], pp[1], vv[0], vv[1], c->tb[0],
+                           c->tb[1]);
+    };
     auto matvec = [&](auto MC, const double* xa, const double* xb, double* ya, double* yb,
                       const double* wa, const double* wb) {
+        if (tschur) {
+            if (sys == 1)
+                launch_schur(MC, std::integral_constant<int, 1>{}, xa, xb, ya, yb, wa, wb);
+            else if (sys == 2)
+                launch_schur(MC, std::integral_constant<int, 2>{}, xa, xb, ya, yb, wa, wb);
+            else
+                launch_schur(MC, std::integral_constant<int, 3>{}, xa, xb, ya, yb, wa, wb);
+            return nbh;
+        }
         if (stored) {
             if (sys == 1)
                 launch_rows(MC, std::integral_constant<int, 1>{}, xa, xb, ya, yb, wa, wb);
@@ -852,6 +1150,12 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     };
     // r = b - A x, rh = r, p = v = 0, partials (b.b, r.r)
     auto start = [&](bool build) {
+        if (tschur) {
+            sys = 3;
+            launch_schur(std::integral_constant<int, 2>{}, std::integral_constant<int, 3>{},
+                         xs[0], xs[1], r[0], r[1], bS[0], bS[1]);
+            return nbh;
+        }
         if (stored) {
             sys = 3;
             launch_rows(std::integral_constant<int, 2>{}, std::integral_constant<int, 3>{}, x[0],
@@ -876,8 +1180,10 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         if (first && (c->h_bicg->bnorm2[0] == 0.0 || c->h_bicg->bnorm2[1] == 0.0)) {
             // a system with b = 0 has x = 0 exactly; restart on the zeroed guess
             for (int f = 0; f < 2; ++f)
-                if (c->h_bicg->bnorm2[f] == 0.0)
+                if (c->h_bicg->bnorm2[f] == 0.0) {
                     LS_HIP(hipMemsetAsync(x[f], 0, c->N * sizeof(double), c->stream));
+                    if (tschur) LS_HIP(hipMemsetAsync(xs[f], 0, Nh * sizeof(double), c->stream));
+                }
             first = false;
             continue;
         }
@@ -886,25 +1192,25 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             sys = (c->h_bicg->done[0] ? 0 : 1) | (c->h_bicg->done[1] ? 0 : 2);
             if (sys == 0) sys = 3;
             for (int it = 0; it < batch; ++it) {
-                hipLaunchKernelGGL(k_bicg_p, dim3(nb), dim3(EW_BLOCK), 0, c->stream, pp[0], pp[1],
-                                   r[0], r[1], vv[0], vv[1], ph[0], ph[1], c->tinvd, c->bicg, N);
+                hipLaunchKernelGGL(k_bicg_p, dim3(nbv), dim3(EW_BLOCK), 0, c->stream, pp[0], pp[1],
+                                   r[0], r[1], vv[0], vv[1], ph[0], ph[1], invd_v, c->bicg, Nv);
                 const int n0 = matvec(std::integral_constant<int, 0>{}, ph[0], ph[1], vv[0],
                                       vv[1], rh[0], rh[1]);
                 hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1024), 0, c->stream, c->partials,
                                    n0, c->bicg);
-                hipLaunchKernelGGL(k_bicg_s, dim3(nb), dim3(EW_BLOCK), 0, c->stream, r[0], r[1],
-                                   vv[0], vv[1], s[0], s[1], sh[0], sh[1], c->tinvd, c->bicg, N,
+                hipLaunchKernelGGL(k_bicg_s, dim3(nbv), dim3(EW_BLOCK), 0, c->stream, r[0], r[1],
+                                   vv[0], vv[1], s[0], s[1], sh[0], sh[1], invd_v, c->bicg, Nv,
                                    c->partials);
                 hipLaunchKernelGGL(k_bicg_scheck, dim3(1), dim3(1024), 0, c->stream, c->partials,
-                                   nb, c->bicg);
+                                   nbv, c->bicg);
                 const int n1 = matvec(std::integral_constant<int, 1>{}, sh[0], sh[1], t[0], t[1],
                                       s[0], s[1]);
                 hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1024), 0, c->stream, c->partials,
                                    n1, c->bicg);
-                hipLaunchKernelGGL(k_bicg_x, dim3(nb), dim3(EW_BLOCK), 0, c->stream, x[0], x[1],
+                hipLaunchKernelGGL(k_bicg_x, dim3(nbv), dim3(EW_BLOCK), 0, c->stream, xv[0], xv[1],
                                    r[0], r[1], ph[0], ph[1], sh[0], sh[1], s[0], s[1], t[0], t[1],
-                                   rh[0], rh[1], c->bicg, N, c->partials);
-                hipLaunchKernelGGL(k_bicg_rho, dim3(1), dim3(1024), 0, c->stream, c->partials, nb,
+                                   rh[0], rh[1], c->bicg, Nv, c->partials);
+                hipLaunchKernelGGL(k_bicg_rho, dim3(1), dim3(1024), 0, c->stream, c->partials, nbv,
                                    c->bicg);
             }
             LS_LAUNCH_CHECK();
@@ -928,6 +1234,11 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         ++restarts;
         hipLaunchKernelGGL(k_bicg_first_rho, dim3(1), dim3(1), 0, c->stream, c->bicg);
         // loop re-enters with the start kernel (rebuild of b skipped)
+    }
+    if (tschur) {   // x_b, and x_r from the red rows, into the fields
+        hipLaunchKernelGGL(k_tschur_recover, dim3(nbh), dim3(EW_BLOCK), 0, c->stream, c->arows, g,
+                           Nh, xs[0], xs[1], c->tb[0], c->tb[1], x[0], x[1]);
+        LS_LAUNCH_CHECK();
     }
     const BicgScalars& h = *c->h_bicg;
     if (iters_out) *iters_out = h.iter[0] > h.iter[1] ? h.iter[0] : h.iter[1];

@@ -94,6 +94,7 @@ void ls_default_params(ls_params* p) {
     p->learned_race = 1.0;
     p->implicit_sat = 0;
     p->mixed_schur = 1;
+    p->adr_schur = 1;
     p->mixed_inner_rtol = 0.1;
 }
 

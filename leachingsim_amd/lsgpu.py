@@ -47,7 +47,7 @@ class Params(C.Structure):
         ("warm_start", C.c_int), ("press_rtol", C.c_double), ("press_maxit", C.c_int),
         ("adr_rtol", C.c_double), ("adr_maxit", C.c_int), ("warm_history", C.c_int),
         ("mixed_inner_rtol", C.c_double), ("learned_race", C.c_double),
-        ("implicit_sat", C.c_int), ("mixed_schur", C.c_int),
+        ("implicit_sat", C.c_int), ("mixed_schur", C.c_int), ("adr_schur", C.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -93,7 +93,7 @@ def default_params(**overrides) -> Params:
                solver=SOLVER_PLAIN64, warm_start=WARM_PREVIOUS, press_rtol=1e-8,
                press_maxit=10000, adr_rtol=1e-8, adr_maxit=10000, warm_history=4,
                mixed_inner_rtol=0.1, learned_race=1.0, implicit_sat=0,
-               mixed_schur=1)
+               mixed_schur=1, adr_schur=1)
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)

@@ -114,14 +114,16 @@ def test_surface_precipitation_bit_exact(rng):
 
 
 @pytest.mark.parametrize("theta", [1.0, 0.5])
-@pytest.mark.parametrize("nk,acid_zero", [(12, False), (12, True), (13, False)])
-def test_transport_parity(rng, theta, nk, acid_zero):
-    """nk even: rows stored once per solve, vectorised matvec; acid = 0: the
-    acid system converges at the start and the matvec runs conc alone;
-    nk odd: rows rebuilt in every matvec."""
+@pytest.mark.parametrize("nk,acid_zero,schur", [(12, False, 1), (12, True, 1), (12, False, 0),
+                                               (12, True, 0), (13, False, 1)])
+def test_transport_parity(rng, theta, nk, acid_zero, schur):
+    """nk even: rows stored once per solve, BiCGStab on the red-black Schur
+    complement (schur 1) or on the full system with the vectorised matvec
+    (schur 0); acid = 0: the acid system converges at the start and the
+    kernels run conc alone; nk odd: full system, rows rebuilt per matvec."""
     shape = (11, 10, nk)
     img = small_mixed_geometry(shape, 13)
-    P = L.default_params(adr_rtol=1e-13, theta=theta)
+    P = L.default_params(adr_rtol=1e-13, theta=theta, adr_schur=schur)
     g, o = both(shape, P, img)
     q = [rng.uniform(-0.3, 0.3, size=shape) for _ in range(3)]
     conc = rng.uniform(0.0, 1.0, size=shape)
