@@ -651,12 +651,6 @@ __device__ __forceinline__ SPair spair(uint32_t m, const TGeo& g) {
     return s;
 }
 
-// neighbour f (0 +i, 1 -i, 2 +j, 3 -j, 4 +k, 5 -k) of voxel v at (i, j, k)
-__device__ __forceinline__ bool snbr(const TGeo& g, int i, int j, int k, uint32_t v, int f,
-                                     uint32_t& n) {
-    return tnbr(g, i, j, k, v, f, n);
-}
-
 __device__ __forceinline__ double srow(const double* __restrict__ rows, uint32_t Nh, int f,
                                        int colour, uint32_t m) {
     return rows[((size_t)f * 2 + colour) * Nh + m];
@@ -725,7 +719,7 @@ __global__ __launch_bounds__(EW_BLOCK) void k_tschur_prep(
         double r0 = b0[P.vb], r1 = b1[P.vb];
         for (int f = 0; f < 6; ++f) {
             uint32_t n;
-            if (!snbr(g, P.i, P.j, P.kb, P.vb, f, n)) continue;
+            if (!tnbr(g, P.i, P.j, P.kb, P.vb, f, n)) continue;
             const double o = srow(rows, Nh, f + 1, 0, m);
             const uint32_t mn = n >> 1;
             const double dn = srow(rows, Nh, 0, 1, mn);
@@ -754,7 +748,7 @@ __global__ __launch_bounds__(EW_BLOCK) void k_tschur_red(
         double a0 = 0.0, a1 = 0.0;
         for (int f = 0; f < 6; ++f) {
             uint32_t n;
-            if (!snbr(g, P.i, P.j, P.kr, P.vr, f, n)) continue;
+            if (!tnbr(g, P.i, P.j, P.kr, P.vr, f, n)) continue;
             const double o = srow(rows, Nh, f + 1, 1, m);
             if (SYS & 1) a0 += o * p0[n >> 1];
             if (SYS & 2) a1 += o * p1[n >> 1];
@@ -789,7 +783,7 @@ __global__ __launch_bounds__(EW_BLOCK) void k_tschur_black(
         double ya = S0 ? d * x0[m] : 0.0, yb = S1 ? d * x1[m] : 0.0;
         for (int f = 0; f < 6; ++f) {
             uint32_t n;
-            if (!snbr(g, P.i, P.j, P.kb, P.vb, f, n)) continue;
+            if (!tnbr(g, P.i, P.j, P.kb, P.vb, f, n)) continue;
             const double o = srow(rows, Nh, f + 1, 0, m);
             if (S0) ya -= o * wr0[n >> 1];
             if (S1) yb -= o * wr1[n >> 1];
@@ -854,7 +848,7 @@ __global__ __launch_bounds__(EW_BLOCK) void k_tschur_recover(
         double a0 = b0[P.vr], a1 = b1[P.vr];
         for (int f = 0; f < 6; ++f) {
             uint32_t n;
-            if (!snbr(g, P.i, P.j, P.kr, P.vr, f, n)) continue;
+            if (!tnbr(g, P.i, P.j, P.kr, P.vr, f, n)) continue;
             const double o = srow(rows, Nh, f + 1, 1, m);
             a0 -= o * xb0[n >> 1];
             a1 -= o * xb1[n >> 1];
