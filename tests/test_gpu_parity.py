@@ -115,7 +115,7 @@ def test_surface_precipitation_bit_exact(rng):
 
 @pytest.mark.parametrize("theta", [1.0, 0.5])
 @pytest.mark.parametrize("nk,acid_zero,schur", [(12, False, 1), (12, True, 1), (12, False, 0),
-                                               (12, True, 0), (13, False, 1)])
+                                               (12, True, 0), (13, False, 1), (2, False, 1)])
 def test_transport_parity(rng, theta, nk, acid_zero, schur):
     """nk even: rows stored once per solve, BiCGStab on the red-black Schur
     complement (schur 1) or on the full system with the vectorised matvec
