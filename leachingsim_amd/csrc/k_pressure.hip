@@ -1382,7 +1382,7 @@ static double effective_top_pressure(double top, double t) {
 
 static int solve_learned(ls_ctx* c, ls_step_report* rep);
 static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep);
-static int warm_push(ls_ctx* c, double ptop);
+static int warm_push(ls_ctx* c, double ptop, bool replace);
 
 int solve_pressure(ls_ctx* c, double t, ls_step_report* rep) {
     const double ptop = effective_top_pressure(c->P.top_pressure, t);
@@ -1395,7 +1395,11 @@ int solve_pressure(ls_ctx* c, double t, ls_step_report* rep) {
     if (st < 0) return st;
     int st2 = finalize_pressure(c, ptop);
     if (st2) return st2;
-    if (st == 0 && (st2 = warm_push(c, ptop))) return st2;
+    // a solve that ended on the projection itself adds no direction: it takes
+    // the newest slot instead of a new one, so the history (and the cost of
+    // the next projection) does not grow with copies of one span
+    const bool same_span = rep->press_iters == 0 && rep->warm_basis > 0 && !rep->warm_net;
+    if (st == 0 && (st2 = warm_push(c, ptop, same_span))) return st2;
     c->p_top_prev = ptop;
     c->have_prev = true;
     rep->top_pressure_eff = ptop;
@@ -2325,10 +2329,18 @@ static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
     return 0;
 }
 
-// After a solve: the new pressure joins the history.
-static int warm_push(ls_ctx* c, double ptop) {
+// After a solve: the new pressure joins the history (replace: it overwrites
+// the newest entry).
+static int warm_push(ls_ctx* c, double ptop, bool replace) {
     if (c->P.warm_start == LS_WARM_PREVIOUS) {
         c->n_hist = 0;
+        return 0;
+    }
+    if (replace && c->n_hist > 0) {
+        const int newest = (c->hist_head + LS_WARM_MAX - 1) % LS_WARM_MAX;
+        LS_HIP(hipMemcpyAsync(c->hist[newest], c->p, c->N * sizeof(double),
+                              hipMemcpyDeviceToDevice, c->stream));
+        c->hist_ptop[newest] = ptop;
         return 0;
     }
     const int slot = c->hist_head;

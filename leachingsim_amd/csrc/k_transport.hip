@@ -1031,6 +1031,18 @@ __global__ void k_bicg_first_rho(BicgScalars* bs) {
     }
 }
 
+// A system with b = 0 is solved by x = 0, which the host has just written:
+// it takes no iteration, and the other system keeps its start state.
+__global__ void k_bicg_zero_rhs(BicgScalars* bs) {
+    for (int f = 0; f < 2; ++f)
+        if (bs->bnorm2[f] == 0.0) {
+            bs->rr[f] = 0.0;
+            bs->done[f] = 1;
+            bs->converged[f] = 1;
+        }
+    bs->all_done = bs->done[0] && bs->done[1];
+}
+
 // Lock-step BiCGStab of the two systems x0, x1 (KIND 0: conc, acid; KIND 1:
 // the implicit saturation in x0, x1 idle).  Returns 0 converged, 1 not.
 template <int KIND>
@@ -1172,14 +1184,18 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
                               c->stream));
         LS_HIP(hipStreamSynchronize(c->stream));
         if (first && (c->h_bicg->bnorm2[0] == 0.0 || c->h_bicg->bnorm2[1] == 0.0)) {
-            // a system with b = 0 has x = 0 exactly; restart on the zeroed guess
+            // a system with b = 0 has x = 0 exactly: write it and mark the
+            // system solved (the start state of the other one stands)
             for (int f = 0; f < 2; ++f)
                 if (c->h_bicg->bnorm2[f] == 0.0) {
                     LS_HIP(hipMemsetAsync(x[f], 0, c->N * sizeof(double), c->stream));
                     if (tschur) LS_HIP(hipMemsetAsync(xs[f], 0, Nh * sizeof(double), c->stream));
                 }
-            first = false;
-            continue;
+            hipLaunchKernelGGL(k_bicg_zero_rhs, dim3(1), dim3(1), 0, c->stream, c->bicg);
+            LS_LAUNCH_CHECK();
+            LS_HIP(hipMemcpyAsync(c->h_bicg, c->bicg, sizeof(BicgScalars), hipMemcpyDeviceToHost,
+                                  c->stream));
+            LS_HIP(hipStreamSynchronize(c->stream));
         }
         first = false;
         while (!c->h_bicg->all_done) {
