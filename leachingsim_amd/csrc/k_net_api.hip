@@ -254,6 +254,12 @@ int ls_net_step(ls_net* n, const float* gout, float lr) {
     return st;
 }
 
+int ls_net_set_conv(ls_net* n, int tiled) {
+    if (!n) return -1;
+    n->conv_tiled = tiled != 0;
+    return 0;
+}
+
 // Time `reps` forward passes on the device-resident input (HIP events).
 int ls_net_time_forward(ls_net* n, int reps, double* ms) {
     if (!n || !ms || reps < 1) return -1;

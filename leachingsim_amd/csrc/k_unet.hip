@@ -131,6 +131,134 @@ __global__ __launch_bounds__(256) void k_conv(const TIN* __restrict__ A, const T
 }
 
 // ---------------------------------------------------------------------------
+// Tiled 16 -> 16 channel convolution (CA = 16, no concatenation, no
+// upsampling): the full-resolution layers e0a, e0b, d0b, out and their
+// backward-data.  A block of 4 waves owns CT_TJ = 4 rows (j) x CT_TK = 64
+// voxels (k) of an output plane and marches CT_ICH planes along i.  The input
+// planes i - 1 .. i + 1 of the tile and its one-voxel halo sit in LDS as BF16
+// (four rotating slots: the plane after next is fetched into registers while
+// the current one is computed, one barrier per plane); each lane keeps its 27
+// weight fragments in registers.  A slot is laid out [row][channel quad]
+// [voxel][4 channels], so the 16 lanes that read one quad of a B operand
+// read 128 contiguous bytes (no bank conflict).  The MFMA sequence
+// of an output tile (taps in order, one 16-channel chunk) is that of k_conv,
+// so both give the same bits.  Needs ni, nj, nk to be multiples of 16.
+// ---------------------------------------------------------------------------
+constexpr int CT_TJ = 4, CT_TK = 64, CT_ICH = 16;
+constexpr int CT_ROWS = CT_TJ + 2, CT_VOX = CT_TK + 2;
+constexpr int CT_PLANE = CT_ROWS * CT_VOX * 16;            // BF16 elements per slot
+constexpr int CT_CHUNKS = CT_ROWS * CT_VOX * 2;            // 8-channel chunks per plane
+constexpr int CT_PER_THREAD = (CT_CHUNKS + 255) / 256;
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf8 load8(const bf16* p) { return *(const bf8*)p; }
+__device__ __forceinline__ bf8 load8(const float* p) {
+    const f4 a = *(const f4*)p, b = *(const f4*)(p + 4);
+    bf8 r;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        r[e] = (bf16)a[e];
+        r[e + 4] = (bf16)b[e];
+    }
+    return r;
+}
+
+template <bool RELU, typename TIN, typename TOUT>
+__global__ __launch_bounds__(256) void k_conv16_tiled(const TIN* __restrict__ X,
+                                                      const bf16* __restrict__ W,
+                                                      const float* __restrict__ bias,
+                                                      TOUT* __restrict__ Y, NetLvl L) {
+    __shared__ bf16 tile[4 * CT_PLANE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, q = lane >> 4;
+    const int nkt = L.nk / CT_TK + (L.nk % CT_TK ? 1 : 0);
+    const int k0 = (int)(blockIdx.x % (unsigned)nkt) * CT_TK;
+    const int j0 = (int)(blockIdx.x / (unsigned)nkt) * CT_TJ;
+    const int i0 = blockIdx.y * CT_ICH, i1 = min(i0 + CT_ICH, L.ni);
+    const size_t xoff = (size_t)blockIdx.z * L.n;
+    bf4 wr[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) wr[t] = load4(W + ((size_t)t * 16 + col) * 16 + 4 * q);
+    // chunk c of a plane: row c / (2 CT_VOX), voxel (c / 2) % CT_VOX, channels
+    // 8 h .. 8 h + 7 with h = c % 2, i.e. quads 2 h and 2 h + 1
+    bf8 pre[CT_PER_THREAD];
+    auto fetch = [&](int ip) {
+#pragma unroll
+        for (int r = 0; r < CT_PER_THREAD; ++r) {
+            const int c = threadIdx.x + 256 * r;
+            const int row = c / (2 * CT_VOX), rem = c - row * (2 * CT_VOX);
+            const int jj = j0 - 1 + row, kk = k0 - 1 + (rem >> 1);
+            bf8 v;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = (bf16)0.0f;
+            if (c < CT_CHUNKS && ip >= 0 && ip < L.ni && jj >= 0 && jj < L.nj && kk >= 0 &&
+                kk < L.nk)
+                v = load8(X + (xoff + ((size_t)ip * L.nj + jj) * L.nk + kk) * 16 + (rem & 1) * 8);
+            pre[r] = v;
+        }
+    };
+    auto publish = [&](int slot) {
+#pragma unroll
+        for (int r = 0; r < CT_PER_THREAD; ++r) {
+            const int c = threadIdx.x + 256 * r;
+            if (c < CT_CHUNKS) {
+                const int row = c / (2 * CT_VOX), rem = c - row * (2 * CT_VOX);
+                bf16* d = tile + slot * CT_PLANE +
+                          ((row * 4 + 2 * (rem & 1)) * CT_VOX + (rem >> 1)) * 4;
+                *(bf4*)d = bf4{pre[r][0], pre[r][1], pre[r][2], pre[r][3]};
+                *(bf4*)(d + 4 * CT_VOX) = bf4{pre[r][4], pre[r][5], pre[r][6], pre[r][7]};
+            }
+        }
+    };
+    fetch(i0 - 1);
+    publish((i0 - 1) & 3);
+    fetch(i0);
+    publish(i0 & 3);
+    fetch(i0 + 1);
+    const int j = j0 + wave;
+    f4 bv = f4{0.f, 0.f, 0.f, 0.f};
+    if (bias != nullptr) bv = f4{bias[4 * q], bias[4 * q + 1], bias[4 * q + 2], bias[4 * q + 3]};
+    for (int i = i0; i < i1; ++i) {
+        // slot (i + 1) & 3 held plane i - 3, last read for output plane
+        // i - 2, which every wave finished before the barrier of plane i - 1
+        publish((i + 1) & 3);
+        if (i + 1 < i1) fetch(i + 2);
+        __syncthreads();
+        f4 acc[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 27; ++t) {
+            const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
+            const bf16* base = tile + ((i + di) & 3) * CT_PLANE +
+                               (((wave + 1 + dj) * 4 + q) * CT_VOX + 1 + dk + col) * 4;
+            // all four columns, also those past nk in a partial k-tile (their
+            // LDS voxels are zero and their results are not stored): a branch
+            // here would serialise the LDS reads
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                const bf4 xb = *(const bf4*)(base + n * 64);
+                acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[t], xb, acc[n], 0, 0, 0);
+            }
+        }
+        if (j < L.nj) {
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                const int k = k0 + 16 * n + col;
+                if (k0 + 16 * n >= L.nk) continue;
+                f4 o = acc[n] + bv;
+                if (RELU) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) o[e] = o[e] > 0.f ? o[e] : 0.f;
+                }
+                store4(Y + (xoff + ((size_t)i * L.nj + j) * L.nk + k) * 16 + 4 * q, o);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient: part[c][t][co][ci] = sum over the voxels of chunk c of
 // g[v][co] X[v + off(t)][ci]; one wave per (chunk, tap), 64 voxel blocks of
 // 16 along K.  X is the layer input ([up(A), B]), g the FP32 gradient of the
@@ -408,6 +536,15 @@ static int eb(size_t n) {
 template <int CA, int CB, int COUT, bool UPA, bool RELU, typename TIN, typename TOUT>
 static void conv(UNet* net, const TIN* A, const TIN* B, const bf16* W, const float* bias, TOUT* Y,
                  const NetLvl& L, const NetLvl& LA, int batch) {
+    if constexpr (CA == 16 && CB == 0 && COUT == 16 && !UPA) {
+        if (net->conv_tiled) {
+            const int nkt = (L.nk + CT_TK - 1) / CT_TK, njt = (L.nj + CT_TJ - 1) / CT_TJ;
+            dim3 grd(nkt * njt, (L.ni + CT_ICH - 1) / CT_ICH, batch);
+            hipLaunchKernelGGL((k_conv16_tiled<RELU, TIN, TOUT>), grd, dim3(256), 0, net->stream, A,
+                               W, bias, Y, L);
+            return;
+        }
+    }
     const uint32_t waves = (L.n + 63) / 64;
     dim3 grd((waves + 3) / 4, batch);
     hipLaunchKernelGGL((k_conv<CA, CB, COUT, UPA, RELU, TIN, TOUT>), grd, dim3(256), 0, net->stream,

@@ -25,6 +25,7 @@ struct NetLvl {
 struct UNet {
     int batch = 1;
     NetLvl L[3];
+    bool conv_tiled = true;      // LDS-tiled kernel for the 16 -> 16 channel layers
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string err;

@@ -28,6 +28,7 @@ def _lib():
             "ls_net_backward": (i, [vp, fp, fp]),
             "ls_net_train_step": (i, [vp, fp, fp, fp, C.c_float, C.POINTER(C.c_double)]),
             "ls_net_time_forward": (i, [vp, i, C.POINTER(C.c_double)]),
+            "ls_net_set_conv": (i, [vp, i]),
             "ls_net_step": (i, [vp, fp, C.c_float]),
         }
         for name, (res, args) in sig.items():
@@ -101,6 +102,11 @@ class Net:
     def step(self, gout, lr):
         g = np.ascontiguousarray(gout, dtype=np.float32)
         self._check(self.lib.ls_net_step(self.h, _fp(g), float(lr)))
+
+    def set_conv(self, tiled):
+        """Kernel of the 16 -> 16 channel layers: LDS-tiled (True, default)
+        or the general implicit-GEMM kernel (False); both give the same bits."""
+        self._check(self.lib.ls_net_set_conv(self.h, 1 if tiled else 0))
 
     def time_forward(self, reps=5):
         ms = C.c_double()

@@ -92,3 +92,21 @@ def test_unet_gradient_euler_identity():
         v = np.array([vals[n] for n in group])
         scale = max(abs(v.mean()), abs(Lval))
         assert v.max() - v.min() <= 0.15 * scale, (group, Lval, vals)
+
+
+@pytest.mark.parametrize("shape,batch", [((16, 16, 32), 2), ((32, 48, 80), 1), ((16, 32, 64), 3)])
+def test_tiled_conv_matches_direct(shape, batch):
+    """The LDS-tiled kernel of the 16 -> 16 channel layers (forward e0a, e0b,
+    d0b, out and their backward-data) issues the MFMA sequence of the general
+    kernel, so forward output and weight gradient agree bit for bit.  nk = 80
+    leaves a k-tile with one valid 16-voxel column of four."""
+    net, w, x, rng = setup(5, shape=shape, batch=batch)
+    gout = rng.standard_normal((batch,) + shape).astype(np.float32)
+    res = {}
+    for tiled in (True, False):
+        net.set_conv(tiled)
+        out = net.forward(x)
+        res[tiled] = (out, net.backward(gout))
+    assert np.array_equal(res[True][0], res[False][0])
+    assert np.array_equal(res[True][1], res[False][1])
+    assert np.abs(res[True][0]).max() > 0
