@@ -326,9 +326,10 @@ int ls_net_train_step(ls_net* net, const float* in, const float* target, const f
 int ls_net_step(ls_net* net, const float* gout, float lr);
 /* Average time of `reps` forward passes on the device-resident input. */
 int ls_net_time_forward(ls_net* net, int reps, double* ms_per_forward);
-/* Kernel of the 16 -> 16 channel layers (forward and backward-data):
- * 1 (default) LDS-tiled, 0 the general implicit-GEMM kernel.  Both give the
- * same bits; the switch is for tests and timing. */
+/* Convolution kernels (forward and backward-data): 1 (default) the
+ * LDS-tiled kernels for the layers that have one (the 16 -> 16 channel
+ * layers, e1a, d0a), 0 the general implicit-GEMM kernel everywhere.  Both
+ * give the same bits; the switch is for tests and timing. */
 int ls_net_set_conv(ls_net* net, int tiled);
 
 /* Weights of the learned coarse-level preconditioner used by

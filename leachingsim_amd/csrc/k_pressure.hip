@@ -772,9 +772,13 @@ __global__ void k_cg_beta(const double* __restrict__ partials, int n, CgScalars*
     if (cs->done) return;
     __shared__ double lds[2 * 1024 / WAVE];
     double a0 = 0.0, a1 = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        a0 += partials[i];
-        a1 += partials[n + i];
+    if (use_w) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            a0 += partials[i];
+            a1 += partials[n + i];
+        }
+    } else {   // the update kernel wrote zeros there (no weights): not read
+        for (int i = threadIdx.x; i < n; i += blockDim.x) a0 += partials[i];
     }
     double a[2] = {a0, a1};
     block_reduce<2, RED_SUM, RED_SUM>(a, lds);

@@ -131,24 +131,27 @@ __global__ __launch_bounds__(256) void k_conv(const TIN* __restrict__ A, const T
 }
 
 // ---------------------------------------------------------------------------
-// Tiled 16 -> 16 channel convolution (CA = 16, no concatenation, no
-// upsampling): the full-resolution layers e0a, e0b, d0b, out and their
-// backward-data.  A block of 4 waves owns CT_TJ = 4 rows (j) x CT_TK = 64
-// voxels (k) of an output plane and marches CT_ICH planes along i.  The input
-// planes i - 1 .. i + 1 of the tile and its one-voxel halo sit in LDS as BF16
-// (four rotating slots: the plane after next is fetched into registers while
-// the current one is computed, one barrier per plane); each lane keeps its 27
-// weight fragments in registers.  A slot is laid out [row][channel quad]
-// [voxel][4 channels], so the 16 lanes that read one quad of a B operand
-// read 128 contiguous bytes (no bank conflict).  The MFMA sequence
-// of an output tile (taps in order, one 16-channel chunk) is that of k_conv,
-// so both give the same bits.  Needs ni, nj, nk to be multiples of 16.
+// Tiled convolution (forward, and backward-data with the mirrored weights):
+// the same result as k_conv, bit for bit.  A block of
+// TJ waves owns TJ rows (j) x TK voxels (k) of an output plane and marches
+// CT_ICH planes along i.  The input planes i - 1 .. i + 1 of the tile and its
+// one-voxel halo sit in LDS as BF16 ([up2(A), B] channels for a decoder
+// layer): four rotating slots, the plane after next fetched into registers
+// while the current one is computed, one barrier per plane.  A slot is laid
+// out [row][channel quad][voxel][4 channels], so the 16 lanes that read one
+// quad of a B operand read 128 contiguous bytes (no bank conflict).  The 27
+// weight fragments stay in registers when the layer is one 16 x 16 block;
+// otherwise each tap's fragments are loaded (L1/L2) one tap ahead.  The MFMA
+// sequence of an output tile (taps, then 16-channel chunks, in order) is that
+// of k_conv.  Tile shapes (per input channel count) keep a block's LDS near
+// 52 KB (three blocks per CU).  The grid of the first level has extents that
+// are multiples of 16, so a level's extents are multiples of 4.
 // ---------------------------------------------------------------------------
-constexpr int CT_TJ = 4, CT_TK = 64, CT_ICH = 16;
-constexpr int CT_ROWS = CT_TJ + 2, CT_VOX = CT_TK + 2;
-constexpr int CT_PLANE = CT_ROWS * CT_VOX * 16;            // BF16 elements per slot
-constexpr int CT_CHUNKS = CT_ROWS * CT_VOX * 2;            // 8-channel chunks per plane
-constexpr int CT_PER_THREAD = (CT_CHUNKS + 255) / 256;
+constexpr int CT_ICH = 16;
+
+template <int CIN> struct ConvTile;
+template <> struct ConvTile<16> { static constexpr int TJ = 4, TK = 64; };
+template <> struct ConvTile<48> { static constexpr int TJ = 2, TK = 32; };
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
@@ -163,6 +166,163 @@ __device__ __forceinline__ bf8 load8(const float* p) {
     }
     return r;
 }
+
+template <int CA, int CB, int COUT, bool UPA, bool RELU, typename TIN, typename TOUT>
+__global__ __launch_bounds__(64 * ConvTile<CA + CB>::TJ) void k_conv_tiled(
+    const TIN* __restrict__ A, const TIN* __restrict__ B, const bf16* __restrict__ W,
+    const float* __restrict__ bias, TOUT* __restrict__ Y, NetLvl L, NetLvl LA) {
+    constexpr int CIN = CA + CB, NQ = CIN / 4, NCK = CIN / 16, MT = COUT / 16;
+    constexpr int TJ = ConvTile<CIN>::TJ, TK = ConvTile<CIN>::TK, NT = TK / 16;
+    constexpr int NTH = 64 * TJ, VOX = TK + 2, G8 = CIN / 8;
+    constexpr int PLANE = (TJ + 2) * NQ * VOX * 4;           // BF16 elements per slot
+    constexpr int CHUNKS = (TJ + 2) * VOX * G8;              // 8-channel chunks per plane
+    constexpr int PER = (CHUNKS + NTH - 1) / NTH;
+    constexpr bool WREG = MT * NCK == 1;
+    __shared__ bf16 tile[4 * PLANE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, q = lane >> 4;
+    const int nkt = L.nk / TK + (L.nk % TK ? 1 : 0);
+    const int k0 = (int)(blockIdx.x % (unsigned)nkt) * TK;
+    const int j0 = (int)(blockIdx.x / (unsigned)nkt) * TJ;
+    const int i0 = blockIdx.y * CT_ICH, i1 = min(i0 + CT_ICH, L.ni);
+    const int b = blockIdx.z;
+    const size_t offA = (size_t)b * (UPA ? LA.n : L.n), offB = (size_t)b * L.n;
+    bf4 wr[WREG ? 27 : 1];
+    if constexpr (WREG) {
+#pragma unroll
+        for (int t = 0; t < 27; ++t) wr[t] = load4(W + ((size_t)t * 16 + col) * 16 + 4 * q);
+    }
+    // chunk c of a plane: row c / (VOX G8), voxel (c / G8) % VOX, channels
+    // 8 g .. 8 g + 7 with g = c % G8 (quads 2 g, 2 g + 1)
+    bf8 pre[PER];
+    auto fetch = [&](int ip) {
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int c = threadIdx.x + NTH * r;
+            const int row = c / (VOX * G8), rem = c - row * (VOX * G8);
+            const int vx = rem / G8, g = rem - vx * G8;
+            const int jj = j0 - 1 + row, kk = k0 - 1 + vx;
+            bf8 v;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = (bf16)0.0f;
+            if (c < CHUNKS && ip >= 0 && ip < L.ni && jj >= 0 && jj < L.nj && kk >= 0 &&
+                kk < L.nk) {
+                if (8 * g < CA) {
+                    const size_t n = UPA ? ((size_t)(ip >> 1) * LA.nj + (jj >> 1)) * LA.nk + (kk >> 1)
+                                         : ((size_t)ip * L.nj + jj) * L.nk + kk;
+                    v = load8(A + (offA + n) * CA + 8 * g);
+                } else {
+                    const size_t n = ((size_t)ip * L.nj + jj) * L.nk + kk;
+                    v = load8(B + (offB + n) * CB + (8 * g - CA));
+                }
+            }
+            pre[r] = v;
+        }
+    };
+    auto publish = [&](int slot) {
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int c = threadIdx.x + NTH * r;
+            if (c < CHUNKS) {
+                const int row = c / (VOX * G8), rem = c - row * (VOX * G8);
+                const int vx = rem / G8, g = rem - vx * G8;
+                bf16* d = tile + slot * PLANE + ((row * NQ + 2 * g) * VOX + vx) * 4;
+                *(bf4*)d = bf4{pre[r][0], pre[r][1], pre[r][2], pre[r][3]};
+                *(bf4*)(d + 4 * VOX) = bf4{pre[r][4], pre[r][5], pre[r][6], pre[r][7]};
+            }
+        }
+    };
+    auto wload = [&](int t, bf4 (&wa)[MT][NCK]) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int ck = 0; ck < NCK; ++ck)
+                wa[m][ck] = load4(W + ((size_t)t * COUT + m * 16 + col) * CIN + 16 * ck + 4 * q);
+    };
+    fetch(i0 - 1);
+    publish((i0 - 1) & 3);
+    fetch(i0);
+    publish(i0 & 3);
+    fetch(i0 + 1);
+    const int j = j0 + wave;
+    for (int i = i0; i < i1; ++i) {
+        // slot (i + 1) & 3 held plane i - 3, last read for output plane
+        // i - 2, which every wave finished before the barrier of plane i - 1
+        publish((i + 1) & 3);
+        if (i + 1 < i1) fetch(i + 2);
+        __syncthreads();
+        f4 acc[MT][NT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
+        // one tap: every column, also those past nk in a partial k-tile (their
+        // LDS voxels are zero and their results are not stored; a branch here
+        // would serialise the LDS reads)
+        auto tap = [&](int t, const bf4 (&wa)[MT][NCK]) {
+            const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
+            const bf16* base = tile + ((i + di) & 3) * PLANE +
+                               (((wave + 1 + dj) * NQ + q) * VOX + 1 + dk + col) * 4;
+#pragma unroll
+            for (int ck = 0; ck < NCK; ++ck)
+#pragma unroll
+                for (int n = 0; n < NT; ++n) {
+                    const bf4 xb = *(const bf4*)(base + ck * 16 * VOX + n * 64);
+#pragma unroll
+                    for (int m = 0; m < MT; ++m)
+                        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[m][ck], xb, acc[m][n],
+                                                                            0, 0, 0);
+                }
+        };
+        if constexpr (WREG) {
+#pragma unroll
+            for (int t = 0; t < 27; ++t) {
+                const bf4 w1[1][1] = {{wr[t]}};
+                tap(t, w1);
+            }
+        } else {   // weights one tap ahead; the tap loop stays rolled
+            bf4 wa[MT][NCK], wn[MT][NCK];
+            wload(0, wa);
+#pragma unroll 1
+            for (int t = 0; t < 27; ++t) {
+                if (t + 1 < 27) wload(t + 1, wn);
+                tap(t, wa);
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+#pragma unroll
+                    for (int ck = 0; ck < NCK; ++ck) wa[m][ck] = wn[m][ck];
+            }
+        }
+        if (j < L.nj) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                const int co0 = m * 16 + 4 * q;
+                f4 bv = f4{0.f, 0.f, 0.f, 0.f};
+                if (bias != nullptr) bv = f4{bias[co0], bias[co0 + 1], bias[co0 + 2], bias[co0 + 3]};
+#pragma unroll
+                for (int n = 0; n < NT; ++n) {
+                    const int k = k0 + 16 * n + col;   // nk is a multiple of 4 only
+                    if (k >= L.nk) continue;           // at the coarser levels
+                    f4 o = acc[m][n] + bv;
+                    if (RELU) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) o[e] = o[e] > 0.f ? o[e] : 0.f;
+                    }
+                    store4(Y + (offB + ((size_t)i * L.nj + j) * L.nk + k) * COUT + co0, o);
+                }
+            }
+        }
+    }
+}
+
+// The one-block case (16 -> 16 channels: e0a, e0b, d0b, out and their
+// backward-data) as a kernel of its own: weights in registers, 4 rows x 64
+// voxels, the taps unrolled; the general kernel's form of it is ~60 % slower.
+constexpr int CT_TJ = 4, CT_TK = 64;
+constexpr int CT_ROWS = CT_TJ + 2, CT_VOX = CT_TK + 2;
+constexpr int CT_PLANE = CT_ROWS * CT_VOX * 16;            // BF16 elements per slot
+constexpr int CT_CHUNKS = CT_ROWS * CT_VOX * 2;            // 8-channel chunks per plane
+constexpr int CT_PER_THREAD = (CT_CHUNKS + 255) / 256;
 
 template <bool RELU, typename TIN, typename TOUT>
 __global__ __launch_bounds__(256) void k_conv16_tiled(const TIN* __restrict__ X,
@@ -536,12 +696,22 @@ static int eb(size_t n) {
 template <int CA, int CB, int COUT, bool UPA, bool RELU, typename TIN, typename TOUT>
 static void conv(UNet* net, const TIN* A, const TIN* B, const bf16* W, const float* bias, TOUT* Y,
                  const NetLvl& L, const NetLvl& LA, int batch) {
-    if constexpr (CA == 16 && CB == 0 && COUT == 16 && !UPA) {
-        if (net->conv_tiled) {
+    if (net->conv_tiled) {
+        if constexpr (CA == 16 && CB == 0 && COUT == 16 && !UPA) {
             const int nkt = (L.nk + CT_TK - 1) / CT_TK, njt = (L.nj + CT_TJ - 1) / CT_TJ;
             dim3 grd(nkt * njt, (L.ni + CT_ICH - 1) / CT_ICH, batch);
             hipLaunchKernelGGL((k_conv16_tiled<RELU, TIN, TOUT>), grd, dim3(256), 0, net->stream, A,
                                W, bias, Y, L);
+            return;
+        } else if constexpr ((CA == 16 && CB == 0 && COUT == 32 && !UPA) ||
+                             (CA == 32 && CB == 16 && COUT == 16 && UPA)) {
+            // the layers where the tiled form was measured faster (e1a, d0a);
+            // with more channels the general kernel's 64-voxel weight reuse wins
+            constexpr int TJ = ConvTile<CA + CB>::TJ, TK = ConvTile<CA + CB>::TK;
+            const int nkt = (L.nk + TK - 1) / TK, njt = (L.nj + TJ - 1) / TJ;
+            dim3 grd(nkt * njt, (L.ni + CT_ICH - 1) / CT_ICH, batch);
+            hipLaunchKernelGGL((k_conv_tiled<CA, CB, COUT, UPA, RELU, TIN, TOUT>), grd, dim3(64 * TJ),
+                               0, net->stream, A, B, W, bias, Y, L, LA);
             return;
         }
     }

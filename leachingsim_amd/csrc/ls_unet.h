@@ -25,7 +25,7 @@ struct NetLvl {
 struct UNet {
     int batch = 1;
     NetLvl L[3];
-    bool conv_tiled = true;      // LDS-tiled kernel for the 16 -> 16 channel layers
+    bool conv_tiled = true;      // LDS-tiled convolution kernels (else the general one)
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string err;

@@ -104,8 +104,8 @@ class Net:
         self._check(self.lib.ls_net_step(self.h, _fp(g), float(lr)))
 
     def set_conv(self, tiled):
-        """Kernel of the 16 -> 16 channel layers: LDS-tiled (True, default)
-        or the general implicit-GEMM kernel (False); both give the same bits."""
+        """Convolution kernels: LDS-tiled (True, default) or the general
+        implicit-GEMM kernel (False); both give the same bits."""
         self._check(self.lib.ls_net_set_conv(self.h, 1 if tiled else 0))
 
     def time_forward(self, reps=5):

@@ -96,10 +96,10 @@ def test_unet_gradient_euler_identity():
 
 @pytest.mark.parametrize("shape,batch", [((16, 16, 32), 2), ((32, 48, 80), 1), ((16, 32, 64), 3)])
 def test_tiled_conv_matches_direct(shape, batch):
-    """The LDS-tiled kernel of the 16 -> 16 channel layers (forward e0a, e0b,
-    d0b, out and their backward-data) issues the MFMA sequence of the general
-    kernel, so forward output and weight gradient agree bit for bit.  nk = 80
-    leaves a k-tile with one valid 16-voxel column of four."""
+    """The LDS-tiled convolution kernels (every layer shape: plain, with the
+    upsampled concatenation, and backward-data) issue the MFMA sequence of
+    the general kernel, so forward output and weight gradient agree bit for
+    bit.  nk = 80 leaves a partial k-tile at every level."""
     net, w, x, rng = setup(5, shape=shape, batch=batch)
     gout = rng.standard_normal((batch,) + shape).astype(np.float32)
     res = {}
