@@ -246,6 +246,31 @@ def stencil_only(img, label_bytes, kernel, reps):
           flush=True)
 
 
+def unet_only(size, reps):
+    """Time the U-Net forward on the grid the learned preconditioner uses for a
+    size^3 pressure grid (halved, padded to multiples of 16), with the tiled
+    convolution kernels and with the general kernel everywhere (HIP events,
+    trained weights, seeded input)."""
+    from leachingsim_amd import unet_spec as S
+    from leachingsim_amd.learn import WEIGHTS, load_weights
+    from leachingsim_amd.unet import Net
+    n = -(-((size + 1) // 2) // 16) * 16
+    net = Net((n, n, n), batch=1, trainable=False, weights=load_weights(WEIGHTS))
+    x = np.random.default_rng(0).standard_normal((1, n, n, n, S.CIN)).astype(np.float32)
+    net.forward(x)
+    del x
+    flop = sum(2.0 * 27 * L["cin"] * L["cout"] * n ** 3 / 8 ** L["level"] for L in S.LAYERS)
+    out = {"kernel": "unet_forward", "grid": [n, n, n], "reps": reps, "gflop": flop / 1e9}
+    for tiled in (True, False):
+        net.set_conv(tiled)
+        ms = net.time_forward(reps)
+        key = "tiled" if tiled else "general"
+        out[f"ms_{key}"] = ms
+        out[f"tflops_{key}"] = flop / ms / 1e9
+    net.close()
+    print(json.dumps(out), flush=True)
+
+
 def summary(sec, reps, steps):
     return {"steps_per_s": steps / sec,
             "press_iters_per_step": [r["press_iters"] for r in reps],
@@ -283,9 +308,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-crop", type=int, default=96)
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--mode", default="steps", choices=["steps", "stencil"],
+    ap.add_argument("--mode", default="steps", choices=["steps", "stencil", "unet"],
                     help="stencil: assemble the pressure operator once and time only the "
-                         "stencil and a copy kernel (for counter collection)")
+                         "stencil and a copy kernel (for counter collection); unet: time "
+                         "the U-Net forward on the preconditioner's grid")
     ap.add_argument("--kernel", default="schur", choices=list(KERNELS),
                     help="kernel timed in --mode stencil")
     ap.add_argument("--reps", type=int, default=20)
@@ -295,6 +321,9 @@ def main():
               "(the i-slab decomposition across GPUs is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
+    if args.mode == "unet":
+        unet_only(args.size or 512, args.reps)
+        return
     t0 = time.perf_counter()
     img, lb, dt, desc = workload(args.workload, args.size)
     log(f"workload {desc}: generated in {time.perf_counter() - t0:.1f} s; labels "
