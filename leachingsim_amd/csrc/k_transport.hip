@@ -28,6 +28,11 @@ struct TGeo {
 struct AdrConst {
     double inv_dt, alpha, theta, D, Dpf, dx;
     double phi, evap;      // implicit saturation: porosity, evaporative flux
+    // -D_eff / dx^2 next to a Pore and to other phases, as the row computes
+    // them (the host evaluates the same expression, so the bits agree)
+    double diff_pore, diff_other;
+    double inv_dx;         // 1 / dx, exact when dx_pow2
+    int dx_pow2;           // dx is a power of two: q / dx == q * (1 / dx)
 };
 
 __device__ __forceinline__ void tdecode(uint32_t v, const TGeo& g, int& i, int& j, int& k) {
@@ -108,11 +113,11 @@ __device__ __forceinline__ void adr_row(const uint8_t* __restrict__ img, const d
         }
         double actual_q_face = q_face;
         if (neigh_type == PREC || neigh_type == SUL) actual_q_face = 0.0;
-        double D_eff = (neigh_type == PORE) ? (a.D * a.Dpf) : a.D;
-        double diffusion_coeff = -D_eff / (a.dx * a.dx);
+        // D_eff = D Dpf next to a Pore, else D; diffusion_coeff = -D_eff / dx^2
+        double diffusion_coeff = (neigh_type == PORE) ? a.diff_pore : a.diff_other;
         double coeff_neighbor = diffusion_coeff;
         double coeff_self = -diffusion_coeff;
-        double adv_coeff = actual_q_face / a.dx;
+        double adv_coeff = a.dx_pow2 ? actual_q_face * a.inv_dx : actual_q_face / a.dx;
         if (adv_coeff > 0) coeff_self += adv_coeff;
         else coeff_neighbor -= adv_coeff;
         R.off[f] = a.theta * coeff_neighbor;
@@ -1270,6 +1275,11 @@ static AdrConst adr_const(const ls_ctx* c, double dt) {
     a.dx = c->P.dx;
     a.phi = c->P.porosity;
     a.evap = c->P.evap_flux;
+    a.diff_pore = -(a.D * a.Dpf) / (a.dx * a.dx);
+    a.diff_other = -a.D / (a.dx * a.dx);
+    int e2 = 0;
+    a.dx_pow2 = std::frexp(a.dx, &e2) == 0.5 ? 1 : 0;
+    a.inv_dx = 1.0 / a.dx;
     return a;
 }
 
