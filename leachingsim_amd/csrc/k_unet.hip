@@ -151,7 +151,7 @@ constexpr int CT_ICH = 16;
 
 template <int CIN> struct ConvTile;
 template <> struct ConvTile<16> { static constexpr int TJ = 4, TK = 64; };
-template <> struct ConvTile<48> { static constexpr int TJ = 2, TK = 32; };
+template <> struct ConvTile<48> { static constexpr int TJ = 4, TK = 16; };
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
@@ -168,7 +168,7 @@ __device__ __forceinline__ bf8 load8(const float* p) {
 }
 
 template <int CA, int CB, int COUT, bool UPA, bool RELU, typename TIN, typename TOUT>
-__global__ __launch_bounds__(64 * ConvTile<CA + CB>::TJ) void k_conv_tiled(
+__global__ __launch_bounds__(64 * ConvTile<CA + CB>::TJ, 2) void k_conv_tiled(
     const TIN* __restrict__ A, const TIN* __restrict__ B, const bf16* __restrict__ W,
     const float* __restrict__ bias, TOUT* __restrict__ Y, NetLvl L, NetLvl LA) {
     constexpr int CIN = CA + CB, NQ = CIN / 4, NCK = CIN / 16, MT = COUT / 16;
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(64 * ConvTile<CA + CB>::TJ) void k_conv_tiled(
     constexpr int PLANE = (TJ + 2) * NQ * VOX * 4;           // BF16 elements per slot
     constexpr int CHUNKS = (TJ + 2) * VOX * G8;              // 8-channel chunks per plane
     constexpr int PER = (CHUNKS + NTH - 1) / NTH;
-    constexpr bool WREG = MT * NCK == 1;
+    constexpr bool WREG = MT == 1 && NCK <= 3;              // weights in registers
     __shared__ bf16 tile[4 * PLANE];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, q = lane >> 4;
@@ -187,10 +187,13 @@ __global__ __launch_bounds__(64 * ConvTile<CA + CB>::TJ) void k_conv_tiled(
     const int i0 = blockIdx.y * CT_ICH, i1 = min(i0 + CT_ICH, L.ni);
     const int b = blockIdx.z;
     const size_t offA = (size_t)b * (UPA ? LA.n : L.n), offB = (size_t)b * L.n;
-    bf4 wr[WREG ? 27 : 1];
+    bf4 wr[WREG ? 27 * NCK : 1];
     if constexpr (WREG) {
 #pragma unroll
-        for (int t = 0; t < 27; ++t) wr[t] = load4(W + ((size_t)t * 16 + col) * 16 + 4 * q);
+        for (int t = 0; t < 27; ++t)
+#pragma unroll
+            for (int ck = 0; ck < NCK; ++ck)
+                wr[t * NCK + ck] = load4(W + ((size_t)t * 16 + col) * CIN + 16 * ck + 4 * q);
     }
     // chunk c of a plane: row c / (VOX G8), voxel (c / G8) % VOX, channels
     // 8 g .. 8 g + 7 with g = c % G8 (quads 2 g, 2 g + 1)
@@ -277,7 +280,9 @@ __global__ __launch_bounds__(64 * ConvTile<CA + CB>::TJ) void k_conv_tiled(
         if constexpr (WREG) {
 #pragma unroll
             for (int t = 0; t < 27; ++t) {
-                const bf4 w1[1][1] = {{wr[t]}};
+                bf4 w1[1][NCK];
+#pragma unroll
+                for (int ck = 0; ck < NCK; ++ck) w1[0][ck] = wr[t * NCK + ck];
                 tap(t, w1);
             }
         } else {   // weights one tap ahead; the tap loop stays rolled
