@@ -139,13 +139,15 @@ __global__ __launch_bounds__(256) void k_conv(const TIN* __restrict__ A, const T
 // layer): four rotating slots, the plane after next fetched into registers
 // while the current one is computed, one barrier per plane.  A slot is laid
 // out [row][channel quad][voxel][4 channels], so the 16 lanes that read one
-// quad of a B operand read 128 contiguous bytes (no bank conflict).  The 27
-// weight fragments stay in registers when the layer is one 16 x 16 block;
-// otherwise each tap's fragments are loaded (L1/L2) one tap ahead.  The MFMA
-// sequence of an output tile (taps, then 16-channel chunks, in order) is that
-// of k_conv.  Tile shapes (per input channel count) keep a block's LDS near
-// 52 KB (three blocks per CU).  The grid of the first level has extents that
-// are multiples of 16, so a level's extents are multiples of 4.
+// quad of a B operand read 128 contiguous bytes (no bank conflict).  With
+// one 16-channel output block and at most 48 input channels the weight
+// fragments (27 per 16 input channels) stay in registers, at most 256 per
+// lane (two waves per SIMD); otherwise each tap's fragments are loaded
+// (L1/L2) one tap ahead.  The MFMA sequence of an output tile (taps, then
+// 16-channel chunks, in order) is that of k_conv.  Tile shapes per input
+// channel count keep a block's LDS at 41-51 KB.  The grid of the first level
+// has extents that are multiples of 16, so a level's extents are multiples
+// of 4.
 // ---------------------------------------------------------------------------
 constexpr int CT_ICH = 16;
 
