@@ -327,9 +327,9 @@ int ls_net_step(ls_net* net, const float* gout, float lr);
 /* Average time of `reps` forward passes on the device-resident input. */
 int ls_net_time_forward(ls_net* net, int reps, double* ms_per_forward);
 /* Convolution kernels (forward and backward-data): 1 (default) the
- * LDS-tiled kernels for the layers that have one (the 16 -> 16 channel
- * layers, e1a, d0a), 0 the general implicit-GEMM kernel everywhere.  Both
- * give the same bits; the switch is for tests and timing. */
+ * LDS-tiled kernels for the layers that have one (at most 48 input
+ * channels), 0 the general implicit-GEMM kernel everywhere.  Both give the
+ * same bits; the switch is for tests and timing. */
 int ls_net_set_conv(ls_net* net, int tiled);
 
 /* Weights of the learned coarse-level preconditioner used by

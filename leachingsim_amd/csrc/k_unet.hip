@@ -132,28 +132,32 @@ __global__ __launch_bounds__(256) void k_conv(const TIN* __restrict__ A, const T
 
 // ---------------------------------------------------------------------------
 // Tiled convolution (forward, and backward-data with the mirrored weights):
-// the same result as k_conv, bit for bit.  A block of
-// TJ waves owns TJ rows (j) x TK voxels (k) of an output plane and marches
-// CT_ICH planes along i.  The input planes i - 1 .. i + 1 of the tile and its
+// the same result as k_conv, bit for bit.  A block of TJ x MS waves owns TJ
+// rows (j) x TK voxels (k) of an output plane and marches CT_ICH planes along
+// i; the MS waves of a row split its output channels.  The input planes i - 1 .. i + 1 of the tile and its
 // one-voxel halo sit in LDS as BF16 ([up2(A), B] channels for a decoder
 // layer): four rotating slots, the plane after next fetched into registers
 // while the current one is computed, one barrier per plane.  A slot is laid
 // out [row][channel quad][voxel][4 channels], so the 16 lanes that read one
 // quad of a B operand read 128 contiguous bytes (no bank conflict).  With
-// one 16-channel output block and at most 48 input channels the weight
-// fragments (27 per 16 input channels) stay in registers, at most 256 per
-// lane (two waves per SIMD); otherwise each tap's fragments are loaded
+// one 16-channel output block per wave and at most 48 input channels the
+// weight fragments (27 per 16 input channels) stay in registers, at most 256
+// per lane (two waves per SIMD); otherwise each tap's fragments are loaded
 // (L1/L2) one tap ahead.  The MFMA sequence of an output tile (taps, then
-// 16-channel chunks, in order) is that of k_conv.  Tile shapes per input
-// channel count keep a block's LDS at 41-51 KB.  The grid of the first level
-// has extents that are multiples of 16, so a level's extents are multiples
-// of 4.
+// 16-channel chunks, in order) is that of k_conv.  Tile shapes keep a
+// block's LDS at 26-41 KB.  The grid of the first level has extents that are
+// multiples of 16, so a level's extents are multiples of 4.
 // ---------------------------------------------------------------------------
 constexpr int CT_ICH = 16;
 
-template <int CIN> struct ConvTile;
-template <> struct ConvTile<16> { static constexpr int TJ = 4, TK = 64; };
-template <> struct ConvTile<48> { static constexpr int TJ = 4, TK = 16; };
+// Tile of a layer: TJ rows x TK voxels per block, and MS waves per row that
+// split the output channels (each wave computes COUT / MS of them from the
+// shared LDS tile, so that its weight fragments fit in registers).
+template <int CIN, int COUT> struct ConvTile;
+template <> struct ConvTile<16, 32> { static constexpr int TJ = 2, TK = 64, MS = 2; };
+template <> struct ConvTile<32, 32> { static constexpr int TJ = 2, TK = 32, MS = 2; };
+template <> struct ConvTile<32, 64> { static constexpr int TJ = 1, TK = 32, MS = 4; };
+template <> struct ConvTile<48, 16> { static constexpr int TJ = 4, TK = 16, MS = 1; };
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
@@ -170,18 +174,19 @@ __device__ __forceinline__ bf8 load8(const float* p) {
 }
 
 template <int CA, int CB, int COUT, bool UPA, bool RELU, typename TIN, typename TOUT>
-__global__ __launch_bounds__(64 * ConvTile<CA + CB>::TJ, 2) void k_conv_tiled(
+__global__ __launch_bounds__((64 * ConvTile<CA + CB, COUT>::TJ * ConvTile<CA + CB, COUT>::MS), 2) void k_conv_tiled(
     const TIN* __restrict__ A, const TIN* __restrict__ B, const bf16* __restrict__ W,
     const float* __restrict__ bias, TOUT* __restrict__ Y, NetLvl L, NetLvl LA) {
     constexpr int CIN = CA + CB, NQ = CIN / 4, NCK = CIN / 16, MT = COUT / 16;
-    constexpr int TJ = ConvTile<CIN>::TJ, TK = ConvTile<CIN>::TK, NT = TK / 16;
-    constexpr int NTH = 64 * TJ, VOX = TK + 2, G8 = CIN / 8;
+    constexpr int TJ = ConvTile<CIN, COUT>::TJ, TK = ConvTile<CIN, COUT>::TK, NT = TK / 16;
+    constexpr int MS = ConvTile<CIN, COUT>::MS, MW = MT / MS;   // output blocks per wave
+    constexpr int NTH = 64 * TJ * MS, VOX = TK + 2, G8 = CIN / 8;
     constexpr int PLANE = (TJ + 2) * NQ * VOX * 4;           // BF16 elements per slot
     constexpr int CHUNKS = (TJ + 2) * VOX * G8;              // 8-channel chunks per plane
     constexpr int PER = (CHUNKS + NTH - 1) / NTH;
-    constexpr bool WREG = MT == 1 && NCK <= 3;              // weights in registers
+    constexpr bool WREG = MW == 1 && NCK <= 3;              // weights in registers
     __shared__ bf16 tile[4 * PLANE];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) / MS, mg = (threadIdx.x >> 6) % MS;
     const int col = lane & 15, q = lane >> 4;
     const int nkt = L.nk / TK + (L.nk % TK ? 1 : 0);
     const int k0 = (int)(blockIdx.x % (unsigned)nkt) * TK;
@@ -195,7 +200,8 @@ __global__ __launch_bounds__(64 * ConvTile<CA + CB>::TJ, 2) void k_conv_tiled(
         for (int t = 0; t < 27; ++t)
 #pragma unroll
             for (int ck = 0; ck < NCK; ++ck)
-                wr[t * NCK + ck] = load4(W + ((size_t)t * 16 + col) * CIN + 16 * ck + 4 * q);
+                wr[t * NCK + ck] =
+                    load4(W + ((size_t)t * COUT + mg * 16 + col) * CIN + 16 * ck + 4 * q);
     }
     // chunk c of a plane: row c / (VOX G8), voxel (c / G8) % VOX, channels
     // 8 g .. 8 g + 7 with g = c % G8 (quads 2 g, 2 g + 1)
@@ -237,12 +243,14 @@ __global__ __launch_bounds__(64 * ConvTile<CA + CB>::TJ, 2) void k_conv_tiled(
             }
         }
     };
-    auto wload = [&](int t, bf4 (&wa)[MT][NCK]) {
+    // this wave's output blocks are mg MW .. mg MW + MW - 1
+    auto wload = [&](int t, bf4 (&wa)[MW][NCK]) {
 #pragma unroll
-        for (int m = 0; m < MT; ++m)
+        for (int m = 0; m < MW; ++m)
 #pragma unroll
             for (int ck = 0; ck < NCK; ++ck)
-                wa[m][ck] = load4(W + ((size_t)t * COUT + m * 16 + col) * CIN + 16 * ck + 4 * q);
+                wa[m][ck] = load4(W + ((size_t)t * COUT + (mg * MW + m) * 16 + col) * CIN + 16 * ck +
+                                  4 * q);
     };
     fetch(i0 - 1);
     publish((i0 - 1) & 3);
@@ -256,15 +264,15 @@ __global__ __launch_bounds__(64 * ConvTile<CA + CB>::TJ, 2) void k_conv_tiled(
         publish((i + 1) & 3);
         if (i + 1 < i1) fetch(i + 2);
         __syncthreads();
-        f4 acc[MT][NT];
+        f4 acc[MW][NT];
 #pragma unroll
-        for (int m = 0; m < MT; ++m)
+        for (int m = 0; m < MW; ++m)
 #pragma unroll
             for (int n = 0; n < NT; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
         // one tap: every column, also those past nk in a partial k-tile (their
         // LDS voxels are zero and their results are not stored; a branch here
         // would serialise the LDS reads)
-        auto tap = [&](int t, const bf4 (&wa)[MT][NCK]) {
+        auto tap = [&](int t, const bf4 (&wa)[MW][NCK]) {
             const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
             const bf16* base = tile + ((i + di) & 3) * PLANE +
                                (((wave + 1 + dj) * NQ + q) * VOX + 1 + dk + col) * 4;
@@ -274,7 +282,7 @@ __global__ __launch_bounds__(64 * ConvTile<CA + CB>::TJ, 2) void k_conv_tiled(
                 for (int n = 0; n < NT; ++n) {
                     const bf4 xb = *(const bf4*)(base + ck * 16 * VOX + n * 64);
 #pragma unroll
-                    for (int m = 0; m < MT; ++m)
+                    for (int m = 0; m < MW; ++m)
                         acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[m][ck], xb, acc[m][n],
                                                                             0, 0, 0);
                 }
@@ -288,22 +296,22 @@ __global__ __launch_bounds__(64 * ConvTile<CA + CB>::TJ, 2) void k_conv_tiled(
                 tap(t, w1);
             }
         } else {   // weights one tap ahead; the tap loop stays rolled
-            bf4 wa[MT][NCK], wn[MT][NCK];
+            bf4 wa[MW][NCK], wn[MW][NCK];
             wload(0, wa);
 #pragma unroll 1
             for (int t = 0; t < 27; ++t) {
                 if (t + 1 < 27) wload(t + 1, wn);
                 tap(t, wa);
 #pragma unroll
-                for (int m = 0; m < MT; ++m)
+                for (int m = 0; m < MW; ++m)
 #pragma unroll
                     for (int ck = 0; ck < NCK; ++ck) wa[m][ck] = wn[m][ck];
             }
         }
         if (j < L.nj) {
 #pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                const int co0 = m * 16 + 4 * q;
+            for (int m = 0; m < MW; ++m) {
+                const int co0 = (mg * MW + m) * 16 + 4 * q;
                 f4 bv = f4{0.f, 0.f, 0.f, 0.f};
                 if (bias != nullptr) bv = f4{bias[co0], bias[co0 + 1], bias[co0 + 2], bias[co0 + 3]};
 #pragma unroll
@@ -711,14 +719,18 @@ static void conv(UNet* net, const TIN* A, const TIN* B, const bf16* W, const flo
                                W, bias, Y, L);
             return;
         } else if constexpr ((CA == 16 && CB == 0 && COUT == 32 && !UPA) ||
+                             (CA == 32 && CB == 0 && COUT == 32 && !UPA) ||
+                             (CA == 32 && CB == 0 && COUT == 64 && !UPA) ||
                              (CA == 32 && CB == 16 && COUT == 16 && UPA)) {
-            // the layers where the tiled form was measured faster (e1a, d0a);
-            // with more channels the general kernel's 64-voxel weight reuse wins
-            constexpr int TJ = ConvTile<CA + CB>::TJ, TK = ConvTile<CA + CB>::TK;
-            const int nkt = (L.nk + TK - 1) / TK, njt = (L.nj + TJ - 1) / TJ;
+            // the layers where the tiled form was measured faster: e1a, e1b,
+            // d1b (and their backward-data), e2a, d0a; with 64 or more input
+            // channels the weights no longer fit in registers and the general
+            // kernel's 64-voxel weight reuse wins
+            using T = ConvTile<CA + CB, COUT>;
+            const int nkt = (L.nk + T::TK - 1) / T::TK, njt = (L.nj + T::TJ - 1) / T::TJ;
             dim3 grd(nkt * njt, (L.ni + CT_ICH - 1) / CT_ICH, batch);
-            hipLaunchKernelGGL((k_conv_tiled<CA, CB, COUT, UPA, RELU, TIN, TOUT>), grd, dim3(64 * TJ),
-                               0, net->stream, A, B, W, bias, Y, L, LA);
+            hipLaunchKernelGGL((k_conv_tiled<CA, CB, COUT, UPA, RELU, TIN, TOUT>), grd,
+                               dim3(64 * T::TJ * T::MS), 0, net->stream, A, B, W, bias, Y, L, LA);
             return;
         }
     }
