@@ -77,34 +77,42 @@ __device__ __forceinline__ void adr_row(const uint8_t* __restrict__ img, const d
         R.diag = 1.0;
         return;
     }
+    // The neighbours' labels (and, for a fluid voxel, the face fluxes) are
+    // all loaded before any decision on them, so the loads overlap; a missing
+    // neighbour points at v itself and is not used.
+    bool has[6];
+    uint32_t nb[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        has[f] = tnbr(g, i, j, k, v, f, nb[f]);
+        if (!has[f]) nb[f] = v;
+    }
+    uint8_t nt[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) nt[f] = img[nb[f]];
     R.diag = a.inv_dt;
     if (tv == ROCK) {          // 2. Rock: 0-D Warren-Root bin
+#pragma unroll
         for (int f = 0; f < 6; ++f) {
-            uint32_t n;
-            if (tnbr(g, i, j, k, v, f, n) && img[n] == PORE) {
+            if (has[f] && nt[f] == PORE) {
                 R.diag += a.alpha;
                 R.off[f] = -a.alpha;
-                R.nb[f] = n;
+                R.nb[f] = nb[f];
                 R.mask |= 1 << f;
             }
         }
         return;
     }
-    // 3. Pore / Air / Sulphide: active fluid network
+    // 3. Pore / Air / Sulphide: active fluid network.  Outward face flux:
+    // +face q[v], -face -q[v - s] (faces +i, -i, +j, -j, +k, -k)
+    const double qp[3] = {qx[v], qy[v], qz[v]};
+    const double qm[3] = {qx[nb[1]], qy[nb[3]], qz[nb[5]]};
+#pragma unroll
     for (int f = 0; f < 6; ++f) {
-        uint32_t n;
-        if (!tnbr(g, i, j, k, v, f, n)) continue;
-        double q_face;
-        switch (f) {
-            case 0: q_face = qx[v]; break;
-            case 1: q_face = -qx[n]; break;
-            case 2: q_face = qy[v]; break;
-            case 3: q_face = -qy[n]; break;
-            case 4: q_face = qz[v]; break;
-            default: q_face = -qz[n]; break;
-        }
-        const uint8_t neigh_type = img[n];
-        R.nb[f] = n;
+        if (!has[f]) continue;
+        const double q_face = (f & 1) ? -qm[f >> 1] : qp[f >> 1];
+        const uint8_t neigh_type = nt[f];
+        R.nb[f] = nb[f];
         R.mask |= 1 << f;
         if (neigh_type == ROCK) {
             R.diag += a.alpha;
@@ -125,6 +133,30 @@ __device__ __forceinline__ void adr_row(const uint8_t* __restrict__ img, const d
         R.cs[f] = coeff_self;
         R.cn[f] = coeff_neighbor;
         R.explicit_mask |= 1 << f;
+    }
+}
+
+// The explicit (1 - theta) part of the right-hand sides of both systems
+// (src/Simulation.cpp:1221-1222): the neighbours' old values are loaded
+// together before the sum, which runs over the faces in order.
+__device__ __forceinline__ void explicit_rhs(const AdrRow& R, uint32_t v,
+                                             const double* __restrict__ x0,
+                                             const double* __restrict__ x1, double xa, double xb,
+                                             const AdrConst& a, double& ba, double& bbv) {
+    double n0[6], n1[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const uint32_t n = (R.explicit_mask & (1 << f)) ? R.nb[f] : v;
+        n0[f] = x0[n];
+        n1[f] = x1[n];
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        if (!(R.explicit_mask & (1 << f))) continue;
+        double e0 = -(R.cs[f] * xa + R.cn[f] * n0[f]);
+        double e1 = -(R.cs[f] * xb + R.cn[f] * n1[f]);
+        ba += (1.0 - a.theta) * e0;
+        bbv += (1.0 - a.theta) * e1;
     }
 }
 
@@ -211,14 +243,7 @@ __global__ void k_adr_start(const uint8_t* __restrict__ img, const double* __res
             } else {
                 ba = xa * a.inv_dt;
                 bbv = xb * a.inv_dt;
-                for (int f = 0; f < 6; ++f) {
-                    if (!(R.explicit_mask & (1 << f))) continue;
-                    const uint32_t n = R.nb[f];
-                    double e0 = -(R.cs[f] * xa + R.cn[f] * x0[n]);
-                    double e1 = -(R.cs[f] * xb + R.cn[f] * x1[n]);
-                    ba += (1.0 - a.theta) * e0;
-                    bbv += (1.0 - a.theta) * e1;
-                }
+                explicit_rhs(R, v, x0, x1, xa, xb, a, ba, bbv);
             }
             b0[v] = ba;
             b1[v] = bbv;
@@ -418,14 +443,7 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows(
         } else {
             ba = xa * a.inv_dt;
             bbv = xb * a.inv_dt;
-            for (int f = 0; f < 6; ++f) {
-                if (!(R.explicit_mask & (1 << f))) continue;
-                const uint32_t n = R.nb[f];
-                double e0 = -(R.cs[f] * xa + R.cn[f] * x0[n]);
-                double e1 = -(R.cs[f] * xb + R.cn[f] * x1[n]);
-                ba += (1.0 - a.theta) * e0;
-                bbv += (1.0 - a.theta) * e1;
-            }
+            explicit_rhs(R, v, x0, x1, xa, xb, a, ba, bbv);
         }
         b0[v] = ba;
         b1[v] = bbv;
@@ -691,14 +709,7 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
         } else {
             ba = xa * a.inv_dt;
             bbv = xb * a.inv_dt;
-            for (int f = 0; f < 6; ++f) {
-                if (!(R.explicit_mask & (1 << f))) continue;
-                const uint32_t n = R.nb[f];
-                double e0 = -(R.cs[f] * xa + R.cn[f] * x0[n]);
-                double e1 = -(R.cs[f] * xb + R.cn[f] * x1[n]);
-                ba += (1.0 - a.theta) * e0;
-                bbv += (1.0 - a.theta) * e1;
-            }
+            explicit_rhs(R, v, x0, x1, xa, xb, a, ba, bbv);
         }
         b0[v] = ba;
         b1[v] = bbv;
