@@ -128,9 +128,12 @@ typedef struct ls_params {
                                 Schur complement of the scaled operator (about
                                 half the iterations of Jacobi-PCG), on vectors
                                 compacted to the black voxels with one fused
-                                pass per operator apply when nk % 8 == 0 and
-                                nk <= 1024 (else as 2); 2: the
-                                same on full-length vectors in two passes; both
+                                pass per operator apply when nk % 8 == 0,
+                                nk <= 1024 and a row of nk / 2 values fills at
+                                least 3/8 of the fused kernel's 256-value wave
+                                rows (so 192 <= nk <= 1024), else as 2; 3: the
+                                fused form wherever nk allows it; 2: the same
+                                on full-length vectors in two passes; 1-3
                                 need nk % 4 == 0; 0: inner CG on the operator  */
     int adr_schur;           /* transport and implicit saturation (1): BiCGStab on
                                 the red-black Schur complement of the stored rows

@@ -1188,7 +1188,15 @@ static bool schur_shape(ls_ctx* c, SchurShape* sh) {
 static int schur_form(ls_ctx* c) {
     if (!c->P.mixed_schur || !use_vec<float>(c)) return 0;
     SchurShape sh;
-    if (c->P.mixed_schur == 1 && schur_shape(c, &sh)) return 1;
+    if ((c->P.mixed_schur == 1 || c->P.mixed_schur == 3) && schur_shape(c, &sh)) {
+        // 1 takes the fused form only where a row fills at least 3/8 of the
+        // lanes of its waves (each covers SCHUR_VW * WAVE compacted values):
+        // cold solves on smooth cubes measured two-pass 14-32 % faster at
+        // 64^3-128^3 (rows 1/8-1/4 full), fused 27-38 % faster at 192^3 and
+        // 256^3 (3/8 and 1/2); 3 takes it wherever it applies
+        const int lanes = sh.wr * SCHUR_VW * WAVE;
+        if (c->P.mixed_schur == 3 || 8 * (c->nk / 2) >= 3 * lanes) return 1;
+    }
     return 2;
 }
 

@@ -110,7 +110,7 @@ def test_fused_schur_rejects_wide_rows():
 def test_schur_matches_plain_in_fewer_iterations(seed):
     img = G.smoothed_threshold((40, 36, 48), 2.0, 0.4, seed)     # nk % 8 == 0: fused form
     r64, p64 = solve(img, 1500.0, solver=L.SOLVER_PLAIN64)
-    rs, ps = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=1)
+    rs, ps = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=3)
     r2, p2 = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=2)
     rm, pm = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=0)
     for r in (r64, rs, r2, rm):
@@ -123,11 +123,11 @@ def test_schur_matches_plain_in_fewer_iterations(seed):
     assert abs(rs.press_iters - r2.press_iters) <= max(3, 0.03 * r2.press_iters)
 
 
-@pytest.mark.parametrize("nk", [28, 32])     # two-pass form, fused form
-def test_schur_coupled_steps_match_oracle(nk):
+@pytest.mark.parametrize("nk,form", [(28, 1), (32, 1), (32, 3)])   # two-pass, two-pass, fused
+def test_schur_coupled_steps_match_oracle(nk, form):
     from tests.helpers import both
     img = small_mixed_geometry((24, 20, nk), 9)
-    P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED, mixed_schur=1,
+    P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED, mixed_schur=form,
                          press_rtol=1e-12)
     g, o = both(img.shape, P, img)
     t = 0.0
@@ -139,3 +139,24 @@ def test_schur_coupled_steps_match_oracle(nk):
         t += 30.0
     assert relerr(g.download(L.F_PRESSURE), o.download(L.F_PRESSURE)) <= 1e-7
     assert relerr(g.download(L.F_CONC), o.download(L.F_CONC)) <= 1e-7
+
+
+@pytest.mark.parametrize("nk", [64, 192])
+def test_schur_form_choice(nk):
+    """mixed_schur = 1 picks the two-pass form for rows of 64 and the fused
+    form for rows of 192 (the fused kernel's waves cover 256 compacted
+    values; below 3/8 of that the two passes are faster).  Whichever it
+    picks, the solution equals the forced form's to the solve tolerance and
+    the iteration counts agree up to the dot products' summation order."""
+    img = G.smoothed_threshold((12, 10, nk), 2.0, 0.4, 3)
+    r1, p1 = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=1)
+    r2, p2 = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=2)
+    r3, p3 = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=3)
+    for r in (r1, r2, r3):
+        assert r.status == 0 and r.press_relres <= 1e-12
+    picked, other = ((r2, p2), (r3, p3)) if nk < 192 else ((r3, p3), (r2, p2))
+    # the chosen form runs the same kernels as the forced one: same bits
+    assert r1.press_iters == picked[0].press_iters
+    assert np.array_equal(p1, picked[1])
+    assert relerr(p1, other[1]) <= 1e-7
+    assert abs(r1.press_iters - other[0].press_iters) <= max(3, 0.03 * r1.press_iters)
