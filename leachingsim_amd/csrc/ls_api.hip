@@ -133,6 +133,10 @@ static int validate_params(ls_ctx* c, const ls_params* p) {
         c->err = "learned_race must be >= 0";
         return -3;
     }
+    if (p->mixed_schur < 0 || p->mixed_schur > 3 || p->adr_schur < 0 || p->adr_schur > 1) {
+        c->err = "invalid mixed_schur (0..3) or adr_schur (0..1)";
+        return -3;
+    }
     return 0;
 }
 

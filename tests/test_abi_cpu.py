@@ -60,3 +60,18 @@ def test_create_without_device_fails_cleanly(lib):
         pytest.skip("a HIP device is present")
     with pytest.raises(L.LsError):
         L.GpuSimulation((8, 8, 8))
+
+
+@pytest.mark.parametrize("field,value", [("mixed_schur", 4), ("mixed_schur", -1),
+                                         ("adr_schur", 2), ("solver", 3),
+                                         ("warm_history", 0), ("learned_race", -1.0)])
+def test_create_rejects_out_of_range_settings(lib, field, value):
+    """ls_create checks its settings before it touches the device, so the
+    refusal (-3, no context returned) is the same with or without a GPU."""
+    kw = {field: value}
+    if field == "warm_history":
+        kw["warm_start"] = L.WARM_PROJECTED
+    p = L.default_params(**kw)
+    h = C.c_void_p()
+    assert lib.ls_create(C.byref(h), 8, 8, 8, 1, C.byref(p)) == -3
+    assert not h.value
