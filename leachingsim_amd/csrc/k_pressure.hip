@@ -1000,6 +1000,33 @@ static int batch_size(const ls_ctx* c) {
     return c->N >= (size_t)64 * 64 * 64 * 8 ? 16 : 32;
 }
 
+// Batch sizing for the inner FP32 CG of the mixed solver.  The device stops at
+// the trigger by itself, so the launches queued past it return at once, but
+// each still costs a dispatch: at 64^3 about as much as a working kernel, and
+// a reliable-update cycle of ~50 iterations wasted a third of its launches in
+// batches of 32.  The next batch is sized from the residual's decay per
+// iteration over the last batch (kept across cycles) to reach the trigger
+// with a 25 % margin, within [4, bs].  Where the host polls changes no
+// iterate: every stop test runs on the device.
+struct InnerBatch {
+    int bs;
+    double rate = 0.0;       // ln(rr) decrease per iteration over the last batch
+    double last_rr = 0.0;
+    int last_iter = 0;
+    explicit InnerBatch(int b) : bs(b) {}
+    void start(double rr, int iter) { last_rr = rr; last_iter = iter; }
+    int next(double rr, int iter, double trig) {
+        if (last_rr > 0.0 && rr > 0.0 && rr < last_rr && iter > last_iter)
+            rate = std::log(last_rr / rr) / (iter - last_iter);
+        last_rr = rr;
+        last_iter = iter;
+        if (!(rate > 0.0) || !(rr > trig) || !(trig > 0.0)) return bs;
+        const double need = std::log(rr / trig) / rate;
+        if (!(need < (double)bs)) return bs;
+        return std::max(4, std::min(bs, (int)std::ceil(1.25 * need) + 2));
+    }
+};
+
 // Plain FP64 Jacobi-PCG (as CG on A^).  Returns 0 converged, 1 not.
 static int solve_plain64(ls_ctx* c, ls_step_report* rep, int maxit) {
     int st = cg64_residual(c, c->P.press_rtol, maxit);
@@ -1047,7 +1074,7 @@ static int solve_plain64(ls_ctx* c, ls_step_report* rep, int maxit) {
 static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
     uint32_t N = (uint32_t)c->N;
     int nbe = ew_blocks(N);
-    const int bs = batch_size(c);
+    InnerBatch ib(batch_size(c));
     const double delta = c->P.mixed_inner_rtol > 0 ? c->P.mixed_inner_rtol : 0.1;
     int st;
     if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
@@ -1106,11 +1133,14 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
         LS_LAUNCH_CHECK();
         first = false;
         if ((st = poll_cg(c))) return st;
+        ib.start(c->h_cg->rr, c->h_cg->iter);
+        int nb = ib.next(c->h_cg->rr, c->h_cg->iter, trig);
         while (!c->h_cg->done) {
-            if ((st = cg_iterations<float>(c, c->x32, c->r32, c->p32, c->ap32, c->c32, nullptr, bs)))
+            if ((st = cg_iterations<float>(c, c->x32, c->r32, c->p32, c->ap32, c->c32, nullptr, nb)))
                 return st;
             if ((st = poll_cg(c))) return st;
             record(c->h_cg->rr);
+            nb = ib.next(c->h_cg->rr, c->h_cg->iter, trig);
         }
         const int iters = c->h_cg->iter;
         // stopped without reaching the trigger or the cap: breakdown / non-finite
@@ -1266,7 +1296,7 @@ static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep, bool fused) {
     const uint32_t N = (uint32_t)c->N;
     const size_t Nv = fused ? c->N / 2 : c->N;
     const int nbe = ew_blocks(N);
-    const int bs = batch_size(c);
+    InnerBatch ib(batch_size(c));
     const double delta = c->P.mixed_inner_rtol > 0 ? c->P.mixed_inner_rtol : 0.1;
     float* rs = (float*)c->tv[0][0];    // Schur residual (FP32 work vectors borrowed from
     float* w = (float*)c->tv[0][1];     // the idle transport buffers)
@@ -1331,11 +1361,14 @@ static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep, bool fused) {
         LS_LAUNCH_CHECK();
         first = false;
         if (!(rrs <= trig)) {
+            ib.start(rrs, c->h_cg->iter);
+            int nb = ib.next(rrs, c->h_cg->iter, trig);
             while (!c->h_cg->done) {
-                if ((st = cg_iterations_rb(c, fused, c->x32, rs, c->p32, c->ap32, w, bs)))
+                if ((st = cg_iterations_rb(c, fused, c->x32, rs, c->p32, c->ap32, w, nb)))
                     return st;
                 if ((st = poll_cg(c))) return st;
                 record(c->h_cg->rr);
+                nb = ib.next(c->h_cg->rr, c->h_cg->iter, trig);
             }
         }
         const int iters = c->h_cg->iter;
