@@ -391,6 +391,8 @@ def main():
         kname = "k_stencil<float, 0>"
     if kernel == "stencil64" and img.shape[2] % 2 != 0:
         kname = "k_stencil<double, 0>"
+    if kernel == "rb_gather" and img.shape[2] % 4 != 0:
+        kname = "k_rb_scalar<1>"
     achieved = bytes_per_launch / (ms * 1e-3) / 1e9 if n else 0.0
     traffic = None
     tr = load_traffic()

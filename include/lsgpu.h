@@ -133,8 +133,9 @@ typedef struct ls_params {
                                 least 3/8 of the fused kernel's 256-value wave
                                 rows (so 192 <= nk <= 1024), else as 2; 3: the
                                 fused form wherever nk allows it; 2: the same
-                                on full-length vectors in two passes; 1-3
-                                need nk % 4 == 0; 0: inner CG on the operator  */
+                                on full-length vectors in two passes (any nk,
+                                vectorised when nk % 4 == 0); 0: inner CG on
+                                the operator                                    */
     int adr_schur;           /* transport and implicit saturation (1): BiCGStab on
                                 the red-black Schur complement of the stored rows
                                 (black voxels, about half the iterations), the

@@ -339,6 +339,70 @@ __global__ __launch_bounds__(WAVE * StencilShape<float>::TJV) void k_rb_vec(
     }
 }
 
+// The red-black pass for any nk (modes 1-4, full-length vectors): one voxel
+// per thread, marching ICHUNK planes along i on k_stencil's grid, with the
+// neighbour sum n in k_rb_vec's order.  One FP64 block partial per block.
+template <int MODE>
+__global__ __launch_bounds__(TK * TJ) void k_rb_scalar(
+    const float* __restrict__ xn, const float* __restrict__ xc, float* __restrict__ y,
+    const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
+    SGeo g, double* __restrict__ partials, const int* __restrict__ done) {
+    static_assert(MODE >= 1 && MODE <= 4, "compacted output needs k_rb_vec");
+    if (done != nullptr && *done) return;
+    __shared__ double lds[(TK * TJ) / WAVE];
+    const int nbx = gridDim.x, nby = gridDim.y;
+    const int nb = nbx * nby * gridDim.z;
+    const int hw = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    const int t = xcd_remap(hw, nb);
+    const int bx = t % nbx, rest = t / nbx, by = rest % nby, bz = rest / nby;
+    const int k = bx * TK + threadIdx.x;
+    const int j = by * TJ + threadIdx.y;
+    const int i0 = bz * ICHUNK;
+    const int i1 = min(i0 + ICHUNK, g.ni);
+    double acc = 0.0;
+    if (k < g.nk && j < g.nj) {
+        uint32_t v = (uint32_t)i0 * g.si + (uint32_t)j * g.sj + (uint32_t)k;
+        const bool hj = j > 0, hk = k > 0, pj = j + 1 < g.nj, pk = k + 1 < g.nk;
+        float xm = (i0 > 0) ? xn[v - g.si] : 0.0f;
+        float cxm = (i0 > 0) ? cx[v - g.si] : 0.0f;
+        float xcur = xn[v];
+        for (int i = i0; i < i1; ++i, v += g.si) {
+            const float xp = (i + 1 < g.ni) ? xn[v + g.si] : 0.0f;
+            const float cxp = cx[v];
+            const float cyp = cy[v];
+            const float czp = cz[v];
+            const float cym = hj ? cy[v - g.sj] : 0.0f;
+            const float czm = hk ? cz[v - 1] : 0.0f;
+            const float xjp = pj ? xn[v + g.sj] : 0.0f;
+            const float xjm = hj ? xn[v - g.sj] : 0.0f;
+            const float xkp = pk ? xn[v + 1] : 0.0f;
+            const float xkm = hk ? xn[v - 1] : 0.0f;
+            const float n = cxp * xp + cxm * xm + cyp * xjp + cym * xjm + czp * xkp + czm * xkm;
+            const bool red = ((i + j + k) & 1) != 0;
+            float o;
+            if (MODE == 1) {
+                o = red ? n : 0.0f;
+            } else {
+                const float cen = xc[v];
+                if (MODE == 2) o = red ? 0.0f : cen - n;
+                else if (MODE == 3) o = red ? 0.0f : cen + n;
+                else o = red ? cen + n : xcur;
+                if (MODE == 2) acc += (double)cen * (double)o;
+                if (MODE == 3) acc += (double)o * (double)o;
+            }
+            y[v] = o;
+            xm = xcur;
+            xcur = xp;
+            cxm = cxp;
+        }
+    }
+    if (MODE == 2 || MODE == 3) {
+        double a[1] = {acc};
+        block_reduce<1, RED_SUM>(a, lds);
+        if (flat_tid() == 0) partials[t] = a[0];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // The Schur complement on colour-compacted vectors.
 //
@@ -1182,6 +1246,13 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
 // block partials (MODE 2 and 3 write them).
 template <int MODE>
 static int launch_rb(ls_ctx* c, const float* xn, const float* xc, float* y, const int* done) {
+    if (MODE != 5 && !use_vec<float>(c)) {
+        const dim3 grd = stencil_grid(c);
+        hipLaunchKernelGGL(k_rb_scalar<(MODE == 5 ? 1 : MODE)>, grd, dim3(TK, TJ), 0, c->stream,
+                           xn, xc, y, c->c32[0], c->c32[1], c->c32[2], sgeo(c), c->partials,
+                           done);
+        return (int)(grd.x * grd.y * grd.z);
+    }
     const dim3 grd = vec_grid<float>(c);
     hipLaunchKernelGGL(k_rb_vec<MODE>, grd, dim3(WAVE, StencilShape<float>::TJV), 0, c->stream,
                        xn, xc, y, c->c32[0], c->c32[1], c->c32[2], sgeo(c), c->partials, done);
@@ -1216,7 +1287,8 @@ static bool schur_shape(ls_ctx* c, SchurShape* sh) {
 // The mixed solver's Schur form: 0 none, 1 fused on compacted vectors,
 // 2 two passes on full-length vectors.
 static int schur_form(ls_ctx* c) {
-    if (!c->P.mixed_schur || !use_vec<float>(c)) return 0;
+    if (!c->P.mixed_schur) return 0;
+    if (!use_vec<float>(c)) return 2;     // one voxel per thread (k_rb_scalar)
     SchurShape sh;
     if ((c->P.mixed_schur == 1 || c->P.mixed_schur == 3) && schur_shape(c, &sh)) {
         // 1 takes the fused form only where a row fills at least 3/8 of the
@@ -1257,7 +1329,8 @@ __global__ void k_set_tol2(CgScalars* cs, double tol2) { cs->tol2 = tol2; }
 static int cg_iterations_rb(ls_ctx* c, bool fused, float* e, float* rs, float* p, float* q,
                             float* w, int batch) {
     const uint32_t Nv = fused ? (uint32_t)(c->N / 2) : (uint32_t)c->N;
-    const int nbe = vec_blocks<float>(Nv);
+    const bool vec = fused || use_vec<float>(c);
+    const int nbe = vec ? vec_blocks<float>(Nv) : ew_blocks(Nv);
     for (int b = 0; b < batch; ++b) {
         const bool sample = c->profiling && b == 0;
         if (sample) LS_HIP(hipEventRecord(c->ev[12], c->stream));
@@ -1274,11 +1347,19 @@ static int cg_iterations_rb(ls_ctx* c, bool fused, float* e, float* rs, float* p
         }
         if (!fused) nbs = launch_rb<2>(c, w, p, q, &c->cg->done);    // q = S p, p.q
         hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1024), 0, c->stream, c->partials, nbs, c->cg);
-        hipLaunchKernelGGL((k_cg_update_vec<float, false>), dim3(nbe), dim3(EW_BLOCK), 0,
-                           c->stream, e, rs, p, q, nullptr, c->cg, Nv, c->partials);
+        if (vec)
+            hipLaunchKernelGGL((k_cg_update_vec<float, false>), dim3(nbe), dim3(EW_BLOCK), 0,
+                               c->stream, e, rs, p, q, nullptr, c->cg, Nv, c->partials);
+        else
+            hipLaunchKernelGGL((k_cg_update<float, false>), dim3(nbe), dim3(EW_BLOCK), 0,
+                               c->stream, e, rs, p, q, nullptr, c->cg, Nv, c->partials);
         hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1024), 0, c->stream, c->partials, nbe, c->cg, 0);
-        hipLaunchKernelGGL(k_cg_direction_vec<float>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, p,
-                           rs, e, c->cg, Nv);
+        if (vec)
+            hipLaunchKernelGGL(k_cg_direction_vec<float>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream,
+                               p, rs, e, c->cg, Nv);
+        else
+            hipLaunchKernelGGL(k_cg_direction<float>, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, p,
+                               rs, e, c->cg, Nv);
     }
     LS_LAUNCH_CHECK();
     return 0;
@@ -1290,8 +1371,9 @@ static int cg_iterations_rb(ls_ctx* c, bool fused, float* e, float* rs, float* p
 // r_b + N_br r_r, CG on S runs until the Schur residual has fallen by delta
 // (or its norm reaches the tolerance), then the red correction is
 // e_r = r_r + N_rb e_b, the full correction is added to the FP64 iterate and
-// the FP64 residual recomputed.  Needs the vectorised layout (nk % 4 == 0);
-// fused (nk % 8 == 0): the inner vectors are compacted (k_schur_apply).
+// the FP64 residual recomputed.  Two-pass: k_rb_vec when nk % 4 == 0, else
+// k_rb_scalar; fused (nk % 8 == 0): the inner vectors are compacted
+// (k_schur_apply).
 static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep, bool fused) {
     const uint32_t N = (uint32_t)c->N;
     const size_t Nv = fused ? c->N / 2 : c->N;
@@ -1341,8 +1423,13 @@ static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep, bool fused) {
         if (first) {
             LS_HIP(hipMemcpyAsync(c->p32, rs, Nv * sizeof(float), hipMemcpyDeviceToDevice,
                                   c->stream));
-        } else {   // xlive = 0 here: only the direction carries over
+        } else if (fused || use_vec<float>(c)) {   // xlive = 0 here: only the direction
             hipLaunchKernelGGL(k_cg_direction_vec<float>, dim3(vec_blocks<float>((uint32_t)Nv)),
+                               dim3(EW_BLOCK), 0, c->stream, c->p32, rs, c->x32, c->cg,
+                               (uint32_t)Nv);                  // carries over
+            LS_LAUNCH_CHECK();
+        } else {
+            hipLaunchKernelGGL(k_cg_direction<float>, dim3(ew_blocks((uint32_t)Nv)),
                                dim3(EW_BLOCK), 0, c->stream, c->p32, rs, c->x32, c->cg,
                                (uint32_t)Nv);
             LS_LAUNCH_CHECK();

@@ -123,7 +123,8 @@ def test_schur_matches_plain_in_fewer_iterations(seed):
     assert abs(rs.press_iters - r2.press_iters) <= max(3, 0.03 * r2.press_iters)
 
 
-@pytest.mark.parametrize("nk,form", [(28, 1), (32, 1), (32, 3)])   # two-pass, two-pass, fused
+@pytest.mark.parametrize("nk,form", [(27, 1), (28, 1), (32, 1), (32, 3)])   # two-pass: one
+# voxel per thread, vectorised (twice); fused
 def test_schur_coupled_steps_match_oracle(nk, form):
     from tests.helpers import both
     img = small_mixed_geometry((24, 20, nk), 9)
@@ -160,3 +161,20 @@ def test_schur_form_choice(nk):
     assert np.array_equal(p1, picked[1])
     assert relerr(p1, other[1]) <= 1e-7
     assert abs(r1.press_iters - other[0].press_iters) <= max(3, 0.03 * r1.press_iters)
+
+
+@pytest.mark.parametrize("shape", [(20, 18, 27), (17, 22, 30)])
+def test_two_pass_schur_any_nk(shape):
+    """nk % 4 != 0: the two-pass form runs one voxel per thread
+    (k_rb_scalar).  Same solution as plain FP64 PCG and as the inner CG on
+    the operator itself, in about half the latter's iterations."""
+    img = G.smoothed_threshold(shape, 2.0, 0.4, 5)
+    r64, p64 = solve(img, 1500.0, solver=L.SOLVER_PLAIN64)
+    r2, p2 = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=1)
+    rm, pm = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=0)
+    for r in (r64, r2, rm):
+        assert r.status == 0 and r.press_relres <= 1e-12
+    assert relerr(p2, p64) <= 1e-7 and relerr(pm, p64) <= 1e-7
+    print(f"iterations: plain FP64 {r64.press_iters}, mixed {rm.press_iters}, "
+          f"mixed Schur two-pass {r2.press_iters}")
+    assert r2.press_iters <= 0.7 * rm.press_iters
