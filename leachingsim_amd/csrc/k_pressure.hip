@@ -390,7 +390,7 @@ __global__ __launch_bounds__(TK * TJ) void k_rb_scalar(
                 if (MODE == 2) acc += (double)cen * (double)o;
                 if (MODE == 3) acc += (double)o * (double)o;
             }
-            y[v] = o;
+            __builtin_nontemporal_store(o, y + v);
             xm = xcur;
             xcur = xp;
             cxm = cxp;
