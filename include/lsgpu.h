@@ -287,12 +287,15 @@ int ls_debug_pressure_faces(ls_ctx* ctx, double* faces);
  * last assembled FP32 scaled operator, on host vectors compacted to the black
  * voxels (N/2 floats: black voxel v at index v >> 1), with *dot = p.q.
  * form 1: the fused kernel (nk % 8 == 0, nk <= 1024); form 2: the gather and apply passes
- * on full-length vectors (nk % 4 == 0). */
+ * on full-length vectors (nk % 4 == 0); form 4: the same passes on full-length host
+ * vectors (N floats; the red entries of p are ignored and come back zero in q), for any
+ * nk (one voxel per thread when nk % 4 != 0, as the solver runs them). */
 int ls_debug_schur_apply(ls_ctx* ctx, int form, const float* p, float* q, double* dot);
 /* Time `reps` launches of a pressure kernel on the context's stream with HIP
  * events: the stencil y = A x (precision 64 or 32), or one of the FP32
  * red-black Schur passes (LS_TIME_RB_GATHER: w = N_rb p; LS_TIME_RB_APPLY:
- * q = p - N_br w with its dot; LS_TIME_SCHUR: both fused); returns ms per launch and the kernel's
+ * q = p - N_br w with its dot, vectorised when nk % 4 == 0, else one voxel per
+ * thread; LS_TIME_SCHUR: both fused); returns ms per launch and the kernel's
  * algorithmic bytes per launch. */
 #define LS_TIME_RB_GATHER 1
 #define LS_TIME_RB_APPLY 2

@@ -1569,16 +1569,36 @@ int debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot
         c->err = "the fused Schur apply needs nk % " + std::to_string(2 * SCHUR_VW) + " == 0 and nk <= " + std::to_string(SCHUR_MAX_NK);
         return -3;
     }
-    if (!use_vec<float>(c)) {
-        c->err = "the red-black passes need nk % 4 == 0";
+    if (form == 2 && (c->nk % 2 != 0 || !use_vec<float>(c))) {
+        c->err = "the compacted two-pass form needs nk % 4 == 0 (form 4 takes any nk)";
         return -3;
     }
     const size_t N = c->N, Nh = N / 2;
     float* pc = (float*)c->tv[0][0];         // 2N floats per transport buffer
     float* qc = pc + Nh;
-    LS_HIP(hipMemcpyAsync(pc, p, Nh * sizeof(float), hipMemcpyHostToDevice, c->stream));
     int nbs;
     std::vector<float> full;
+    if (form == 4) {
+        // full-length vectors, any nk: the gather and apply passes the
+        // solver runs (k_rb_vec for nk % 4 == 0, else k_rb_scalar); the red
+        // entries of p are never read, those of q come back zero
+        float* pf = (float*)c->tv[0][1];
+        float* wf = pf + N;
+        LS_HIP(hipMemcpyAsync(pf, p, N * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        launch_rb<1>(c, pf, nullptr, wf, nullptr);
+        nbs = launch_rb<2>(c, wf, pf, c->ap32, nullptr);
+        LS_LAUNCH_CHECK();
+        LS_HIP(hipMemcpyAsync(q, c->ap32, N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        std::vector<double> part(nbs);
+        LS_HIP(hipMemcpyAsync(part.data(), c->partials, nbs * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+        LS_HIP(hipStreamSynchronize(c->stream));
+        double a = 0.0;
+        for (double x : part) a += x;
+        *dot = a;
+        return 0;
+    }
+    LS_HIP(hipMemcpyAsync(pc, p, Nh * sizeof(float), hipMemcpyHostToDevice, c->stream));
     if (form == 1) {
         nbs = launch_schur(c, pc, qc, nullptr);
         LS_LAUNCH_CHECK();
@@ -1615,10 +1635,6 @@ int time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) 
     if (!c->press_assembled) {
         c->err = "no pressure system assembled";
         return -1;
-    }
-    if ((precision == LS_TIME_RB_GATHER || precision == LS_TIME_RB_APPLY) && !use_vec<float>(c)) {
-        c->err = "the red-black passes need nk % 4 == 0";
-        return -3;
     }
     SchurShape sh;
     if (precision == LS_TIME_SCHUR && !schur_shape(c, &sh)) {

@@ -619,8 +619,9 @@ int ls_debug_apply_pressure(ls_ctx* c, const double* x, double* y) {
 
 int ls_debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot) {
     if (!c || !p || !q || !dot) return -1;
-    if (form != 1 && form != 2) {
-        c->err = "ls_debug_schur_apply: form must be 1 (fused) or 2 (two passes)";
+    if (form != 1 && form != 2 && form != 4) {
+        c->err = "ls_debug_schur_apply: form must be 1 (fused), 2 (two passes) or 4 (two "
+                 "passes on full-length vectors)";
         return -3;
     }
     return debug_schur_apply(c, form, p, q, dot);

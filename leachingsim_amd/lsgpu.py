@@ -342,9 +342,12 @@ class GpuSimulation:
 
     def schur_apply(self, p_black, form: int = 1):
         """q = S p on a vector compacted to the black voxels (N/2 float32,
-        black voxel v at v >> 1) with the dot p.q; form 1 fused, 2 two-pass."""
+        black voxel v at v >> 1) with the dot p.q; form 1 fused, 2 two-pass;
+        form 4: two-pass on a full-length vector (N float32, any nk; red
+        entries ignored on input, zero on output)."""
         a = np.ascontiguousarray(p_black, dtype=np.float32)
-        assert a.size * 2 == int(np.prod(self.shape))
+        n = int(np.prod(self.shape))
+        assert a.size == (n if form == 4 else n // 2) and (form == 4 or n % 2 == 0)
         q = np.empty_like(a)
         dot = C.c_double()
         self._check(self.lib.ls_debug_schur_apply(self.h, form, a.ctypes.data, q.ctypes.data,

@@ -31,13 +31,20 @@ def relerr(a, b):
     return d / nb if nb > 0 else d
 
 
-def small_mixed_geometry(shape, seed, air_slabs=True, sulphide=True):
-    """Pore/Rock field with Air slabs on the i faces and a Sulphide pocket."""
+def small_mixed_geometry(shape, seed, air_slabs=True, sulphide=True, air_axis=0):
+    """Pore/Rock field with a Sulphide pocket and one-plane Air slabs on both
+    faces of axis `air_axis` (0: the i faces, whose Air voxels carry the
+    reference's Dirichlet data; 1: the j faces, which leave the i-face
+    inlet/outlet planes of the z-face boundary in contact with Pore, so the
+    pressure keeps a drive after the pores next to Air have precipitated)."""
     img = G.smoothed_threshold(shape, 1.5, 0.45, seed)
     if sulphide:
         f = G.smooth_field(shape, 1.5, seed + 7)
         img[(f < np.quantile(f, 0.12)) & (img == G.ROCK)] = G.SULPHIDE
     if air_slabs:
-        img[0] = G.AIR
-        img[-1] = G.AIR
+        sl = [slice(None)] * 3
+        sl[air_axis] = 0
+        img[tuple(sl)] = G.AIR
+        sl[air_axis] = -1
+        img[tuple(sl)] = G.AIR
     return img

@@ -90,7 +90,9 @@ def test_learned_solver_reaches_tolerance_and_matches_plain(seed):
         sol[solver] = g.download(L.F_PRESSURE)
         g.close()
     # two different Krylov paths stopped at 1e-12 differ by ~ cond(A) x 1e-12
-    assert relerr(sol[L.SOLVER_LEARNED], sol[L.SOLVER_PLAIN64]) <= 1e-7
+    err = relerr(sol[L.SOLVER_LEARNED], sol[L.SOLVER_PLAIN64])
+    print(f"learned vs plain FP64 at rtol 1e-12: |dp|/|p| {err:.2e}")
+    assert err <= 1e-8
 
 
 def test_race_hands_over_to_mixed_pcg():
@@ -121,7 +123,7 @@ def test_race_hands_over_to_mixed_pcg():
     h.init_state()
     h.phase(L.PH_PROPERTIES)
     h.phase(L.PH_PRESSURE, 2500.0)
-    assert relerr(p, h.download(L.F_PRESSURE)) <= 1e-7
+    assert relerr(p, h.download(L.F_PRESSURE)) <= 1e-9
     h.close()
 
 

@@ -180,33 +180,50 @@ def test_leaching_fast_path_is_noop():
 
 
 @pytest.mark.parametrize("solver", [L.SOLVER_PLAIN64, L.SOLVER_MIXED])
-def test_step_parity_smooth64(solver):
-    """64^3 geometry (sigma 2, porosity 0.40), z-face boundary,
-    Air slabs added so surface effects and transport are active."""
+@pytest.mark.parametrize("rtol", [1e-8, 1e-12])
+def test_step_parity_smooth64(solver, rtol):
+    """The 64^3 workload (sigma 2, porosity 0.40), z-face boundary, Air
+    slabs added on the j faces so surface effects and transport are active
+    while the inlet/outlet planes keep driving the pressure, 10 steps.
+
+    At rtol 1e-12 the solution error of either side is ~cond * 1e-12, so p
+    and conc must agree to 1e-9 / 1e-10.  At the workload's own rtol 1e-8
+    two correct Krylov solves differ by up to cond * 1e-8 (cond of the
+    pressure system ~1e2-1e3 here), so that run asserts 1e-6 / 1e-7; both
+    hold the masks bit-exact."""
     img = G.named_geometry("smooth64")
-    img[0:2] = G.AIR
-    P = L.default_params(bc_mode=L.BC_ZFACES, solver=solver)
+    img[:, 0:2] = G.AIR
+    P = L.default_params(bc_mode=L.BC_ZFACES, solver=solver, press_rtol=rtol, adr_rtol=rtol)
     g, o = both(img.shape, P, img)
+    tol_p, tol_c = (1e-9, 1e-10) if rtol < 1e-11 else (1e-6, 1e-7)
     t, dt = 0.0, 1.0
+    worst = [0.0, 0.0]
     for n in range(10):
         rg = g.step(t, dt)
         ro = o.step(t, dt)
         assert rg.status == 0 and ro.status == 0
-        assert rg.press_relres <= 1e-8 and rg.adr_relres <= 1e-8
+        assert rg.press_relres <= rtol and rg.adr_relres <= rtol
         assert np.array_equal(g.download(L.F_IMG), o.download(L.F_IMG)), n
         assert np.array_equal(g.download(L.F_FRAC), o.download(L.F_FRAC)), n
-        assert relerr(g.download(L.F_PRESSURE), o.download(L.F_PRESSURE)) <= 1e-6
-        assert relerr(g.download(L.F_CONC), o.download(L.F_CONC)) <= 1e-7
+        ep = relerr(g.download(L.F_PRESSURE), o.download(L.F_PRESSURE))
+        ec = relerr(g.download(L.F_CONC), o.download(L.F_CONC))
+        worst = [max(worst[0], ep), max(worst[1], ec)]
+        assert ep <= tol_p and ec <= tol_c, (n, ep, ec)
         t += dt
+    print(f"smooth64 solver {solver} rtol {rtol}: worst |dp|/|p| {worst[0]:.2e}, "
+          f"|dc|/|c| {worst[1]:.2e}")
 
 
 @pytest.mark.parametrize("shape", [(21, 18, 27), (20, 18, 30)])
-def test_step_parity_scalar_paths(shape):
-    """nk odd: scalar stencil, FP32 CG on the operator itself, transport rows
-    rebuilt per matvec; nk = 30 (even, not a multiple of 4): scalar stencil,
-    stored transport rows."""
-    img = small_mixed_geometry(shape, 21)
-    P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED,
+@pytest.mark.parametrize("schur", [0, 1])
+def test_step_parity_scalar_paths(shape, schur):
+    """nk % 4 != 0: scalar stencils.  mixed_schur 1: the FP32 inner CG runs
+    on the red-black Schur complement in two one-voxel-per-thread passes
+    (k_rb_scalar); mixed_schur 0: FP32 CG on the operator itself.  nk odd:
+    transport on the full system with rows rebuilt per matvec; nk = 30:
+    stored transport rows on the Schur complement."""
+    img = small_mixed_geometry(shape, 21, air_axis=1)
+    P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED, mixed_schur=schur,
                          warm_start=L.WARM_PROJECTED, press_rtol=1e-12, adr_rtol=1e-12)
     g, o = both(shape, P, img)
     t = 0.0
@@ -216,8 +233,12 @@ def test_step_parity_scalar_paths(shape):
         assert rg.status == ro.status == 0, (n, rg.status, ro.status)
         assert np.array_equal(g.download(L.F_IMG), o.download(L.F_IMG)), n
         t += 30.0
-    assert relerr(g.download(L.F_PRESSURE), o.download(L.F_PRESSURE)) <= 1e-7
-    assert relerr(g.download(L.F_CONC), o.download(L.F_CONC)) <= 1e-7
+    po = o.download(L.F_PRESSURE)
+    assert np.abs(po).max() > 0          # the inlet/outlet planes still drive the flow
+    ep = relerr(g.download(L.F_PRESSURE), po)
+    ec = relerr(g.download(L.F_CONC), o.download(L.F_CONC))
+    print(f"shape {shape} schur {schur}: |dp|/|p| {ep:.2e}, |dc|/|c| {ec:.2e}")
+    assert ep <= 1e-9 and ec <= 1e-10
 
 
 def test_rollback_restores_state():
@@ -263,7 +284,7 @@ def test_implicit_saturation_phase_parity():
 def test_implicit_saturation_step_parity():
     """Coupled steps with the implicit saturation in 3 Picard passes."""
     img = G.named_geometry("smooth64")[:32, :32, :32].copy()
-    img[0:2] = G.AIR
+    img[:, 0:2] = G.AIR
     P = L.default_params(bc_mode=L.BC_ZFACES, implicit_sat=1, press_rtol=1e-12, adr_rtol=1e-12)
     g, o = both(img.shape, P, img)
     t = 0.0
@@ -274,6 +295,10 @@ def test_implicit_saturation_step_parity():
         assert rg.sat_valid == ro.sat_valid == 1
         assert np.array_equal(g.download(L.F_IMG), o.download(L.F_IMG)), n
         t += 1.0
-    assert relerr(g.download(L.F_SATURATION), o.download(L.F_SATURATION)) <= 1e-9
-    assert relerr(g.download(L.F_PRESSURE), o.download(L.F_PRESSURE)) <= 1e-7
-    assert relerr(g.download(L.F_CONC), o.download(L.F_CONC)) <= 1e-7
+    es = relerr(g.download(L.F_SATURATION), o.download(L.F_SATURATION))
+    po = o.download(L.F_PRESSURE)
+    assert np.abs(po).max() > 0
+    ep = relerr(g.download(L.F_PRESSURE), po)
+    ec = relerr(g.download(L.F_CONC), o.download(L.F_CONC))
+    print(f"implicit saturation steps: |ds|/|s| {es:.2e}, |dp|/|p| {ep:.2e}, |dc|/|c| {ec:.2e}")
+    assert es <= 1e-10 and ep <= 1e-9 and ec <= 1e-10

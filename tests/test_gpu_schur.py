@@ -31,6 +31,33 @@ def solve(img, t, **kw):
     return rep, p
 
 
+def schur_numpy_full(faces32, p_full, shape):
+    """S p = p - N_br N_rb p in float64 on a full-length vector whose red
+    entries are ignored; returns q (zero at red voxels) and p.q."""
+    ni, nj, nk = shape
+    ii, jj, kk = np.meshgrid(np.arange(ni), np.arange(nj), np.arange(nk), indexing="ij")
+    black = ((ii + jj + kk) % 2 == 0).reshape(-1)
+    cx, cy, cz = (f.astype(np.float64) for f in faces32)
+
+    def nmul(x):
+        X = x.reshape(shape)
+        y = np.zeros(shape)
+        y[:-1] += cx[:-1] * X[1:]
+        y[1:] += cx[:-1] * X[:-1]
+        y[:, :-1] += cy[:, :-1] * X[:, 1:]
+        y[:, 1:] += cy[:, :-1] * X[:, :-1]
+        y[:, :, :-1] += cz[:, :, :-1] * X[:, :, 1:]
+        y[:, :, 1:] += cz[:, :, :-1] * X[:, :, :-1]
+        return y.reshape(-1)
+
+    pf = np.where(black, p_full.astype(np.float64), 0.0)
+    w = nmul(pf)
+    w[black] = 0.0
+    qf = pf - nmul(w)
+    qf[~black] = 0.0
+    return qf, float(pf @ qf)
+
+
 def schur_numpy(faces32, p_black, shape):
     """S p = p - N_br N_rb p in float64 on the compacted black vector."""
     ni, nj, nk = shape
@@ -86,6 +113,36 @@ def test_fused_schur_apply(shape):
     assert abs(d1 - dr) <= 1e-5 * abs(dr)
 
 
+@pytest.mark.parametrize("shape", [(21, 18, 27), (20, 18, 30), (9, 7, 5), (33, 5, 131),
+                                   (17, 22, 28)])
+def test_two_pass_schur_apply_any_nk(shape):
+    """The two-pass form on full-length vectors (what the solver runs when
+    nk % 4 != 0: k_rb_scalar, one voxel per thread; nk = 28: k_rb_vec),
+    against numpy on the same FP32 faces.  Ragged tiles: nj not a multiple
+    of the block's rows, ni not a multiple of its planes, nk not a multiple
+    of the wave width."""
+    img = small_mixed_geometry(shape, 4, air_slabs=shape[0] > 4)
+    P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED)
+    g = L.GpuSimulation(shape, P)
+    g.read_labels(img)
+    g.init_state()
+    g.phase(L.PH_PROPERTIES)
+    g.phase(L.PH_PRESSURE, 1500.0)
+    p = np.random.default_rng(sum(shape) + 1).standard_normal(img.size).astype(np.float32)
+    print("shape", shape, flush=True)
+    q, d = g.schur_apply(p, 4)
+    faces = g.pressure_faces().astype(np.float32)
+    g.close()
+    qr, dr = schur_numpy_full(faces, p, shape)
+    ii, jj, kk = np.meshgrid(*(np.arange(n) for n in shape), indexing="ij")
+    red = ((ii + jj + kk) % 2 == 1).reshape(-1)
+    assert np.all(q[red] == 0.0)
+    assert np.abs(q - qr).max() <= 1e-5 * np.abs(qr).max()
+    assert abs(d - dr) <= 1e-5 * abs(dr)
+    # S = I - N_br N_rb is not the identity on these faces
+    assert np.abs(qr - np.where(red, 0.0, p)).max() > 1e-3 * np.abs(qr).max()
+
+
 def test_fused_schur_rejects_wide_rows():
     """Rows wider than the fused kernel's block (nk > 1024): the debug
     entry refuses form 1, the solver runs the two-pass form."""
@@ -115,7 +172,9 @@ def test_schur_matches_plain_in_fewer_iterations(seed):
     rm, pm = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=0)
     for r in (r64, rs, r2, rm):
         assert r.status == 0 and r.press_relres <= 1e-12
-    assert relerr(ps, p64) <= 1e-7 and relerr(pm, p64) <= 1e-7 and relerr(p2, p64) <= 1e-7
+    errs = (relerr(ps, p64), relerr(pm, p64), relerr(p2, p64))
+    print("relative p error vs plain FP64 (fused, inner CG on A, two-pass):", errs)
+    assert max(errs) <= 1e-9
     print(f"iterations: plain FP64 {r64.press_iters}, mixed {rm.press_iters}, "
           f"mixed Schur fused {rs.press_iters}, two-pass {r2.press_iters}")
     assert rs.press_iters <= 0.7 * rm.press_iters
@@ -127,9 +186,9 @@ def test_schur_matches_plain_in_fewer_iterations(seed):
 # voxel per thread, vectorised (twice); fused
 def test_schur_coupled_steps_match_oracle(nk, form):
     from tests.helpers import both
-    img = small_mixed_geometry((24, 20, nk), 9)
+    img = small_mixed_geometry((24, 20, nk), 9, air_axis=1)
     P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED, mixed_schur=form,
-                         press_rtol=1e-12)
+                         press_rtol=1e-12, adr_rtol=1e-12)
     g, o = both(img.shape, P, img)
     t = 0.0
     for n in range(4):
@@ -138,8 +197,12 @@ def test_schur_coupled_steps_match_oracle(nk, form):
         assert rg.status == ro.status == 0
         assert np.array_equal(g.download(L.F_IMG), o.download(L.F_IMG)), n
         t += 30.0
-    assert relerr(g.download(L.F_PRESSURE), o.download(L.F_PRESSURE)) <= 1e-7
-    assert relerr(g.download(L.F_CONC), o.download(L.F_CONC)) <= 1e-7
+    po = o.download(L.F_PRESSURE)
+    assert np.abs(po).max() > 0
+    ep = relerr(g.download(L.F_PRESSURE), po)
+    ec = relerr(g.download(L.F_CONC), o.download(L.F_CONC))
+    print(f"nk {nk} form {form}: |dp|/|p| {ep:.2e}, |dc|/|c| {ec:.2e}")
+    assert ep <= 1e-9 and ec <= 1e-10
 
 
 @pytest.mark.parametrize("nk", [64, 192])
@@ -159,7 +222,7 @@ def test_schur_form_choice(nk):
     # the chosen form runs the same kernels as the forced one: same bits
     assert r1.press_iters == picked[0].press_iters
     assert np.array_equal(p1, picked[1])
-    assert relerr(p1, other[1]) <= 1e-7
+    assert relerr(p1, other[1]) <= 1e-9
     assert abs(r1.press_iters - other[0].press_iters) <= max(3, 0.03 * r1.press_iters)
 
 
@@ -174,7 +237,8 @@ def test_two_pass_schur_any_nk(shape):
     rm, pm = solve(img, 1500.0, solver=L.SOLVER_MIXED, mixed_schur=0)
     for r in (r64, r2, rm):
         assert r.status == 0 and r.press_relres <= 1e-12
-    assert relerr(p2, p64) <= 1e-7 and relerr(pm, p64) <= 1e-7
     print(f"iterations: plain FP64 {r64.press_iters}, mixed {rm.press_iters}, "
-          f"mixed Schur two-pass {r2.press_iters}")
+          f"mixed Schur two-pass {r2.press_iters}; |dp|/|p| {relerr(p2, p64):.2e}, "
+          f"{relerr(pm, p64):.2e}")
+    assert relerr(p2, p64) <= 1e-9 and relerr(pm, p64) <= 1e-9
     assert r2.press_iters <= 0.7 * rm.press_iters
