@@ -306,6 +306,31 @@ int ls_time_stencil(ls_ctx* ctx, int precision, int reps, double* ms_per_launch,
  * events (a copy kernel of known size, for bandwidth calibration). */
 int ls_time_copy(ls_ctx* ctx, size_t bytes, int reps, double* ms_per_launch);
 
+/* ---- Allocation-free entry point.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It is the
+ * saturation phase of ls_phase (LS_PH_SATURATION, updateSaturation,
+ * src/Simulation.cpp:1672-1856) with the same kernels and the same bits, on
+ * caller-owned DEVICE fields of an ni x nj x nk grid (k-fastest, no
+ * context): labels as uint8 (1..5), the three cell fluxes, and the
+ * saturation, updated in place.  It calls no hipMalloc/hipFree: the scratch
+ * of its global MAX (block partials) and sub-step plan is a static device
+ * array of LS_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.
+ * Limits: 1 <= ni <= 65535, nj, nk >= 1, ni*nj*nk < 2^32 (else -2); the
+ * partials never exceed the scratch at any such size.  Every call on one GPU
+ * shares that GPU's scratch: calls that overlap on one GPU race (serialise
+ * them, e.g. on one stream).  `stream` is a hipStream_t (NULL: the null
+ * stream); the kernels are enqueued on it.  nsub / max_rate, if not NULL,
+ * receive the sub-step count and max |div q| + evap; reading them
+ * synchronises the stream, so pass NULL for both under graph capture.
+ * Errors: ls_noalloc_last_error() (per thread). */
+#define LS_NOALLOC_SCRATCH_BYTES 65792
+int ls_update_saturation_noalloc(int ni, int nj, int nk, const uint8_t* img, const double* qx,
+                                 const double* qy, const double* qz, double* s, double dt,
+                                 double dx, double porosity, double evap_flux, void* stream,
+                                 int* nsub, double* max_rate);
+const char* ls_noalloc_last_error(void);
+
 /* ---- The learned part: a 3-level 3D U-Net (16/32/64 channels, 3x3x3
  * convolutions, BF16 on MFMA with FP32 accumulation), no reference
  * counterpart.  Layers and the flat FP32 weight layout are those of

@@ -775,3 +775,132 @@ int summarize(ls_ctx* c, ls_step_report* rep) {
 }
 
 }  // namespace ls
+
+// ---------------------------------------------------------------------------
+// updateSaturation on caller-owned device fields, allocating nothing.  The
+// only device memory beyond the caller's inputs and outputs is the scratch
+// of the global MAX (block partials) and the sub-step plan (a StepScalars),
+// held in a file-scope device array.  Its device address is looked up once
+// per device with hipGetSymbolAddress (the array's name is a host-side
+// stand-in, not a device address) and only that address is used from the
+// host.  Every call on one device shares the array: calls that overlap on
+// one device race.  Nothing in it needs clearing: k_sat_max writes every
+// partial that k_sat_finalize reads, and k_sat_finalize writes every field
+// of the plan that k_sat_update reads.
+// ---------------------------------------------------------------------------
+#include <cstdio>
+#include <mutex>
+
+namespace {
+constexpr int NOALLOC_PLAN_BYTES = 256;
+constexpr int NOALLOC_MAX_PARTIALS = 8192;     // ew_blocks() never launches more
+constexpr size_t NOALLOC_SCRATCH_BYTES =
+    NOALLOC_PLAN_BYTES + sizeof(double) * (size_t)NOALLOC_MAX_PARTIALS;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static_assert(sizeof(StepScalars) <= NOALLOC_PLAN_BYTES, "plan slot too small");
+static_assert(NOALLOC_SCRATCH_BYTES == LS_NOALLOC_SCRATCH_BYTES, "header and kernel disagree");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device (looked up once per
+// device, with that device current); nullptr and a message on failure.
+void* scratch_on_current_device(const char** why) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        *why = "no current HIP device";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || ptr == nullptr) {
+            *why = "hipGetSymbolAddress(scratch) failed";
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// error text of the last failed call on this thread (a fixed buffer: the
+// entry point allocates no memory on the host either)
+thread_local char g_noalloc_err[256];
+
+void set_err(const char* fmt, const char* a = "", long long b = 0, long long c = 0) {
+    snprintf(g_noalloc_err, sizeof g_noalloc_err, fmt, a, b, c);
+}
+}  // namespace
+
+extern "C" const char* ls_noalloc_last_error(void) { return g_noalloc_err; }
+
+extern "C" int ls_update_saturation_noalloc(int ni, int nj, int nk, const uint8_t* img,
+                                            const double* qx, const double* qy,
+                                            const double* qz, double* s, double dt, double dx,
+                                            double porosity, double evap_flux, void* stream,
+                                            int* nsub, double* max_rate) {
+    using namespace ls;
+    g_noalloc_err[0] = 0;
+    if (!img || !qx || !qy || !qz || !s) {
+        set_err("ls_update_saturation_noalloc: null field pointer%s");
+        return -1;
+    }
+    if (ni < 1 || nj < 1 || nk < 1 || ni > 65535 || (double)ni * nj * nk >= 4294967295.0) {
+        set_err("ls_update_saturation_noalloc: grid must have 1 <= ni <= 65535, "
+                "nj, nk >= 1 and ni*nj*nk < 2^32%s");
+        return -2;
+    }
+    if (!(dt > 0.0) || !(dx > 0.0) || !(porosity > 0.0)) {
+        set_err("ls_update_saturation_noalloc: dt, dx and porosity must be > 0%s");
+        return -2;
+    }
+    const uint32_t N = (uint32_t)((size_t)ni * nj * nk);
+    const int nb = ew_blocks(N);
+    if (nb > NOALLOC_MAX_PARTIALS) {
+        set_err("ls_update_saturation_noalloc:%s %lld block partials exceed the scratch's %lld",
+                "", nb, NOALLOC_MAX_PARTIALS);
+        return -2;
+    }
+    const char* why = "";
+    unsigned char* base = (unsigned char*)scratch_on_current_device(&why);
+    if (!base) {
+        set_err("ls_update_saturation_noalloc: %s", why);
+        return -10;
+    }
+    StepScalars* plan = (StepScalars*)base;
+    double* partials = (double*)(base + NOALLOC_PLAN_BYTES);
+    hipStream_t st = (hipStream_t)stream;
+    Geo g;
+    g.ni = ni;
+    g.nj = nj;
+    g.nk = nk;
+    g.sj = (uint32_t)nk;
+    g.si = (uint32_t)nk * (uint32_t)nj;
+    hipLaunchKernelGGL(k_sat_max, dim3(nb), dim3(EW_BLOCK), 0, st, img, qx, qy, qz, g, N, dx,
+                       evap_flux, partials);
+    hipLaunchKernelGGL(k_sat_finalize, dim3(1), dim3(1024), 0, st, partials, nb, dt, porosity,
+                       plan);
+    dim3 blk(TK, TJ), grd((nk + TK - 1) / TK, (nj + TJ - 1) / TJ, ni);
+    hipLaunchKernelGGL(k_sat_update, grd, blk, 0, st, img, qx, qy, qz, s, g, dx, evap_flux, plan);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_err("ls_update_saturation_noalloc: kernel launch: %s", hipGetErrorString(e));
+        return -11;
+    }
+    if (nsub || max_rate) {
+        // the plan's first fields; a synchronous read, so leave both null
+        // where the call must stay asynchronous (under graph capture)
+        StepScalars h;
+        e = hipMemcpyAsync(&h, plan, sizeof(StepScalars), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            set_err("ls_update_saturation_noalloc: %s", hipGetErrorString(e));
+            return -10;
+        }
+        if (nsub) *nsub = h.sat_nsub;
+        if (max_rate) *max_rate = h.sat_max;
+    }
+    return 0;
+}

@@ -140,6 +140,9 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "ls_debug_pressure_system": (i, [vp, vp, vp, vp]),
         "ls_debug_pressure_faces": (i, [vp, vp]),
         "ls_debug_schur_apply": (i, [vp, i, vp, vp, C.POINTER(d)]),
+        "ls_update_saturation_noalloc": (i, [i, i, i, vp, vp, vp, vp, vp, d, d, d, d, vp,
+                                             C.POINTER(i), C.POINTER(d)]),
+        "ls_noalloc_last_error": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -160,6 +163,26 @@ def exported_symbols() -> list[str]:
     import re
     hdr = (PKG.parent / "include" / "lsgpu.h").read_text()
     return sorted(set(re.findall(r"^\s*(?:int|void|size_t|const char\*)\s+(ls_\w+)\(", hdr, re.M)))
+
+
+def update_saturation_noalloc(shape, img_ptr: int, qx_ptr: int, qy_ptr: int, qz_ptr: int,
+                              s_ptr: int, dt: float, dx: float = 1.0, porosity: float = 0.1,
+                              evap_flux: float = 0.0, stream: int | None = None,
+                              read_plan: bool = True):
+    """updateSaturation on caller-owned device fields (raw device addresses:
+    uint8 labels, FP64 fluxes and saturation, k-fastest) without allocating
+    device memory (ls_update_saturation_noalloc).  Returns (n_sub, max rate)
+    when read_plan, else None (the call then stays asynchronous on `stream`)."""
+    lib = load_library()
+    ni, nj, nk = (int(x) for x in shape)
+    ns, mx = C.c_int(0), C.c_double(0.0)
+    st = lib.ls_update_saturation_noalloc(ni, nj, nk, img_ptr, qx_ptr, qy_ptr, qz_ptr, s_ptr,
+                                          dt, dx, porosity, evap_flux, stream,
+                                          C.byref(ns) if read_plan else None,
+                                          C.byref(mx) if read_plan else None)
+    if st != 0:
+        raise LsError(st, lib.ls_noalloc_last_error().decode())
+    return (ns.value, mx.value) if read_plan else None
 
 
 class LsError(RuntimeError):
