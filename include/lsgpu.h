@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LSGPU_ABI_VERSION 7
+#define LSGPU_ABI_VERSION 8
 
 /* Labels (src/compiler_opts.h:69-82). */
 enum ls_label {
@@ -222,6 +222,9 @@ typedef struct ls_step_report {
     double prof_schur_ms;
     int prof_schur_n;
     double prof_schur_bytes;
+    /* Iterations of each transport system (conc, acid), for the reference's
+     * per-solve log lines (src/Simulation.cpp:1278, 2208). */
+    int adr_iters_conc, adr_iters_acid;
 } ls_step_report;
 
 typedef struct ls_ctx ls_ctx;

@@ -80,7 +80,7 @@ def build_host() -> Path | None:
     if not HOST.exists():
         return None
     cxx = ["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-I", str(ROOT / "include")]
-    io = [str(HOST / "io.cpp")]
+    io = [str(HOST / "io.cpp"), str(HOST / "steplog.cpp")]
     tmp = HOSTLIB.with_suffix(".so.tmp")
     _run([*cxx, "-shared", *io, str(HOST / "capi.cpp"), "-o", str(tmp)])
     os.replace(tmp, HOSTLIB)

@@ -1063,7 +1063,7 @@ __global__ void k_bicg_zero_rhs(BicgScalars* bs) {
 // the implicit saturation in x0, x1 idle).  Returns 0 converged, 1 not.
 template <int KIND>
 static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, const AdrConst& a,
-                     int* iters_out, double* relres_out) {
+                     int* iters_out, double* relres_out, int* sys_iters = nullptr) {
     TGeo g;
     g.ni = c->ni; g.nj = c->nj; g.nk = c->nk;
     g.sj = (uint32_t)c->nk;
@@ -1268,6 +1268,10 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     }
     const BicgScalars& h = *c->h_bicg;
     if (iters_out) *iters_out = h.iter[0] > h.iter[1] ? h.iter[0] : h.iter[1];
+    if (sys_iters) {
+        sys_iters[0] = h.iter[0];
+        sys_iters[1] = h.iter[1];
+    }
     if (relres_out) {
         double r0v = h.bnorm2[0] > 0 ? sqrt(h.rr[0] / h.bnorm2[0]) : 0.0;
         double r1v = h.bnorm2[1] > 0 ? sqrt(h.rr[1] / h.bnorm2[1]) : 0.0;
@@ -1297,12 +1301,14 @@ static AdrConst adr_const(const ls_ctx* c, double dt) {
 int phase_transport(ls_ctx* c, double dt, ls_step_report* rep) {
     int st = zero_flux_rock_precip(c);
     if (st) return st;
-    int iters = 0;
+    int iters = 0, sys_iters[2] = {0, 0};
     double relres = 0.0;
-    st = bicg_pair<0>(c, c->conc, c->acid, nullptr, adr_const(c, dt), &iters, &relres);
+    st = bicg_pair<0>(c, c->conc, c->acid, nullptr, adr_const(c, dt), &iters, &relres, sys_iters);
     if (rep) {
         rep->adr_iters = iters;
         rep->adr_relres = relres;
+        rep->adr_iters_conc = sys_iters[0];
+        rep->adr_iters_acid = sys_iters[1];
     }
     if (st) c->err = "transport solve did not converge";
     return st;

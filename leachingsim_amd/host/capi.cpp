@@ -25,6 +25,19 @@ extern "C" {
 
 const char* lshost_last_error(void) { return g_err.c_str(); }
 
+// The reference's log lines for one step (written to buf, NUL-terminated;
+// returns the full length, so a short buffer can be retried).
+int lshost_step_log(const ls_step_report* r, double dt, const ls_params* P, char* buf,
+                    size_t n) {
+    const std::string s = lshost::step_log(*r, dt, *P);
+    if (buf && n > 0) {
+        const size_t m = s.size() < n - 1 ? s.size() : n - 1;
+        std::memcpy(buf, s.data(), m);
+        buf[m] = 0;
+    }
+    return (int)s.size();
+}
+
 int lshost_read_raw(const char* path, int ni, int nj, int nk, int label_bytes, size_t header,
                     void* out) {
     try {

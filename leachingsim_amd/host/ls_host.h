@@ -23,6 +23,9 @@ struct Grid {
     size_t n() const { return (size_t)ni * (size_t)nj * (size_t)nk; }
 };
 
+// ---- the reference's per-solve log lines of one step (steplog.cpp) --------
+std::string step_log(const ls_step_report& r, double dt, const ls_params& P);
+
 // ---- .raw input (src/MPIRawLoader.h:48-82) --------------------------------
 // Headerless (or header-skipped) voxel file, k fastest, label_bytes per voxel
 // (1: uint8 build, 2: uint16 build).  Throws if the file is short.

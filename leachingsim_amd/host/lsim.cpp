@@ -293,9 +293,10 @@ int main(int argc, char** argv) {
                           << std::endl;
                 ok = sim.step(t, current_dt);   // snapshots the state first (saveState)
                 const ls_step_report& r = sim.report();
-                std::cout << "    Pressure: " << r.press_iters << " iterations, relative residual "
-                          << r.press_relres << (r.press_fallback ? " (fallback)" : "")
-                          << "; transport " << r.adr_iters << " iterations" << std::endl;
+                std::cout << lshost::step_log(r, current_dt, p)
+                          << "    [GPU] pressure relative residual " << r.press_relres
+                          << (r.press_fallback ? " (fallback)" : "") << ", "
+                          << r.t_total_ms << " ms" << std::endl;
                 if (ok) {
                     sim.total_leached_count += (unsigned long long)r.leached;
                     const double pct = sim.initial_sulphide_count > 0

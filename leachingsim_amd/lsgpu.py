@@ -77,7 +77,8 @@ class StepReport(C.Structure):
         ("press_relres0", C.c_double), ("warm_basis", C.c_int), ("warm_net", C.c_int),
         ("sat_iters", C.c_int), ("sat_valid", C.c_int), ("sat_raw_min", C.c_double),
         ("sat_raw_max", C.c_double), ("prof_schur_ms", C.c_double), ("prof_schur_n", C.c_int),
-        ("prof_schur_bytes", C.c_double),
+        ("prof_schur_bytes", C.c_double), ("adr_iters_conc", C.c_int),
+        ("adr_iters_acid", C.c_int),
     ]
 
     def as_dict(self) -> dict:

@@ -846,6 +846,8 @@ static int phase_transport(lso_ctx& c, double dt, ls_step_report* rep) {
     }
     if (rep) {
         rep->adr_iters = std::max(it1, it2);
+        rep->adr_iters_conc = it1;
+        rep->adr_iters_acid = it2;
         rep->adr_relres = std::max(rr1, rr2);
         double a = 1e300, bb = -1e300, e = 1e300, f = -1e300;
         for (size_t v = 0; v < c.N; ++v) {

@@ -4,6 +4,7 @@ steps run through the Python binding, and a restart from a checkpoint must
 continue as the reference's does (src/main.cpp:151-181: n = restart_step,
 t = the checkpoint's step time, so that step runs again)."""
 import ctypes as C
+import re
 import struct
 import subprocess
 import xml.etree.ElementTree as ET
@@ -68,6 +69,14 @@ def test_lsim_matches_binding_and_restarts(tmp_path):
     img.astype(np.uint8).tofile(raw)
     log = run_lsim(tmp_path, raw, "--tmax", "240", "--nout", "2", "--chk_freq", "2")
     assert "*** Simulation Finished ***" in log and log.count("Done iteration.") == 4
+    # the reference's per-solve lines, read with its benchmark parser's regex
+    # (plot_benchmarks.py:22-23): one [Saturation] and one [Leaching] line per
+    # step, whose sub-step counts it sums
+    assert log.count("    [Pressure] Converged in ") == 4
+    assert log.count("    [Concentration] Converged in ") == 4
+    assert log.count("    [Acid Transport] Converged in ") == 4
+    subs = [int(m) for m in re.findall(r"Sub-steps:\s+(\d+)", log)]
+    assert len(subs) == 8 and all(n >= 1 for n in subs)
     ck = load_own_checkpoint(tmp_path / "checkpoint_000002.lsck")
     assert ck["step"] == 2 and ck["t"] == 2 * DT
     ref = python_run(img, [0.0, 60.0, 120.0])           # steps n = 0, 1, 2

@@ -193,3 +193,63 @@ def test_checkpoint_round_trip_and_reference_layout(lib, tmp_path, lb):
                                       ptrs(out), C.byref(t), C.byref(dt)) == -1
     assert lib.lshost_load_checkpoint(d, 42, *g, lb, img2.ctypes.data, frac2.ctypes.data,
                                       ptrs(out), C.byref(t), C.byref(dt)) == -1
+
+
+# ---------------------------------------------------------------------------
+# The reference's per-solve log lines (src/Simulation.cpp:457,838,1278,1772,
+# 2018,2208) as lsim prints them, read back with the regexes of the
+# reference's benchmark parser (plot_benchmarks.py:22-24, restated): it sums
+# every "Sub-steps: N" of an explicit-saturation log (the [Saturation] and
+# [Leaching] lines) and every "[Implicit Saturation] Converged in N".
+# ---------------------------------------------------------------------------
+REGEX_EXPLICIT_STEPS = re.compile(r"Sub-steps:\s+(\d+)")
+REGEX_IMPLICIT_STEPS = re.compile(r"\[Implicit Saturation\] Converged in\s+(\d+)")
+
+
+def step_log(lib, rep, dt, params):
+    from leachingsim_amd import lsgpu as L
+    lib.lshost_step_log.argtypes = [C.POINTER(L.StepReport), C.c_double, C.POINTER(L.Params),
+                                    C.c_char_p, C.c_size_t]
+    buf = C.create_string_buffer(4096)
+    n = lib.lshost_step_log(C.byref(rep), dt, C.byref(params), buf, len(buf))
+    assert 0 < n < len(buf)
+    return buf.value.decode()
+
+
+def parse_like_reference(text, explicit):
+    total = 0
+    for line in text.splitlines():
+        m = (REGEX_EXPLICIT_STEPS if explicit else REGEX_IMPLICIT_STEPS).search(line)
+        if m:
+            total += int(m.group(1))
+    return total
+
+
+def test_step_log_lines_match_reference_formats(lib):
+    from leachingsim_amd import lsgpu as L
+    P = L.default_params()
+    r = L.StepReport(press_iters=17, press_relres=3e-9, p_min=0.0, p_max=2.5, sat_substeps=7,
+                     sat_max_divq=0.445, adr_iters=5, adr_iters_conc=5, adr_iters_acid=0,
+                     adr_relres=1e-9, conc_min=0.0, conc_max=1.0, acid_min=0.0, acid_max=0.0,
+                     leach_substeps=3, leach_max_rate=0.25)
+    text = step_log(lib, r, 0.5, P)
+    lines = text.splitlines()
+    assert lines[0] == "    [Pressure] Converged in 17 iterations. (Reason Code: 2)"
+    assert lines[1] == "    [Pressure] Min Value: 0 Pa | Max Value: 2.5 Pa"
+    assert lines[2] == "    [Saturation] Max div(q): 0.445 | Sub-steps: 7 (0.0714286s)"
+    assert lines[3] == "    [Saturation] Sub-stepping required: 7 steps of 0.0714286s."
+    assert lines[4] == "    [Concentration] Converged in 5 iterations. (Reason Code: 2)"
+    assert lines[6] == "    [Acid Transport] Converged in 0 iterations. (Reason Code: 2)"
+    assert lines[8] == "    [Leaching] Max rate: 0.25 | Sub-steps: 3 (0.166667s)"
+    assert parse_like_reference(text, explicit=True) == 7 + 3
+    # implicit saturation: the parser counts the solver's iterations instead
+    Pi = L.default_params(implicit_sat=1)
+    r.sat_iters, r.sat_valid, r.sat_raw_min, r.sat_raw_max = 9, 1, -0.01, 0.93
+    text = step_log(lib, r, 0.5, Pi)
+    assert "    [Implicit Saturation] Converged in 9 iterations. (Reason Code: 2)" in text
+    assert parse_like_reference(text, explicit=False) == 9
+    assert "[Saturation] Max div(q)" not in text
+    # a pressure solve that missed the tolerance ends the step's lines
+    r.press_relres = 1e-3
+    text = step_log(lib, r, 0.5, P)
+    assert text == "    [Pressure] DIVERGED/FAILED in 17 iterations! (Error Code: -3)\n"
