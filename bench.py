@@ -13,24 +13,26 @@ saturation, surface effects, conc + acid transport, precipitation,
 leaching) on device-resident state.  The timed region holds `--steps` steps
 after `--warmup` untimed steps, bracketed by device synchronisation.
 
-Runs (each on a fresh context, same workload and window):
-  headline  learned+mixed: flexible CG with the U-Net coarse-level
-            preconditioner, raced against the recorded curve of
-            mixed-precision PCG (FP32 CG on the red-black Schur complement
-            inside FP64 reliable updates), which takes over when the network
-            falls behind; warm start = projection onto earlier solutions
-            plus the network direction (trained weights from
-            leachingsim_amd/weights/);
-  baseline  plain FP64 Jacobi-PCG warm-started from the previous pressure
-            (the reference's algorithm, src/Simulation.cpp:131-159, 808);
-  ablations mixed-precision PCG + projected warm start (no network), and
-            plain FP64 PCG + projected warm start.
+Each run is a fresh context making the workload's first 100 steps (the
+timed window lies inside them, so the same steps are compared); the cold
+solves of steps 0 and 1 are reported apart, with the whole 100-step time.
+  headline  mixed precision: FP32 CG on the red-black Schur complement inside
+            FP64 reliable updates, warm start = projection onto earlier
+            solutions (the fastest configuration measured);
+  baseline  plain FP64 Jacobi-PCG with the same warm start (like for like),
+            and plain FP64 Jacobi-PCG from the previous pressure, the
+            reference's algorithm (src/Simulation.cpp:131-159, 808), timed
+            on its first steps (each is a full solve);
+  ablation  the learned solver: flexible CG with the U-Net coarse-level
+            preconditioner raced against mixed PCG, warm start with the
+            network direction (trained weights from leachingsim_amd/weights/).
 
-Prints one JSON line: steps/s of the headline, the baseline and ablations,
-the roofline of the dominant pressure kernel measured with HIP events
-during the timed region (HBM traffic from the committed rocprofv3 counter
-summary), the plain FP32 / FP64 stencil y = A x timed alone on the same
-operator, and the CPU oracle baseline.
+Prints one JSON line: steps/s of the headline over the window, every run's
+window and 100-step figures, the speedups over plain FP64, the roofline of
+the kernel with the largest share of the window (the transport's red Schur
+pass) and of the pressure kernel of the cold solves, both timed live with
+HIP events, the plain FP32 / FP64 stencil y = A x timed alone on the same
+operator, and the CPU oracle baseline over the same window.
 """
 from __future__ import annotations
 
@@ -62,13 +64,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+_hip = None
+
+
 def device_sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    """hipDeviceSynchronize through the HIP runtime the library links."""
+    global _hip
+    import ctypes
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so.7")
+    if _hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
 
 
 def workload(name: str, size: int | None):
@@ -113,60 +119,139 @@ def field_errors(a, b):
     return out
 
 
-def run_solver(img, label_bytes, dt, solver, steps, warmup, profile=False, warm=L.WARM_PREVIOUS,
-               weights=None, keep_fields=False):
-    P = L.default_params(bc_mode=L.BC_ZFACES, solver=solver, warm_start=warm)
-    sim = L.GpuSimulation(img.shape, P, label_bytes)
-    sim.read_labels(img)
-    sim.init_state()
-    if weights is not None:
-        sim.set_network(weights)
-    name = f"{SOLVER_NAMES[solver]}/{WARM_NAMES[warm]}"
-    t = 0.0
-    reps = []
-    for n in range(warmup):
-        r = sim.step(t, dt)
-        if r.status != 0:
-            raise RuntimeError(f"warmup step {n} failed")
-        t += dt
-        log(f"  [{name}] warmup {n}: {r.t_total_ms:.1f} ms, press iters {r.press_iters}")
-    sim.set_profiling(profile)
-    device_sync()
-    t0 = time.perf_counter()
-    for n in range(steps):
-        r = sim.step(t, dt)
-        if r.status != 0:
-            raise RuntimeError(f"timed step {n} failed: {r.as_dict()}")
-        reps.append(r.as_dict())
-        t += dt
-    device_sync()
-    t1 = time.perf_counter()
-    for n, r in enumerate(reps):
-        log(f"  [{name}] step {warmup + n}: {r['t_total_ms']:.1f} ms "
-            f"(pressure {r['t_pressure_ms']:.1f}, transport {r['t_transport_ms']:.1f}), "
-            f"press iters {r['press_iters']} outer {r['press_outer']} "
-            f"fallback {r['press_fallback']} unet {r['press_unet_applies']} "
-            f"warm relres {r['press_relres0']:.2e} basis {r['warm_basis']}+{r['warm_net']} "
-            f"relres {r['press_relres']:.2e}, adr iters {r['adr_iters']}")
-    copy_gbs = None
-    if profile:
-        nb = min(img.size * 8, 1 << 30)
-        ms = sim.time_copy(nb, 20)
-        copy_gbs = 2 * nb / ms / 1e6
-    fields = None
-    if keep_fields:
-        fields = {name: sim.download(f) for name, f in COMPARE_FIELDS}
-        fields["img"] = sim.download(L.F_IMG)
-        fields["frac"] = sim.download(L.F_FRAC)
-    sim.close()
-    return (t1 - t0), reps, copy_gbs, fields
+class Run:
+    """One configuration on a fresh context: steps 0 .. total-1 of the
+    workload; the timed window is steps warmup .. warmup+steps-1 (device
+    synchronised on both sides); every step's wall time and report kept."""
+
+    def __init__(self, solver, warm, **params):
+        self.solver, self.warm, self.params = solver, warm, params
+        self.name = f"{SOLVER_NAMES[solver]}/{WARM_NAMES[warm]}"
+
+    def run(self, img, label_bytes, dt, warmup, steps, total, profile=False, weights=None,
+            keep_fields=False):
+        P = L.default_params(bc_mode=L.BC_ZFACES, solver=self.solver, warm_start=self.warm,
+                             **self.params)
+        sim = L.GpuSimulation(img.shape, P, label_bytes)
+        sim.read_labels(img)
+        sim.init_state()
+        if weights is not None:
+            sim.set_network(weights)
+        self.reports, self.wall = [], []
+        self.fields = None
+        self.window_s = None
+        t = 0.0
+        total = max(total, warmup + steps)
+        # sampled kernel timing (one event pair per solver batch) for the
+        # whole run: the cold solves' pressure kernels and the window's
+        sim.set_profiling(profile)
+        for n in range(total):
+            if n == warmup:
+                device_sync()
+                w0 = time.perf_counter()
+            t0 = time.perf_counter()
+            r = sim.step(t, dt)
+            t1 = time.perf_counter()
+            if r.status != 0:
+                raise RuntimeError(f"[{self.name}] step {n} failed: {sim_error(sim)}")
+            t += dt
+            self.reports.append(r.as_dict())
+            self.wall.append(t1 - t0)
+            if n == warmup + steps - 1:
+                device_sync()
+                self.window_s = time.perf_counter() - w0
+                if keep_fields:
+                    self.fields = {name: sim.download(f) for name, f in COMPARE_FIELDS}
+                    self.fields["img"] = sim.download(L.F_IMG)
+                    self.fields["frac"] = sim.download(L.F_FRAC)
+            if n < 3 or warmup <= n < warmup + steps or n == total - 1 or n % 20 == 0:
+                log_step(self.name, n, r.as_dict(), 1e3 * (t1 - t0))
+        self.copy_gbs = None
+        if profile:
+            nb = min(img.size * 8, 1 << 30)
+            ms = sim.time_copy(nb, 20)
+            self.copy_gbs = 2 * nb / ms / 1e6
+        sim.close()
+        self.warmup, self.steps, self.total = warmup, steps, total
+        return self
+
+    def window(self):
+        return self.reports[self.warmup:self.warmup + self.steps]
+
+    def summary(self):
+        reps = self.window()
+        out = {"steps_per_s": self.steps / self.window_s,
+               "ms_per_step": 1e3 * self.window_s / self.steps,
+               "press_iters_per_step": [r["press_iters"] for r in reps],
+               "adr_iters_per_step": [r["adr_iters"] for r in reps],
+               "warm_relres": [r["press_relres0"] for r in reps],
+               "final_relres_max": max(r["press_relres"] for r in reps),
+               "fallbacks": sum(r["press_fallback"] for r in reps),
+               "unet_applies": sum(r["press_unet_applies"] for r in reps)}
+        # the two solves from a cold start (step 0: p = 0; step 1: the ramp has
+        # grown the drive 1e6-fold, so no earlier solution is used)
+        out["hard_solves"] = [{"step": n, "ms": 1e3 * self.wall[n],
+                               "press_ms": self.reports[n]["t_pressure_ms"],
+                               "press_iters": self.reports[n]["press_iters"]}
+                              for n in range(min(2, len(self.reports)))]
+        if self.total >= 100:
+            out["first_100_steps"] = {"s": sum(self.wall[:100]),
+                                      "steps_per_s": 100.0 / sum(self.wall[:100])}
+        return out
 
 
-def cpu_baseline(img, label_bytes, dt, crop: int, nsteps: int):
-    """The FP64 oracle (C++/OpenMP) on a bounded sample: a centred crop of
-    the same geometry, `nsteps` coupled steps, scaled to the full grid by
-    voxel count (the CPU's PCG iteration count grows with the grid, so the
-    scaled figure flatters the CPU)."""
+def sim_error(sim):
+    return sim.lib.ls_last_error(sim.h).decode()
+
+
+def log_step(name, n, r, ms):
+    log(f"  [{name}] step {n}: {ms:.1f} ms (pressure {r['t_pressure_ms']:.1f}, "
+        f"transport {r['t_transport_ms']:.1f}), press iters {r['press_iters']} "
+        f"outer {r['press_outer']} fallback {r['press_fallback']} unet {r['press_unet_applies']} "
+        f"warm relres {r['press_relres0']:.2e} basis {r['warm_basis']}+{r['warm_net']} "
+        f"relres {r['press_relres']:.2e}, adr iters {r['adr_iters']}")
+
+
+def host_cpu():
+    """Model name, sockets and logical CPUs of this host (lscpu, else
+    /proc/cpuinfo)."""
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+            elif k == "Core(s) per socket":
+                info["cores_per_socket"] = int(v)
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+    except (OSError, ValueError):
+        pass
+    if "model" not in info:
+        try:
+            for line in Path("/proc/cpuinfo").read_text().splitlines():
+                if line.startswith("model name"):
+                    info["model"] = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    return info
+
+
+def cpu_baseline(img, label_bytes, dt, crop: int, warmup: int, nsteps: int):
+    """The FP64 oracle (C++/OpenMP, the reference's algorithm: Jacobi-PCG
+    from the previous pressure, BiCGStab transport, both to 1e-8) on a
+    bounded sample of the workload: the centred crop^3 of the same geometry
+    over the same step window as the GPU line (warmup untimed steps, then
+    nsteps timed), scaled to the full grid by voxel count (the CPU's PCG
+    iteration count grows with the grid, so the scaled figure flatters the
+    CPU)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")     # read when the oracle loads
     from oracle.oracle import OracleSimulation, set_threads
     ncpu = len(os.sched_getaffinity(0))
     set_threads(ncpu)
@@ -178,19 +263,23 @@ def cpu_baseline(img, label_bytes, dt, crop: int, nsteps: int):
     o.init_state()
     t = 0.0
     times = []
-    for _ in range(nsteps):
+    for n in range(warmup + nsteps):
         t0 = time.perf_counter()
         r = o.step(t, dt)
-        times.append(time.perf_counter() - t0)
+        if n >= warmup:
+            times.append(time.perf_counter() - t0)
         t += dt
         if r.status != 0:
             break
     sec = float(np.mean(times))
     scale = sub.size / img.size
     return {"value": scale / sec, "unit": "steps/s", "cores": ncpu, "kind": "port",
-            "sample": f"oracle FP64 Jacobi-PCG, {nsteps} coupled steps on the centred "
-                      f"{'x'.join(map(str, sub.shape))} crop ({sec:.2f} s/step), scaled "
-                      f"by voxel count to {'x'.join(map(str, img.shape))}"}
+            "host": host_cpu(),
+            "sample": f"oracle FP64 Jacobi-PCG + BiCGStab to 1e-8, steps {warmup}.."
+                      f"{warmup + len(times) - 1} (after {warmup} untimed) of the workload on "
+                      f"its centred {'x'.join(map(str, sub.shape))} crop, {sec:.3f} s/step, "
+                      f"scaled by voxel count to {'x'.join(map(str, img.shape))}; "
+                      f"OMP threads {ncpu}"}
 
 
 KERNELS = {"stencil64": (64, "k_stencil_vec<double>"), "stencil32": (32, "k_stencil_vec<float>"),
@@ -276,15 +365,6 @@ def unet_only(size, reps):
     print(json.dumps(out), flush=True)
 
 
-def summary(sec, reps, steps):
-    return {"steps_per_s": steps / sec,
-            "press_iters_per_step": [r["press_iters"] for r in reps],
-            "warm_relres": [r["press_relres0"] for r in reps],
-            "final_relres_max": max(r["press_relres"] for r in reps),
-            "fallbacks": sum(r["press_fallback"] for r in reps),
-            "unet_applies": sum(r["press_unet_applies"] for r in reps)}
-
-
 def load_traffic():
     """HBM bytes per stencil launch from the rocprofv3 FETCH_SIZE/WRITE_SIZE
     runs summarised in profiles/ (None if absent)."""
@@ -294,25 +374,48 @@ def load_traffic():
         return None
 
 
+def sampled(reps, key):
+    n = sum(r[key + "_n"] for r in reps)
+    ms = sum(r[key + "_ms"] for r in reps) / max(n, 1)
+    return n, ms
+
+
+def kernel_roofline(name, bytes_per_launch, ms, n, traffic=None):
+    achieved = bytes_per_launch / (ms * 1e-3) / 1e9 if n else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": name,
+            "bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms, "sampled_launches": n}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--total", type=int, default=100,
+                    help="steps each full run makes (the workload's 100 steps at dt 0.5); the "
+                         "timed window is steps warmup .. warmup+steps-1 of them")
     ap.add_argument("--workload", default="fractured512", choices=list(G.WORKLOADS))
     ap.add_argument("--size", type=int, default=None,
                     help="edge length of the fractured workload (default 512)")
-    ap.add_argument("--solver", default="learned", choices=["plain", "mixed", "learned"])
+    ap.add_argument("--solver", default="mixed", choices=["plain", "mixed", "learned"],
+                    help="headline solver (default: mixed, the fastest measured)")
     ap.add_argument("--warm", default=None, choices=["previous", "projected", "learned"],
                     help="warm start of the headline (default: learned for the learned "
                          "solver, projected otherwise)")
+    ap.add_argument("--warm-history", type=int, default=None,
+                    help="projection basis size of every projected run (default: the "
+                         "library's)")
     ap.add_argument("--weights", default=None, help="U-Net weights (float32 file)")
-    ap.add_argument("--no-baseline", action="store_true", help="skip the plain FP64 run")
-    ap.add_argument("--no-ablation", action="store_true",
-                    help="skip the mixed+projected and plain+projected runs")
+    ap.add_argument("--no-baseline", action="store_true",
+                    help="skip the plain FP64 runs (projected and previous-pressure)")
+    ap.add_argument("--previous-steps", type=int, default=3,
+                    help="steps of the plain FP64 previous-pressure run (every one is a "
+                         "full solve of ~4000 iterations; timed from step 0)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned ablation")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-crop", type=int, default=96)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--mode", default="steps", choices=["steps", "stencil", "unet"],
                     help="stencil: assemble the pressure operator once and time only the "
                          "stencil and a copy kernel (for counter collection); unet: time "
@@ -342,94 +445,113 @@ def main():
             "learned": L.WARM_LEARNED, None: None}[args.warm]
     if warm is None:
         warm = L.WARM_LEARNED if head == L.SOLVER_LEARNED else L.WARM_PROJECTED
+    extra = {} if args.warm_history is None else {"warm_history": args.warm_history}
     weights = None
-    if head == L.SOLVER_LEARNED or warm == L.WARM_LEARNED:
-        from leachingsim_amd.learn import WEIGHTS, load_weights
+    from leachingsim_amd.learn import WEIGHTS, load_weights
+    if head == L.SOLVER_LEARNED or warm == L.WARM_LEARNED or not args.no_learned:
         weights = load_weights(args.weights or WEIGHTS)
-
+    W, K, T = args.warmup, args.steps, args.total
     keep = not args.no_baseline
-    sec, reps, copy_gbs, hfields = run_solver(img, lb, dt, head, args.steps, args.warmup,
-                                              profile=True, warm=warm, weights=weights,
-                                              keep_fields=keep)
-    value = args.steps / sec
-    base = None
+
+    hr = Run(head, warm, **extra).run(img, lb, dt, W, K, T, profile=True,
+                                      weights=weights if head == L.SOLVER_LEARNED or
+                                      warm == L.WARM_LEARNED else None, keep_fields=keep)
+    value = K / hr.window_s
+    runs = {hr.name: hr}
+    base = prev = None
     errors = None
     if not args.no_baseline:
-        bsec, breps, _, bfields = run_solver(img, lb, dt, L.SOLVER_PLAIN64, args.steps,
-                                             args.warmup, keep_fields=True)
-        base = summary(bsec, breps, args.steps)
-        errors = field_errors(hfields, bfields)
-        del hfields, bfields
-    ablations = {}
-    if not args.no_ablation:
-        for sv in (L.SOLVER_MIXED, L.SOLVER_PLAIN64):
-            if (sv, L.WARM_PROJECTED) == (head, warm):
-                continue
-            asec, areps, _, _ = run_solver(img, lb, dt, sv, args.steps, args.warmup,
-                                           warm=L.WARM_PROJECTED)
-            ablations[f"{SOLVER_NAMES[sv]}/projected"] = summary(asec, areps, args.steps)
+        # like for like: plain FP64 PCG with the headline's warm start
+        bw = warm if warm != L.WARM_LEARNED else L.WARM_PROJECTED
+        base = Run(L.SOLVER_PLAIN64, bw, **extra).run(img, lb, dt, W, K, T, keep_fields=True)
+        runs[base.name] = base
+        errors = field_errors(hr.fields, base.fields)
+        hr.fields = base.fields = None
+        # the reference's algorithm: plain FP64 PCG from the previous pressure;
+        # every step is a full solve, so a short run from step 0 is timed
+        npv = max(1, args.previous_steps)
+        prev = Run(L.SOLVER_PLAIN64, L.WARM_PREVIOUS).run(img, lb, dt, 0, npv, npv)
+        runs[prev.name] = prev
+    learned = None
+    if not args.no_learned and head != L.SOLVER_LEARNED:
+        learned = Run(L.SOLVER_LEARNED, L.WARM_LEARNED, **extra).run(img, lb, dt, W, K, T,
+                                                                     weights=weights)
+        runs[learned.name] = learned
 
-    # roofline of the sampled pressure kernel (HIP events in the timed
-    # region): the fused Schur apply on compacted vectors when the mixed
-    # solver runs on the Schur complement (3 faces + half-length p and q:
-    # 16 B per voxel), its gather pass in the two-pass form, else the FP32
-    # or FP64 stencil (x, 3 faces, y: 20 B / 40 B per voxel)
-    if sum(r["prof_schur_n"] for r in reps) > 0:
-        key = "prof_schur"
-        bytes_per_launch = reps[-1]["prof_schur_bytes"]
-        kernel = "schur" if bytes_per_launch == img.size * 16 else "rb_gather"
-    elif head == L.SOLVER_PLAIN64 or sum(r["prof_stencil32_n"] for r in reps) == 0:
-        key, kernel, bytes_per_launch = "prof_stencil64", "stencil64", img.size * 40
-    else:
-        key, kernel, bytes_per_launch = "prof_stencil32", "stencil32", img.size * 20
-    n = sum(r[key + "_n"] for r in reps)
-    ms = sum(r[key + "_ms"] for r in reps) / max(n, 1)
-    kname = KERNELS[kernel][1]
-    # the vectorised stencils need nk % 4 == 0 (FP32) / nk % 2 == 0 (FP64);
-    # otherwise the solver ran the scalar stencil
-    if kernel == "stencil32" and img.shape[2] % 4 != 0:
-        kname = "k_stencil<float, 0>"
-    if kernel == "stencil64" and img.shape[2] % 2 != 0:
-        kname = "k_stencil<double, 0>"
-    if kernel == "rb_gather" and img.shape[2] % 4 != 0:
-        kname = "k_rb_scalar<1>"
-    achieved = bytes_per_launch / (ms * 1e-3) / 1e9 if n else 0.0
-    traffic = None
-    tr = load_traffic()
-    if tr is not None and kernel in tr and tr[kernel].get("grid") == list(img.shape):
-        traffic = tr[kernel]["bytes_per_launch"]
+    # roofline: the kernel with the largest share of the timed window, the
+    # transport's red Schur pass (k_tschur_red, 36 B per voxel of the grid
+    # for one system: the red rows' 7 coefficients and p, w at half length),
+    # timed live with HIP events on the context's stream; the pressure
+    # kernels that dominate the cold solves beside it
+    reps = hr.window()
+    rl = {}
+    n, ms = sampled(reps, "prof_tred")
+    if n:
+        rl["transport_red_pass"] = kernel_roofline(
+            "k_tschur_red<1>" if reps[-1]["prof_tred_bytes"] == img.size * 36 else
+            "k_tschur_red<3>", reps[-1]["prof_tred_bytes"], ms, n)
+    hard = hr.reports[:min(2, len(hr.reports))]
+    n, ms = sampled(hard, "prof_schur")
+    if n:
+        b = hard[-1]["prof_schur_bytes"]
+        kname = "k_schur_apply<16, 4>" if b == img.size * 16 else (
+            "k_rb_vec<1>" if img.shape[2] % 4 == 0 else "k_rb_scalar<1>")
+        tr = load_traffic() or {}
+        t = tr.get("schur" if b == img.size * 16 else "rb_gather", {})
+        rl["pressure_schur_apply_cold_solves"] = kernel_roofline(
+            kname, b, ms, n, t.get("bytes_per_launch") if t.get("grid") == list(img.shape)
+            else None)
+    tr = load_traffic() or {}
+    t = tr.get("transport_red", {})
+    if "transport_red_pass" in rl and t.get("grid") == list(img.shape):
+        rl["transport_red_pass"]["traffic"] = t.get("bytes_per_launch")
+    roof = rl.get("transport_red_pass") or rl.get("pressure_schur_apply_cold_solves") or \
+        kernel_roofline("none", 0.0, 0.0, 0)
 
     stencils = stencil_rooflines(img, lb)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(img, lb, dt, min(args.cpu_crop, *img.shape), args.cpu_steps)
+        cpu = cpu_baseline(img, lb, dt, min(args.cpu_crop, *img.shape), W,
+                           min(args.cpu_steps, K))
 
+    summ = {name: r.summary() for name, r in runs.items()}
+    if prev is not None:
+        summ[prev.name]["note"] = (f"steps 0..{prev.steps - 1} from the initial state; every "
+                                   "step is a full solve from the previous pressure")
+    full = {name: summ[name].get("first_100_steps") for name in summ
+            if summ[name].get("first_100_steps")}
     out = {
         "metric": "steps/s at 512^3, 1e-8 rel residual: learned+mixed vs plain FP64 CG",
         "value": value,
         "unit": "steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * sec / args.steps,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * hr.window_s / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == L.SOLVER_PLAIN64 else "f32+f64+bf16",
+        "dtype": "f64" if head == L.SOLVER_PLAIN64 else (
+            "f32+f64+bf16" if head == L.SOLVER_LEARNED else "f32+f64"),
         "data": "synthetic",
         "config": {"workload": desc, "grid": list(img.shape), "solver": SOLVER_NAMES[head],
                    "warm_start": WARM_NAMES[warm], "pressure_rtol": 1e-8,
-                   "headline": summary(sec, reps, args.steps),
-                   "plain_fp64_pcg_previous": base,
-                   "speedup_vs_plain_fp64": (value / base["steps_per_s"]) if base else None,
-                   "error_vs_plain_fp64_after_window": errors,
-                   "ablations": ablations},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "kernel": kname,
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": ms,
-                     "sampled_launches": n, "measured_copy_GBs": copy_gbs, "traffic": traffic},
+                   "transport_rtol": 1e-8, "window": f"steps {W}..{W + K - 1}",
+                   "runs": summ,
+                   "speedup_vs_plain_fp64": (value / summ[base.name]["steps_per_s"])
+                   if base else None,
+                   "speedup_vs_plain_fp64_note": "plain FP64 Jacobi-PCG with the headline's "
+                   "warm start, same window",
+                   "speedup_vs_plain_fp64_previous_pressure":
+                       (value / summ[prev.name]["steps_per_s"]) if prev else None,
+                   "full_run_first_100_steps": full,
+                   "full_run_speedup_vs_plain_fp64":
+                       (full[hr.name]["steps_per_s"] / full[base.name]["steps_per_s"])
+                       if base and hr.name in full and base.name in full else None,
+                   "error_vs_plain_fp64_after_window": errors},
+        "roofline": roof,
+        "rooflines": rl,
         "stencil_roofline": stencils,
         "cpu_baseline": cpu,
     }

@@ -110,9 +110,16 @@ typedef struct ls_params {
     int press_maxit;         /* 10000 (src/Simulation.cpp:156)       */
     double adr_rtol;         /* 1e-8                                 */
     int adr_maxit;           /* 10000                                */
-    int warm_history;        /* solutions kept for LS_WARM_PROJECTED / LEARNED (4),
-                                1 .. LS_WARM_MAX (LEARNED uses one slot for the
-                                network correction)                              */
+    int warm_history;        /* solutions kept for LS_WARM_PROJECTED / LEARNED (1:
+                                the newest solution, optimally scaled), 1 ..
+                                LS_WARM_MAX (LEARNED uses one slot for the
+                                network correction).  More vectors make the
+                                start exact when the right-hand side moves in
+                                their span under a fixed operator (a ramp plus a
+                                fixed evaporation source); on the 512^3
+                                fractured workload, whose operator drifts every
+                                step, 2 and 4 vectors left the start's residual
+                                no lower and made a step 2-13 ms slower         */
     double mixed_inner_rtol; /* mixed solver: the FP64 residual replaces the FP32 one
                                 whenever the latter has fallen by this factor (0.1) */
     double learned_race;    /* learned solver: a race against the residual curve
@@ -141,6 +148,11 @@ typedef struct ls_params {
                                 (black voxels, about half the iterations), the
                                 red values recovered after convergence; needs
                                 nk even; 0: BiCGStab on the full system         */
+    int adr_extrapolate;     /* transport on the Schur complement (1): conc starts
+                                from its linear extrapolation in time through the
+                                last two solves, x + (dt / dt_prev) (x - x_prev),
+                                instead of the old field (the reference's start,
+                                src/Simulation.cpp:1247); 0: the old field       */
 } ls_params;
 
 /* Fields that can be uploaded / downloaded.  Element types are fixed:
@@ -225,6 +237,12 @@ typedef struct ls_step_report {
     /* Iterations of each transport system (conc, acid), for the reference's
      * per-solve log lines (src/Simulation.cpp:1278, 2208). */
     int adr_iters_conc, adr_iters_acid;
+    /* With profiling on: HIP-event durations of sampled launches of the
+     * transport's red Schur pass (w = D_r^-1 A_rb p, the first launch of every
+     * BiCGStab batch), their count and its algorithmic HBM bytes per launch. */
+    double prof_tred_ms;
+    int prof_tred_n;
+    double prof_tred_bytes;
 } ls_step_report;
 
 typedef struct ls_ctx ls_ctx;

@@ -159,6 +159,18 @@ __global__ void k_leach_wave(uint8_t* __restrict__ img, float* __restrict__ frac
 }
 
 int phase_leaching(ls_ctx* c, double dt, ls_step_report* rep) {
+    if (c->acid_zero && c->frac_clean) {
+        // every surface flux is k_eff * 0 and no fraction sits at or below
+        // the threshold: the scan would find max rate 0 (one sub-step) and
+        // nothing to do, and the sweep would change nothing
+        if (rep) {
+            rep->leach_substeps = 1;
+            rep->leach_max_rate = 0.0;
+            rep->leached = 0;
+            rep->leach_general_path = 0;
+        }
+        return 0;
+    }
     LGeo g;
     g.ni = c->ni; g.nj = c->nj; g.nk = c->nk;
     g.sj = (uint32_t)c->nk;
@@ -179,6 +191,7 @@ int phase_leaching(ls_ctx* c, double dt, ls_step_report* rep) {
     int general = 0;
     if (c->h_ss->leach_needed) {
         general = 1;
+        c->frac_clean = false;    // the sweep may leave fractions anywhere in (0, 1]
         const int W = (c->ni - 1) + 2 * (c->nj - 1) + 3 * (c->nk - 1) + 1;
         dim3 blk(64, 4), grd((c->nk + 63) / 64, (c->nj + 3) / 4);
         for (int step = 0; step < nsub; ++step) {

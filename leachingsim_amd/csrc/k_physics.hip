@@ -59,6 +59,15 @@ static Geo geo_of(const ls_ctx* c) {
 // ---------------------------------------------------------------------------
 struct PropConst {
     double m, s_res, vg_alpha, vg_n, log_k_rock, log_k_pore;
+    // 10^log_k_pore and 10^log_k_rock, evaluated once on the host (the
+    // intrinsic permeability of a pore with no inventory, of a sulphide with
+    // fraction 0 or 1, and of a fully clogged pore)
+    double k_pore, k_rock;
+    // vg_n == 2 (the reference's value, src/main.cpp:89), so m = 1/2: the
+    // van Genuchten powers are x^2 = x * x, x^(1/2) = sqrt(x) and x^-2 =
+    // 1 / (x * x), each one rounding (correctly rounded, as pow is to within
+    // an ulp) instead of a general pow
+    int vg2;
 };
 
 __device__ __forceinline__ double rel_perm(uint8_t t, double s, const PropConst& pc) {
@@ -66,7 +75,8 @@ __device__ __forceinline__ double rel_perm(uint8_t t, double s, const PropConst&
         double Se = (s - pc.s_res) / (1.0 - pc.s_res);
         Se = dmax(0.0, dmin(1.0, Se));
         if (Se <= 0.0) return 0.0;
-        double term1 = 1.0 - pow(1.0 - pow(Se, 1.0 / pc.m), pc.m);
+        double term1 = pc.vg2 ? 1.0 - sqrt(1.0 - Se * Se)
+                              : 1.0 - pow(1.0 - pow(Se, 1.0 / pc.m), pc.m);
         return sqrt(Se) * term1 * term1;
     }
     return 0.0;
@@ -77,6 +87,7 @@ __device__ __forceinline__ double cap_pressure(uint8_t t, double s, const PropCo
         double Se = (s - pc.s_res) / (1.0 - pc.s_res);
         Se = dmax(1e-6, dmin(1.0, Se));
         if (Se >= 1.0) return 0.0;
+        if (pc.vg2) return (1.0 / pc.vg_alpha) * sqrt(1.0 / (Se * Se) - 1.0);
         double Se_inv_m = pow(Se, -1.0 / pc.m);
         return (1.0 / pc.vg_alpha) * pow(Se_inv_m - 1.0, 1.0 / pc.vg_n);
     }
@@ -94,12 +105,25 @@ __device__ __forceinline__ double permeability(uint8_t type, double krel, double
         K_intrinsic = pore_permeability;
     } else if (type == PORE) {
         double p_frac = dmax(0.0, dmin(1.0, inv));
-        double log_k_eff = (p_frac * pc.log_k_rock) + ((1.0 - p_frac) * pc.log_k_pore);
-        K_intrinsic = pow(10.0, log_k_eff);
+        // p_frac 0 or 1 makes log_k_eff exactly log_k_pore or log_k_rock
+        if (p_frac == 0.0) {
+            K_intrinsic = pc.k_pore;
+        } else if (p_frac == 1.0) {
+            K_intrinsic = pc.k_rock;
+        } else {
+            double log_k_eff = (p_frac * pc.log_k_rock) + ((1.0 - p_frac) * pc.log_k_pore);
+            K_intrinsic = pow(10.0, log_k_eff);
+        }
     } else if (type == SUL) {
         double f = fr;
-        double log_k_eff = (f * pc.log_k_rock) + ((1.0 - f) * pc.log_k_pore);
-        K_intrinsic = pow(10.0, log_k_eff);
+        if (f == 1.0) {
+            K_intrinsic = pc.k_rock;
+        } else if (f == 0.0) {
+            K_intrinsic = pc.k_pore;
+        } else {
+            double log_k_eff = (f * pc.log_k_rock) + ((1.0 - f) * pc.log_k_pore);
+            K_intrinsic = pow(10.0, log_k_eff);
+        }
     }
     double K_rel = dmax(krel, 1e-15);
     return K_intrinsic * K_rel;
@@ -113,6 +137,9 @@ static PropConst prop_const(const ls_ctx* c) {
     pc.vg_n = c->P.vg_n;
     pc.log_k_rock = c->log_k_rock;
     pc.log_k_pore = c->log_k_pore;
+    pc.k_pore = std::pow(10.0, c->log_k_pore);
+    pc.k_rock = std::pow(10.0, c->log_k_rock);
+    pc.vg2 = c->P.vg_n == 2.0 ? 1 : 0;
     return pc;
 }
 
@@ -208,13 +235,22 @@ int init_state(ls_ctx* c, long long* nsul) {
     c->have_prev = false;
     c->n_hist = 0;
     c->press_assembled = false;
+    c->acid_zero = true;      // k_init: acid = 0, frac in {0, 1}
+    c->tprev_valid = false;
+    c->frac_clean = true;
     return phase_properties(c);
 }
 
 // ---------------------------------------------------------------------------
 // Pressure operator (src/Simulation.cpp:658-798), reduced to the solved Pore
-// rows.  Stored per voxel: the three +face conductances as the Pore row
-// computes them, the coupling sum d, the row class and the Dirichlet value.
+// rows.  Two passes over the grid: k_press_diag stores per voxel the row
+// class, the Dirichlet value and sqrt(d) of the coupling sum d on the solved
+// rows; k_press_rhs_scale
+// then builds, per solved row, the right-hand side and the Jacobi-scaled
+// +face couplings (FP64 and FP32).  The face conductances themselves are
+// recomputed from the permeability where needed, never stored: a face has
+// the same bits from either side (see face_cond), so every pass sees the
+// coupling the reference's Pore row computes.
 // ---------------------------------------------------------------------------
 
 // Harmonic-mean Pore link (src/Simulation.cpp:737-768).
@@ -225,39 +261,27 @@ __device__ __forceinline__ double pore_link(double perm_self, double k_neighbor,
     return K_avg / mu_self;
 }
 
-// Conductance of the face between v and n as seen by the Pore side.
+// Conductance of the face between v and n as seen by the Pore side.  Between
+// two Pore voxels it is symmetric bit for bit: 2 K_a is exact, so
+// (2 K_a) K_b and (2 K_b) K_a round the same product, and the sum commutes.
 __device__ __forceinline__ double face_cond(uint8_t tv, uint8_t tn, double pv, double pn) {
     if (tv == PORE) return pore_link(pv, (tn == AIR) ? 1.0e-9 : pn, viscosity(tv));
     if (tn == PORE) return pore_link(pn, (tv == AIR) ? 1.0e-9 : pv, viscosity(tn));
     return 0.0;
 }
 
-__global__ void k_press_faces(const uint8_t* __restrict__ img, const double* __restrict__ perm,
-                              const double* __restrict__ p, double* __restrict__ cfx,
-                              double* __restrict__ cfy, double* __restrict__ cfz,
-                              double* __restrict__ cin, double* __restrict__ dsum,
-                              int8_t* __restrict__ cls, double* __restrict__ pD, Geo g, int zfaces,
-                              double ptop) {
+__global__ void k_press_diag(const uint8_t* __restrict__ img, const double* __restrict__ perm,
+                             const double* __restrict__ p,
+                             int8_t* __restrict__ cls, double* __restrict__ pD,
+                             double* __restrict__ sd, Geo g, int zfaces, double ptop) {
     const int k = blockIdx.x * TK + threadIdx.x;
     const int j = blockIdx.y * TJ + threadIdx.y;
     const int i = blockIdx.z;
     if (k >= g.nk || j >= g.nj) return;
     const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
     const uint8_t tv = img[v];
-    const double pv = perm[v];
-    // +faces, value from the Pore side (identical from both sides when both
-    // are Pore: the harmonic mean is symmetric bit for bit)
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    if (i + 1 < g.ni) fx = face_cond(tv, img[v + g.si], pv, perm[v + g.si]);
-    else if (zfaces && tv == PORE) fx = pore_link(pv, 1.0e-9, viscosity(tv));  // outlet plane
-    if (j + 1 < g.nj) fy = face_cond(tv, img[v + g.sj], pv, perm[v + g.sj]);
-    if (k + 1 < g.nk) fz = face_cond(tv, img[v + 1], pv, perm[v + 1]);
-    cfx[v] = fx;
-    cfy[v] = fy;
-    cfz[v] = fz;
-    if (i == 0) cin[(uint32_t)j * g.sj + k] = (zfaces && tv == PORE) ? pore_link(pv, 1.0e-9, viscosity(tv)) : 0.0;
-
     if (tv == PORE) {
+        const double pv = perm[v];
         double diagonal_term = 0.0;
         for (int f = 0; f < 6; ++f) {
             uint32_t n;
@@ -266,15 +290,16 @@ __global__ void k_press_faces(const uint8_t* __restrict__ img, const double* __r
             double K_neighbor = (tn == AIR) ? 1.0e-9 : perm[n];
             diagonal_term -= pore_link(pv, K_neighbor, viscosity(tv));
         }
+        // the virtual inlet / outlet planes of the z-face boundary
         if (zfaces && i == 0) diagonal_term -= pore_link(pv, 1.0e-9, viscosity(tv));
         if (zfaces && i == g.ni - 1) diagonal_term -= pore_link(pv, 1.0e-9, viscosity(tv));
-        dsum[v] = -diagonal_term;
         int8_t cl = (fabs(diagonal_term) < 1e-16) ? 2 : 1;
         cls[v] = cl;
         pD[v] = (cl == 2) ? p[v] : 0.0;  // floored rows: lagged value
+        sd[v] = (cl == 1) ? sqrt(-diagonal_term) : 0.0;
     } else {
-        dsum[v] = 0.0;
         cls[v] = 0;
+        sd[v] = 0.0;
         double val = 0.0;
         if (tv == AIR) {  // surface Air: any in-grid neighbour is not Air (:696-711)
             for (int f = 0; f < 6; ++f) {
@@ -286,76 +311,53 @@ __global__ void k_press_faces(const uint8_t* __restrict__ img, const double* __r
     }
 }
 
-// Face conductance of face f of v from the stored +face arrays.
-__device__ __forceinline__ double face_raw(const double* cfx, const double* cfy, const double* cfz,
-                                           uint32_t v, uint32_t n, int f) {
-    switch (f) {
-        case 0: return cfx[v];
-        case 1: return cfx[n];
-        case 2: return cfy[v];
-        case 3: return cfy[n];
-        case 4: return cfz[v];
-        default: return cfz[n];
-    }
-}
-
-__global__ void k_press_rhs(const uint8_t* __restrict__ img, const double* __restrict__ cfx,
-                            const double* __restrict__ cfy, const double* __restrict__ cfz,
-                            const double* __restrict__ cin, const double* __restrict__ dsum,
-                            const int8_t* __restrict__ cls, const double* __restrict__ pD,
-                            const double* __restrict__ p, double* __restrict__ sd,
-                            double* __restrict__ bhat, double* __restrict__ x, Geo g, int zfaces,
-                            double ptop, double evap_dx) {
+// Right-hand side (Air neighbours' evaporation source, Dirichlet neighbours,
+// the inlet plane), the scaled warm start, and the scaled +faces
+// c / (sqrt(d_v) sqrt(d_n)) between two solved rows.  Only the faces these
+// need are evaluated: those to unsolved neighbours and the solved +faces.
+__global__ void k_press_rhs_scale(const uint8_t* __restrict__ img, const double* __restrict__ perm,
+                                  const int8_t* __restrict__ cls, const double* __restrict__ pD,
+                                  const double* __restrict__ p, const double* __restrict__ sd,
+                                  double* __restrict__ bhat, double* __restrict__ x,
+                                  double* __restrict__ hx, double* __restrict__ hy,
+                                  double* __restrict__ hz, float* __restrict__ fx,
+                                  float* __restrict__ fy, float* __restrict__ fz, Geo g,
+                                  int zfaces, double ptop, double evap_dx) {
     const int k = blockIdx.x * TK + threadIdx.x;
     const int j = blockIdx.y * TJ + threadIdx.y;
     const int i = blockIdx.z;
     if (k >= g.nk || j >= g.nj) return;
     const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
     if (cls[v] != 1) {
-        sd[v] = 0.0;
         bhat[v] = 0.0;
         x[v] = 0.0;
+        hx[v] = 0.0; hy[v] = 0.0; hz[v] = 0.0;
+        fx[v] = 0.0f; fy[v] = 0.0f; fz[v] = 0.0f;
         return;
     }
+    const double pv = perm[v];
+    const double sv = sd[v];
     double source_term = 0.0;
     for (int f = 0; f < 6; ++f) {
         uint32_t n;
         if (nbr(g, i, j, k, v, f, n) && img[n] == AIR) source_term -= evap_dx;
     }
     double rhs = -source_term;
+    double sc[3] = {0.0, 0.0, 0.0};
     for (int f = 0; f < 6; ++f) {
         uint32_t n;
         if (!nbr(g, i, j, k, v, f, n)) continue;
-        if (cls[n] == 1) continue;
-        rhs += face_raw(cfx, cfy, cfz, v, n, f) * pD[n];
+        const int8_t cl = cls[n];
+        if (cl == 1 && (f & 1)) continue;       // a solved -face: the neighbour's +face
+        const double fc = face_cond(PORE, img[n], pv, perm[n]);
+        if (cl != 1) rhs += fc * pD[n];
+        else sc[f >> 1] = fc / (sv * sd[n]);
     }
-    if (zfaces && i == 0) rhs += cin[(uint32_t)j * g.sj + k] * ptop;
-    double sv = sqrt(dsum[v]);
-    sd[v] = sv;
+    if (zfaces && i == 0) rhs += pore_link(pv, 1.0e-9, viscosity(PORE)) * ptop;
     bhat[v] = rhs / sv;
     x[v] = sv * p[v];  // warm start from the previous pressure, scaled variables
-}
-
-// Scaled operator faces: c / (sqrt(d_v) sqrt(d_n)) between two solved rows.
-__global__ void k_press_scale(const double* __restrict__ cfx, const double* __restrict__ cfy,
-                              const double* __restrict__ cfz, const int8_t* __restrict__ cls,
-                              const double* __restrict__ sd, double* __restrict__ hx,
-                              double* __restrict__ hy, double* __restrict__ hz,
-                              float* __restrict__ fx, float* __restrict__ fy,
-                              float* __restrict__ fz, Geo g) {
-    const int k = blockIdx.x * TK + threadIdx.x;
-    const int j = blockIdx.y * TJ + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= g.nk || j >= g.nj) return;
-    const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
-    const bool rv = cls[v] == 1;
-    const double sv = sd[v];
-    double a = 0.0, b = 0.0, cc = 0.0;
-    if (rv && i + 1 < g.ni && cls[v + g.si] == 1) a = cfx[v] / (sv * sd[v + g.si]);
-    if (rv && j + 1 < g.nj && cls[v + g.sj] == 1) b = cfy[v] / (sv * sd[v + g.sj]);
-    if (rv && k + 1 < g.nk && cls[v + 1] == 1) cc = cfz[v] / (sv * sd[v + 1]);
-    hx[v] = a; hy[v] = b; hz[v] = cc;
-    fx[v] = (float)a; fy[v] = (float)b; fz[v] = (float)cc;
+    hx[v] = sc[0]; hy[v] = sc[1]; hz[v] = sc[2];
+    fx[v] = (float)sc[0]; fy[v] = (float)sc[1]; fz[v] = (float)sc[2];
 }
 
 __global__ void k_count_floored(const int8_t* __restrict__ cls, uint32_t N, double* partials) {
@@ -380,16 +382,12 @@ int assemble_pressure(ls_ctx* c, double t) {
     const double ptop = effective_top_pressure(c->P.top_pressure, t);
     const int zf = c->P.bc_mode == LS_BC_ZFACES;
     dim3 blk(TK, TJ), grd((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, c->ni);
-    hipLaunchKernelGGL(k_press_faces, grd, blk, 0, c->stream, c->img, c->perm, c->p, c->craw[0],
-                       c->craw[1], c->craw[2], c->cin, c->dsum, c->cls, c->pD, g, zf, ptop);
+    hipLaunchKernelGGL(k_press_diag, grd, blk, 0, c->stream, c->img, c->perm, c->p, c->cls,
+                       c->pD, c->sd, g, zf, ptop);
     LS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_press_rhs, grd, blk, 0, c->stream, c->img, c->craw[0], c->craw[1],
-                       c->craw[2], c->cin, c->dsum, c->cls, c->pD, c->p, c->sd, c->bhat, c->x64, g,
-                       zf, ptop, c->P.evap_flux * c->P.dx);
-    LS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_press_scale, grd, blk, 0, c->stream, c->craw[0], c->craw[1], c->craw[2],
-                       c->cls, c->sd, c->c64[0], c->c64[1], c->c64[2], c->c32[0], c->c32[1],
-                       c->c32[2], g);
+    hipLaunchKernelGGL(k_press_rhs_scale, grd, blk, 0, c->stream, c->img, c->perm, c->cls, c->pD,
+                       c->p, c->sd, c->bhat, c->x64, c->c64[0], c->c64[1], c->c64[2], c->c32[0],
+                       c->c32[1], c->c32[2], g, zf, ptop, c->P.evap_flux * c->P.dx);
     LS_LAUNCH_CHECK();
     uint32_t N = (uint32_t)c->N;
     int nb = ew_blocks(N);
@@ -406,12 +404,11 @@ int assemble_pressure(ls_ctx* c, double t) {
 // After the solve: p = x/sqrt(d) on solved rows, the floored rows from their
 // own reference row 1e-16 p_f + sum_n c p_n = source (src/Simulation.cpp:
 // 783-797), the Dirichlet value everywhere else.
-__global__ void k_press_final(const uint8_t* __restrict__ img, const double* __restrict__ cfx,
-                              const double* __restrict__ cfy, const double* __restrict__ cfz,
-                              const double* __restrict__ cin, const int8_t* __restrict__ cls,
-                              const double* __restrict__ pD, const double* __restrict__ sd,
-                              const double* __restrict__ x, double* __restrict__ p, Geo g,
-                              int zfaces, double ptop, double evap_dx) {
+__global__ void k_press_final(const uint8_t* __restrict__ img, const double* __restrict__ perm,
+                              const int8_t* __restrict__ cls, const double* __restrict__ pD,
+                              const double* __restrict__ sd, const double* __restrict__ x,
+                              double* __restrict__ p, Geo g, int zfaces, double ptop,
+                              double evap_dx) {
     const int k = blockIdx.x * TK + threadIdx.x;
     const int j = blockIdx.y * TJ + threadIdx.y;
     const int i = blockIdx.z;
@@ -421,15 +418,16 @@ __global__ void k_press_final(const uint8_t* __restrict__ img, const double* __r
     if (cl == 1) {
         p[v] = x[v] / sd[v];
     } else if (cl == 2) {
+        const double pv = perm[v];
         double source_term = 0.0, sum = 0.0;
         for (int f = 0; f < 6; ++f) {
             uint32_t n;
             if (!nbr(g, i, j, k, v, f, n)) continue;
             if (img[n] == AIR) source_term -= evap_dx;
             double pn = (cls[n] == 1) ? x[n] / sd[n] : pD[n];
-            sum += face_raw(cfx, cfy, cfz, v, n, f) * pn;
+            sum += face_cond(PORE, img[n], pv, perm[n]) * pn;
         }
-        if (zfaces && i == 0) sum += cin[(uint32_t)j * g.sj + k] * ptop;
+        if (zfaces && i == 0) sum += pore_link(pv, 1.0e-9, viscosity(PORE)) * ptop;
         p[v] = (source_term - sum) / 1e-16;
     } else {
         p[v] = pD[v];
@@ -439,9 +437,9 @@ __global__ void k_press_final(const uint8_t* __restrict__ img, const double* __r
 int finalize_pressure(ls_ctx* c, double ptop) {
     Geo g = geo_of(c);
     dim3 blk(TK, TJ), grd((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, c->ni);
-    hipLaunchKernelGGL(k_press_final, grd, blk, 0, c->stream, c->img, c->craw[0], c->craw[1],
-                       c->craw[2], c->cin, c->cls, c->pD, c->sd, c->x64, c->p, g,
-                       c->P.bc_mode == LS_BC_ZFACES, ptop, c->P.evap_flux * c->P.dx);
+    hipLaunchKernelGGL(k_press_final, grd, blk, 0, c->stream, c->img, c->perm, c->cls, c->pD,
+                       c->sd, c->x64, c->p, g, c->P.bc_mode == LS_BC_ZFACES, ptop,
+                       c->P.evap_flux * c->P.dx);
     LS_LAUNCH_CHECK();
     return 0;
 }

@@ -136,16 +136,20 @@ template <typename T> struct StencilShape;
 template <> struct StencilShape<float> { static constexpr int W = 4, ICH = 4, TJV = 8; };
 template <> struct StencilShape<double> { static constexpr int W = 2, ICH = 8, TJV = 4; };
 
-template <typename T>
+// MODE 0: y = A^ x with the block partials of x.y.  MODE 1 (FP64, the
+// residual of k_stencil<double, 1>): y = p2 = b - A^ x with the partials of
+// r.r, |sd r|^2 and |sd b|^2 (three arrays of nb).
+template <typename T, int MODE = 0>
 __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
     const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ cx,
     const T* __restrict__ cy, const T* __restrict__ cz, SGeo g, double* __restrict__ partials,
-    const int* __restrict__ done) {
+    const int* __restrict__ done, const T* __restrict__ b = nullptr, T* __restrict__ p2 = nullptr,
+    const T* __restrict__ sd = nullptr) {
     constexpr int W = StencilShape<T>::W, ICH = StencilShape<T>::ICH;
     constexpr int TJV = StencilShape<T>::TJV;
     typedef typename VecOf<T, W>::type V;
     if (done != nullptr && *done) return;
-    __shared__ double lds[TJV];
+    __shared__ double lds[3 * TJV];
     const int nbx = gridDim.x, nby = gridDim.y;
     const int nb = nbx * nby * gridDim.z;
     const int hw = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
@@ -155,7 +159,7 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
     const int k0 = (bx * WAVE + lane) * W;
     const int j = by * TJV + threadIdx.y;
     const int i0 = bz * ICH, i1 = min(i0 + ICH, g.ni);
-    double acc = 0.0;
+    double acc = 0.0, acc1 = 0.0, acc2 = 0.0;
     // rows past nj idle as whole waves (threadIdx.y is uniform per wave)
     if (j < g.nj) {
         const bool ok = k0 < g.nk;
@@ -187,6 +191,11 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
             }
             if (lane == WAVE - 1) right = (ok && k0 + W < g.nk) ? x[v + W] : T(0);
             V ax;
+            V bv = zero, sv = zero;
+            if (MODE == 1 && ok) {
+                bv = *(const V*)(b + v);
+                sv = *(const V*)(sd + v);
+            }
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 const T xkm = (w == 0) ? left : xc[w - 1];
@@ -194,9 +203,21 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
                 const T czm = (w == 0) ? czl : czp[w - 1];
                 ax[w] = xc[w] - (cxp[w] * xp[w] + cxm[w] * xm[w] + cyp[w] * xjp[w] +
                                  cym[w] * xjm[w] + czp[w] * xkp + czm * xkm);
-                acc += (double)xc[w] * (double)ax[w];
+                if (MODE == 0) {
+                    acc += (double)xc[w] * (double)ax[w];
+                } else {
+                    const double r = (double)bv[w] - (double)ax[w];
+                    ax[w] = (T)r;
+                    const double sr = (double)sv[w] * r, sb = (double)sv[w] * (double)bv[w];
+                    acc += r * r;
+                    acc1 += sr * sr;
+                    acc2 += sb * sb;
+                }
             }
-            if (ok) __builtin_nontemporal_store(ax, (V*)(y + v));
+            if (ok) {
+                __builtin_nontemporal_store(ax, (V*)(y + v));
+                if (MODE == 1) __builtin_nontemporal_store(ax, (V*)(p2 + v));
+            }
             xm = xc;
             xc = xp;
             cxm = cxp;
@@ -211,12 +232,31 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
         }
     }
     acc = wave_sum(acc);
-    if (lane == 0) lds[threadIdx.y] = acc;
+    if (MODE == 1) {
+        acc1 = wave_sum(acc1);
+        acc2 = wave_sum(acc2);
+    }
+    if (lane == 0) {
+        lds[threadIdx.y] = acc;
+        if (MODE == 1) {
+            lds[TJV + threadIdx.y] = acc1;
+            lds[2 * TJV + threadIdx.y] = acc2;
+        }
+    }
     __syncthreads();
     if (flat_tid() == 0) {
         double s = 0.0;
         for (int q = 0; q < TJV; ++q) s += lds[q];
         partials[t] = s;
+        if (MODE == 1) {
+            double s1 = 0.0, s2 = 0.0;
+            for (int q = 0; q < TJV; ++q) {
+                s1 += lds[TJV + q];
+                s2 += lds[2 * TJV + q];
+            }
+            partials[nb + t] = s1;
+            partials[2 * nb + t] = s2;
+        }
     }
 }
 
@@ -983,10 +1023,21 @@ static int nblocks_stencil(const ls_ctx* c) {
 // r = b^ - A^ x^ (FP64), p = r; fills cs (rr, wrr, bnorm2, tol2, done).
 static int cg64_residual(ls_ctx* c, double rtol, int maxit) {
     SGeo g = sgeo(c);
-    dim3 grd = stencil_grid(c), blk(TK, TJ);
-    int nb = nblocks_stencil(c);
-    hipLaunchKernelGGL((k_stencil<double, 1>), grd, blk, 0, c->stream, c->x64, c->r64, c->c64[0],
-                       c->c64[1], c->c64[2], g, c->partials, nullptr, c->bhat, c->p64, c->sd);
+    int nb;
+    if (use_vec<double>(c)) {
+        const dim3 grd = vec_grid<double>(c);
+        nb = (int)(grd.x * grd.y * grd.z);
+        hipLaunchKernelGGL((k_stencil_vec<double, 1>), grd,
+                           dim3(WAVE, StencilShape<double>::TJV), 0, c->stream, c->x64, c->r64,
+                           c->c64[0], c->c64[1], c->c64[2], g, c->partials, nullptr, c->bhat,
+                           c->p64, c->sd);
+    } else {
+        dim3 grd = stencil_grid(c), blk(TK, TJ);
+        nb = nblocks_stencil(c);
+        hipLaunchKernelGGL((k_stencil<double, 1>), grd, blk, 0, c->stream, c->x64, c->r64,
+                           c->c64[0], c->c64[1], c->c64[2], g, c->partials, nullptr, c->bhat,
+                           c->p64, c->sd);
+    }
     LS_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(1024), 0, c->stream, c->partials, nb, c->cg, rtol,
                        maxit);

@@ -643,11 +643,17 @@ __global__ __launch_bounds__(WAVE * ADR_TJ) void k_adr_matvec_rows(
 // colour, A = [D_b A_br; A_rb D_r] with diagonal D_b, D_r, and
 //   S x_b = b_b - A_br D_r^-1 b_r,   S = D_b - A_br D_r^-1 A_rb,
 //   x_r   = D_r^-1 (b_r - A_rb x_b).
-// BiCGStab with Jacobi on diag(S) runs on the black unknowns: the
-// spectral radius of the Jacobi iteration matrix squares, so about half
-// the iterations, on half-length vectors.  After the red recovery the full
-// residual is (b_b - A_bb x_b - A_br x_r, 0) = (r_S, 0), so the solver stops
-// on ||r_S|| <= rtol ||b|| with the full system's b.
+// BiCGStab with Jacobi on D_b runs on the black unknowns: the spectral
+// radius of the Jacobi iteration matrix squares, so about half the
+// iterations, on half-length vectors.  (diag(S) = D_b - sum a_bf a_fb / d_f
+// lies within 0.83..1 of D_b on the transport rows, and BiCGStab took the
+// same iteration counts with either on the oracle's systems of the
+// fractured workload at 48^3.)  With the red values implied by the black
+// ones the full residual is (b_b - A_bb x_b - A_br x_r, 0) = (r_S, 0), so the
+// solver stops on ||r_S|| <= rtol ||b|| with the full system's b.  b_S is
+// never formed: a start pass computes u = D_r^-1 (b_r - A_rb x) at the red
+// voxels and r_S = b_b - D_b x - A_br u at the black ones; the final one
+// also writes u and x into the fields.
 //
 // Layout (nk even): voxel v of colour c is stored at v >> 1 of its colour's
 // half-length arrays; the pair (2m, 2m + 1) holds one voxel of each colour.
@@ -679,15 +685,28 @@ __device__ __forceinline__ double srow(const double* __restrict__ rows, uint32_t
     return rows[((size_t)f * 2 + colour) * Nh + m];
 }
 
-// The stored rows, colour-split, with b and D^-1 (full layout) as k_adr_rows.
+// The stored rows, colour-split, with b (full layout) as k_adr_rows; at the
+// black voxels also the solver's Jacobi inverse diagonal 1 / D_b and its
+// start.  one: system 1 is known to be 0 with b = 0 (its b and x are not
+// touched).  With xprev (the black part of system 0's field at the previous
+// transport solve) system 0 starts from the linear extrapolation
+// x + ratio (x - xprev), ratio = dt / dt_prev (extrap), and xprev takes the
+// current field for the next solve.
+// Transport (KIND 0) also zeroes the flux at Rock and Precipitate voxels
+// here, as the reference's assembly does (src/Simulation.cpp:1134-1136,
+// 1147-1149): no row reads the flux of such a voxel (a Rock neighbour is an
+// exchange link, a Precipitate neighbour a flux barrier), so the writes race
+// with no read whose value is used.
 template <int KIND>
 __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
-    const uint8_t* __restrict__ img, const double* __restrict__ qx,
-    const double* __restrict__ qy, const double* __restrict__ qz,
+    const uint8_t* __restrict__ img, double* qx, double* qy, double* qz,
     const double* __restrict__ sf, TGeo g, uint32_t N, AdrConst a,
     double* __restrict__ rows, const double* __restrict__ x0, const double* __restrict__ x1,
-    double* __restrict__ b0, double* __restrict__ b1, double* __restrict__ invd) {
+    double* __restrict__ b0, double* __restrict__ b1, int one, double* __restrict__ invdb,
+    double* __restrict__ xs0, double* __restrict__ xs1, double* __restrict__ xprev, int extrap,
+    double ratio) {
     const uint32_t Nh = N / 2;
+    const double* x1r = one ? x0 : x1;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
         int i, j, k;
         tdecode(v, g, i, j, k);
@@ -697,8 +716,13 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
             sat_row(img, qx, qy, qz, sf, g, i, j, k, v, a, R, srhs);
         } else {
             adr_row(img, qx, qy, qz, g, i, j, k, v, a, R);
+            if (R.type == PREC || R.type == ROCK) {
+                qx[v] = 0.0;
+                qy[v] = 0.0;
+                qz[v] = 0.0;
+            }
         }
-        const double xa = x0[v], xb = x1[v];
+        const double xa = x0[v], xb = x1r[v];
         double ba, bbv;
         if (KIND == 1) {
             ba = srhs;
@@ -709,56 +733,40 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
         } else {
             ba = xa * a.inv_dt;
             bbv = xb * a.inv_dt;
-            explicit_rhs(R, v, x0, x1, xa, xb, a, ba, bbv);
+            explicit_rhs(R, v, x0, x1r, xa, xb, a, ba, bbv);
         }
         b0[v] = ba;
-        b1[v] = bbv;
-        invd[v] = 1.0 / R.diag;
+        if (!one) b1[v] = bbv;
         const int colour = (i + j + k) & 1;
         const uint32_t m = v >> 1;
         rows[((size_t)0 * 2 + colour) * Nh + m] = R.diag;
 #pragma unroll
         for (int f = 0; f < 6; ++f)
             rows[((size_t)(f + 1) * 2 + colour) * Nh + m] = (R.mask & (1 << f)) ? R.off[f] : 0.0;
-    }
-}
-
-// diag(S)^-1, the Schur right-hand sides and the black part of x.
-__global__ __launch_bounds__(EW_BLOCK) void k_tschur_prep(
-    const double* __restrict__ rows, TGeo g, uint32_t Nh, const double* __restrict__ b0,
-    const double* __restrict__ b1, const double* __restrict__ x0, const double* __restrict__ x1,
-    double* __restrict__ invdS, double* __restrict__ bS0, double* __restrict__ bS1,
-    double* __restrict__ xb0, double* __restrict__ xb1) {
-    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < Nh; m += gridDim.x * blockDim.x) {
-        const SPair P = spair(m, g);
-        double dS = srow(rows, Nh, 0, 0, m);
-        double r0 = b0[P.vb], r1 = b1[P.vb];
-        for (int f = 0; f < 6; ++f) {
-            uint32_t n;
-            if (!tnbr(g, P.i, P.j, P.kb, P.vb, f, n)) continue;
-            const double o = srow(rows, Nh, f + 1, 0, m);
-            const uint32_t mn = n >> 1;
-            const double dn = srow(rows, Nh, 0, 1, mn);
-            const double back = srow(rows, Nh, (f ^ 1) + 1, 1, mn);   // the neighbour's face toward v
-            dS -= o * back / dn;
-            r0 -= o * (b0[n] / dn);
-            r1 -= o * (b1[n] / dn);
+        if (colour == 0) {
+            invdb[m] = 1.0 / R.diag;
+            if (xprev != nullptr) {
+                xs0[m] = extrap ? xa + ratio * (xa - xprev[m]) : xa;
+                xprev[m] = xa;
+            } else {
+                xs0[m] = xa;
+            }
+            if (!one) xs1[m] = xb;
         }
-        invdS[m] = 1.0 / dS;
-        bS0[m] = r0;
-        bS1[m] = r1;
-        xb0[m] = x0[P.vb];
-        xb1[m] = x1[P.vb];
     }
 }
 
-// w = D_r^-1 A_rb p at the red voxels (compacted), for the systems in SYS.
-template <int SYS>
+// The red pass for the systems in SYS.  MODE 0 (apply): w = D_r^-1 A_rb p.
+// MODE 1 (start): w = D_r^-1 (b_r - A_rb x), the red unknowns implied by the
+// black ones x; MODE 2 (final): the same, also written into the full field
+// xf at the red voxels.
+template <int SYS, int MODE>
 __global__ __launch_bounds__(EW_BLOCK) void k_tschur_red(
     const double* __restrict__ rows, TGeo g, uint32_t Nh, const double* __restrict__ p0,
     const double* __restrict__ p1, double* __restrict__ w0, double* __restrict__ w1,
-    const BicgScalars* __restrict__ bs, int start) {
-    if (!start && bs->all_done) return;
+    const BicgScalars* __restrict__ bs, const double* __restrict__ bf0,
+    const double* __restrict__ bf1, double* __restrict__ xf0, double* __restrict__ xf1) {
+    if (MODE == 0 && bs->all_done) return;
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < Nh; m += gridDim.x * blockDim.x) {
         const SPair P = spair(m, g);
         double a0 = 0.0, a1 = 0.0;
@@ -770,15 +778,31 @@ __global__ __launch_bounds__(EW_BLOCK) void k_tschur_red(
             if (SYS & 2) a1 += o * p1[n >> 1];
         }
         const double d = srow(rows, Nh, 0, 1, m);
-        if (SYS & 1) w0[m] = a0 / d;
-        if (SYS & 2) w1[m] = a1 / d;
+        if (MODE == 0) {
+            if (SYS & 1) w0[m] = a0 / d;
+            if (SYS & 2) w1[m] = a1 / d;
+        } else {
+            if (SYS & 1) {
+                const double u = (bf0[P.vr] - a0) / d;
+                w0[m] = u;
+                if (MODE == 2) xf0[P.vr] = u;
+            }
+            if (SYS & 2) {
+                const double u = (bf1[P.vr] - a1) / d;
+                w1[m] = u;
+                if (MODE == 2) xf1[P.vr] = u;
+            }
+        }
     }
 }
 
-// y = S x = D_b x - A_br w at the black voxels; partials as
-// k_adr_matvec_rows: rh.y (MODE 0), (y.y, y.s) (MODE 1); MODE 2 is the
-// start r = bS - S x, rh = r, p = v = 0 with partials (b.b over the whole
-// grid, r.r).
+// The black pass.  MODE 0/1 (apply): y = S x = D_b x - A_br w at the black
+// voxels, with partials rh.y (MODE 0) or (y.y, y.s) (MODE 1), w from the red
+// pass in mode 0.  MODE 2 (start): the Schur residual r = b_b - D_b x - A_br u
+// with u from the red pass in mode 1 (r_S = b_S - S x, b_S = b_b -
+// A_br D_r^-1 b_r), rh = r, p = v = 0, partials (b.b over the whole grid, r.r);
+// MODE 3 (final): the same, also writing x into the full field xf at the
+// black voxels.
 template <int MODE, int SYS>
 __global__ __launch_bounds__(EW_BLOCK) void k_tschur_black(
     const double* __restrict__ rows, TGeo g, uint32_t Nh, const double* __restrict__ x0,
@@ -787,42 +811,56 @@ __global__ __launch_bounds__(EW_BLOCK) void k_tschur_black(
     const double* __restrict__ w1, const BicgScalars* __restrict__ bs,
     double* __restrict__ partials, double* __restrict__ rh0, double* __restrict__ rh1,
     double* __restrict__ p0, double* __restrict__ p1, double* __restrict__ v0,
-    double* __restrict__ v1, const double* __restrict__ bf0, const double* __restrict__ bf1) {
+    double* __restrict__ v1, const double* __restrict__ bf0, const double* __restrict__ bf1,
+    double* __restrict__ xf0, double* __restrict__ xf1) {
     constexpr bool S0 = (SYS & 1) != 0, S1 = (SYS & 2) != 0;
-    if (MODE != 2 && bs->all_done) return;
+    constexpr bool START = MODE >= 2;
+    if (!START && bs->all_done) return;
     __shared__ double lds[4 * EW_BLOCK / WAVE];
-    const bool a0 = MODE == 2 || !bs->done[0], a1 = MODE == 2 || !bs->done[1];
+    const bool a0 = START || !bs->done[0], a1 = START || !bs->done[1];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < Nh; m += gridDim.x * blockDim.x) {
         const SPair P = spair(m, g);
         const double d = srow(rows, Nh, 0, 0, m);
-        double ya = S0 ? d * x0[m] : 0.0, yb = S1 ? d * x1[m] : 0.0;
+        const double xa = S0 ? x0[m] : 0.0, xb = S1 ? x1[m] : 0.0;
+        double ya = S0 ? d * xa : 0.0, yb = S1 ? d * xb : 0.0;
         for (int f = 0; f < 6; ++f) {
             uint32_t n;
             if (!tnbr(g, P.i, P.j, P.kb, P.vb, f, n)) continue;
             const double o = srow(rows, Nh, f + 1, 0, m);
-            if (S0) ya -= o * wr0[n >> 1];
-            if (S1) yb -= o * wr1[n >> 1];
+            if (START) {
+                if (S0) ya += o * wr0[n >> 1];
+                if (S1) yb += o * wr1[n >> 1];
+            } else {
+                if (S0) ya -= o * wr0[n >> 1];
+                if (S1) yb -= o * wr1[n >> 1];
+            }
         }
-        if (MODE == 2) {
-            const double ra = S0 ? w0[m] - ya : 0.0, rb = S1 ? w1[m] - yb : 0.0;
+        if (START) {
+            const double ra = S0 ? bf0[P.vb] - ya : 0.0, rb = S1 ? bf1[P.vb] - yb : 0.0;
             if (S0) {
                 y0[m] = ra;
                 rh0[m] = ra;
                 p0[m] = 0.0;
                 v0[m] = 0.0;
+                if (MODE == 3) xf0[P.vb] = xa;
             }
             if (S1) {
                 y1[m] = rb;
                 rh1[m] = rb;
                 p1[m] = 0.0;
                 v1[m] = 0.0;
+                if (MODE == 3) xf1[P.vb] = xb;
             }
             // ||b|| of the full system: the pair (2m, 2m + 1)
-            const double ba = bf0[2 * m], bb = bf0[2 * m + 1];
-            const double bc = bf1[2 * m], bd = bf1[2 * m + 1];
-            s0 += ba * ba + bb * bb;
-            s1 += bc * bc + bd * bd;
+            if (S0) {
+                const double ba = bf0[2 * m], bb = bf0[2 * m + 1];
+                s0 += ba * ba + bb * bb;
+            }
+            if (S1) {
+                const double bc = bf1[2 * m], bd = bf1[2 * m + 1];
+                s1 += bc * bc + bd * bd;
+            }
             s2 += ra * ra;
             s3 += rb * rb;
         } else {
@@ -851,29 +889,6 @@ __global__ __launch_bounds__(EW_BLOCK) void k_tschur_black(
         partials[nb + blockIdx.x] = q[1];
         partials[2 * nb + blockIdx.x] = q[2];
         partials[3 * nb + blockIdx.x] = q[3];
-    }
-}
-
-// x_b from the solver, x_r = D_r^-1 (b_r - A_rb x_b), into the full fields.
-__global__ __launch_bounds__(EW_BLOCK) void k_tschur_recover(
-    const double* __restrict__ rows, TGeo g, uint32_t Nh, const double* __restrict__ xb0,
-    const double* __restrict__ xb1, const double* __restrict__ b0, const double* __restrict__ b1,
-    double* __restrict__ x0, double* __restrict__ x1) {
-    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < Nh; m += gridDim.x * blockDim.x) {
-        const SPair P = spair(m, g);
-        double a0 = b0[P.vr], a1 = b1[P.vr];
-        for (int f = 0; f < 6; ++f) {
-            uint32_t n;
-            if (!tnbr(g, P.i, P.j, P.kr, P.vr, f, n)) continue;
-            const double o = srow(rows, Nh, f + 1, 1, m);
-            a0 -= o * xb0[n >> 1];
-            a1 -= o * xb1[n >> 1];
-        }
-        const double d = srow(rows, Nh, 0, 1, m);
-        x0[P.vb] = xb0[m];
-        x1[P.vb] = xb1[m];
-        x0[P.vr] = a0 / d;
-        x1[P.vr] = a1 / d;
     }
 }
 
@@ -1096,22 +1111,47 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     // extra half-length arrays use the second halves of work vectors whose
     // first halves hold the half-length BiCGStab vectors
     const bool tschur = stored && c->P.adr_schur != 0;
+    // system 1 known to be 0 with b = 0 (the idle system of the implicit
+    // saturation; acid that has never had a source): the Schur form then
+    // builds, starts and recovers system 0 alone
+    const bool one = tschur && (KIND == 1 || c->acid_zero);
+    // transport zeroes the flux at Rock and Precipitate voxels (the Schur
+    // form's row builder does it itself)
+    if (KIND == 0 && !tschur) {
+        if (int st = zero_flux_rock_precip(c)) return st;
+    }
     const uint32_t Nh = N / 2;
     const int nbh = ew_blocks(Nh);
-    double *xs[2] = {nullptr, nullptr}, *bS[2] = {nullptr, nullptr}, *wr[2] = {nullptr, nullptr};
-    double* invdS = nullptr;
+    double *xs[2] = {nullptr, nullptr}, *wr[2] = {nullptr, nullptr};
+    double* invdb = nullptr;
     if (tschur) {
         for (int f = 0; f < 2; ++f) {
             xs[f] = c->tv[f][0] + Nh;
-            bS[f] = c->tv[f][1] + Nh;
             wr[f] = c->tv[f][2] + Nh;
         }
-        invdS = c->tv[0][3] + Nh;
+        invdb = c->tv[0][3] + Nh;
+        // transport: conc starts from its linear extrapolation in time
+        double* xprev = nullptr;
+        if (KIND == 0 && c->P.adr_extrapolate) {
+            if (c->tprev == nullptr) {
+                if (hipMalloc((void**)&c->tprev, Nh * sizeof(double)) == hipSuccess) {
+                    c->allocations.push_back(c->tprev);
+                } else {        // out of memory: start from the old field instead
+                    c->tprev = nullptr;
+                    (void)hipGetLastError();
+                }
+            }
+            xprev = c->tprev;
+        }
+        const bool extrap = xprev != nullptr && c->tprev_valid && c->tprev_dt > 0.0;
         hipLaunchKernelGGL((k_adr_rows_split<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
                            c->img, c->qx, c->qy, c->qz, sf, g, N, a, c->arows, x[0], x[1],
-                           c->tb[0], c->tb[1], c->tinvd);
-        hipLaunchKernelGGL(k_tschur_prep, dim3(nbh), dim3(EW_BLOCK), 0, c->stream, c->arows, g, Nh,
-                           c->tb[0], c->tb[1], x[0], x[1], invdS, bS[0], bS[1], xs[0], xs[1]);
+                           c->tb[0], c->tb[1], one ? 1 : 0, invdb, xs[0], xs[1], xprev,
+                           extrap ? 1 : 0, extrap ? (1.0 / a.inv_dt) / c->tprev_dt : 0.0);
+        if (KIND == 0) {
+            c->tprev_valid = xprev != nullptr;
+            c->tprev_dt = 1.0 / a.inv_dt;
+        }
         LS_LAUNCH_CHECK();
     } else if (stored) {       // rows, right-hand sides and D^-1 once per solve
         hipLaunchKernelGGL((k_adr_rows<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->img,
@@ -1123,7 +1163,7 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     double* xv[2] = {tschur ? xs[0] : x[0], tschur ? xs[1] : x[1]};
     const uint32_t Nv = tschur ? Nh : N;
     const int nbv = tschur ? nbh : nb;
-    const double* invd_v = tschur ? invdS : c->tinvd;
+    const double* invd_v = tschur ? invdb : c->tinvd;
     // systems still iterating, as the host last saw them (a system that
     // converges inside a batch is masked on the device)
     int sys = 3;
@@ -1134,15 +1174,32 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
                            c->arows, xa, xb, ya, yb, wa, wb, g, N, c->bicg, c->partials, rh[0],
                            rh[1], pp[0], pp[1], vv[0], vv[1]);
     };
+    // profiling: the first red pass of every batch is timed with events
+    bool sample_red = false, red_pending = false;
     auto launch_schur = [&](auto MC, auto SC, const double* xa, const double* xb, double* ya,
                             double* yb, const double* wa, const double* wb) {
         constexpr int M = decltype(MC)::value, S = decltype(SC)::value;
-        hipLaunchKernelGGL((k_tschur_red<S>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream, c->arows,
-                           g, Nh, xa, xb, wr[0], wr[1], c->bicg, M == 2 ? 1 : 0);
+        const bool sample = sample_red;
+        if (sample) {
+            (void)hipEventRecord(c->ev[16], c->stream);
+            sample_red = false;
+        }
+        // black mode M: 0, 1 apply; 2 start; 3 final start (fields written)
+        constexpr int RM = M == 2 ? 1 : (M == 3 ? 2 : 0);
+        hipLaunchKernelGGL((k_tschur_red<S, RM>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream,
+                           c->arows, g, Nh, xa, xb, wr[0], wr[1], c->bicg, c->tb[0], c->tb[1],
+                           x[0], x[1]);
+        if (sample) {
+            (void)hipEventRecord(c->ev[17], c->stream);
+            red_pending = true;
+            // red rows (7 coefficients of the red voxels), p at the black
+            // voxels and w at the red ones, per system iterated
+            c->prof_tred_bytes = (double)c->N * (28.0 + (S == 3 ? 16.0 : 8.0));
+        }
         hipLaunchKernelGGL((k_tschur_black<M, S>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream,
                            c->arows, g, Nh, xa, xb, wr[0], wr[1], ya, yb, wa, wb, c->bicg,
                            c->partials, rh[0], rh[1], pp[0], pp[1], vv[0], vv[1], c->tb[0],
-                           c->tb[1]);
+                           c->tb[1], x[0], x[1]);
     };
     auto matvec = [&](auto MC, const double* xa, const double* xb, double* ya, double* yb,
                       const double* wa, const double* wb) {
@@ -1170,12 +1227,28 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         }
         return stored ? nbm : nb;
     };
-    // r = b - A x, rh = r, p = v = 0, partials (b.b, r.r)
+    // r = b - A x, rh = r, p = v = 0, partials (b.b, r.r); in the Schur form
+    // every start after the first also writes the iterate into the fields
+    // (the red values from the black ones), so the last one leaves the
+    // solution there
     auto start = [&](bool build) {
         if (tschur) {
-            sys = 3;
-            launch_schur(std::integral_constant<int, 2>{}, std::integral_constant<int, 3>{},
-                         xs[0], xs[1], r[0], r[1], bS[0], bS[1]);
+            sys = one ? 1 : 3;
+            if (build) {
+                if (one)
+                    launch_schur(std::integral_constant<int, 2>{}, std::integral_constant<int, 1>{},
+                                 xs[0], xs[1], r[0], r[1], nullptr, nullptr);
+                else
+                    launch_schur(std::integral_constant<int, 2>{}, std::integral_constant<int, 3>{},
+                                 xs[0], xs[1], r[0], r[1], nullptr, nullptr);
+            } else {
+                if (one)
+                    launch_schur(std::integral_constant<int, 3>{}, std::integral_constant<int, 1>{},
+                                 xs[0], xs[1], r[0], r[1], nullptr, nullptr);
+                else
+                    launch_schur(std::integral_constant<int, 3>{}, std::integral_constant<int, 3>{},
+                                 xs[0], xs[1], r[0], r[1], nullptr, nullptr);
+            }
             return nbh;
         }
         if (stored) {
@@ -1203,7 +1276,7 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             // a system with b = 0 has x = 0 exactly: write it and mark the
             // system solved (the start state of the other one stands)
             for (int f = 0; f < 2; ++f)
-                if (c->h_bicg->bnorm2[f] == 0.0) {
+                if (c->h_bicg->bnorm2[f] == 0.0 && !(one && f == 1)) {
                     LS_HIP(hipMemsetAsync(x[f], 0, c->N * sizeof(double), c->stream));
                     if (tschur) LS_HIP(hipMemsetAsync(xs[f], 0, Nh * sizeof(double), c->stream));
                 }
@@ -1217,6 +1290,7 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         while (!c->h_bicg->all_done) {
             sys = (c->h_bicg->done[0] ? 0 : 1) | (c->h_bicg->done[1] ? 0 : 2);
             if (sys == 0) sys = 3;
+            sample_red = c->profiling && tschur;
             for (int it = 0; it < batch; ++it) {
                 hipLaunchKernelGGL(k_bicg_p, dim3(nbv), dim3(EW_BLOCK), 0, c->stream, pp[0], pp[1],
                                    r[0], r[1], vv[0], vv[1], ph[0], ph[1], invd_v, c->bicg, Nv);
@@ -1243,6 +1317,13 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             LS_HIP(hipMemcpyAsync(c->h_bicg, c->bicg, sizeof(BicgScalars), hipMemcpyDeviceToHost,
                                   c->stream));
             LS_HIP(hipStreamSynchronize(c->stream));
+            if (red_pending) {
+                float ms = 0.0f;
+                LS_HIP(hipEventElapsedTime(&ms, c->ev[16], c->ev[17]));
+                c->prof_ms[3] += ms;
+                c->prof_n[3] += 1;
+                red_pending = false;
+            }
         }
         // confirm on the recomputed residual (the start kernel with the
         // stored right-hand sides); restart systems that fall short
@@ -1260,11 +1341,6 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         ++restarts;
         hipLaunchKernelGGL(k_bicg_first_rho, dim3(1), dim3(1), 0, c->stream, c->bicg);
         // loop re-enters with the start kernel (rebuild of b skipped)
-    }
-    if (tschur) {   // x_b, and x_r from the red rows, into the fields
-        hipLaunchKernelGGL(k_tschur_recover, dim3(nbh), dim3(EW_BLOCK), 0, c->stream, c->arows, g,
-                           Nh, xs[0], xs[1], c->tb[0], c->tb[1], x[0], x[1]);
-        LS_LAUNCH_CHECK();
     }
     const BicgScalars& h = *c->h_bicg;
     if (iters_out) *iters_out = h.iter[0] > h.iter[1] ? h.iter[0] : h.iter[1];
@@ -1299,8 +1375,7 @@ static AdrConst adr_const(const ls_ctx* c, double dt) {
 }
 
 int phase_transport(ls_ctx* c, double dt, ls_step_report* rep) {
-    int st = zero_flux_rock_precip(c);
-    if (st) return st;
+    int st = 0;
     int iters = 0, sys_iters[2] = {0, 0};
     double relres = 0.0;
     st = bicg_pair<0>(c, c->conc, c->acid, nullptr, adr_const(c, dt), &iters, &relres, sys_iters);

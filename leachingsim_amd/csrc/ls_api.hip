@@ -53,8 +53,8 @@ struct Timer {
 };
 
 void reset_prof(ls_ctx* c) {
-    c->prof_ms[0] = c->prof_ms[1] = c->prof_ms[2] = 0.0;
-    c->prof_n[0] = c->prof_n[1] = c->prof_n[2] = 0;
+    c->prof_ms[0] = c->prof_ms[1] = c->prof_ms[2] = c->prof_ms[3] = 0.0;
+    c->prof_n[0] = c->prof_n[1] = c->prof_n[2] = c->prof_n[3] = 0;
     c->prof_pending = 0;
 }
 
@@ -66,6 +66,9 @@ void report_prof(ls_ctx* c, ls_step_report* r) {
     r->prof_schur_ms = c->prof_ms[2];
     r->prof_schur_n = c->prof_n[2];
     r->prof_schur_bytes = c->prof_schur_bytes;
+    r->prof_tred_ms = c->prof_ms[3];
+    r->prof_tred_n = c->prof_n[3];
+    r->prof_tred_bytes = c->prof_tred_bytes;
 }
 
 }  // namespace
@@ -99,11 +102,12 @@ void ls_default_params(ls_params* p) {
     p->press_maxit = 10000;
     p->adr_rtol = 1e-8;
     p->adr_maxit = 10000;
-    p->warm_history = 4;
+    p->warm_history = 1;
     p->learned_race = 1.0;
     p->implicit_sat = 0;
     p->mixed_schur = 1;
     p->adr_schur = 1;
+    p->adr_extrapolate = 1;
     p->mixed_inner_rtol = 0.1;
 }
 
@@ -133,8 +137,9 @@ static int validate_params(ls_ctx* c, const ls_params* p) {
         c->err = "learned_race must be >= 0";
         return -3;
     }
-    if (p->mixed_schur < 0 || p->mixed_schur > 3 || p->adr_schur < 0 || p->adr_schur > 1) {
-        c->err = "invalid mixed_schur (0..3) or adr_schur (0..1)";
+    if (p->mixed_schur < 0 || p->mixed_schur > 3 || p->adr_schur < 0 || p->adr_schur > 1 ||
+        p->adr_extrapolate < 0 || p->adr_extrapolate > 1) {
+        c->err = "invalid mixed_schur (0..3), adr_schur (0..1) or adr_extrapolate (0..1)";
         return -3;
     }
     return 0;
@@ -162,7 +167,7 @@ int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes, const ls_pa
         *out = c;
         return -4;
     }
-    for (int i = 0; i < 16; ++i) (void)hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < 18; ++i) (void)hipEventCreate(&c->ev[i]);
     const size_t N = c->N;
     int st = 0;
     st |= dalloc(c, &c->img, N);
@@ -178,12 +183,9 @@ int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes, const ls_pa
     st |= dalloc(c, &c->perm, N);
     st |= dalloc(c, &c->pc, N);
     for (int a = 0; a < 3; ++a) {
-        st |= dalloc(c, &c->craw[a], N);
         st |= dalloc(c, &c->c64[a], N);
         st |= dalloc(c, &c->c32[a], N);
     }
-    st |= dalloc(c, &c->cin, (size_t)nj * nk);
-    st |= dalloc(c, &c->dsum, N);
     st |= dalloc(c, &c->cls, N);
     st |= dalloc(c, &c->pD, N);
     st |= dalloc(c, &c->sd, N);
@@ -242,7 +244,7 @@ void ls_destroy(ls_ctx* c) {
     if (c->h_fcg) (void)hipHostFree(c->h_fcg);
     if (c->h_wsum) (void)hipHostFree(c->h_wsum);
     if (c->stream) {
-        for (int i = 0; i < 16; ++i) (void)hipEventDestroy(c->ev[i]);
+        for (int i = 0; i < 18; ++i) (void)hipEventDestroy(c->ev[i]);
         (void)hipStreamDestroy(c->stream);
     }
     delete c;
@@ -260,6 +262,7 @@ int ls_set_params(ls_ctx* c, const ls_params* p) {
     if (!c || !p) return -1;
     if (int st = validate_params(c, p)) return st;
     c->P = *p;
+    c->tprev_valid = false;   // the transport's path may change with the settings
     return 0;
 }
 
@@ -310,6 +313,19 @@ int ls_upload(ls_ctx* c, int field, const void* host, size_t elem_bytes) {
     LS_HIP(hipMemcpyAsync(d, host, N * eb, hipMemcpyHostToDevice, c->stream));
     LS_HIP(hipStreamSynchronize(c->stream));
     if (field == LS_F_PRESSURE) c->n_hist = 0;   // history no longer matches p
+    if (field == LS_F_CONC) c->tprev_valid = false;
+    if (field == LS_F_ACID) {                    // +0.0 bit patterns only
+        const uint64_t* a = (const uint64_t*)host;
+        bool z = true;
+        for (size_t v = 0; v < N && z; ++v) z = a[v] == 0;
+        c->acid_zero = z;
+    }
+    if (field == LS_F_FRAC) {
+        const float* f = (const float*)host;
+        bool clean = true;
+        for (size_t v = 0; v < N && clean; ++v) clean = !(f[v] > 0.0f && (double)f[v] <= 1e-4);
+        c->frac_clean = clean;
+    }
     return 0;
 }
 
@@ -369,6 +385,8 @@ int ls_snapshot(ls_ctx* c) {
     st |= copy_d2d(c, c->b_qx, c->qx, N * 8);
     st |= copy_d2d(c, c->b_qy, c->qy, N * 8);
     st |= copy_d2d(c, c->b_qz, c->qz, N * 8);
+    c->b_acid_zero = c->acid_zero;
+    c->b_frac_clean = c->frac_clean;
     return st;
 }
 
@@ -387,6 +405,9 @@ int ls_rollback(ls_ctx* c) {
     st |= copy_d2d(c, c->qy, c->b_qy, N * 8);
     st |= copy_d2d(c, c->qz, c->b_qz, N * 8);
     if (st) return st;
+    c->acid_zero = c->b_acid_zero;
+    c->frac_clean = c->b_frac_clean;
+    c->tprev_valid = false;   // the last transport solve belonged to the failed attempt
     c->n_hist = 0;   // the pressure history no longer belongs to this state
     // rebuild the derived properties (src/Simulation.cpp:2347-2352)
     st = phase_properties(c);

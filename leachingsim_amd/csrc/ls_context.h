@@ -91,9 +91,6 @@ struct ls_ctx {
     // --- properties ---
     double *perm = nullptr, *pc = nullptr;
     // --- pressure system ---
-    double* craw[3] = {nullptr, nullptr, nullptr};  // unscaled face conductances
-    double* cin = nullptr;                           // virtual inlet plane faces
-    double* dsum = nullptr;                          // sum of couplings (Pore rows)
     int8_t* cls = nullptr;                           // 0 Dirichlet, 1 solved, 2 floored
     double* pD = nullptr;                            // Dirichlet values
     double* sd = nullptr;                            // sqrt(d) on solved rows, else 0
@@ -108,6 +105,11 @@ struct ls_ctx {
     double* tb[2];      // right-hand sides
     double* tinvd = nullptr;
     double* arows = nullptr;   // stored transport rows, 7 x N (allocated on first use)
+    // black half of conc at the last transport solve (N/2, allocated on first
+    // use), its dt, and whether it belongs to the current state's history
+    double* tprev = nullptr;
+    double tprev_dt = 0.0;
+    bool tprev_valid = false;
     // --- reductions ---
     double* partials = nullptr;     // MAX_PARTIALS * 8 values
     CgScalars* cg = nullptr;
@@ -131,6 +133,17 @@ struct ls_ctx {
     BicgScalars* h_bicg = nullptr;
     StepScalars* h_ss = nullptr;
 
+    // Facts about the state that let phases skip work exactly:
+    // acid_zero: the acid field is 0 everywhere (set by the initial state,
+    //   kept by transport, whose acid system then has b = 0 and x = 0, and by
+    //   leaching, which only moves acid where it is non-zero; an upload sets it
+    //   from the data).  The acid source of the reference is commented out
+    //   (src/Simulation.cpp:997-1128), so this holds unless acid is injected.
+    // frac_clean: no voxel holds 0 < frac <= 1e-4 (the initial state has frac
+    //   0 or 1; any upload of frac or a general leaching sweep clears it).
+    // With both, the leaching sweep cannot change anything.
+    bool acid_zero = false, frac_clean = false;
+    bool b_acid_zero = false, b_frac_clean = false;   // at the last snapshot
     double p_top_prev = 0.0;
     bool have_prev = false;
     bool sat_valid = true;     // physics check of the last implicit saturation solve
@@ -144,14 +157,16 @@ struct ls_ctx {
     double* h_wsum = nullptr;      // pinned mirror
     bool press_assembled = false;
 
-    hipEvent_t ev[16];
+    hipEvent_t ev[18];   // step phase marks (ls_step uses 0-10), sampled stencil and copy
+                         // timing (12-15), the transport's sampled red pass (16-17)
     // sampled stencil timing (profiling on): events 12/13 bracket one launch
     int profiling = 0;
     // prof_pending: 64 / 32 (FP64 / FP32 stencil) or 16 (Schur apply) when an
     // event pair awaits reading; slots 0, 1, 2 accumulate them
     int prof_pending = 0;
-    double prof_ms[3] = {0.0, 0.0, 0.0};
-    int prof_n[3] = {0, 0, 0};
+    double prof_ms[4] = {0.0, 0.0, 0.0, 0.0};   // slot 3: the transport's red Schur pass
+    int prof_n[4] = {0, 0, 0, 0};
+    double prof_tred_bytes = 0.0;  // algorithmic bytes of the sampled red pass
     double prof_schur_bytes = 0.0; // algorithmic bytes of the sampled Schur kernel
     std::vector<void*> allocations;
 };

@@ -48,6 +48,7 @@ class Params(C.Structure):
         ("adr_rtol", C.c_double), ("adr_maxit", C.c_int), ("warm_history", C.c_int),
         ("mixed_inner_rtol", C.c_double), ("learned_race", C.c_double),
         ("implicit_sat", C.c_int), ("mixed_schur", C.c_int), ("adr_schur", C.c_int),
+        ("adr_extrapolate", C.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -78,7 +79,8 @@ class StepReport(C.Structure):
         ("sat_iters", C.c_int), ("sat_valid", C.c_int), ("sat_raw_min", C.c_double),
         ("sat_raw_max", C.c_double), ("prof_schur_ms", C.c_double), ("prof_schur_n", C.c_int),
         ("prof_schur_bytes", C.c_double), ("adr_iters_conc", C.c_int),
-        ("adr_iters_acid", C.c_int),
+        ("adr_iters_acid", C.c_int), ("prof_tred_ms", C.c_double), ("prof_tred_n", C.c_int),
+        ("prof_tred_bytes", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -92,9 +94,9 @@ def default_params(**overrides) -> Params:
                vg_n=2.0, s_res=0.05, porosity=0.1, solid_density=1.0, time_scalar=1.0,
                alpha_transfer=1e-5, instant_precip=0, bc_mode=BC_REFERENCE,
                solver=SOLVER_PLAIN64, warm_start=WARM_PREVIOUS, press_rtol=1e-8,
-               press_maxit=10000, adr_rtol=1e-8, adr_maxit=10000, warm_history=4,
+               press_maxit=10000, adr_rtol=1e-8, adr_maxit=10000, warm_history=1,
                mixed_inner_rtol=0.1, learned_race=1.0, implicit_sat=0,
-               mixed_schur=1, adr_schur=1)
+               mixed_schur=1, adr_schur=1, adr_extrapolate=1)
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)

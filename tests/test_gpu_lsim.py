@@ -50,7 +50,11 @@ def load_own_checkpoint(path):
                 pressure=f[2], conc=f[3], acid=f[4])
 
 
-def python_run(img, times):
+def python_run(img, times, restart_times=()):
+    """Steps at `times` on one context; then, like a restart from a
+    checkpoint, the state moves to a fresh context (no solver history: the
+    transport's extrapolated start begins again from the old field) for the
+    steps at `restart_times`."""
     P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_PLAIN64,
                          warm_start=L.WARM_PREVIOUS)
     g = L.GpuSimulation(SHAPE, P)
@@ -60,6 +64,16 @@ def python_run(img, times):
         assert g.step(t, DT).status == 0
     s = g.state()
     g.close()
+    if restart_times:
+        h = L.GpuSimulation(SHAPE, P)
+        h.read_labels(img)
+        h.init_state()
+        for f, a in s.items():
+            h.upload(f, a)
+        for t in restart_times:
+            assert h.step(t, DT).status == 0
+        s = h.state()
+        h.close()
     return s
 
 
@@ -103,7 +117,7 @@ def test_lsim_matches_binding_and_restarts(tmp_path):
                    "--restart_step", "2", "--max_steps", "1")
     assert "Restarting simulation from step 2" in log
     ck2 = load_own_checkpoint(tmp_path / "checkpoint_000002.lsck")
-    ref2 = python_run(img, [0.0, 60.0, 120.0, 120.0])
+    ref2 = python_run(img, [0.0, 60.0, 120.0], [120.0])
     for key, f in (("img", L.F_IMG), ("saturation", L.F_SATURATION),
                    ("pressure", L.F_PRESSURE), ("conc", L.F_CONC)):
         assert np.array_equal(ck2[key].reshape(SHAPE), ref2[f].reshape(SHAPE)), key

@@ -22,6 +22,7 @@ SHAPE = (24, 20, 28)
 
 
 def ctx(warm, img, evap=1e-12, **kw):
+    kw.setdefault("warm_history", 4)
     P = L.default_params(bc_mode=L.BC_ZFACES, warm_start=warm, press_rtol=1e-12,
                          evap_flux=evap, **kw)
     g = L.GpuSimulation(SHAPE, P)
