@@ -892,6 +892,318 @@ __global__ __launch_bounds__(EW_BLOCK) void k_tschur_black(
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same two passes, vectorised, for nk % 4 == 0: a lane owns two
+// consecutive compacted values (16 B) of one row (i, j), so every row array
+// and every i / j neighbour row is read as aligned double2 streams, with no
+// index division per value.  In a row of parity par = (i + j) & 1 a voxel
+// with compacted index m sits at k = 2m + q (q = par for black, 1 - par for
+// red); its k neighbours (the other colour) are at m - 1 + q and m + q,
+// which for the lane's pair (m0, m0 + 1) are the pair itself and one value
+// of the neighbouring lane (shuffles; the wave's edge lanes load it).  A
+// block is 64 lanes x TSV_TR waves; each wave walks TSV_RPW rows.  The sums
+// run over the faces in the order of the scalar passes (+i, -i, +j, -j, +k,
+// -k), absent neighbours contributing 0 x 0.
+// ---------------------------------------------------------------------------
+constexpr int TSV_TR = 4;     // waves (rows in flight) per block
+constexpr int TSV_RPW = 4;    // rows per wave
+
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+struct TVGeo {
+    uint32_t nhr;       // compacted values per row (nk / 2, even)
+    uint32_t nrows;     // ni * nj
+    uint32_t plane;     // compacted values per i-plane (nj * nhr)
+    int ni, nj;
+};
+
+// the other colour's values at the k neighbours of the lane's pair: (km, kp)
+// for each of the two values; q is the own k parity
+__device__ __forceinline__ void tsv_kn(const double* __restrict__ src, uint32_t idx, uint32_t m0,
+                                       uint32_t nhr, int q, bool ok, d2 c, d2& km, d2& kp) {
+    const int lane = threadIdx.x;
+    double prev = __shfl_up(c.y, 1, WAVE);
+    double next = __shfl_down(c.x, 1, WAVE);
+    if (lane == 0) prev = (ok && q == 0 && m0 >= 1) ? src[idx - 1] : 0.0;
+    if (lane == WAVE - 1) next = (ok && q == 1 && m0 + 2 < nhr) ? src[idx + 2] : 0.0;
+    if (q == 0) {
+        km = d2{m0 >= 1 ? prev : 0.0, c.x};
+        kp = d2{c.x, c.y};
+    } else {
+        km = d2{c.x, c.y};
+        kp = d2{c.y, m0 + 2 < nhr ? next : 0.0};
+    }
+}
+
+// Red pass.  MODE 0: w = D_r^-1 A_rb p; MODE 1: w = D_r^-1 (b_r - A_rb x);
+// MODE 2: as 1, also written into the full field at the red voxels.
+template <int SYS, int MODE>
+__global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
+    const double* __restrict__ rows, TVGeo g, uint32_t Nh, const double* __restrict__ p0,
+    const double* __restrict__ p1, double* __restrict__ w0, double* __restrict__ w1,
+    const BicgScalars* __restrict__ bs, const double* __restrict__ bf0,
+    const double* __restrict__ bf1, double* __restrict__ xf0, double* __restrict__ xf1) {
+    if (MODE == 0 && bs->all_done) return;
+    const uint32_t m0 = 2u * (blockIdx.x * WAVE + threadIdx.x);
+    const bool ok = m0 < g.nhr;
+    const d2 zero = d2{0.0, 0.0};
+    for (int q8 = 0; q8 < TSV_RPW; ++q8) {
+        const uint32_t row = (blockIdx.y * TSV_RPW + q8) * TSV_TR + threadIdx.y;
+        if (row >= g.nrows) break;                      // uniform per wave
+        const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
+        const int q = 1 - ((i + j) & 1);
+        const uint32_t idx = row * g.nhr + m0;
+        d2 o[6], d = d2{1.0, 1.0};
+        for (int f = 0; f < 6; ++f) o[f] = zero;
+        d2 a = zero, b = zero;
+        d2 c0 = zero, c1 = zero;
+        if (ok) {
+            d = *(const d2*)(rows + (size_t)1 * Nh + idx);
+#pragma unroll
+            for (int f = 0; f < 6; ++f) o[f] = *(const d2*)(rows + ((size_t)(f + 1) * 2 + 1) * Nh + idx);
+            if (SYS & 1) c0 = *(const d2*)(p0 + idx);
+            if (SYS & 2) c1 = *(const d2*)(p1 + idx);
+        }
+        const bool hip_ = i + 1 < g.ni, him = i > 0, hjp = j + 1 < g.nj, hjm = j > 0;
+        if (SYS & 1) {
+            d2 ip = (ok && hip_) ? *(const d2*)(p0 + idx + g.plane) : zero;
+            d2 im = (ok && him) ? *(const d2*)(p0 + idx - g.plane) : zero;
+            d2 jp = (ok && hjp) ? *(const d2*)(p0 + idx + g.nhr) : zero;
+            d2 jm = (ok && hjm) ? *(const d2*)(p0 + idx - g.nhr) : zero;
+            d2 km, kp;
+            tsv_kn(p0, idx, m0, g.nhr, q, ok, c0, km, kp);
+            a = o[0] * ip;
+            a = a + o[1] * im;
+            a = a + o[2] * jp;
+            a = a + o[3] * jm;
+            a = a + o[4] * kp;
+            a = a + o[5] * km;
+        }
+        if (SYS & 2) {
+            d2 ip = (ok && hip_) ? *(const d2*)(p1 + idx + g.plane) : zero;
+            d2 im = (ok && him) ? *(const d2*)(p1 + idx - g.plane) : zero;
+            d2 jp = (ok && hjp) ? *(const d2*)(p1 + idx + g.nhr) : zero;
+            d2 jm = (ok && hjm) ? *(const d2*)(p1 + idx - g.nhr) : zero;
+            d2 km, kp;
+            tsv_kn(p1, idx, m0, g.nhr, q, ok, c1, km, kp);
+            b = o[0] * ip;
+            b = b + o[1] * im;
+            b = b + o[2] * jp;
+            b = b + o[3] * jm;
+            b = b + o[4] * kp;
+            b = b + o[5] * km;
+        }
+        if (!ok) continue;
+        if (MODE == 0) {
+            if (SYS & 1) *(d2*)(w0 + idx) = a / d;
+            if (SYS & 2) *(d2*)(w1 + idx) = b / d;
+        } else {
+            // the red voxels of the pair in the full layout: 2m + 1 - par
+            const uint32_t vr = 2u * idx + (uint32_t)q;
+            if (SYS & 1) {
+                const d2 u = d2{bf0[vr] - a.x, bf0[vr + 2] - a.y} / d;
+                *(d2*)(w0 + idx) = u;
+                if (MODE == 2) {
+                    xf0[vr] = u.x;
+                    xf0[vr + 2] = u.y;
+                }
+            }
+            if (SYS & 2) {
+                const d2 u = d2{bf1[vr] - b.x, bf1[vr + 2] - b.y} / d;
+                *(d2*)(w1 + idx) = u;
+                if (MODE == 2) {
+                    xf1[vr] = u.x;
+                    xf1[vr + 2] = u.y;
+                }
+            }
+        }
+    }
+}
+
+// Black pass, the modes of k_tschur_black.
+template <int MODE, int SYS>
+__global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
+    const double* __restrict__ rows, TVGeo g, uint32_t Nh, const double* __restrict__ x0,
+    const double* __restrict__ x1, const double* __restrict__ wr0, const double* __restrict__ wr1,
+    double* __restrict__ y0, double* __restrict__ y1, const double* __restrict__ w0,
+    const double* __restrict__ w1, const BicgScalars* __restrict__ bs,
+    double* __restrict__ partials, double* __restrict__ rh0, double* __restrict__ rh1,
+    double* __restrict__ p0, double* __restrict__ p1, double* __restrict__ v0,
+    double* __restrict__ v1, const double* __restrict__ bf0, const double* __restrict__ bf1,
+    double* __restrict__ xf0, double* __restrict__ xf1) {
+    constexpr bool S0 = (SYS & 1) != 0, S1 = (SYS & 2) != 0;
+    constexpr bool START = MODE >= 2;
+    if (!START && bs->all_done) return;
+    __shared__ double lds[4 * TSV_TR];
+    const bool a0 = START || !bs->done[0], a1 = START || !bs->done[1];
+    const uint32_t m0 = 2u * (blockIdx.x * WAVE + threadIdx.x);
+    const bool ok = m0 < g.nhr;
+    const d2 zero = d2{0.0, 0.0};
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int q8 = 0; q8 < TSV_RPW; ++q8) {
+        const uint32_t row = (blockIdx.y * TSV_RPW + q8) * TSV_TR + threadIdx.y;
+        if (row >= g.nrows) break;                      // uniform per wave
+        const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
+        const int q = (i + j) & 1;
+        const uint32_t idx = row * g.nhr + m0;
+        d2 o[6], d = zero;
+        for (int f = 0; f < 6; ++f) o[f] = zero;
+        d2 xa = zero, xb = zero, c0 = zero, c1 = zero;
+        if (ok) {
+            d = *(const d2*)(rows + idx);
+#pragma unroll
+            for (int f = 0; f < 6; ++f) o[f] = *(const d2*)(rows + ((size_t)(f + 1) * 2) * Nh + idx);
+            if (S0) {
+                xa = *(const d2*)(x0 + idx);
+                c0 = *(const d2*)(wr0 + idx);
+            }
+            if (S1) {
+                xb = *(const d2*)(x1 + idx);
+                c1 = *(const d2*)(wr1 + idx);
+            }
+        }
+        const bool hip_ = i + 1 < g.ni, him = i > 0, hjp = j + 1 < g.nj, hjm = j > 0;
+        d2 ya = zero, yb = zero;
+        if (S0) {
+            d2 ip = (ok && hip_) ? *(const d2*)(wr0 + idx + g.plane) : zero;
+            d2 im = (ok && him) ? *(const d2*)(wr0 + idx - g.plane) : zero;
+            d2 jp = (ok && hjp) ? *(const d2*)(wr0 + idx + g.nhr) : zero;
+            d2 jm = (ok && hjm) ? *(const d2*)(wr0 + idx - g.nhr) : zero;
+            d2 km, kp;
+            tsv_kn(wr0, idx, m0, g.nhr, q, ok, c0, km, kp);
+            ya = d * xa;
+            if (START) {
+                ya = ya + o[0] * ip;
+                ya = ya + o[1] * im;
+                ya = ya + o[2] * jp;
+                ya = ya + o[3] * jm;
+                ya = ya + o[4] * kp;
+                ya = ya + o[5] * km;
+            } else {
+                ya = ya - o[0] * ip;
+                ya = ya - o[1] * im;
+                ya = ya - o[2] * jp;
+                ya = ya - o[3] * jm;
+                ya = ya - o[4] * kp;
+                ya = ya - o[5] * km;
+            }
+        }
+        if (S1) {
+            d2 ip = (ok && hip_) ? *(const d2*)(wr1 + idx + g.plane) : zero;
+            d2 im = (ok && him) ? *(const d2*)(wr1 + idx - g.plane) : zero;
+            d2 jp = (ok && hjp) ? *(const d2*)(wr1 + idx + g.nhr) : zero;
+            d2 jm = (ok && hjm) ? *(const d2*)(wr1 + idx - g.nhr) : zero;
+            d2 km, kp;
+            tsv_kn(wr1, idx, m0, g.nhr, q, ok, c1, km, kp);
+            yb = d * xb;
+            if (START) {
+                yb = yb + o[0] * ip;
+                yb = yb + o[1] * im;
+                yb = yb + o[2] * jp;
+                yb = yb + o[3] * jm;
+                yb = yb + o[4] * kp;
+                yb = yb + o[5] * km;
+            } else {
+                yb = yb - o[0] * ip;
+                yb = yb - o[1] * im;
+                yb = yb - o[2] * jp;
+                yb = yb - o[3] * jm;
+                yb = yb - o[4] * kp;
+                yb = yb - o[5] * km;
+            }
+        }
+        if (!ok) continue;
+        if (START) {
+            // the black voxels of the pair in the full layout: 2m + par
+            const uint32_t vb = 2u * idx + (uint32_t)q;
+            if (S0) {
+                const d2 ra = d2{bf0[vb] - ya.x, bf0[vb + 2] - ya.y};
+                *(d2*)(y0 + idx) = ra;
+                *(d2*)(rh0 + idx) = ra;
+                *(d2*)(p0 + idx) = zero;
+                *(d2*)(v0 + idx) = zero;
+                if (MODE == 3) {
+                    xf0[vb] = xa.x;
+                    xf0[vb + 2] = xa.y;
+                }
+                // ||b|| of the full system: the voxels 2 idx .. 2 idx + 3
+                const d2 ba = *(const d2*)(bf0 + 2u * idx), bb = *(const d2*)(bf0 + 2u * idx + 2);
+                s0 += ba.x * ba.x + ba.y * ba.y;
+                s0 += bb.x * bb.x + bb.y * bb.y;
+                s2 += ra.x * ra.x;
+                s2 += ra.y * ra.y;
+            }
+            if (S1) {
+                const d2 rb = d2{bf1[vb] - yb.x, bf1[vb + 2] - yb.y};
+                *(d2*)(y1 + idx) = rb;
+                *(d2*)(rh1 + idx) = rb;
+                *(d2*)(p1 + idx) = zero;
+                *(d2*)(v1 + idx) = zero;
+                if (MODE == 3) {
+                    xf1[vb] = xb.x;
+                    xf1[vb + 2] = xb.y;
+                }
+                const d2 bc = *(const d2*)(bf1 + 2u * idx), bd = *(const d2*)(bf1 + 2u * idx + 2);
+                s1 += bc.x * bc.x + bc.y * bc.y;
+                s1 += bd.x * bd.x + bd.y * bd.y;
+                s3 += rb.x * rb.x;
+                s3 += rb.y * rb.y;
+            }
+        } else {
+            if (S0 && a0) *(d2*)(y0 + idx) = ya;
+            if (S1 && a1) *(d2*)(y1 + idx) = yb;
+            if (MODE == 0) {
+                if (S0) {
+                    const d2 r = *(const d2*)(w0 + idx);
+                    s0 += r.x * ya.x;
+                    s0 += r.y * ya.y;
+                }
+                if (S1) {
+                    const d2 r = *(const d2*)(w1 + idx);
+                    s1 += r.x * yb.x;
+                    s1 += r.y * yb.y;
+                }
+            } else {
+                if (S0) {
+                    const d2 sv = *(const d2*)(w0 + idx);
+                    s0 += ya.x * ya.x;
+                    s0 += ya.y * ya.y;
+                    s2 += ya.x * sv.x;
+                    s2 += ya.y * sv.y;
+                }
+                if (S1) {
+                    const d2 sv = *(const d2*)(w1 + idx);
+                    s1 += yb.x * yb.x;
+                    s1 += yb.y * yb.y;
+                    s3 += yb.x * sv.x;
+                    s3 += yb.y * sv.y;
+                }
+            }
+        }
+    }
+    // block partials: wave sums, then the waves in order
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    s3 = wave_sum(s3);
+    if (threadIdx.x == 0) {
+        lds[threadIdx.y] = s0;
+        lds[TSV_TR + threadIdx.y] = s1;
+        lds[2 * TSV_TR + threadIdx.y] = s2;
+        lds[3 * TSV_TR + threadIdx.y] = s3;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const int nb = gridDim.x * gridDim.y;
+        const int b = blockIdx.x + gridDim.x * blockIdx.y;
+        for (int a = 0; a < 4; ++a) {
+            double t = 0.0;
+            for (int w = 0; w < TSV_TR; ++w) t += lds[a * TSV_TR + w];
+            partials[a * nb + b] = t;
+        }
+    }
+}
+
 __global__ void k_bicg_alpha(const double* __restrict__ partials, int n, BicgScalars* bs) {
     if (bs->all_done) return;
     __shared__ double lds[1024 / WAVE];
@@ -1164,6 +1476,19 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     const uint32_t Nv = tschur ? Nh : N;
     const int nbv = tschur ? nbh : nb;
     const double* invd_v = tschur ? invdb : c->tinvd;
+    // the vectorised Schur passes: nk % 4 == 0 (16 B aligned pairs per row)
+    // and the row count within the grid's y limit
+    TVGeo tvg;
+    tvg.nhr = (uint32_t)(c->nk / 2);
+    tvg.nrows = (uint32_t)c->ni * (uint32_t)c->nj;
+    tvg.plane = (uint32_t)c->nj * tvg.nhr;
+    tvg.ni = c->ni;
+    tvg.nj = c->nj;
+    const dim3 tvgrid((tvg.nhr / 2 + WAVE - 1) / WAVE,
+                      (tvg.nrows + TSV_TR * TSV_RPW - 1) / (TSV_TR * TSV_RPW));
+    const bool tvec = tschur && c->nk % 4 == 0 && tvgrid.y <= 65535 &&
+                      4 * (size_t)tvgrid.x * tvgrid.y <= 8 * (size_t)MAX_PARTIALS;
+    const int nbs_schur = tvec ? (int)(tvgrid.x * tvgrid.y) : nbh;
     // systems still iterating, as the host last saw them (a system that
     // converges inside a batch is masked on the device)
     int sys = 3;
@@ -1186,9 +1511,14 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         }
         // black mode M: 0, 1 apply; 2 start; 3 final start (fields written)
         constexpr int RM = M == 2 ? 1 : (M == 3 ? 2 : 0);
-        hipLaunchKernelGGL((k_tschur_red<S, RM>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream,
-                           c->arows, g, Nh, xa, xb, wr[0], wr[1], c->bicg, c->tb[0], c->tb[1],
-                           x[0], x[1]);
+        if (tvec)
+            hipLaunchKernelGGL((k_tschur_red_v<S, RM>), tvgrid, dim3(WAVE, TSV_TR), 0, c->stream,
+                               c->arows, tvg, Nh, xa, xb, wr[0], wr[1], c->bicg, c->tb[0],
+                               c->tb[1], x[0], x[1]);
+        else
+            hipLaunchKernelGGL((k_tschur_red<S, RM>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream,
+                               c->arows, g, Nh, xa, xb, wr[0], wr[1], c->bicg, c->tb[0], c->tb[1],
+                               x[0], x[1]);
         if (sample) {
             (void)hipEventRecord(c->ev[17], c->stream);
             red_pending = true;
@@ -1196,10 +1526,16 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             // voxels and w at the red ones, per system iterated
             c->prof_tred_bytes = (double)c->N * (28.0 + (S == 3 ? 16.0 : 8.0));
         }
-        hipLaunchKernelGGL((k_tschur_black<M, S>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream,
-                           c->arows, g, Nh, xa, xb, wr[0], wr[1], ya, yb, wa, wb, c->bicg,
-                           c->partials, rh[0], rh[1], pp[0], pp[1], vv[0], vv[1], c->tb[0],
-                           c->tb[1], x[0], x[1]);
+        if (tvec)
+            hipLaunchKernelGGL((k_tschur_black_v<M, S>), tvgrid, dim3(WAVE, TSV_TR), 0, c->stream,
+                               c->arows, tvg, Nh, xa, xb, wr[0], wr[1], ya, yb, wa, wb, c->bicg,
+                               c->partials, rh[0], rh[1], pp[0], pp[1], vv[0], vv[1], c->tb[0],
+                               c->tb[1], x[0], x[1]);
+        else
+            hipLaunchKernelGGL((k_tschur_black<M, S>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream,
+                               c->arows, g, Nh, xa, xb, wr[0], wr[1], ya, yb, wa, wb, c->bicg,
+                               c->partials, rh[0], rh[1], pp[0], pp[1], vv[0], vv[1], c->tb[0],
+                               c->tb[1], x[0], x[1]);
     };
     auto matvec = [&](auto MC, const double* xa, const double* xb, double* ya, double* yb,
                       const double* wa, const double* wb) {
@@ -1210,7 +1546,7 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
                 launch_schur(MC, std::integral_constant<int, 2>{}, xa, xb, ya, yb, wa, wb);
             else
                 launch_schur(MC, std::integral_constant<int, 3>{}, xa, xb, ya, yb, wa, wb);
-            return nbh;
+            return nbs_schur;
         }
         if (stored) {
             if (sys == 1)
@@ -1249,7 +1585,7 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
                     launch_schur(std::integral_constant<int, 3>{}, std::integral_constant<int, 3>{},
                                  xs[0], xs[1], r[0], r[1], nullptr, nullptr);
             }
-            return nbh;
+            return nbs_schur;
         }
         if (stored) {
             sys = 3;
