@@ -487,9 +487,10 @@ def main():
     rl = {}
     n, ms = sampled(reps, "prof_tred")
     if n:
+        sysn = "1" if reps[-1]["prof_tred_bytes"] == img.size * 36 else "3"
         rl["transport_red_pass"] = kernel_roofline(
-            "k_tschur_red<1>" if reps[-1]["prof_tred_bytes"] == img.size * 36 else
-            "k_tschur_red<3>", reps[-1]["prof_tred_bytes"], ms, n)
+            f"k_tschur_red_v<{sysn}, 0>" if img.shape[2] % 4 == 0 else f"k_tschur_red<{sysn}, 0>",
+            reps[-1]["prof_tred_bytes"], ms, n)
     hard = hr.reports[:min(2, len(hr.reports))]
     n, ms = sampled(hard, "prof_schur")
     if n:

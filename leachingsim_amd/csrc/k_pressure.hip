@@ -138,7 +138,8 @@ template <> struct StencilShape<double> { static constexpr int W = 2, ICH = 8, T
 
 // MODE 0: y = A^ x with the block partials of x.y.  MODE 1 (FP64, the
 // residual of k_stencil<double, 1>): y = p2 = b - A^ x with the partials of
-// r.r, |sd r|^2 and |sd b|^2 (three arrays of nb).
+// r.r, |sd r|^2 and |sd b|^2 (three arrays of nb).  MODE 2: y = A^ x with
+// the partials of x.y and x.b (two arrays of nb; the one-vector projection).
 template <typename T, int MODE = 0>
 __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
     const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ cx,
@@ -196,6 +197,7 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
                 bv = *(const V*)(b + v);
                 sv = *(const V*)(sd + v);
             }
+            if (MODE == 2 && ok) bv = *(const V*)(b + v);
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 const T xkm = (w == 0) ? left : xc[w - 1];
@@ -205,6 +207,9 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
                                  cym[w] * xjm[w] + czp[w] * xkp + czm * xkm);
                 if (MODE == 0) {
                     acc += (double)xc[w] * (double)ax[w];
+                } else if (MODE == 2) {
+                    acc += (double)xc[w] * (double)ax[w];
+                    acc1 += (double)xc[w] * (double)bv[w];
                 } else {
                     const double r = (double)bv[w] - (double)ax[w];
                     ax[w] = (T)r;
@@ -232,13 +237,13 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
         }
     }
     acc = wave_sum(acc);
-    if (MODE == 1) {
+    if (MODE != 0) {
         acc1 = wave_sum(acc1);
         acc2 = wave_sum(acc2);
     }
     if (lane == 0) {
         lds[threadIdx.y] = acc;
-        if (MODE == 1) {
+        if (MODE != 0) {
             lds[TJV + threadIdx.y] = acc1;
             lds[2 * TJV + threadIdx.y] = acc2;
         }
@@ -248,14 +253,14 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
         double s = 0.0;
         for (int q = 0; q < TJV; ++q) s += lds[q];
         partials[t] = s;
-        if (MODE == 1) {
+        if (MODE != 0) {
             double s1 = 0.0, s2 = 0.0;
             for (int q = 0; q < TJV; ++q) {
                 s1 += lds[TJV + q];
                 s2 += lds[2 * TJV + q];
             }
             partials[nb + t] = s1;
-            partials[2 * nb + t] = s2;
+            if (MODE == 1) partials[2 * nb + t] = s2;
         }
     }
 }
@@ -2471,6 +2476,36 @@ static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
     // correction of it, measured at 512^3, left an error that took mixed PCG
     // past 10000 iterations against 6007 without it)
     if (m == 0) return 0;
+    // one vector and no network: x0 = (w.b^ / w.A^w) w with w = sqrt(d) p_newest,
+    // its energy and projection from one stencil pass with both dots
+    if (m == 1 && c->P.warm_start == LS_WARM_PROJECTED && use_vec<double>(c)) {
+        const int slot = (c->hist_head + LS_WARM_MAX - 1) % LS_WARM_MAX;
+        double* w = c->tv[0][0];
+        double* aw = c->tv[1][0];
+        hipLaunchKernelGGL(k_wscale, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->hist[slot],
+                           nullptr, c->sd, w, N);
+        const dim3 grd = vec_grid<double>(c);
+        const int nbv = (int)(grd.x * grd.y * grd.z);
+        hipLaunchKernelGGL((k_stencil_vec<double, 2>), grd, dim3(WAVE, StencilShape<double>::TJV),
+                           0, c->stream, w, aw, c->c64[0], c->c64[1], c->c64[2], sgeo(c),
+                           c->partials, nullptr, c->bhat, nullptr, nullptr);
+        hipLaunchKernelGGL(k_wsum, dim3(2), dim3(1024), 0, c->stream, c->partials, nbv, c->wsum);
+        LS_LAUNCH_CHECK();
+        LS_HIP(hipMemcpyAsync(c->h_wsum, c->wsum, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        LS_HIP(hipStreamSynchronize(c->stream));
+        const double e = c->h_wsum[0], gb = c->h_wsum[1];
+        if (!(e > 0.0) || !std::isfinite(e) || !std::isfinite(gb)) return 0;
+        WBasis B{};
+        WCoef C{};
+        B.v[0] = w;
+        C.c[0] = gb / e;
+        hipLaunchKernelGGL(k_wcombine, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, B, C, 1, c->x64,
+                           N, 0);
+        LS_LAUNCH_CHECK();
+        rep->warm_basis = 1;
+        return 0;
+    }
     WarmSet S;
     int slot_used = 0;
     for (int a = 0; a < m; ++a) {   // newest first, then successive differences

@@ -935,8 +935,8 @@ __device__ __forceinline__ void tsv_kn(const double* __restrict__ src, uint32_t 
     }
 }
 
-// Red pass.  MODE 0: w = D_r^-1 A_rb p; MODE 1: w = D_r^-1 (b_r - A_rb x);
-// MODE 2: as 1, also written into the full field at the red voxels.
+// Red pass.  MODE 0: w = D_r^-1 A_rb p; MODE 1 and 2: w = D_r^-1 (b_r -
+// A_rb x) (the final black pass writes it into the field).
 template <int SYS, int MODE>
 __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
     const double* __restrict__ rows, TVGeo g, uint32_t Nh, const double* __restrict__ p0,
@@ -998,29 +998,24 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
             if (SYS & 1) *(d2*)(w0 + idx) = a / d;
             if (SYS & 2) *(d2*)(w1 + idx) = b / d;
         } else {
-            // the red voxels of the pair in the full layout: 2m + 1 - par
-            const uint32_t vr = 2u * idx + (uint32_t)q;
+            // the pair's voxels of both colours are the full-layout quad
+            // 2 idx .. 2 idx + 3; the red ones are at 2 idx + q, 2 idx + 2 + q
             if (SYS & 1) {
-                const d2 u = d2{bf0[vr] - a.x, bf0[vr + 2] - a.y} / d;
-                *(d2*)(w0 + idx) = u;
-                if (MODE == 2) {
-                    xf0[vr] = u.x;
-                    xf0[vr + 2] = u.y;
-                }
+                const d2 lo = *(const d2*)(bf0 + 2u * idx), hi = *(const d2*)(bf0 + 2u * idx + 2);
+                const d2 br = q ? d2{lo.y, hi.y} : d2{lo.x, hi.x};
+                *(d2*)(w0 + idx) = (br - a) / d;
             }
             if (SYS & 2) {
-                const d2 u = d2{bf1[vr] - b.x, bf1[vr + 2] - b.y} / d;
-                *(d2*)(w1 + idx) = u;
-                if (MODE == 2) {
-                    xf1[vr] = u.x;
-                    xf1[vr + 2] = u.y;
-                }
+                const d2 lo = *(const d2*)(bf1 + 2u * idx), hi = *(const d2*)(bf1 + 2u * idx + 2);
+                const d2 br = q ? d2{lo.y, hi.y} : d2{lo.x, hi.x};
+                *(d2*)(w1 + idx) = (br - b) / d;
             }
         }
     }
 }
 
-// Black pass, the modes of k_tschur_black.
+// Black pass, the modes of k_tschur_black; the final one (MODE 3) writes the
+// full-layout quad of both colours.
 template <int MODE, int SYS>
 __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
     const double* __restrict__ rows, TVGeo g, uint32_t Nh, const double* __restrict__ x0,
@@ -1114,38 +1109,42 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
         }
         if (!ok) continue;
         if (START) {
-            // the black voxels of the pair in the full layout: 2m + par
-            const uint32_t vb = 2u * idx + (uint32_t)q;
+            // the pair's voxels of both colours are the full-layout quad
+            // 2 idx .. 2 idx + 3, the black ones at 2 idx + q, 2 idx + 2 + q;
+            // the final pass writes the whole quad: x (black) and the red
+            // values u the red pass left in wr at the same compacted index
             if (S0) {
-                const d2 ra = d2{bf0[vb] - ya.x, bf0[vb + 2] - ya.y};
+                const d2 lo = *(const d2*)(bf0 + 2u * idx), hi = *(const d2*)(bf0 + 2u * idx + 2);
+                const d2 bbk = q ? d2{lo.y, hi.y} : d2{lo.x, hi.x};
+                const d2 ra = bbk - ya;
                 *(d2*)(y0 + idx) = ra;
                 *(d2*)(rh0 + idx) = ra;
                 *(d2*)(p0 + idx) = zero;
                 *(d2*)(v0 + idx) = zero;
                 if (MODE == 3) {
-                    xf0[vb] = xa.x;
-                    xf0[vb + 2] = xa.y;
+                    *(d2*)(xf0 + 2u * idx) = q ? d2{c0.x, xa.x} : d2{xa.x, c0.x};
+                    *(d2*)(xf0 + 2u * idx + 2) = q ? d2{c0.y, xa.y} : d2{xa.y, c0.y};
                 }
-                // ||b|| of the full system: the voxels 2 idx .. 2 idx + 3
-                const d2 ba = *(const d2*)(bf0 + 2u * idx), bb = *(const d2*)(bf0 + 2u * idx + 2);
-                s0 += ba.x * ba.x + ba.y * ba.y;
-                s0 += bb.x * bb.x + bb.y * bb.y;
+                // ||b|| of the full system over the quad
+                s0 += lo.x * lo.x + lo.y * lo.y;
+                s0 += hi.x * hi.x + hi.y * hi.y;
                 s2 += ra.x * ra.x;
                 s2 += ra.y * ra.y;
             }
             if (S1) {
-                const d2 rb = d2{bf1[vb] - yb.x, bf1[vb + 2] - yb.y};
+                const d2 lo = *(const d2*)(bf1 + 2u * idx), hi = *(const d2*)(bf1 + 2u * idx + 2);
+                const d2 bbk = q ? d2{lo.y, hi.y} : d2{lo.x, hi.x};
+                const d2 rb = bbk - yb;
                 *(d2*)(y1 + idx) = rb;
                 *(d2*)(rh1 + idx) = rb;
                 *(d2*)(p1 + idx) = zero;
                 *(d2*)(v1 + idx) = zero;
                 if (MODE == 3) {
-                    xf1[vb] = xb.x;
-                    xf1[vb + 2] = xb.y;
+                    *(d2*)(xf1 + 2u * idx) = q ? d2{c1.x, xb.x} : d2{xb.x, c1.x};
+                    *(d2*)(xf1 + 2u * idx + 2) = q ? d2{c1.y, xb.y} : d2{xb.y, c1.y};
                 }
-                const d2 bc = *(const d2*)(bf1 + 2u * idx), bd = *(const d2*)(bf1 + 2u * idx + 2);
-                s1 += bc.x * bc.x + bc.y * bc.y;
-                s1 += bd.x * bd.x + bd.y * bd.y;
+                s1 += lo.x * lo.x + lo.y * lo.y;
+                s1 += hi.x * hi.x + hi.y * hi.y;
                 s3 += rb.x * rb.x;
                 s3 += rb.y * rb.y;
             }
