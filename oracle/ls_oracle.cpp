@@ -130,6 +130,12 @@ struct lso_ctx {
     double p_top_prev = 0.0;
     bool have_prev = false;
     bool sat_valid = true;      /* physics_valid of the last implicit saturation */
+    /* linear solvers: 0 Jacobi-PCG on the reduced SPD pressure system (stop
+     * test ||b - A x|| / ||b|| <= press_rtol) and BiCGStab for transport;
+     * 1 the reference's KSP for both: GMRES(30) with left Jacobi, pressure
+     * on the full reference-form system, stop test on the preconditioned
+     * residual norm <= rtol ||D^-1 b||, each KSPSolve issued twice */
+    int ksp = 0;
 
     size_t idx(int i, int j, int k) const {
         return (size_t)k + (size_t)nk * ((size_t)j + (size_t)nj * (size_t)i);
@@ -404,7 +410,149 @@ static int pcg_reduced(lso_ctx& c, const std::vector<double>& b, std::vector<dou
     return 1;
 }
 
+long long lso_pressure_system(lso_ctx* c, double t, int* rows, int* cols, double* vals,
+                              double* rhs);
+
+/* Restarted GMRES(m) with left Jacobi preconditioning: the algorithm of
+ * PETSc's KSPGMRES + PCJACOBI that the reference configures
+ * (src/Simulation.cpp:146-157; PETSc 3.20 defaults: restart 30, left
+ * preconditioning, convergence when the preconditioned residual norm of the
+ * Arnoldi recurrence falls to rtol ||D^-1 b||, maxit iterations).  The
+ * Arnoldi vectors are orthogonalised by modified Gram-Schmidt (PETSc's
+ * default is classical Gram-Schmidt; in exact arithmetic both give the same
+ * iterates).  Warm start from x.  Returns 0 converged, 1 not. */
+static int gmres_left_jacobi(const Csr& A, const std::vector<double>& b, std::vector<double>& x,
+                             double rtol, int maxit, int m, int& iters, double& prec_relres) {
+    const size_t N = b.size();
+    std::vector<double> dinv(N);
+    for (size_t i = 0; i < N; ++i) dinv[i] = A.diag[i] != 0.0 ? 1.0 / A.diag[i] : 1.0;
+    std::vector<double> z(N), Ax(N);
+    for (size_t i = 0; i < N; ++i) z[i] = dinv[i] * b[i];
+    const double bnorm = std::sqrt(dot(z, z));
+    const double ttol = std::max(rtol * bnorm, 1e-50);
+    iters = 0;
+    prec_relres = 0.0;
+    if (bnorm == 0.0) {
+        std::fill(x.begin(), x.end(), 0.0);
+        return 0;
+    }
+    std::vector<std::vector<double>> V(m + 1, std::vector<double>(N));
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    auto h = [&](int i, int j) -> double& { return H[(size_t)i * m + j]; };
+    while (true) {
+        A.apply(x.data(), Ax.data());
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)N; ++i) V[0][i] = dinv[i] * (b[i] - Ax[i]);
+        const double beta = std::sqrt(dot(V[0], V[0]));
+        prec_relres = beta / bnorm;
+        if (beta <= ttol) return 0;
+        if (iters >= maxit) return 1;
+        for (size_t i = 0; i < N; ++i) V[0][i] /= beta;
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;
+        bool done = false;
+        for (; k < m && iters < maxit; ++k) {
+            A.apply(V[k].data(), Ax.data());
+            std::vector<double>& w = V[k + 1];
+#pragma omp parallel for schedule(static)
+            for (long long i = 0; i < (long long)N; ++i) w[i] = dinv[i] * Ax[i];
+            for (int i = 0; i <= k; ++i) {
+                const double hij = dot(w, V[i]);
+                h(i, k) = hij;
+#pragma omp parallel for schedule(static)
+                for (long long q = 0; q < (long long)N; ++q) w[q] -= hij * V[i][q];
+            }
+            const double hn = std::sqrt(dot(w, w));
+            h(k + 1, k) = hn;
+            if (hn > 0.0)
+                for (size_t q = 0; q < N; ++q) w[q] /= hn;
+            for (int i = 0; i < k; ++i) {          /* previous rotations */
+                const double a = h(i, k), bb = h(i + 1, k);
+                h(i, k) = cs[i] * a + sn[i] * bb;
+                h(i + 1, k) = -sn[i] * a + cs[i] * bb;
+            }
+            const double a = h(k, k), bb = h(k + 1, k);
+            const double r = std::hypot(a, bb);
+            cs[k] = r > 0.0 ? a / r : 1.0;
+            sn[k] = r > 0.0 ? bb / r : 0.0;
+            h(k, k) = r;
+            h(k + 1, k) = 0.0;
+            g[k + 1] = -sn[k] * g[k];
+            g[k] = cs[k] * g[k];
+            ++iters;
+            prec_relres = std::fabs(g[k + 1]) / bnorm;
+            if (std::fabs(g[k + 1]) <= ttol || hn == 0.0) {
+                ++k;
+                done = true;
+                break;
+            }
+        }
+        for (int i = k - 1; i >= 0; --i) {       /* back substitution */
+            double acc = g[i];
+            for (int j = i + 1; j < k; ++j) acc -= h(i, j) * y[j];
+            y[i] = acc / h(i, i);
+        }
+        for (int j = 0; j < k; ++j) {
+#pragma omp parallel for schedule(static)
+            for (long long q = 0; q < (long long)N; ++q) x[q] += y[j] * V[j][q];
+        }
+        if (done) return 0;
+        if (iters >= maxit) return 1;
+    }
+}
+
+/* The reference issues each KSPSolve twice in a row (pressure
+ * src/Simulation.cpp:823-825, conc :1262-1264): iterations summed, relres of
+ * the last solve. */
+static int ksp_twice(const Csr& A, const std::vector<double>& b, std::vector<double>& x,
+                     double rtol, int maxit, int& iters, double& relres) {
+    int it1 = 0, it2 = 0;
+    double rr1 = 0.0, rr2 = 0.0;
+    int st = gmres_left_jacobi(A, b, x, rtol, maxit, 30, it1, rr1);
+    if (st == 0) st = gmres_left_jacobi(A, b, x, rtol, maxit, 30, it2, rr2);
+    iters = it1 + it2;
+    relres = st == 0 ? rr2 : rr1;
+    return st;
+}
+
+/* solvePressure with the reference's KSP (src/Simulation.cpp:804-855): the
+ * full reference-form system of every voxel, warm start = previous pressure,
+ * KSPSolve called twice (:823, :825). */
+static int phase_pressure_reference(lso_ctx& c, double t, ls_step_report* rep) {
+    const long long nnz = lso_pressure_system(&c, t, nullptr, nullptr, nullptr, nullptr);
+    std::vector<int> R(nnz), C(nnz);
+    std::vector<double> V(nnz), b(c.N);
+    lso_pressure_system(&c, t, R.data(), C.data(), V.data(), b.data());
+    Csr A;
+    A.build((int)c.N, R, C, V);
+    std::vector<double> x = c.p;
+    int its = 0;
+    double rr = 0.0;
+    const int st = ksp_twice(A, b, x, c.P.press_rtol, c.P.press_maxit, its, rr);
+    c.p = x;
+    const double ptop = effective_top_pressure(c.P.top_pressure, t);
+    c.p_top_prev = ptop;
+    c.have_prev = true;
+    if (rep) {
+        rep->press_iters = its;
+        rep->press_outer = 2;
+        rep->press_relres = rr;
+        rep->top_pressure_eff = ptop;
+        double mn = 1e300, mx = -1e300;
+        for (size_t v = 0; v < c.N; ++v) { mn = std::min(mn, c.p[v]); mx = std::max(mx, c.p[v]); }
+        rep->p_min = mn;
+        rep->p_max = mx;
+    }
+    if (st != 0) {
+        c.err = "pressure solve did not converge";
+        return 1;
+    }
+    return 0;
+}
+
 static int phase_pressure(lso_ctx& c, double t, ls_step_report* rep) {
+    if (c.ksp == 1) return phase_pressure_reference(c, t, rep);
     std::vector<double> pD, b;
     const double ptop = effective_top_pressure(c.P.top_pressure, t);
     assemble_pressure(c, t, pD, b);
@@ -836,13 +984,15 @@ static int phase_transport(lso_ctx& c, double dt, ls_step_report* rep) {
     assemble_transport(c, dt, c.conc, R, C, V, b);
     Csr A;
     A.build((int)c.N, R, C, V);
-    int st = bicgstab(A, b, c.conc, c.P.adr_rtol, c.P.adr_maxit, it1, rr1);
+    int st = c.ksp == 1 ? ksp_twice(A, b, c.conc, c.P.adr_rtol, c.P.adr_maxit, it1, rr1)
+                        : bicgstab(A, b, c.conc, c.P.adr_rtol, c.P.adr_maxit, it1, rr1);
     if (st == 0) {
         /* acid: setupConcentrationEqns(dt, conc_acid, 1, 0) + solveAcid */
         assemble_transport(c, dt, c.acid, R, C, V, b);
         Csr A2;
         A2.build((int)c.N, R, C, V);
-        st = bicgstab(A2, b, c.acid, c.P.adr_rtol, c.P.adr_maxit, it2, rr2);
+        st = c.ksp == 1 ? ksp_twice(A2, b, c.acid, c.P.adr_rtol, c.P.adr_maxit, it2, rr2)
+                        : bicgstab(A2, b, c.acid, c.P.adr_rtol, c.P.adr_maxit, it2, rr2);
     }
     if (rep) {
         rep->adr_iters = std::max(it1, it2);
@@ -997,6 +1147,12 @@ static void summarize(lso_ctx& c, ls_step_report* rep) {
 /* ------------------------------------------------------------------------ */
 
 extern "C" {
+
+int lso_set_ksp(lso_ctx* c, int which) {
+    if (which != 0 && which != 1) return -1;
+    c->ksp = which;
+    return 0;
+}
 
 void lso_set_threads(int nthreads) {
 #ifdef _OPENMP

@@ -38,6 +38,13 @@ int lso_phase(lso_ctx* ctx, int phase, double t, double dt, ls_step_report* repo
 int lso_snapshot(lso_ctx* ctx);
 int lso_rollback(lso_ctx* ctx);
 void lso_set_threads(int nthreads);
+/* Linear solvers: 0 (the default) Jacobi-PCG on the reduced pressure
+ * system to press_rtol on ||b - A x|| / ||b|| and BiCGStab for transport;
+ * 1 the reference's KSP for pressure and transport: restarted GMRES(30)
+ * with left Jacobi (pressure on the full reference-form system), stopping
+ * on the preconditioned residual norm <= rtol ||D^-1 b||, each solve issued
+ * twice (src/Simulation.cpp:146-157, 804-855, 1247-1275). */
+int lso_set_ksp(lso_ctx* ctx, int which);
 
 /* The reference's full pressure system (src/Simulation.cpp:658-798) for the
  * current state, as assembled by MatSetValue/VecSetValue (ADD_VALUES), in

@@ -51,6 +51,7 @@ def load() -> C.CDLL:
         "lso_snapshot": (i, [vp]),
         "lso_rollback": (i, [vp]),
         "lso_set_threads": (None, [i]),
+        "lso_set_ksp": (i, [vp, i]),
         "lso_pressure_system": (ll, [vp, d, vp, vp, vp, vp]),
         "lso_transport_system": (ll, [vp, d, i, vp, vp, vp, vp]),
         "lso_saturation_system": (ll, [vp, d, vp, vp, vp, vp]),
@@ -99,6 +100,12 @@ class OracleSimulation:
         if st < 0 or (st > 0 and not allow_positive):
             raise RuntimeError(f"oracle status {st}: {self.lib.lso_last_error(self.h).decode()}")
         return st
+
+    def set_ksp(self, which: int):
+        """0: Jacobi-PCG pressure on the reduced system and BiCGStab transport
+        (default); 1: the reference's GMRES(30) + left Jacobi for both,
+        pressure on the full reference-form system, each solve issued twice."""
+        self._check(self.lib.lso_set_ksp(self.h, which))
 
     def set_params(self, params: Params):
         self._check(self.lib.lso_set_params(self.h, C.byref(params)))

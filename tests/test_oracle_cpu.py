@@ -173,6 +173,38 @@ def test_pressure_matches_dense_solve_of_reference_system(seed, bc):
         assert np.all(np.abs(res[fl]) <= 1e-6 * np.abs(np.diag(A)[fl] * p[fl]).max() + 1e-12)
 
 
+@pytest.mark.parametrize("rtol", [1e-6, 1e-13])
+def test_reference_gmres_solver_meets_its_stop_test(rtol):
+    """The reference's KSP setting (GMRES(30) + left Jacobi on the full
+    reference-form system, src/Simulation.cpp:146-157): after the two solves
+    the preconditioned residual recomputed from the assembled system meets
+    ||D^-1 (b - A p)|| <= rtol ||D^-1 b||; at a tight rtol it is the dense
+    solution; the problem needs several restart cycles."""
+    shape = (10, 9, 12)
+    img = G.smoothed_threshold(shape, 1.0, 0.55, 5)
+    img[0] = G.AIR
+    o = make(shape, img, bc_mode=L.BC_ZFACES, press_rtol=rtol)
+    o.set_ksp(1)
+    o.upload(L.F_SATURATION, np.random.default_rng(5).uniform(0.08, 0.6, size=shape))
+    o.phase(L.PH_PROPERTIES)
+    t = 77.0
+    r, c, v, b = o.pressure_system(t)
+    rep = o.phase(L.PH_PRESSURE, t)
+    p = o.download(L.F_PRESSURE).ravel()
+    A = dense(r, c, v, p.size)
+    d = np.diag(A)
+    pres = np.linalg.norm((b - A @ p) / d) / np.linalg.norm(b / d)
+    assert pres <= rtol * 1.001, pres
+    assert abs(rep.press_relres - pres) <= 1e-3 * rtol + 1e-15
+    assert rep.press_outer == 2 and rep.press_iters > 30
+    if rtol < 1e-10:
+        x = np.linalg.solve(A, b)
+        np.testing.assert_allclose(p, x, rtol=0, atol=1e-8 * np.abs(x).max())
+        # a converged warm start needs no iteration
+        rep2 = o.phase(L.PH_PRESSURE, t)
+        assert rep2.press_iters == 0
+
+
 def test_reduced_operator_is_symmetric_positive():
     shape = (6, 6, 6)
     img = G.smoothed_threshold(shape, 1.0, 0.6, 4)
@@ -318,12 +350,15 @@ def test_precipitation_rules(instant):
 # Transport: dense solve, maximum principle, conservation, shared operator
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("theta", [1.0, 0.5])
-def test_transport_matches_dense_solve(theta):
+@pytest.mark.parametrize("ksp", [0, 1])
+def test_transport_matches_dense_solve(theta, ksp):
+    """ksp 0: BiCGStab; ksp 1: the reference's GMRES(30) + left Jacobi."""
     shape = (5, 6, 4)
     rng = np.random.default_rng(3)
     img = rng.choice([G.AIR, G.PORE, G.PORE, G.ROCK, G.SULPHIDE, G.PRECIPITATE],
                      size=shape).astype(np.uint8)
     o = make(shape, img, theta=theta, adr_rtol=1e-14, alpha_transfer=0.3, Dpore_fac=1.7)
+    o.set_ksp(ksp)
     for f in (L.F_QX, L.F_QY, L.F_QZ):
         o.upload(f, rng.uniform(-1, 1, size=shape))
     c0 = rng.uniform(0, 1, size=shape)
