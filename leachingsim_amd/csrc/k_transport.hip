@@ -680,9 +680,17 @@ __device__ __forceinline__ SPair spair(uint32_t m, const TGeo& g) {
     return s;
 }
 
+// The 14 colour-split row arrays are TROW_PAD doubles apart beyond their
+// N/2 length, so that arrays streamed together start at different offsets
+// modulo the HBM channel interleave (at 512^3, N/2 doubles are exactly
+// 2^29 B, and unpadded every row array of one voxel would map to the same
+// channel).
+constexpr uint32_t TROW_PAD = 544;     // 4352 B
+__device__ __forceinline__ size_t rstride(uint32_t Nh) { return (size_t)Nh + TROW_PAD; }
+
 __device__ __forceinline__ double srow(const double* __restrict__ rows, uint32_t Nh, int f,
                                        int colour, uint32_t m) {
-    return rows[((size_t)f * 2 + colour) * Nh + m];
+    return rows[((size_t)f * 2 + colour) * rstride(Nh) + m];
 }
 
 // The stored rows, colour-split, with b (full layout) as k_adr_rows; at the
@@ -739,10 +747,11 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
         if (!one) b1[v] = bbv;
         const int colour = (i + j + k) & 1;
         const uint32_t m = v >> 1;
-        rows[((size_t)0 * 2 + colour) * Nh + m] = R.diag;
+        rows[((size_t)0 * 2 + colour) * rstride(Nh) + m] = R.diag;
 #pragma unroll
         for (int f = 0; f < 6; ++f)
-            rows[((size_t)(f + 1) * 2 + colour) * Nh + m] = (R.mask & (1 << f)) ? R.off[f] : 0.0;
+            rows[((size_t)(f + 1) * 2 + colour) * rstride(Nh) + m] =
+                (R.mask & (1 << f)) ? R.off[f] : 0.0;
         if (colour == 0) {
             invdb[m] = 1.0 / R.diag;
             if (xprev != nullptr) {
@@ -910,6 +919,12 @@ constexpr int TSV_RPW = 4;    // rows per wave
 
 typedef double d2 __attribute__((ext_vector_type(2)));
 
+// Row coefficients are read once per pass: nontemporal loads keep them from
+// displacing the reused vectors in L2 and the last-level cache.
+__device__ __forceinline__ d2 ld_once(const double* p) {
+    return __builtin_nontemporal_load((const d2*)p);
+}
+
 struct TVGeo {
     uint32_t nhr;       // compacted values per row (nk / 2, even)
     uint32_t nrows;     // ni * nj
@@ -947,9 +962,10 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
     const uint32_t m0 = 2u * (blockIdx.x * WAVE + threadIdx.x);
     const bool ok = m0 < g.nhr;
     const d2 zero = d2{0.0, 0.0};
+#pragma unroll
     for (int q8 = 0; q8 < TSV_RPW; ++q8) {
         const uint32_t row = (blockIdx.y * TSV_RPW + q8) * TSV_TR + threadIdx.y;
-        if (row >= g.nrows) break;                      // uniform per wave
+        if (row >= g.nrows) continue;                   // uniform per wave
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const int q = 1 - ((i + j) & 1);
         const uint32_t idx = row * g.nhr + m0;
@@ -958,9 +974,10 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
         d2 a = zero, b = zero;
         d2 c0 = zero, c1 = zero;
         if (ok) {
-            d = *(const d2*)(rows + (size_t)1 * Nh + idx);
+            d = ld_once(rows + (size_t)1 * rstride(Nh) + idx);
 #pragma unroll
-            for (int f = 0; f < 6; ++f) o[f] = *(const d2*)(rows + ((size_t)(f + 1) * 2 + 1) * Nh + idx);
+            for (int f = 0; f < 6; ++f)
+                o[f] = ld_once(rows + ((size_t)(f + 1) * 2 + 1) * rstride(Nh) + idx);
             if (SYS & 1) c0 = *(const d2*)(p0 + idx);
             if (SYS & 2) c1 = *(const d2*)(p1 + idx);
         }
@@ -1035,9 +1052,10 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
     const bool ok = m0 < g.nhr;
     const d2 zero = d2{0.0, 0.0};
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
     for (int q8 = 0; q8 < TSV_RPW; ++q8) {
         const uint32_t row = (blockIdx.y * TSV_RPW + q8) * TSV_TR + threadIdx.y;
-        if (row >= g.nrows) break;                      // uniform per wave
+        if (row >= g.nrows) continue;                   // uniform per wave
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const int q = (i + j) & 1;
         const uint32_t idx = row * g.nhr + m0;
@@ -1045,9 +1063,10 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
         for (int f = 0; f < 6; ++f) o[f] = zero;
         d2 xa = zero, xb = zero, c0 = zero, c1 = zero;
         if (ok) {
-            d = *(const d2*)(rows + idx);
+            d = ld_once(rows + idx);
 #pragma unroll
-            for (int f = 0; f < 6; ++f) o[f] = *(const d2*)(rows + ((size_t)(f + 1) * 2) * Nh + idx);
+            for (int f = 0; f < 6; ++f)
+                o[f] = ld_once(rows + ((size_t)(f + 1) * 2) * rstride(Nh) + idx);
             if (S0) {
                 xa = *(const d2*)(x0 + idx);
                 c0 = *(const d2*)(wr0 + idx);
@@ -1410,7 +1429,10 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     const int nbm = (int)(vg.x * vg.y * vg.z);
     bool stored = c->nk % 2 == 0 && 4 * (size_t)nbm <= 8 * (size_t)MAX_PARTIALS;
     if (stored && c->arows == nullptr) {
-        if (hipMalloc((void**)&c->arows, 7 * c->N * sizeof(double)) == hipSuccess) {
+        // 7 N doubles for the full-system layout, 14 (N/2 + TROW_PAD) for the
+        // colour-split one
+        const size_t nrow = std::max(7 * c->N, 14 * (c->N / 2 + TROW_PAD));
+        if (hipMalloc((void**)&c->arows, nrow * sizeof(double)) == hipSuccess) {
             c->allocations.push_back(c->arows);
         } else {            // out of memory: rebuild rows per matvec instead
             c->arows = nullptr;
@@ -1436,11 +1458,13 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     double *xs[2] = {nullptr, nullptr}, *wr[2] = {nullptr, nullptr};
     double* invdb = nullptr;
     if (tschur) {
+        // second halves TROW_PAD beyond N/2 (the allocations carry that much
+        // slack), off the first halves' channel alignment
         for (int f = 0; f < 2; ++f) {
-            xs[f] = c->tv[f][0] + Nh;
-            wr[f] = c->tv[f][2] + Nh;
+            xs[f] = c->tv[f][0] + Nh + TROW_PAD;
+            wr[f] = c->tv[f][2] + Nh + TROW_PAD;
         }
-        invdb = c->tv[0][3] + Nh;
+        invdb = c->tv[0][3] + Nh + TROW_PAD;
         // transport: conc starts from its linear extrapolation in time
         double* xprev = nullptr;
         if (KIND == 0 && c->P.adr_extrapolate) {
