@@ -769,12 +769,16 @@ __global__ __launch_bounds__(EW_BLOCK) void k_cg_direction_vec(
     const T a = (T)cs->alpha;
     const uint32_t nv = N / W;
     const uint32_t base = blockIdx.x * (uint32_t)(EW_BLOCK * VU) + threadIdx.x;
+    // beta == 0 (a fresh direction): p = r, and p is read only for an owed
+    // x update, so a direction vector that holds another phase's bytes is
+    // never read
+    const bool rp = xl || beta != (T)0;
     V pv[VU], rv[VU], xv[VU];
 #pragma unroll
     for (int u = 0; u < VU; ++u) {
         const uint32_t q = base + u * EW_BLOCK;
         if (q < nv) {
-            pv[u] = __builtin_nontemporal_load((const V*)p + q);
+            pv[u] = rp ? __builtin_nontemporal_load((const V*)p + q) : V{};
             if (pl) rv[u] = __builtin_nontemporal_load((const V*)r + q);
             if (xl) xv[u] = __builtin_nontemporal_load((const V*)x + q);
         }
@@ -784,14 +788,15 @@ __global__ __launch_bounds__(EW_BLOCK) void k_cg_direction_vec(
         const uint32_t q = base + u * EW_BLOCK;
         if (q < nv) {
             if (xl) __builtin_nontemporal_store(xv[u] + a * pv[u], (V*)x + q);
-            if (pl) __builtin_nontemporal_store(rv[u] + beta * pv[u], (V*)p + q);
+            if (pl) __builtin_nontemporal_store(beta != (T)0 ? rv[u] + beta * pv[u] : rv[u],
+                                                (V*)p + q);
         }
     }
     if (blockIdx.x == 0 && threadIdx.x < N - nv * W) {
         const uint32_t v = nv * W + threadIdx.x;
-        const T pv0 = p[v];
+        const T pv0 = rp ? p[v] : (T)0;
         if (xl) x[v] = x[v] + a * pv0;
-        if (pl) p[v] = r[v] + beta * pv0;
+        if (pl) p[v] = beta != (T)0 ? r[v] + beta * pv0 : r[v];
     }
 }
 
@@ -921,10 +926,11 @@ __global__ void k_cg_direction(T* __restrict__ p, const T* __restrict__ r, T* __
     if (!xl && !pl) return;
     const T beta = (T)cs->beta;
     const T a = (T)cs->alpha;
+    const bool rp = xl || beta != (T)0;     // as k_cg_direction_vec
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
-        const T pv = p[v];
+        const T pv = rp ? p[v] : (T)0;
         if (xl) x[v] = x[v] + a * pv;
-        if (pl) p[v] = r[v] + beta * pv;
+        if (pl) p[v] = beta != (T)0 ? r[v] + beta * pv : r[v];
     }
 }
 
@@ -1436,8 +1442,8 @@ static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep, bool fused) {
     const int nbe = ew_blocks(N);
     InnerBatch ib(batch_size(c));
     const double delta = c->P.mixed_inner_rtol > 0 ? c->P.mixed_inner_rtol : 0.1;
-    float* rs = (float*)c->tv[0][0];    // Schur residual (FP32 work vectors borrowed from
-    float* w = (float*)c->tv[0][1];     // the idle transport buffers)
+    float* rs = (float*)c->pw[0];       // Schur residual (FP32, in the pressure scratch)
+    float* w = (float*)c->pw[1];
     int st;
     if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
     if ((st = poll_cg(c))) return st;
@@ -1630,7 +1636,7 @@ int debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot
         return -3;
     }
     const size_t N = c->N, Nh = N / 2;
-    float* pc = (float*)c->tv[0][0];         // 2N floats per transport buffer
+    float* pc = (float*)c->pw[0];            // 2N floats per scratch vector
     float* qc = pc + Nh;
     int nbs;
     std::vector<float> full;
@@ -1638,7 +1644,7 @@ int debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot
         // full-length vectors, any nk: the gather and apply passes the
         // solver runs (k_rb_vec for nk % 4 == 0, else k_rb_scalar); the red
         // entries of p are never read, those of q come back zero
-        float* pf = (float*)c->tv[0][1];
+        float* pf = (float*)c->pw[1];
         float* wf = pf + N;
         LS_HIP(hipMemcpyAsync(pf, p, N * sizeof(float), hipMemcpyHostToDevice, c->stream));
         launch_rb<1>(c, pf, nullptr, wf, nullptr);
@@ -1660,7 +1666,7 @@ int debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot
         LS_LAUNCH_CHECK();
         LS_HIP(hipMemcpyAsync(q, qc, Nh * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     } else {
-        float* pf = (float*)c->tv[0][1];
+        float* pf = (float*)c->pw[1];
         float* wf = pf + N;
         launch_unpack(c, pc, pf);
         launch_rb<1>(c, pf, nullptr, wf, nullptr);
@@ -1907,7 +1913,8 @@ __global__ void k_lp(const float* __restrict__ z, const float* __restrict__ ppre
     const float b = (float)fs->beta;
     double a0 = 0.0;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
-        const float pv = z[v] - b * pprev[v];
+        // the first direction reads no pprev (its vector holds no earlier value)
+        const float pv = b != 0.0f ? z[v] - b * pprev[v] : z[v];
         p[v] = pv;
         a0 += (double)pv * (double)r[v];
     }
@@ -2135,16 +2142,16 @@ static int solve_learned(ls_ctx* c, ls_step_report* rep) {
     }
     const uint32_t N = (uint32_t)c->N;
     const int nbe = ew_blocks(N);
-    // FP32 work vectors borrowed from the (idle) transport buffers
-    float* z = (float*)c->tv[0][0];
-    float* pa = (float*)c->tv[0][1];
-    float* pb = (float*)c->tv[0][2];
-    float* qa = (float*)c->tv[0][3];
-    float* qb = (float*)c->tv[0][4];
-    float* sr = (float*)c->tv[0][5];
-    float* w = (float*)c->tv[0][6];
-    float* sw = (float*)c->tv[0][7];
-    double* xbackup = c->tv[1][0];
+    // FP32 work vectors in the pressure scratch, two per N-double vector
+    float* z = (float*)c->pw[0];
+    float* pa = z + N;
+    float* pb = (float*)c->pw[1];
+    float* qa = pb + N;
+    float* qb = (float*)c->pw[2];
+    float* sr = qb + N;
+    float* w = (float*)c->pw[3];
+    float* sw = w + N;
+    double* xbackup = c->pw[4];
     if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
     if ((st = poll_cg(c))) return st;
     const double bnorm2 = c->h_cg->bnorm2;
@@ -2449,8 +2456,8 @@ static int warm_accumulate(ls_ctx* c, const WarmSet& S, int from) {
 }
 
 // x64 <- projected warm start (the assembled system is in place and x64
-// holds the previous-pressure start).  Work vectors: the idle transport
-// buffers (basis in tv[0], A^ basis in tv[1]).
+// holds the previous-pressure start).  Work vectors: the pressure scratch
+// (basis vector a in pw[2a], its image A^ w in pw[2a + 1]).
 static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
     rep->warm_basis = 0;
     rep->warm_net = 0;
@@ -2467,7 +2474,7 @@ static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
     // residual, many more CG iterations.
     int m = 0;
     while (m < std::min(c->n_hist, c->P.warm_history)) {
-        const int slot = (c->hist_head + LS_WARM_MAX - 1 - m) % LS_WARM_MAX;
+        const int slot = (c->hist_head + c->hist_ring - 1 - m) % c->hist_ring;
         const double ratio = c->hist_ptop[slot] / ptop;
         if (!(ratio >= 0.01 && ratio <= 100.0)) break;
         ++m;
@@ -2479,9 +2486,9 @@ static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
     // one vector and no network: x0 = (w.b^ / w.A^w) w with w = sqrt(d) p_newest,
     // its energy and projection from one stencil pass with both dots
     if (m == 1 && c->P.warm_start == LS_WARM_PROJECTED && use_vec<double>(c)) {
-        const int slot = (c->hist_head + LS_WARM_MAX - 1) % LS_WARM_MAX;
-        double* w = c->tv[0][0];
-        double* aw = c->tv[1][0];
+        const int slot = (c->hist_head + c->hist_ring - 1) % c->hist_ring;
+        double* w = c->pw[0];
+        double* aw = c->pw[1];
         hipLaunchKernelGGL(k_wscale, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->hist[slot],
                            nullptr, c->sd, w, N);
         const dim3 grd = vec_grid<double>(c);
@@ -2509,10 +2516,10 @@ static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
     WarmSet S;
     int slot_used = 0;
     for (int a = 0; a < m; ++a) {   // newest first, then successive differences
-        const int slot = (c->hist_head + LS_WARM_MAX - 1 - a) % LS_WARM_MAX;
-        const int newer = (slot + 1) % LS_WARM_MAX;
-        double* w = c->tv[0][slot_used];
-        double* aw = c->tv[1][slot_used];
+        const int slot = (c->hist_head + c->hist_ring - 1 - a) % c->hist_ring;
+        const int newer = (slot + 1) % c->hist_ring;
+        double* w = c->pw[2 * slot_used];
+        double* aw = c->pw[2 * slot_used + 1];
         hipLaunchKernelGGL(k_wscale, dim3(nbe), dim3(EW_BLOCK), 0, c->stream,
                            a == 0 ? c->hist[slot] : c->hist[newer],
                            a == 0 ? nullptr : c->hist[slot], c->sd, w, N);
@@ -2550,8 +2557,8 @@ static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
     }
     hipLaunchKernelGGL(k_lprolong, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->net->out, c->cls,
                        c->x32, g);
-    double* w = c->tv[0][slot_used];
-    double* aw = c->tv[1][slot_used];
+    double* w = c->pw[2 * slot_used];
+    double* aw = c->pw[2 * slot_used + 1];
     hipLaunchKernelGGL(k_wfrom32, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->x32, w, N);
     LS_LAUNCH_CHECK();
     bool added = false;
@@ -2570,8 +2577,28 @@ static int warm_push(ls_ctx* c, double ptop, bool replace) {
         c->n_hist = 0;
         return 0;
     }
+    // the ring holds warm_history solutions: a new length starts it afresh
+    // and frees the slots beyond it
+    const int ring = std::max(1, std::min(c->P.warm_history, LS_WARM_MAX));
+    if (ring != c->hist_ring) {
+        for (int q = ring; q < LS_WARM_MAX; ++q)
+            if (c->hist[q]) {
+                LS_HIP(hipStreamSynchronize(c->stream));
+                for (size_t a = 0; a < c->allocations.size(); ++a)
+                    if (c->allocations[a] == c->hist[q]) {
+                        c->allocations.erase(c->allocations.begin() + a);
+                        break;
+                    }
+                (void)hipFree(c->hist[q]);
+                c->hist[q] = nullptr;
+            }
+        c->hist_ring = ring;
+        c->n_hist = 0;
+        c->hist_head = 0;
+        replace = false;
+    }
     if (replace && c->n_hist > 0) {
-        const int newest = (c->hist_head + LS_WARM_MAX - 1) % LS_WARM_MAX;
+        const int newest = (c->hist_head + c->hist_ring - 1) % c->hist_ring;
         LS_HIP(hipMemcpyAsync(c->hist[newest], c->p, c->N * sizeof(double),
                               hipMemcpyDeviceToDevice, c->stream));
         c->hist_ptop[newest] = ptop;
@@ -2588,8 +2615,8 @@ static int warm_push(ls_ctx* c, double ptop, bool replace) {
     LS_HIP(hipMemcpyAsync(c->hist[slot], c->p, c->N * sizeof(double), hipMemcpyDeviceToDevice,
                           c->stream));
     c->hist_ptop[slot] = ptop;
-    c->hist_head = (slot + 1) % LS_WARM_MAX;
-    c->n_hist = std::min(c->n_hist + 1, LS_WARM_MAX);
+    c->hist_head = (slot + 1) % c->hist_ring;
+    c->n_hist = std::min(c->n_hist + 1, c->hist_ring);
     return 0;
 }
 

@@ -457,6 +457,17 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows(
 
 constexpr int ADR_TJ = 4, ADR_ICH = 8;
 
+static dim3 adr_vec_grid(const ls_ctx* c);
+
+// Stored rows for nk even when the row kernels' block partials fit; the
+// red-black Schur form on them unless switched off.
+void transport_form(const ls_ctx* c, bool* stored, bool* schur) {
+    const dim3 vg = adr_vec_grid(c);
+    const size_t nbm = (size_t)vg.x * vg.y * vg.z;
+    *stored = c->nk % 2 == 0 && 4 * nbm <= 8 * (size_t)MAX_PARTIALS;
+    *schur = *stored && c->P.adr_schur != 0;
+}
+
 static dim3 adr_vec_grid(const ls_ctx* c) {
     return dim3((c->nk / 2 + WAVE - 1) / WAVE, (c->nj + ADR_TJ - 1) / ADR_TJ,
                 (c->ni + ADR_ICH - 1) / ADR_ICH);
@@ -687,6 +698,12 @@ __device__ __forceinline__ SPair spair(uint32_t m, const TGeo& g) {
 // channel).
 constexpr uint32_t TROW_PAD = 544;     // 4352 B
 __device__ __forceinline__ size_t rstride(uint32_t Nh) { return (size_t)Nh + TROW_PAD; }
+
+// Doubles of the stored rows: 7 N for the full-system layout, 14 (N/2 +
+// TROW_PAD) for the colour-split one.
+size_t transport_rows_doubles(const ls_ctx* c) {
+    return std::max(7 * c->N, 14 * (c->N / 2 + TROW_PAD));
+}
 
 __device__ __forceinline__ double srow(const double* __restrict__ rows, uint32_t Nh, int f,
                                        int colour, uint32_t m) {
@@ -1424,26 +1441,14 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     const int batch = 4;
     int restarts = 0;
     bool first = true;
-    // stored rows (nk even, 7 N doubles allocated on first use)
+    // stored rows (nk even) and the Schur form, as the pool was cut for:
+    // the solver then runs on the black unknowns (N/2 each)
     const dim3 vg = adr_vec_grid(c);
     const int nbm = (int)(vg.x * vg.y * vg.z);
-    bool stored = c->nk % 2 == 0 && 4 * (size_t)nbm <= 8 * (size_t)MAX_PARTIALS;
-    if (stored && c->arows == nullptr) {
-        // 7 N doubles for the full-system layout, 14 (N/2 + TROW_PAD) for the
-        // colour-split one
-        const size_t nrow = std::max(7 * c->N, 14 * (c->N / 2 + TROW_PAD));
-        if (hipMalloc((void**)&c->arows, nrow * sizeof(double)) == hipSuccess) {
-            c->allocations.push_back(c->arows);
-        } else {            // out of memory: rebuild rows per matvec instead
-            c->arows = nullptr;
-            (void)hipGetLastError();   // the failed allocation must not fail the next check
-        }
-    }
-    stored = stored && c->arows != nullptr;
-    // the Schur form: the solver runs on the black unknowns (N/2 each); its
-    // extra half-length arrays use the second halves of work vectors whose
-    // first halves hold the half-length BiCGStab vectors
-    const bool tschur = stored && c->P.adr_schur != 0;
+    const bool stored = c->t_stored;
+    const bool tschur = c->t_schur;
+    // every phase using the pool lays its own buffers over the pressure system
+    c->press_assembled = false;
     // system 1 known to be 0 with b = 0 (the idle system of the implicit
     // saturation; acid that has never had a source): the Schur form then
     // builds, starts and recovers system 0 alone
@@ -1455,16 +1460,9 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     }
     const uint32_t Nh = N / 2;
     const int nbh = ew_blocks(Nh);
-    double *xs[2] = {nullptr, nullptr}, *wr[2] = {nullptr, nullptr};
-    double* invdb = nullptr;
+    double *xs[2] = {c->txs[0], c->txs[1]}, *wr[2] = {c->twr[0], c->twr[1]};
+    double* invdb = c->tinvdb;
     if (tschur) {
-        // second halves TROW_PAD beyond N/2 (the allocations carry that much
-        // slack), off the first halves' channel alignment
-        for (int f = 0; f < 2; ++f) {
-            xs[f] = c->tv[f][0] + Nh + TROW_PAD;
-            wr[f] = c->tv[f][2] + Nh + TROW_PAD;
-        }
-        invdb = c->tv[0][3] + Nh + TROW_PAD;
         // transport: conc starts from its linear extrapolation in time
         double* xprev = nullptr;
         if (KIND == 0 && c->P.adr_extrapolate) {
@@ -1788,10 +1786,14 @@ __global__ void k_sat_clamp(double* __restrict__ s, uint32_t N) {
 int phase_saturation_implicit(ls_ctx* c, double dt, ls_step_report* rep) {
     const uint32_t N = (uint32_t)c->N;
     const int nb = ew_blocks(N);
-    // the assembly-time saturation is frozen in the (idle) FP64 CG residual
-    // vector; the second system idles on the zeroed FP64 Ap vector
-    double* sf = c->r64;
-    double* idle = c->ap64;
+    // the assembly-time saturation is frozen in a transport-phase vector of
+    // the pool; the second system idles on another, zeroed
+    double* sf = c->taux[0];
+    double* idle = c->taux[1];
+    if (!sf || !idle) {
+        c->err = "implicit saturation: no pool vectors (set implicit_sat through the parameters)";
+        return -3;
+    }
     LS_HIP(hipMemcpyAsync(sf, c->s, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     LS_HIP(hipMemsetAsync(idle, 0, c->N * sizeof(double), c->stream));
     int iters = 0;

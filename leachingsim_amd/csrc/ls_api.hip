@@ -71,7 +71,138 @@ void report_prof(ls_ctx* c, ls_step_report* r) {
     r->prof_tred_bytes = c->prof_tred_bytes;
 }
 
+// Offsets of the pool's buffers: each starts on a 64 KiB boundary plus the
+// allocation stagger of dalloc, so buffers streamed together do not address
+// the HBM channels in step.
+struct PoolCut {
+    size_t off = 0;
+    int idx = 0;
+    size_t take(size_t bytes) {
+        off = (off + 65535) & ~(size_t)65535;
+        off += (size_t)(idx++ % 16) * LS_ALLOC_STAGGER;
+        const size_t o = off;
+        off += bytes;
+        return o;
+    }
+};
+
+// Pressure scratch vectors the parameters need: the Schur residual pair,
+// the projection basis with its images, the learned solver's FP32 vectors
+// (two per vector) and its iterate backup.
+int pool_pw_count(const ls_params& P) {
+    int n = 2;
+    if (P.warm_start != LS_WARM_PREVIOUS) {
+        const int m = std::max(1, std::min(P.warm_history, LS_WARM_MAX)) +
+                      (P.warm_start == LS_WARM_LEARNED ? 1 : 0);
+        n = std::max(n, 2 * m);
+    }
+    if (P.solver == LS_SOLVER_LEARNED) n = std::max(n, 5);
+    return n;
+}
+
 }  // namespace
+
+namespace ls {
+
+// The phase-scoped pool: the pressure phase's and the transport phase's
+// buffers over the same bytes.  Cut again (and reallocated) when the
+// parameters change the layout; the pool holds nothing between phases.
+int pool_setup(ls_ctx* c) {
+    bool stored = false, schur = false;
+    transport_form(c, &stored, &schur);
+    const int npw = pool_pw_count(c->P);
+    const int key = (stored ? 1 : 0) | (schur ? 2 : 0) | (c->P.implicit_sat ? 4 : 0) | (npw << 3);
+    if (key == c->pool_key && c->pool) return 0;
+    const size_t N = c->N, Nh = N / 2, d = sizeof(double);
+    // pressure phase
+    PoolCut pc;
+    const size_t o_cls = pc.take(N);
+    const size_t o_pD = pc.take(N * d), o_sd = pc.take(N * d), o_bhat = pc.take(N * d);
+    size_t o_c64[3], o_c32[3];
+    for (int a = 0; a < 3; ++a) o_c64[a] = pc.take(N * d);
+    for (int a = 0; a < 3; ++a) o_c32[a] = pc.take(N * sizeof(float));
+    size_t o_v64[4], o_v32[4], o_pw[2 * LS_WARM_MAX + 2];
+    for (int a = 0; a < 4; ++a) o_v64[a] = pc.take(N * d);
+    for (int a = 0; a < 4; ++a) o_v32[a] = pc.take(N * sizeof(float));
+    for (int a = 0; a < npw; ++a) o_pw[a] = pc.take(N * d);
+    // transport phase
+    PoolCut tc;
+    const size_t o_rows = stored ? tc.take(transport_rows_doubles(c) * d) : 0;
+    const size_t o_tb0 = tc.take(N * d), o_tb1 = tc.take(N * d);
+    const size_t nv = schur ? Nh : N;
+    size_t o_tv[2][8], o_xs[2] = {0, 0}, o_wr[2] = {0, 0}, o_invdb = 0, o_tinvd = 0;
+    for (int f = 0; f < 2; ++f)
+        for (int q = 0; q < 8; ++q) o_tv[f][q] = tc.take(nv * d);
+    if (schur) {
+        for (int f = 0; f < 2; ++f) {
+            o_xs[f] = tc.take(Nh * d);
+            o_wr[f] = tc.take(Nh * d);
+        }
+        o_invdb = tc.take(Nh * d);
+    } else {
+        o_tinvd = tc.take(N * d);
+    }
+    size_t o_aux[2] = {0, 0};
+    if (c->P.implicit_sat)
+        for (int f = 0; f < 2; ++f) o_aux[f] = tc.take(N * d);
+    const size_t bytes = std::max(pc.off, tc.off) + 256;
+    if (c->pool) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(c->pool);
+        c->pool = nullptr;
+        c->pool_bytes = 0;
+    }
+    c->pool_key = -1;
+    c->press_assembled = false;
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c->err = std::string("hipMalloc of the phase pool (") + std::to_string(bytes >> 20) +
+                 " MiB): " + hipGetErrorString(e);
+        return -12;
+    }
+    c->pool = q;
+    c->pool_bytes = bytes;
+    c->pool_press_bytes = pc.off;
+    c->pool_trans_bytes = tc.off;
+    c->pool_key = key;
+    char* b = (char*)q;
+    c->cls = (int8_t*)(b + o_cls);
+    c->pD = (double*)(b + o_pD);
+    c->sd = (double*)(b + o_sd);
+    c->bhat = (double*)(b + o_bhat);
+    for (int a = 0; a < 3; ++a) {
+        c->c64[a] = (double*)(b + o_c64[a]);
+        c->c32[a] = (float*)(b + o_c32[a]);
+    }
+    c->x64 = (double*)(b + o_v64[0]);
+    c->r64 = (double*)(b + o_v64[1]);
+    c->p64 = (double*)(b + o_v64[2]);
+    c->ap64 = (double*)(b + o_v64[3]);
+    c->x32 = (float*)(b + o_v32[0]);
+    c->r32 = (float*)(b + o_v32[1]);
+    c->p32 = (float*)(b + o_v32[2]);
+    c->ap32 = (float*)(b + o_v32[3]);
+    c->n_pw = npw;
+    for (int a = 0; a < 2 * LS_WARM_MAX + 2; ++a) c->pw[a] = a < npw ? (double*)(b + o_pw[a]) : nullptr;
+    c->t_stored = stored;
+    c->t_schur = schur;
+    c->arows = stored ? (double*)(b + o_rows) : nullptr;
+    c->tb[0] = (double*)(b + o_tb0);
+    c->tb[1] = (double*)(b + o_tb1);
+    for (int f = 0; f < 2; ++f) {
+        for (int k = 0; k < 8; ++k) c->tv[f][k] = (double*)(b + o_tv[f][k]);
+        c->txs[f] = schur ? (double*)(b + o_xs[f]) : nullptr;
+        c->twr[f] = schur ? (double*)(b + o_wr[f]) : nullptr;
+        c->taux[f] = c->P.implicit_sat ? (double*)(b + o_aux[f]) : nullptr;
+    }
+    c->tinvdb = schur ? (double*)(b + o_invdb) : nullptr;
+    c->tinvd = schur ? nullptr : (double*)(b + o_tinvd);
+    return 0;
+}
+
+}  // namespace ls
 
 extern "C" {
 
@@ -182,27 +313,12 @@ int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes, const ls_pa
         st |= dalloc(c, f, N);
     st |= dalloc(c, &c->perm, N);
     st |= dalloc(c, &c->pc, N);
-    for (int a = 0; a < 3; ++a) {
-        st |= dalloc(c, &c->c64[a], N);
-        st |= dalloc(c, &c->c32[a], N);
-    }
-    st |= dalloc(c, &c->cls, N);
-    st |= dalloc(c, &c->pD, N);
-    st |= dalloc(c, &c->sd, N);
-    st |= dalloc(c, &c->bhat, N);
-    for (double** f : {&c->x64, &c->r64, &c->p64, &c->ap64}) st |= dalloc(c, f, N);
-    for (float** f : {&c->x32, &c->r32, &c->p32, &c->ap32}) st |= dalloc(c, f, N);
-    for (int f = 0; f < 2; ++f) {
-        for (int q = 0; q < 8; ++q) st |= dalloc(c, &c->tv[f][q], N);
-        st |= dalloc(c, &c->tb[f], N);
-    }
-    st |= dalloc(c, &c->tinvd, N);
     st |= dalloc(c, &c->partials, (size_t)MAX_PARTIALS * 8);
     st |= dalloc(c, &c->cg, 1);
     st |= dalloc(c, &c->bicg, 1);
     st |= dalloc(c, &c->ss, 1);
     st |= dalloc(c, &c->fcg, 1);
-    if (st) {
+    if (st || (st = pool_setup(c))) {
         *out = c;
         return -12;
     }
@@ -217,9 +333,8 @@ int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes, const ls_pa
     // a fresh context holds Rock everywhere and zero fields
     (void)hipMemsetAsync(c->img, LS_ROCK, N, c->stream);
     (void)hipMemsetAsync(c->frac, 0, N * sizeof(float), c->stream);
-    for (double* f : {c->inv, c->s, c->p, c->conc, c->acid, c->qx, c->qy, c->qz, c->x64, c->p64})
+    for (double* f : {c->inv, c->s, c->p, c->conc, c->acid, c->qx, c->qy, c->qz})
         (void)hipMemsetAsync(f, 0, N * sizeof(double), c->stream);
-    (void)hipMemsetAsync(c->p32, 0, N * sizeof(float), c->stream);
     (void)hipMemsetAsync(c->cg, 0, sizeof(CgScalars), c->stream);
     (void)hipMemsetAsync(c->bicg, 0, sizeof(BicgScalars), c->stream);
     (void)hipMemsetAsync(c->ss, 0, sizeof(StepScalars), c->stream);
@@ -238,6 +353,7 @@ void ls_destroy(ls_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     learned_free(c);
     for (void* q : c->allocations) (void)hipFree(q);
+    if (c->pool) (void)hipFree(c->pool);
     if (c->h_cg) (void)hipHostFree(c->h_cg);
     if (c->h_bicg) (void)hipHostFree(c->h_bicg);
     if (c->h_ss) (void)hipHostFree(c->h_ss);
@@ -261,8 +377,17 @@ int ls_get_params(const ls_ctx* c, ls_params* out) {
 int ls_set_params(ls_ctx* c, const ls_params* p) {
     if (!c || !p) return -1;
     if (int st = validate_params(c, p)) return st;
+    const ls_params old = c->P;
     c->P = *p;
     c->tprev_valid = false;   // the transport's path may change with the settings
+    if (int st = pool_setup(c)) {
+        // the old layout back (its bytes were freed first)
+        const std::string why = c->err;
+        c->P = old;
+        (void)pool_setup(c);
+        c->err = why;
+        return st;
+    }
     return 0;
 }
 
@@ -347,8 +472,8 @@ int ls_download(ls_ctx* c, int field, void* host, size_t elem_bytes) {
         if (elem_bytes != 8) return -3;
         // derived from the current state, as the reference recomputes them
         // before writing VTK output (src/main.cpp:491-494)
-        if (int st = derived_field(c, field, c->tv[0][0])) return st;
-        LS_HIP(hipMemcpyAsync(host, c->tv[0][0], N * 8, hipMemcpyDeviceToHost, c->stream));
+        if (int st = derived_field(c, field, c->pw[0])) return st;
+        LS_HIP(hipMemcpyAsync(host, c->pw[0], N * 8, hipMemcpyDeviceToHost, c->stream));
         LS_HIP(hipStreamSynchronize(c->stream));
         return 0;
     }
@@ -630,10 +755,10 @@ int ls_set_profiling(ls_ctx* c, int on) {
 int ls_debug_apply_pressure(ls_ctx* c, const double* x, double* y) {
     if (!c || !x || !y) return -1;
     const size_t N = c->N;
-    LS_HIP(hipMemcpyAsync(c->tv[0][0], x, N * 8, hipMemcpyHostToDevice, c->stream));
-    int st = apply_pressure_unscaled(c, c->tv[0][0], c->tv[0][1]);
+    LS_HIP(hipMemcpyAsync(c->pw[0], x, N * 8, hipMemcpyHostToDevice, c->stream));
+    int st = apply_pressure_unscaled(c, c->pw[0], c->pw[1]);
     if (st) return st;
-    LS_HIP(hipMemcpyAsync(y, c->tv[0][1], N * 8, hipMemcpyDeviceToHost, c->stream));
+    LS_HIP(hipMemcpyAsync(y, c->pw[1], N * 8, hipMemcpyDeviceToHost, c->stream));
     LS_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -667,17 +792,18 @@ __global__ void k_copy(const double4* __restrict__ a, double4* __restrict__ b, s
 
 int ls_time_copy(ls_ctx* c, size_t bytes, int reps, double* ms) {
     if (!c || !ms || reps < 1) return -1;
-    // copies between two of the transport work vectors (8 N bytes each)
+    // copies between two pressure scratch vectors of the pool (8 N bytes
+    // each; the assembled pressure system stays)
     size_t cap = c->N * 8;
     if (bytes > cap) bytes = cap;
     size_t n4 = bytes / 32;
     hipEvent_t e0 = c->ev[14], e1 = c->ev[15];
-    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, (const double4*)c->tv[1][0],
-                       (double4*)c->tv[1][1], n4);
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, (const double4*)c->pw[0],
+                       (double4*)c->pw[1], n4);
     LS_HIP(hipEventRecord(e0, c->stream));
     for (int r = 0; r < reps; ++r)
         hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream,
-                           (const double4*)c->tv[1][0], (double4*)c->tv[1][1], n4);
+                           (const double4*)c->pw[0], (double4*)c->pw[1], n4);
     LS_LAUNCH_CHECK();
     LS_HIP(hipEventRecord(e1, c->stream));
     LS_HIP(hipEventSynchronize(e1));

@@ -90,6 +90,21 @@ struct ls_ctx {
            *b_acid = nullptr, *b_qx = nullptr, *b_qy = nullptr, *b_qz = nullptr;
     // --- properties ---
     double *perm = nullptr, *pc = nullptr;
+    // --- phase-scoped pool ---
+    // One allocation holds either the pressure phase's buffers (system, CG
+    // vectors, scratch pw) or the transport phase's (rows, right-hand sides,
+    // BiCGStab vectors), laid over the same bytes: neither phase reads what
+    // the other leaves, and every phase that uses the pool starts by writing
+    // what it reads.  A transport or implicit-saturation solve therefore
+    // ends press_assembled.
+    void* pool = nullptr;
+    size_t pool_bytes = 0;
+    size_t pool_press_bytes = 0, pool_trans_bytes = 0;
+    int pool_key = -1;          // layout the pool was cut for
+    int n_pw = 0;               // pressure scratch vectors (N doubles each)
+    double* pw[2 * LS_WARM_MAX + 2] = {};
+    // transport form of the layout: stored rows (nk even), red-black Schur
+    bool t_stored = false, t_schur = false;
     // --- pressure system ---
     int8_t* cls = nullptr;                           // 0 Dirichlet, 1 solved, 2 floored
     double* pD = nullptr;                            // Dirichlet values
@@ -101,10 +116,17 @@ struct ls_ctx {
     double *x64 = nullptr, *r64 = nullptr, *p64 = nullptr, *ap64 = nullptr;
     float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr;
     // --- transport (two right-hand sides) ---
-    double* tv[2][8];   // r, r0, p, v, s, t, ph, sh  for conc and acid
-    double* tb[2];      // right-hand sides
-    double* tinvd = nullptr;
-    double* arows = nullptr;   // stored transport rows, 7 x N (allocated on first use)
+    // r, r0, p, v, s, t, ph, sh for conc and acid: N/2 each in the Schur
+    // form (black unknowns), else N
+    double* tv[2][8] = {};
+    double* tb[2] = {};      // right-hand sides (N)
+    double* tinvd = nullptr; // Jacobi inverse diagonal (N; not in the Schur form)
+    // Schur form: black unknowns, red values of the last red pass, 1 / D_b
+    double* txs[2] = {};
+    double* twr[2] = {};
+    double* tinvdb = nullptr;
+    double* taux[2] = {};    // implicit saturation: its source field and idle system (N)
+    double* arows = nullptr; // stored transport rows (nk even)
     // black half of conc at the last transport solve (N/2, allocated on first
     // use), its dt, and whether it belongs to the current state's history
     double* tprev = nullptr;
@@ -148,11 +170,12 @@ struct ls_ctx {
     bool have_prev = false;
     bool sat_valid = true;     // physics check of the last implicit saturation solve
     // warm-start history: the last n_hist pressure solutions (physical p,
-    // FP64), newest at hist[(hist_head + LS_WARM_MAX - 1) % LS_WARM_MAX]
+    // FP64), newest at hist[(hist_head + hist_ring - 1) % hist_ring]
     double* hist[LS_WARM_MAX] = {};
     double hist_ptop[LS_WARM_MAX] = {};   // effective top pressure of each
     int n_hist = 0;
     int hist_head = 0;
+    int hist_ring = 1;   // ring length: the configured warm_history (1..LS_WARM_MAX)
     double* wsum = nullptr;        // projection dot products (device)
     double* h_wsum = nullptr;      // pinned mirror
     bool press_assembled = false;
@@ -173,6 +196,12 @@ struct ls_ctx {
 
 // ---- internal entry points (implemented in the kernel files) ----
 namespace ls {
+// ls_api.hip: (re)cut the phase-scoped pool for the current parameters
+int pool_setup(ls_ctx* c);
+// k_transport.hip: the transport form the parameters and grid select, and
+// the doubles its stored rows take
+void transport_form(const ls_ctx* c, bool* stored, bool* schur);
+size_t transport_rows_doubles(const ls_ctx* c);
 // k_physics.hip
 int init_state(ls_ctx* c, long long* nsul);
 int phase_properties(ls_ctx* c);
