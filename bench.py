@@ -67,14 +67,27 @@ def log(*a):
 _hip = None
 
 
-def device_sync():
-    """hipDeviceSynchronize through the HIP runtime the library links."""
+def _hip_rt():
     global _hip
     import ctypes
     if _hip is None:
         _hip = ctypes.CDLL("libamdhip64.so.7")
-    if _hip.hipDeviceSynchronize() != 0:
+    return _hip
+
+
+def device_sync():
+    """hipDeviceSynchronize through the HIP runtime the library links."""
+    if _hip_rt().hipDeviceSynchronize() != 0:
         raise RuntimeError("hipDeviceSynchronize failed")
+
+
+def device_free_bytes():
+    """(free, total) device memory (hipMemGetInfo)."""
+    import ctypes
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    if _hip_rt().hipMemGetInfo(ctypes.byref(f), ctypes.byref(t)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return f.value, t.value
 
 
 def workload(name: str, size: int | None):
@@ -92,6 +105,12 @@ def workload(name: str, size: int | None):
                 np.save(path, img)
         return img, 2, 0.5, (f"{n}^3 uint16, pore + 3 minerals + 4-voxel fracture, Air j-slabs, "
                              "z-face inlet/outlet, dt 0.5")
+    if name == "fractured1024x768":
+        n = size or 1024
+        img = G.named_geometry(name, size=n)
+        return img, 1, 0.5, (f"{n}x{n}x{3 * n // 4} (x, y, z) uint8 sigma 3 porosity 0.25, two "
+                             "orthogonal 6-voxel fractures, Air j-slabs, z-face inlet/outlet, "
+                             "dt 0.5")
     if name == "smooth338":
         return (G.named_geometry(name), 1, 1.0,
                 "338x338x283 uint8 sigma 3 porosity 0.30, z-face inlet/outlet, dt 1")
@@ -132,6 +151,8 @@ class Run:
             keep_fields=False):
         P = L.default_params(bc_mode=L.BC_ZFACES, solver=self.solver, warm_start=self.warm,
                              **self.params)
+        device_sync()
+        free0, self.device_total = device_free_bytes()
         sim = L.GpuSimulation(img.shape, P, label_bytes)
         sim.read_labels(img)
         sim.init_state()
@@ -166,6 +187,8 @@ class Run:
                     self.fields["frac"] = sim.download(L.F_FRAC)
             if n < 3 or warmup <= n < warmup + steps or n == total - 1 or n % 20 == 0:
                 log_step(self.name, n, r.as_dict(), 1e3 * (t1 - t0))
+        device_sync()
+        self.device_bytes = free0 - device_free_bytes()[0]
         self.copy_gbs = None
         if profile:
             nb = min(img.size * 8, 1 << 30)
@@ -173,6 +196,7 @@ class Run:
             self.copy_gbs = 2 * nb / ms / 1e6
         sim.close()
         self.warmup, self.steps, self.total = warmup, steps, total
+        self.voxels = img.size
         return self
 
     def window(self):
@@ -187,7 +211,8 @@ class Run:
                "warm_relres": [r["press_relres0"] for r in reps],
                "final_relres_max": max(r["press_relres"] for r in reps),
                "fallbacks": sum(r["press_fallback"] for r in reps),
-               "unet_applies": sum(r["press_unet_applies"] for r in reps)}
+               "unet_applies": sum(r["press_unet_applies"] for r in reps),
+               "device_bytes_per_voxel": self.device_bytes / self.voxels}
         # the two solves from a cold start (step 0: p = 0; step 1: the ramp has
         # grown the drive 1e6-fold, so no earlier solution is used)
         out["hard_solves"] = [{"step": n, "ms": 1e3 * self.wall[n],
@@ -359,6 +384,47 @@ def stencil_rooflines(img, label_bytes, reps=20, rounds=5):
     return out
 
 
+def big_run(args, img, lb, dt, desc, head, warm, extra, weights):
+    """The 1024x1024x768 workload on one GPU: the headline configuration
+    alone, steps 0 .. warmup+steps-1 from the initial state (cold solves
+    included when warmup is 0), with the device memory the context holds
+    after the last step."""
+    W, K = args.warmup, args.steps
+    r = Run(head, warm, **extra).run(img, lb, dt, W, K, W + K, profile=False,
+                                     weights=weights if head == L.SOLVER_LEARNED or
+                                     warm == L.WARM_LEARNED else None)
+    reps = r.window()
+    summ = r.summary()
+    out = {
+        "metric": "steps/s at 1024x1024x768, 1e-8 rel residual",
+        "value": K / r.window_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * r.window_s / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head == L.SOLVER_PLAIN64 else (
+            "f32+f64+bf16" if head == L.SOLVER_LEARNED else "f32+f64"),
+        "data": "synthetic",
+        "config": {"workload": desc, "grid": list(img.shape), "voxels": int(img.size),
+                   "solver": SOLVER_NAMES[head], "warm_start": WARM_NAMES[warm],
+                   "pressure_rtol": 1e-8, "transport_rtol": 1e-8,
+                   "window": f"steps {W}..{W + K - 1}",
+                   "press_relres_per_step": [x["press_relres"] for x in reps],
+                   "adr_relres_per_step": [x["adr_relres"] for x in reps],
+                   "press_iters_per_step": summ["press_iters_per_step"],
+                   "step_ms": [1e3 * w for w in r.wall[W:W + K]],
+                   "device_bytes": r.device_bytes,
+                   "device_bytes_per_voxel": r.device_bytes / img.size,
+                   "device_total_bytes": r.device_total,
+                   "labels": {str(k): int((img == k).sum()) for k in range(1, 6)}},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def stencil_only(img, label_bytes, kernel, reps):
     """Assemble the pressure operator of the initial state, then time `reps`
     launches of one pressure kernel and `reps` copies of 1 GiB (HIP events)."""
@@ -491,6 +557,9 @@ def main():
     if head == L.SOLVER_LEARNED or warm == L.WARM_LEARNED or not args.no_learned:
         weights = load_weights(args.weights or WEIGHTS)
     W, K, T = args.warmup, args.steps, args.total
+    if args.workload == "fractured1024x768":
+        big_run(args, img, lb, dt, desc, head, warm, extra, weights)
+        return
     keep = not args.no_baseline
 
     hr = Run(head, warm, **extra).run(img, lb, dt, W, K, T, profile=True,

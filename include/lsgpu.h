@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LSGPU_ABI_VERSION 8
+#define LSGPU_ABI_VERSION 9
 
 /* Labels (src/compiler_opts.h:69-82). */
 enum ls_label {
@@ -326,6 +326,13 @@ int ls_time_stencil(ls_ctx* ctx, int precision, int reps, double* ms_per_launch,
 /* Copy bytes within device memory on the context's stream, timed with HIP
  * events (a copy kernel of known size, for bandwidth calibration). */
 int ls_time_copy(ls_ctx* ctx, size_t bytes, int reps, double* ms_per_launch);
+/* Device memory the context holds now: grid_bytes for arrays that scale with
+ * the grid (state, snapshot, properties, the phase pool, the warm-start
+ * history, the transport's previous solution, the learned solver's grids),
+ * other_bytes for the rest (reduction partials, scalars, network weights).
+ * No counterpart in the reference (its PETSc objects live in host memory). */
+int ls_device_bytes(const ls_ctx* ctx, unsigned long long* grid_bytes,
+                    unsigned long long* other_bytes);
 
 /* ---- Allocation-free entry point.  No counterpart in the reference: it is
  * for callers that cannot allocate device memory (code under HIP graph

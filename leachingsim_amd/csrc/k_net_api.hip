@@ -18,6 +18,7 @@ static int nalloc(UNet* net, T** p, size_t n) {
         return -12;
     }
     net->allocations.push_back(q);
+    net->bytes += n * sizeof(T) + 256;
     *p = (T*)q;
     return 0;
 }

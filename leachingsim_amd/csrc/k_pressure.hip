@@ -2014,6 +2014,7 @@ int learned_setup(ls_ctx* c) {
         c->err = "learned solver: hipMalloc";
         return -12;
     }
+    c->grid_bytes += c->net->bytes + cn * sizeof(float);
     if (c->net_weights.size() != c->net->nw) {
         c->err = "learned solver: weight count mismatch";
         return -3;
@@ -2028,6 +2029,8 @@ int learned_setup(ls_ctx* c) {
 
 void learned_free(ls_ctx* c) {
     if (c->net) {
+        if (c->rc) c->grid_bytes -= c->net->bytes + (size_t)c->net_dims[0] * c->net_dims[1] *
+                                                        c->net_dims[2] * sizeof(float);
         unet_free(c->net);
         delete c->net;
         c->net = nullptr;
@@ -2375,6 +2378,7 @@ static int warm_alloc(ls_ctx* c) {
         return -12;
     }
     c->allocations.push_back(c->wsum);
+    c->other_bytes += 64 * sizeof(double);
     return 0;
 }
 
@@ -2591,6 +2595,7 @@ static int warm_push(ls_ctx* c, double ptop, bool replace) {
                     }
                 (void)hipFree(c->hist[q]);
                 c->hist[q] = nullptr;
+                c->grid_bytes -= c->N * sizeof(double);
             }
         c->hist_ring = ring;
         c->n_hist = 0;
@@ -2611,6 +2616,7 @@ static int warm_push(ls_ctx* c, double ptop, bool replace) {
             return -12;
         }
         c->allocations.push_back(c->hist[slot]);
+        c->grid_bytes += c->N * sizeof(double);
     }
     LS_HIP(hipMemcpyAsync(c->hist[slot], c->p, c->N * sizeof(double), hipMemcpyDeviceToDevice,
                           c->stream));

@@ -1469,6 +1469,7 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             if (c->tprev == nullptr) {
                 if (hipMalloc((void**)&c->tprev, Nh * sizeof(double)) == hipSuccess) {
                     c->allocations.push_back(c->tprev);
+                    c->grid_bytes += Nh * sizeof(double);
                 } else {        // out of memory: start from the old field instead
                     c->tprev = nullptr;
                     (void)hipGetLastError();

@@ -31,6 +31,7 @@ int dalloc(ls_ctx* c, T** p, size_t n) {
     }
     c->allocations.push_back(q);
     *p = (T*)((char*)q + stagger);
+    (n >= c->N ? c->grid_bytes : c->other_bytes) += n * sizeof(T) + 256 + 16 * (size_t)LS_ALLOC_STAGGER;
     return 0;
 }
 
@@ -150,6 +151,7 @@ int pool_setup(ls_ctx* c) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipFree(c->pool);
         c->pool = nullptr;
+        c->grid_bytes -= c->pool_bytes;
         c->pool_bytes = 0;
     }
     c->pool_key = -1;
@@ -164,6 +166,7 @@ int pool_setup(ls_ctx* c) {
     }
     c->pool = q;
     c->pool_bytes = bytes;
+    c->grid_bytes += bytes;
     c->pool_press_bytes = pc.off;
     c->pool_trans_bytes = tc.off;
     c->pool_key = key;
@@ -788,6 +791,14 @@ __global__ void k_copy(const double4* __restrict__ a, double4* __restrict__ b, s
     for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
          v += (size_t)gridDim.x * blockDim.x)
         b[v] = a[v];
+}
+
+int ls_device_bytes(const ls_ctx* c, unsigned long long* grid_bytes,
+                    unsigned long long* other_bytes) {
+    if (!c || !grid_bytes || !other_bytes) return -1;
+    *grid_bytes = c->grid_bytes;
+    *other_bytes = c->other_bytes;
+    return 0;
 }
 
 int ls_time_copy(ls_ctx* c, size_t bytes, int reps, double* ms) {

@@ -14,7 +14,9 @@ constexpr uint8_t AIR = LS_AIR, PORE = LS_PORE, ROCK = LS_ROCK, SUL = LS_SULPHID
 
 constexpr int WAVE = 64;          // CDNA wavefront width
 constexpr int EW_BLOCK = 256;     // elementwise / reduction block size
-constexpr int MAX_PARTIALS = 262144;
+// block partials of one reduction: the vectorised kernels of a
+// 1024 x 1024 x 768 grid launch up to 786432 blocks
+constexpr int MAX_PARTIALS = 1 << 20;
 
 // Tile of the i-marching stencil kernels: 64 threads along k (one wave, one
 // coalesced 512 B / 256 B row), TJ rows along j, each thread marches ICHUNK

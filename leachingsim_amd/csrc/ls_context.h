@@ -192,6 +192,8 @@ struct ls_ctx {
     double prof_tred_bytes = 0.0;  // algorithmic bytes of the sampled red pass
     double prof_schur_bytes = 0.0; // algorithmic bytes of the sampled Schur kernel
     std::vector<void*> allocations;
+    // device bytes held: arrays that scale with the grid, and the rest
+    size_t grid_bytes = 0, other_bytes = 0;
 };
 
 // ---- internal entry points (implemented in the kernel files) ----

@@ -56,6 +56,7 @@ struct UNet {
     double mask_count = 0.0;
     bool trainable = false;
     std::vector<void*> allocations;
+    size_t bytes = 0;      // device bytes of the allocations
 };
 
 namespace ls {

@@ -12,6 +12,8 @@ Generators:
                        an optional planar fracture through the centre.
   reference_channel    the 50^3 test geometries of generate_input.py
                        (closed-form slicing, cases 0 and 1).
+  fractured_binary     smoothed_threshold plus two orthogonal planar
+                       fractures through the centre and Air face slabs.
 """
 from __future__ import annotations
 
@@ -109,6 +111,27 @@ def multiphase(shape, fractions=(0.30, 0.25, 0.15), labels=(ROCK, SULPHIDE, PREC
     return img
 
 
+def fractured_binary(shape, sigma: float = 3.0, porosity: float = 0.25, seed: int = 1,
+                     fracture_width: float = 6.0, air_j_planes: int = 2,
+                     dtype=np.uint8) -> np.ndarray:
+    """Pore / Rock from smoothed_threshold, then two orthogonal planar
+    fractures of `fracture_width` voxels through the centre set to Pore (the
+    first at a seeded orientation, the second with a seeded normal in the
+    plane of the first) and `air_j_planes` planes of Air at each j face, the
+    surface that drives evaporation and precipitation."""
+    img = smoothed_threshold(shape, sigma, porosity, seed, dtype=dtype)
+    n1 = random_normal(seed * 101 + 98)
+    v = random_normal(seed * 101 + 97)
+    n2 = v - (v @ n1) * n1
+    n2 /= np.linalg.norm(n2)
+    for nrm in (n1, n2):
+        img[planar_fracture(shape, fracture_width, nrm)] = PORE
+    if air_j_planes > 0:
+        img[:, :air_j_planes, :] = AIR
+        img[:, -air_j_planes:, :] = AIR
+    return img
+
+
 def reference_channel(case: int = 1, n: int = 50, dtype=np.uint8) -> np.ndarray:
     """The synthetic inputs of generate_input.py (closed form).
 
@@ -132,7 +155,7 @@ def reference_channel(case: int = 1, n: int = 50, dtype=np.uint8) -> np.ndarray:
 
 
 # Named workloads
-WORKLOADS = ("smooth64", "smooth338", "fractured512")
+WORKLOADS = ("smooth64", "smooth338", "fractured512", "fractured1024x768")
 
 
 def named_geometry(name: str, seed: int = 1, size: int | None = None) -> np.ndarray:
@@ -140,7 +163,10 @@ def named_geometry(name: str, seed: int = 1, size: int | None = None) -> np.ndar
     CPU-runnable case); smooth338: 338x338x283 (x, y, z) -> (ni, nj, nk) =
     (283, 338, 338) uint8, sigma 3, porosity 0.30 (README scale);
     fractured512: 512^3 (or size^3) uint16 labels, pore + three minerals
-    (0.30 / 0.25 / 0.15), a 4-voxel planar fracture, Air on the j faces."""
+    (0.30 / 0.25 / 0.15), a 4-voxel planar fracture, Air on the j faces;
+    fractured1024x768: 1024x1024x768 (x, y, z) -> (ni, nj, nk) =
+    (768, 1024, 1024) uint8 (or (3 size / 4, size, size)), sigma 3, porosity
+    0.25, two orthogonal 6-voxel fractures, Air on the j faces."""
     if name == "smooth64":
         return smoothed_threshold((64, 64, 64), 2.0, 0.40, seed)
     if name == "smooth338":
@@ -148,6 +174,10 @@ def named_geometry(name: str, seed: int = 1, size: int | None = None) -> np.ndar
     if name == "fractured512":
         n = size or 512
         return multiphase((n, n, n), sigma=3.0, seed=seed, fracture_width=4.0, air_j_planes=2)
+    if name == "fractured1024x768":
+        n = size or 1024
+        return fractured_binary((3 * n // 4, n, n), sigma=3.0, porosity=0.25, seed=seed,
+                                fracture_width=6.0, air_j_planes=2)
     raise KeyError(name)
 
 

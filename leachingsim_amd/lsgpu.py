@@ -121,6 +121,7 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
     sig = {
         "ls_default_params": (None, [C.POINTER(Params)]),
         "ls_abi_info": (i, [C.POINTER(sz), C.POINTER(sz)]),
+        "ls_device_bytes": (i, [vp, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
         "ls_create": (i, [C.POINTER(vp), i, i, i, i, C.POINTER(Params)]),
         "ls_destroy": (None, [vp]),
         "ls_last_error": (C.c_char_p, [vp]),
@@ -382,6 +383,12 @@ class GpuSimulation:
 
     def set_profiling(self, on: bool):
         self._check(self.lib.ls_set_profiling(self.h, 1 if on else 0))
+
+    def device_bytes(self) -> tuple[int, int]:
+        """(grid-proportional, other) device bytes the context holds."""
+        g, o = C.c_ulonglong(), C.c_ulonglong()
+        self._check(self.lib.ls_device_bytes(self.h, C.byref(g), C.byref(o)))
+        return g.value, o.value
 
     def time_copy(self, nbytes: int, reps: int = 20) -> float:
         ms = C.c_double()

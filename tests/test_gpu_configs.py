@@ -96,3 +96,30 @@ def test_smooth338_full_size_properties():
     print(f"  mixed vs plain FP64: |dp|/|p| {err:.2e}")
     # two solves stopped at 1e-8 differ by up to cond * 1e-8
     assert err <= 1e-5
+
+
+def test_device_bytes_per_voxel_of_the_1024x1024x768_class():
+    """The 1024x1024x768 workload's class at 256x256x192 with the mixed
+    solver and the projected warm start: after three steps the context holds
+    at most 330 B per voxel in grid-sized arrays (state, snapshot,
+    properties, the phase pool, history), so the full grid (805M voxels)
+    fits one 288 GiB MI355X with the runtime's own use beside it."""
+    img = G.named_geometry("fractured1024x768", size=256)
+    assert img.shape == (192, 256, 256) and img.dtype == np.uint8
+    P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED,
+                         warm_start=L.WARM_PROJECTED)
+    g = L.GpuSimulation(img.shape, P)
+    g.read_labels(img)
+    g.init_state()
+    print("fractured1024x768 class at 256x256x192", flush=True)
+    t = 0.0
+    for step in range(3):
+        r = g.step(t, 0.5)
+        assert r.status == 0 and r.press_relres <= 1e-8 and r.adr_relres <= 1e-8, step
+        t += 0.5
+    grid, other = g.device_bytes()
+    per_voxel = grid / img.size
+    print(f"  grid bytes {grid / 2**30:.2f} GiB = {per_voxel:.1f} B/voxel, other "
+          f"{other / 2**20:.1f} MiB")
+    assert 300.0 <= per_voxel <= 330.0
+    g.close()
