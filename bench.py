@@ -57,7 +57,7 @@ SOLVER_NAMES = {L.SOLVER_PLAIN64: "plain_fp64_pcg", L.SOLVER_MIXED: "mixed_fp32_
                 L.SOLVER_LEARNED: "learned_fcg_mixed"}
 WARM_NAMES = {L.WARM_PREVIOUS: "previous", L.WARM_PROJECTED: "projected",
               L.WARM_LEARNED: "projected+network"}
-TRAFFIC_FILE = ROOT / "profiles" / "pressure_stencil_traffic.json"
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_fractured512.json"
 
 
 def log(*a):
@@ -349,7 +349,8 @@ def cpu_baseline(img, label_bytes, dt, crop: int, warmup: int, nsteps: int):
 
 KERNELS = {"stencil64": (64, "k_stencil_vec<double>"), "stencil32": (32, "k_stencil_vec<float>"),
            "rb_gather": (1, "k_rb_vec<1>"), "rb_apply": (2, "k_rb_vec<2>"),
-           "schur": (3, "k_schur_apply<16, 4>")}
+           "schur": (3, "k_schur_apply<16, 4>"), "transport_red": (4, "k_tschur_red_v<1, 0>"),
+           "transport_black": (5, "k_tschur_black_v<0, 1>")}
 
 
 def stencil_rooflines(img, label_bytes, reps=20, rounds=5):
@@ -425,16 +426,29 @@ def big_run(args, img, lb, dt, desc, head, warm, extra, weights):
     print(json.dumps(out), flush=True)
 
 
-def stencil_only(img, label_bytes, kernel, reps):
-    """Assemble the pressure operator of the initial state, then time `reps`
-    launches of one pressure kernel and `reps` copies of 1 GiB (HIP events)."""
+def stencil_only(img, label_bytes, kernel, reps, dt=0.5):
+    """Assemble the pressure operator of the initial state (for a transport
+    pass: make three coupled steps of the headline configuration, which
+    leave the conc system's rows in place), then time `reps` launches of the
+    kernel and `reps` copies of 1 GiB (HIP events)."""
     precision, kname = KERNELS[kernel]
-    P = L.default_params(bc_mode=L.BC_ZFACES, press_maxit=1)
-    sim = L.GpuSimulation(img.shape, P, label_bytes)
-    sim.read_labels(img)
-    sim.init_state()
-    sim.phase(L.PH_PROPERTIES)
-    sim.phase(L.PH_PRESSURE, 1.0)      # assembles; the 1-iteration solve is irrelevant
+    if precision in (L.TIME_TRED, L.TIME_TBLACK):
+        P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED,
+                             warm_start=L.WARM_PROJECTED)
+        sim = L.GpuSimulation(img.shape, P, label_bytes)
+        sim.read_labels(img)
+        sim.init_state()
+        for n in range(3):
+            r = sim.step(n * dt, dt)
+            if r.status != 0:
+                raise RuntimeError(f"step {n} failed: {sim_error(sim)}")
+    else:
+        P = L.default_params(bc_mode=L.BC_ZFACES, press_maxit=1)
+        sim = L.GpuSimulation(img.shape, P, label_bytes)
+        sim.read_labels(img)
+        sim.init_state()
+        sim.phase(L.PH_PROPERTIES)
+        sim.phase(L.PH_PRESSURE, 1.0)      # assembles; the 1-iteration solve is irrelevant
     ms, by = sim.time_stencil(precision, reps)
     nb = min(img.size * 8, 1 << 30)
     cms = sim.time_copy(nb, reps)
@@ -543,7 +557,7 @@ def main():
     log(f"workload {desc}: generated in {time.perf_counter() - t0:.1f} s; labels "
         + ", ".join(f"{k}:{int((img == k).sum())}" for k in range(1, 6)))
     if args.mode == "stencil":
-        stencil_only(img, lb, args.kernel, args.reps)
+        stencil_only(img, lb, args.kernel, args.reps, dt)
         return
     head = {"plain": L.SOLVER_PLAIN64, "mixed": L.SOLVER_MIXED,
             "learned": L.SOLVER_LEARNED}[args.solver]

@@ -43,6 +43,20 @@ __device__ __forceinline__ bool nbr(const Geo& g, int i, int j, int k, uint32_t 
     }
 }
 
+// All six neighbour indices of v at once, in the reference order; an index
+// outside the grid is replaced by v itself (in bounds, never used: ok[f]
+// is false), so that a kernel can issue every neighbour load before it
+// looks at any of them.
+__device__ __forceinline__ void nbr6(const Geo& g, int i, int j, int k, uint32_t v,
+                                     uint32_t n[6], bool ok[6]) {
+    ok[0] = i + 1 < g.ni; n[0] = ok[0] ? v + g.si : v;
+    ok[1] = i > 0;        n[1] = ok[1] ? v - g.si : v;
+    ok[2] = j + 1 < g.nj; n[2] = ok[2] ? v + g.sj : v;
+    ok[3] = j > 0;        n[3] = ok[3] ? v - g.sj : v;
+    ok[4] = k + 1 < g.nk; n[4] = ok[4] ? v + 1 : v;
+    ok[5] = k > 0;        n[5] = ok[5] ? v - 1 : v;
+}
+
 static Geo geo_of(const ls_ctx* c) {
     Geo g;
     g.ni = c->ni;
@@ -280,14 +294,24 @@ __global__ void k_press_diag(const uint8_t* __restrict__ img, const double* __re
     if (k >= g.nk || j >= g.nj) return;
     const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
     const uint8_t tv = img[v];
+    uint32_t n[6];
+    bool ok[6];
+    nbr6(g, i, j, k, v, n, ok);
+    // every neighbour's label (and for a Pore voxel its permeability) is
+    // loaded before any is used: one memory round trip per wave
+    uint8_t tn[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) tn[f] = img[n[f]];
     if (tv == PORE) {
+        double kn[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) kn[f] = perm[n[f]];
         const double pv = perm[v];
         double diagonal_term = 0.0;
+#pragma unroll
         for (int f = 0; f < 6; ++f) {
-            uint32_t n;
-            if (!nbr(g, i, j, k, v, f, n)) continue;
-            uint8_t tn = img[n];
-            double K_neighbor = (tn == AIR) ? 1.0e-9 : perm[n];
+            if (!ok[f]) continue;
+            const double K_neighbor = (tn[f] == AIR) ? 1.0e-9 : kn[f];
             diagonal_term -= pore_link(pv, K_neighbor, viscosity(tv));
         }
         // the virtual inlet / outlet planes of the z-face boundary
@@ -302,10 +326,9 @@ __global__ void k_press_diag(const uint8_t* __restrict__ img, const double* __re
         sd[v] = 0.0;
         double val = 0.0;
         if (tv == AIR) {  // surface Air: any in-grid neighbour is not Air (:696-711)
-            for (int f = 0; f < 6; ++f) {
-                uint32_t n;
-                if (nbr(g, i, j, k, v, f, n) && img[n] != AIR) { val = ptop; break; }
-            }
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+                if (ok[f] && tn[f] != AIR) val = ptop;
         }
         pD[v] = val;
     }
@@ -335,27 +358,42 @@ __global__ void k_press_rhs_scale(const uint8_t* __restrict__ img, const double*
         fx[v] = 0.0f; fy[v] = 0.0f; fz[v] = 0.0f;
         return;
     }
+    uint32_t n[6];
+    bool ok[6];
+    nbr6(g, i, j, k, v, n, ok);
+    // all neighbour data first (one memory round trip), then the rows
+    uint8_t tn[6];
+    int8_t cn[6];
+    double kn[6], dn[6], sn[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        tn[f] = img[n[f]];
+        cn[f] = cls[n[f]];
+        kn[f] = perm[n[f]];
+        dn[f] = pD[n[f]];
+        sn[f] = sd[n[f]];
+    }
     const double pv = perm[v];
     const double sv = sd[v];
+    const double pvv = p[v];
     double source_term = 0.0;
-    for (int f = 0; f < 6; ++f) {
-        uint32_t n;
-        if (nbr(g, i, j, k, v, f, n) && img[n] == AIR) source_term -= evap_dx;
-    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+        if (ok[f] && tn[f] == AIR) source_term -= evap_dx;
     double rhs = -source_term;
     double sc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
     for (int f = 0; f < 6; ++f) {
-        uint32_t n;
-        if (!nbr(g, i, j, k, v, f, n)) continue;
-        const int8_t cl = cls[n];
+        if (!ok[f]) continue;
+        const int8_t cl = cn[f];
         if (cl == 1 && (f & 1)) continue;       // a solved -face: the neighbour's +face
-        const double fc = face_cond(PORE, img[n], pv, perm[n]);
-        if (cl != 1) rhs += fc * pD[n];
-        else sc[f >> 1] = fc / (sv * sd[n]);
+        const double fc = face_cond(PORE, tn[f], pv, kn[f]);
+        if (cl != 1) rhs += fc * dn[f];
+        else sc[f >> 1] = fc / (sv * sn[f]);
     }
     if (zfaces && i == 0) rhs += pore_link(pv, 1.0e-9, viscosity(PORE)) * ptop;
     bhat[v] = rhs / sv;
-    x[v] = sv * p[v];  // warm start from the previous pressure, scaled variables
+    x[v] = sv * pvv;  // warm start from the previous pressure, scaled variables
     hx[v] = sc[0]; hy[v] = sc[1]; hz[v] = sc[2];
     fx[v] = (float)sc[0]; fy[v] = (float)sc[1]; fz[v] = (float)sc[2];
 }
@@ -378,6 +416,7 @@ static double effective_top_pressure(double top, double t) {
 }
 
 int assemble_pressure(ls_ctx* c, double t) {
+    c->trans_valid = false;     // the pressure phase lays its buffers over the pool
     Geo g = geo_of(c);
     const double ptop = effective_top_pressure(c->P.top_pressure, t);
     const int zf = c->P.bc_mode == LS_BC_ZFACES;
@@ -448,11 +487,13 @@ int finalize_pressure(ls_ctx* c, double ptop) {
 // Darcy flux (src/Simulation.cpp:866-953); one-sided differences only at the
 // global faces of the single domain.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double grad1(const double* F, uint32_t v, int pos, int ext, uint32_t st,
+// Central difference inside, one-sided at the two ends of the axis; fm, fc,
+// fp are F at pos - 1, pos, pos + 1 (the missing one unused).
+__device__ __forceinline__ double grad1(double fm, double fc, double fp, int pos, int ext,
                                         double dx) {
-    if (pos == 0) return (F[v + st] - F[v]) / dx;
-    if (pos == ext - 1) return (F[v] - F[v - st]) / dx;
-    return (F[v + st] - F[v - st]) / (2.0 * dx);
+    if (pos == 0) return (fp - fc) / dx;
+    if (pos == ext - 1) return (fc - fm) / dx;
+    return (fp - fm) / (2.0 * dx);
 }
 
 __global__ void k_flux(const uint8_t* __restrict__ img, const double* __restrict__ p,
@@ -464,16 +505,28 @@ __global__ void k_flux(const uint8_t* __restrict__ img, const double* __restrict
     const int i = blockIdx.z;
     if (k >= g.nk || j >= g.nj) return;
     const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
-    double gradP_x = grad1(p, v, i, g.ni, g.si, dx);
-    double gradP_y = grad1(p, v, j, g.nj, g.sj, dx);
-    double gradP_z = grad1(p, v, k, g.nk, 1, dx);
-    double gradPc_x = grad1(pcap, v, i, g.ni, g.si, dx);
-    double gradPc_y = grad1(pcap, v, j, g.nj, g.sj, dx);
-    double gradPc_z = grad1(pcap, v, k, g.nk, 1, dx);
+    uint32_t n[6];
+    bool ok[6];
+    nbr6(g, i, j, k, v, n, ok);
+    // all loads first (a missing neighbour reads v itself, unused)
+    double pn[6], cn[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        pn[f] = p[n[f]];
+        cn[f] = pcap[n[f]];
+    }
+    const double pv = p[v], cv = pcap[v], kv = perm[v];
+    const uint8_t tv = img[v];
+    double gradP_x = grad1(pn[1], pv, pn[0], i, g.ni, dx);
+    double gradP_y = grad1(pn[3], pv, pn[2], j, g.nj, dx);
+    double gradP_z = grad1(pn[5], pv, pn[4], k, g.nk, dx);
+    double gradPc_x = grad1(cn[1], cv, cn[0], i, g.ni, dx);
+    double gradPc_y = grad1(cn[3], cv, cn[2], j, g.nj, dx);
+    double gradPc_z = grad1(cn[5], cv, cn[4], k, g.nk, dx);
     gradPc_x = dmax(-mcg, dmin(mcg, gradPc_x));
     gradPc_y = dmax(-mcg, dmin(mcg, gradPc_y));
     gradPc_z = dmax(-mcg, dmin(mcg, gradPc_z));
-    double transmissivity = perm[v] / viscosity(img[v]);
+    double transmissivity = kv / viscosity(tv);
     const double rho_g = 1000.0 * 9.81;
     double liq_grad_x = gradP_x - gradPc_x;
     double liq_grad_y = gradP_y - gradPc_y;
@@ -505,20 +558,30 @@ __device__ __forceinline__ void sat_rates(const uint8_t* img, const double* qx, 
                                           const double* qz, const Geo& g, int i, int j, int k,
                                           uint32_t v, double dx, double evap, double& div_q,
                                           double& evap_sink) {
-    double q_face_xp = (i + 1 < g.ni) ? (qx[v] + qx[v + g.si]) / 2.0 : 0.0;
-    double q_face_xm = (i > 0) ? (qx[v] + qx[v - g.si]) / 2.0 : 0.0;
-    double q_face_yp = (j + 1 < g.nj) ? (qy[v] + qy[v + g.sj]) / 2.0 : 0.0;
-    double q_face_ym = (j > 0) ? (qy[v] + qy[v - g.sj]) / 2.0 : 0.0;
-    double q_face_zp = (k + 1 < g.nk) ? (qz[v] + qz[v + 1]) / 2.0 : 0.0;
-    double q_face_zm = (k > 0) ? (qz[v] + qz[v - 1]) / 2.0 : 0.0;
+    uint32_t n[6];
+    bool ok[6];
+    nbr6(g, i, j, k, v, n, ok);
+    // every load first (neighbour indices outside the grid are v itself)
+    const double qxv = qx[v], qyv = qy[v], qzv = qz[v];
+    const double qn0 = qx[n[0]], qn1 = qx[n[1]], qn2 = qy[n[2]], qn3 = qy[n[3]];
+    const double qn4 = qz[n[4]], qn5 = qz[n[5]];
+    const uint8_t tv = img[v];
+    uint8_t tn[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) tn[f] = img[n[f]];
+    double q_face_xp = ok[0] ? (qxv + qn0) / 2.0 : 0.0;
+    double q_face_xm = ok[1] ? (qxv + qn1) / 2.0 : 0.0;
+    double q_face_yp = ok[2] ? (qyv + qn2) / 2.0 : 0.0;
+    double q_face_ym = ok[3] ? (qyv + qn3) / 2.0 : 0.0;
+    double q_face_zp = ok[4] ? (qzv + qn4) / 2.0 : 0.0;
+    double q_face_zm = ok[5] ? (qzv + qn5) / 2.0 : 0.0;
     div_q = ((q_face_xp - q_face_xm) + (q_face_yp - q_face_ym) + (q_face_zp - q_face_zm)) / dx;
     evap_sink = 0.0;
-    if (img[v] == PORE) {
+    if (tv == PORE) {
         int air_faces = 0;
-        for (int f = 0; f < 6; ++f) {
-            uint32_t n;
-            if (nbr(g, i, j, k, v, f, n) && img[n] == AIR) air_faces++;
-        }
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            if (ok[f] && tn[f] == AIR) air_faces++;
         evap_sink = (air_faces * evap) / dx;
     }
 }

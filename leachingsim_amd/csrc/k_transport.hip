@@ -1701,6 +1701,7 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         // loop re-enters with the start kernel (rebuild of b skipped)
     }
     const BicgScalars& h = *c->h_bicg;
+    c->trans_valid = tschur && KIND == 0;
     if (iters_out) *iters_out = h.iter[0] > h.iter[1] ? h.iter[0] : h.iter[1];
     if (sys_iters) {
         sys_iters[0] = h.iter[0];
@@ -1712,6 +1713,87 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         *relres_out = r0v > r1v ? r0v : r1v;
     }
     return (h.converged[0] && h.converged[1]) ? 0 : 1;
+}
+
+// Re-arm the solver scalars so that apply passes run: system 0 iterating,
+// system 1 done (timing only; the next solve starts afresh).
+__global__ void k_bicg_rearm(BicgScalars* bs) {
+    bs->done[0] = 0;
+    bs->done[1] = 1;
+    bs->all_done = 0;
+}
+
+// Time one Schur pass of the conc system on the rows and vectors the last
+// transport solve left in the pool: LS_TIME_TRED w = D_r^-1 A_rb p (the
+// red rows' 7 coefficients, p at the black voxels, w at the red ones:
+// 36 B per voxel), LS_TIME_TBLACK y = D_b p - A_br w with the dot of y and
+// a third vector (the black rows' 7 coefficients, p, w, y and the dot's
+// vector: 44 B per voxel).  The launch shapes are bicg_pair's.
+int time_transport_pass(ls_ctx* c, int which, int reps, double* ms, double* bytes) {
+    if (!c->t_schur || !c->trans_valid) {
+        c->err = "no transport system in place (the Schur form's rows are built by a "
+                 "transport solve; a pressure phase reuses their memory)";
+        return -1;
+    }
+    TGeo g;
+    g.ni = c->ni; g.nj = c->nj; g.nk = c->nk;
+    g.sj = (uint32_t)c->nk;
+    g.si = (uint32_t)c->nk * (uint32_t)c->nj;
+    const uint32_t N = (uint32_t)c->N, Nh = N / 2;
+    const int nbh = ew_blocks(Nh);
+    TVGeo tvg;
+    tvg.nhr = (uint32_t)(c->nk / 2);
+    tvg.nrows = (uint32_t)c->ni * (uint32_t)c->nj;
+    tvg.plane = (uint32_t)c->nj * tvg.nhr;
+    tvg.ni = c->ni;
+    tvg.nj = c->nj;
+    const dim3 tvgrid((tvg.nhr / 2 + WAVE - 1) / WAVE,
+                      (tvg.nrows + TSV_TR * TSV_RPW - 1) / (TSV_TR * TSV_RPW));
+    const bool tvec = c->nk % 4 == 0 && tvgrid.y <= 65535 &&
+                      4 * (size_t)tvgrid.x * tvgrid.y <= 8 * (size_t)MAX_PARTIALS;
+    // the iteration's first matvec: p -> (red) w -> (black) v, dot with r0
+    const double* xa = c->tv[0][6];
+    double* w = c->twr[0];
+    double* y = c->tv[0][3];
+    const double* wd = c->tv[0][1];
+    hipLaunchKernelGGL(k_bicg_rearm, dim3(1), dim3(1), 0, c->stream, c->bicg);
+    auto launch = [&]() {
+        if (which == LS_TIME_TRED) {
+            if (tvec)
+                hipLaunchKernelGGL((k_tschur_red_v<1, 0>), tvgrid, dim3(WAVE, TSV_TR), 0,
+                                   c->stream, c->arows, tvg, Nh, xa, nullptr, w, nullptr, c->bicg,
+                                   c->tb[0], c->tb[1], nullptr, nullptr);
+            else
+                hipLaunchKernelGGL((k_tschur_red<1, 0>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream,
+                                   c->arows, g, Nh, xa, nullptr, w, nullptr, c->bicg, c->tb[0],
+                                   c->tb[1], nullptr, nullptr);
+        } else {
+            if (tvec)
+                hipLaunchKernelGGL((k_tschur_black_v<0, 1>), tvgrid, dim3(WAVE, TSV_TR), 0,
+                                   c->stream, c->arows, tvg, Nh, xa, nullptr, w, nullptr, y,
+                                   nullptr, wd, nullptr, c->bicg, c->partials, nullptr, nullptr,
+                                   nullptr, nullptr, nullptr, nullptr, c->tb[0], c->tb[1],
+                                   nullptr, nullptr);
+            else
+                hipLaunchKernelGGL((k_tschur_black<0, 1>), dim3(nbh), dim3(EW_BLOCK), 0,
+                                   c->stream, c->arows, g, Nh, xa, nullptr, w, nullptr, y,
+                                   nullptr, wd, nullptr, c->bicg, c->partials, nullptr, nullptr,
+                                   nullptr, nullptr, nullptr, nullptr, c->tb[0], c->tb[1],
+                                   nullptr, nullptr);
+        }
+    };
+    launch();                  // warm
+    hipEvent_t e0 = c->ev[14], e1 = c->ev[15];
+    LS_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) launch();
+    LS_LAUNCH_CHECK();
+    LS_HIP(hipEventRecord(e1, c->stream));
+    LS_HIP(hipEventSynchronize(e1));
+    float f = 0.0f;
+    LS_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = (double)f / reps;
+    *bytes = (double)c->N * (which == LS_TIME_TRED ? 36.0 : 44.0);
+    return 0;
 }
 
 static AdrConst adr_const(const ls_ctx* c, double dt) {

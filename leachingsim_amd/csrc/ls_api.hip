@@ -156,6 +156,7 @@ int pool_setup(ls_ctx* c) {
     }
     c->pool_key = -1;
     c->press_assembled = false;
+    c->trans_valid = false;
     void* q = nullptr;
     const hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) {
@@ -475,6 +476,7 @@ int ls_download(ls_ctx* c, int field, void* host, size_t elem_bytes) {
         if (elem_bytes != 8) return -3;
         // derived from the current state, as the reference recomputes them
         // before writing VTK output (src/main.cpp:491-494)
+        c->trans_valid = false;
         if (int st = derived_field(c, field, c->pw[0])) return st;
         LS_HIP(hipMemcpyAsync(host, c->pw[0], N * 8, hipMemcpyDeviceToHost, c->stream));
         LS_HIP(hipStreamSynchronize(c->stream));
@@ -685,6 +687,7 @@ int ls_set_network(ls_ctx* c, const float* w, size_t count) {
 }
 
 int ls_debug_learned_pack(ls_ctx* c, const float* r_host, float* out_host, size_t out_count) {
+    if (c) c->trans_valid = false;
     if (!c || !r_host || !out_host) return -1;
     if (!c->press_assembled) {
         c->err = "no pressure system assembled";
@@ -758,6 +761,7 @@ int ls_set_profiling(ls_ctx* c, int on) {
 int ls_debug_apply_pressure(ls_ctx* c, const double* x, double* y) {
     if (!c || !x || !y) return -1;
     const size_t N = c->N;
+    c->trans_valid = false;
     LS_HIP(hipMemcpyAsync(c->pw[0], x, N * 8, hipMemcpyHostToDevice, c->stream));
     int st = apply_pressure_unscaled(c, c->pw[0], c->pw[1]);
     if (st) return st;
@@ -767,6 +771,7 @@ int ls_debug_apply_pressure(ls_ctx* c, const double* x, double* y) {
 }
 
 int ls_debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot) {
+    if (c) c->trans_valid = false;
     if (!c || !p || !q || !dot) return -1;
     if (form != 1 && form != 2 && form != 4) {
         c->err = "ls_debug_schur_apply: form must be 1 (fused), 2 (two passes) or 4 (two "
@@ -778,12 +783,15 @@ int ls_debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* 
 
 int ls_time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) {
     if (!c || !ms || !bytes || reps < 1) return -1;
+    if (precision == LS_TIME_TRED || precision == LS_TIME_TBLACK)
+        return time_transport_pass(c, precision, reps, ms, bytes);
     if (precision != 32 && precision != 64 && precision != LS_TIME_RB_GATHER &&
         precision != LS_TIME_RB_APPLY && precision != LS_TIME_SCHUR) {
-        c->err = "ls_time_stencil: precision must be 64, 32, LS_TIME_RB_GATHER, LS_TIME_RB_APPLY "
-                 "or LS_TIME_SCHUR";
+        c->err = "ls_time_stencil: precision must be 64, 32, LS_TIME_RB_GATHER, LS_TIME_RB_APPLY, "
+                 "LS_TIME_SCHUR, LS_TIME_TRED or LS_TIME_TBLACK";
         return -3;
     }
+    c->trans_valid = false;     // pressure scratch of the pool written
     return time_stencil(c, precision, reps, ms, bytes);
 }
 
@@ -808,6 +816,7 @@ int ls_time_copy(ls_ctx* c, size_t bytes, int reps, double* ms) {
     size_t cap = c->N * 8;
     if (bytes > cap) bytes = cap;
     size_t n4 = bytes / 32;
+    c->trans_valid = false;
     hipEvent_t e0 = c->ev[14], e1 = c->ev[15];
     hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, (const double4*)c->pw[0],
                        (double4*)c->pw[1], n4);

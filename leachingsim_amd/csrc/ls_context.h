@@ -105,6 +105,9 @@ struct ls_ctx {
     double* pw[2 * LS_WARM_MAX + 2] = {};
     // transport form of the layout: stored rows (nk even), red-black Schur
     bool t_stored = false, t_schur = false;
+    // the pool holds the rows and vectors of the last transport solve (no
+    // pressure-phase use since)
+    bool trans_valid = false;
     // --- pressure system ---
     int8_t* cls = nullptr;                           // 0 Dirichlet, 1 solved, 2 floored
     double* pD = nullptr;                            // Dirichlet values
@@ -228,6 +231,7 @@ void learned_free(ls_ctx* c);
 int learned_pack_debug(ls_ctx* c, const float* r_dev, float* out_dev);
 // k_transport.hip
 int phase_transport(ls_ctx* c, double dt, ls_step_report* rep);
+int time_transport_pass(ls_ctx* c, int which, int reps, double* ms, double* bytes);
 // k_leach.hip
 int phase_leaching(ls_ctx* c, double dt, ls_step_report* rep);
 }  // namespace ls

@@ -22,6 +22,7 @@ AIR, PORE, ROCK, SULPHIDE, PRECIPITATE = 1, 2, 3, 4, 5
 SOLVER_PLAIN64, SOLVER_MIXED, SOLVER_LEARNED = 0, 1, 2
 BC_REFERENCE, BC_ZFACES = 0, 1
 WARM_PREVIOUS, WARM_PROJECTED, WARM_LEARNED = 0, 1, 2
+TIME_RB_GATHER, TIME_RB_APPLY, TIME_SCHUR, TIME_TRED, TIME_TBLACK = 1, 2, 3, 4, 5
 WARM_MAX = 8
 
 (F_IMG, F_FRAC, F_INVENTORY, F_SATURATION, F_PRESSURE, F_CONC, F_ACID, F_QX, F_QY, F_QZ,
@@ -333,6 +334,10 @@ class GpuSimulation:
         return y
 
     def time_stencil(self, precision: int = 64, reps: int = 20):
+        """ms per launch and algorithmic bytes per launch of one kernel:
+        precision 64 / 32 (the stencil), TIME_RB_GATHER, TIME_RB_APPLY,
+        TIME_SCHUR (pressure), TIME_TRED, TIME_TBLACK (the transport's
+        Schur passes on the rows of the last transport solve)."""
         ms, by = C.c_double(), C.c_double()
         self._check(self.lib.ls_time_stencil(self.h, precision, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
