@@ -602,15 +602,15 @@ def main():
         runs[learned.name] = learned
 
     # roofline: the kernel with the largest share of the timed window, the
-    # transport's red Schur pass (k_tschur_red, 36 B per voxel of the grid
-    # for one system: the red rows' 7 coefficients and p, w at half length),
+    # transport's red Schur pass (k_tschur_red, 32 B per voxel of the grid
+    # for one system: the red rows' 6 coefficients and p, w at half length),
     # timed live with HIP events on the context's stream; the pressure
     # kernels that dominate the cold solves beside it
     reps = hr.window()
     rl = {}
     n, ms = sampled(reps, "prof_tred")
     if n:
-        sysn = "1" if reps[-1]["prof_tred_bytes"] == img.size * 36 else "3"
+        sysn = "1" if reps[-1]["prof_tred_bytes"] == img.size * 32 else "3"
         rl["transport_red_pass"] = kernel_roofline(
             f"k_tschur_red_v<{sysn}, 0>" if img.shape[2] % 4 == 0 else f"k_tschur_red<{sysn}, 0>",
             reps[-1]["prof_tred_bytes"], ms, n)
@@ -627,7 +627,8 @@ def main():
             else None)
     tr = load_traffic() or {}
     t = tr.get("transport_red", {})
-    if "transport_red_pass" in rl and t.get("grid") == list(img.shape):
+    if "transport_red_pass" in rl and t.get("grid") == list(img.shape) and \
+            t.get("algorithmic_bytes") == rl["transport_red_pass"]["bytes_per_launch"]:
         rl["transport_red_pass"]["traffic"] = t.get("bytes_per_launch")
     roof = rl.get("transport_red_pass") or rl.get("pressure_schur_apply_cold_solves") or \
         kernel_roofline("none", 0.0, 0.0, 0)

@@ -323,9 +323,9 @@ int ls_debug_schur_apply(ls_ctx* ctx, int form, const float* p, float* q, double
 #define LS_TIME_SCHUR 3   /* fused q = S p on compacted vectors (nk % 8 == 0) */
 /* The transport's Schur passes on the rows the last transport solve left
  * (run ls_step or the transport phase first; the pressure phase reuses those
- * bytes): LS_TIME_TRED the red pass w = D_r^-1 A_rb p (36 B per voxel of
- * rows, p and w), LS_TIME_TBLACK the black pass y = D_b p - A_br w with its
- * dot (44 B per voxel), both for the conc system alone. */
+ * bytes): LS_TIME_TRED the red pass w = R' p, R' = D_r^-1 A_rb D_b^-1 (32 B
+ * per voxel of rows, p and w), LS_TIME_TBLACK the black pass z = p - A_br w
+ * with its dot (40 B per voxel), both for the conc system alone. */
 #define LS_TIME_TRED 4
 #define LS_TIME_TBLACK 5
 int ls_time_stencil(ls_ctx* ctx, int precision, int reps, double* ms_per_launch,

@@ -324,7 +324,9 @@ __global__ void k_bicg_init(const double* __restrict__ partials, int n, BicgScal
     }
 }
 
-// p = r + beta (p - omega v); ph = p / diag.  beta from the last rho update.
+// p = r + beta (p - omega v); with PRE (Jacobi outside the operator) also
+// ph = p / diag.  beta from the last rho update.
+template <bool PRE>
 __global__ void k_bicg_p(double* __restrict__ p0, double* __restrict__ p1,
                          const double* __restrict__ r0_, const double* __restrict__ r1_,
                          const double* __restrict__ v0, const double* __restrict__ v1,
@@ -335,16 +337,16 @@ __global__ void k_bicg_p(double* __restrict__ p0, double* __restrict__ p1,
     const bool a0 = !bs->done[0], a1 = !bs->done[1];
     const double be0 = bs->beta[0], om0 = bs->omega[0], be1 = bs->beta[1], om1 = bs->omega[1];
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
-        const double id = invd[v];
+        const double id = PRE ? invd[v] : 1.0;
         if (a0) {
             double pv = r0_[v] + be0 * (p0[v] - om0 * v0[v]);
             p0[v] = pv;
-            ph0[v] = pv * id;
+            if (PRE) ph0[v] = pv * id;
         }
         if (a1) {
             double pv = r1_[v] + be1 * (p1[v] - om1 * v1[v]);
             p1[v] = pv;
-            ph1[v] = pv * id;
+            if (PRE) ph1[v] = pv * id;
         }
     }
 }
@@ -666,6 +668,14 @@ __global__ __launch_bounds__(WAVE * ADR_TJ) void k_adr_matvec_rows(
 // voxels and r_S = b_b - D_b x - A_br u at the black ones; the final one
 // also writes u and x into the fields.
 //
+// The Jacobi preconditioner is folded into the operator: right-preconditioned
+// BiCGStab on S is BiCGStab on S' = S D_b^-1 = I - A_br R' with
+// R' = D_r^-1 A_rb D_b^-1, in the unknown y = D_b x_b (the same iterates in
+// exact arithmetic).  The first start pass rescales the red rows in place to
+// R' (and their diagonal slot to 1 / d_r), so an apply reads six
+// coefficients per colour and no diagonal, and the solver keeps no
+// preconditioned copies of its direction vectors.
+//
 // Layout (nk even): voxel v of colour c is stored at v >> 1 of its colour's
 // half-length arrays; the pair (2m, 2m + 1) holds one voxel of each colour.
 // Rows: 7 coefficients (diagonal, then faces +i, -i, +j, -j, +k, -k) x 2
@@ -711,12 +721,11 @@ __device__ __forceinline__ double srow(const double* __restrict__ rows, uint32_t
 }
 
 // The stored rows, colour-split, with b (full layout) as k_adr_rows; at the
-// black voxels also the solver's Jacobi inverse diagonal 1 / D_b and its
-// start.  one: system 1 is known to be 0 with b = 0 (its b and x are not
-// touched).  With xprev (the black part of system 0's field at the previous
-// transport solve) system 0 starts from the linear extrapolation
-// x + ratio (x - xprev), ratio = dt / dt_prev (extrap), and xprev takes the
-// current field for the next solve.
+// black voxels also the solver's start y = D_b x.  one: system 1 is known to
+// be 0 with b = 0 (its b and x are not touched).  With xprev (the black part
+// of system 0's field at the previous transport solve) system 0 starts from
+// the linear extrapolation x + ratio (x - xprev), ratio = dt / dt_prev
+// (extrap), and xprev takes the current field for the next solve.
 // Transport (KIND 0) also zeroes the flux at Rock and Precipitate voxels
 // here, as the reference's assembly does (src/Simulation.cpp:1134-1136,
 // 1147-1149): no row reads the flux of such a voxel (a Rock neighbour is an
@@ -727,9 +736,8 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
     const uint8_t* __restrict__ img, double* qx, double* qy, double* qz,
     const double* __restrict__ sf, TGeo g, uint32_t N, AdrConst a,
     double* __restrict__ rows, const double* __restrict__ x0, const double* __restrict__ x1,
-    double* __restrict__ b0, double* __restrict__ b1, int one, double* __restrict__ invdb,
-    double* __restrict__ xs0, double* __restrict__ xs1, double* __restrict__ xprev, int extrap,
-    double ratio) {
+    double* __restrict__ b0, double* __restrict__ b1, int one, double* __restrict__ xs0,
+    double* __restrict__ xs1, double* __restrict__ xprev, int extrap, double ratio) {
     const uint32_t Nh = N / 2;
     const double* x1r = one ? x0 : x1;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
@@ -770,51 +778,64 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
             rows[((size_t)(f + 1) * 2 + colour) * rstride(Nh) + m] =
                 (R.mask & (1 << f)) ? R.off[f] : 0.0;
         if (colour == 0) {
-            invdb[m] = 1.0 / R.diag;
+            double xstart = xa;
             if (xprev != nullptr) {
-                xs0[m] = extrap ? xa + ratio * (xa - xprev[m]) : xa;
+                if (extrap) xstart = xa + ratio * (xa - xprev[m]);
                 xprev[m] = xa;
-            } else {
-                xs0[m] = xa;
             }
-            if (!one) xs1[m] = xb;
+            xs0[m] = R.diag * xstart;
+            if (!one) xs1[m] = R.diag * xb;
         }
     }
 }
 
-// The red pass for the systems in SYS.  MODE 0 (apply): w = D_r^-1 A_rb p.
-// MODE 1 (start): w = D_r^-1 (b_r - A_rb x), the red unknowns implied by the
-// black ones x; MODE 2 (final): the same, also written into the full field
-// xf at the red voxels.
+// The red pass for the systems in SYS, on the scaled unknowns y = D_b x
+// (the red rows hold R' = D_r^-1 A_rb D_b^-1 and 1 / d_r once the first start
+// pass has rescaled them).  MODE 0 (apply): w = R' p.  MODE 1 (first start):
+// rescales the red rows of the unscaled assembly (a_f -> a_f / (d_r d_f)
+// with d_f the black neighbour's diagonal, d_r -> 1 / d_r), then as MODE 2.
+// MODE 2 (start): w = D_r^-1 b_r - R' y = D_r^-1 (b_r - A_rb x), the red
+// unknowns implied by the black ones, also written into the full field xf
+// at the red voxels (every start after the first).
 template <int SYS, int MODE>
 __global__ __launch_bounds__(EW_BLOCK) void k_tschur_red(
-    const double* __restrict__ rows, TGeo g, uint32_t Nh, const double* __restrict__ p0,
+    double* __restrict__ rows, TGeo g, uint32_t Nh, const double* __restrict__ p0,
     const double* __restrict__ p1, double* __restrict__ w0, double* __restrict__ w1,
     const BicgScalars* __restrict__ bs, const double* __restrict__ bf0,
     const double* __restrict__ bf1, double* __restrict__ xf0, double* __restrict__ xf1) {
     if (MODE == 0 && bs->all_done) return;
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < Nh; m += gridDim.x * blockDim.x) {
         const SPair P = spair(m, g);
+        double inv;
+        if (MODE == 1) {
+            inv = 1.0 / srow(rows, Nh, 0, 1, m);
+            rows[((size_t)0 * 2 + 1) * rstride(Nh) + m] = inv;
+        } else {
+            inv = srow(rows, Nh, 0, 1, m);
+        }
         double a0 = 0.0, a1 = 0.0;
         for (int f = 0; f < 6; ++f) {
             uint32_t n;
             if (!tnbr(g, P.i, P.j, P.kr, P.vr, f, n)) continue;
-            const double o = srow(rows, Nh, f + 1, 1, m);
+            double o = srow(rows, Nh, f + 1, 1, m);
+            if (MODE == 1) {
+                o = (o * inv) / srow(rows, Nh, 0, 0, n >> 1);
+                rows[((size_t)(f + 1) * 2 + 1) * rstride(Nh) + m] = o;
+            }
             if (SYS & 1) a0 += o * p0[n >> 1];
             if (SYS & 2) a1 += o * p1[n >> 1];
         }
-        const double d = srow(rows, Nh, 0, 1, m);
         if (MODE == 0) {
-            if (SYS & 1) w0[m] = a0 / d;
-            if (SYS & 2) w1[m] = a1 / d;
+            if (SYS & 1) w0[m] = a0;
+            if (SYS & 2) w1[m] = a1;
         } else {
             if (SYS & 1) {
-                const double u = (bf0[P.vr] - a0) / d;
+                const double u = bf0[P.vr] * inv - a0;
                 w0[m] = u;
                 if (MODE == 2) xf0[P.vr] = u;
             }
             if (SYS & 2) {
-                const double u = (bf1[P.vr] - a1) / d;
+                const double u = bf1[P.vr] * inv - a1;
                 w1[m] = u;
                 if (MODE == 2) xf1[P.vr] = u;
             }
@@ -822,13 +843,14 @@ __global__ __launch_bounds__(EW_BLOCK) void k_tschur_red(
     }
 }
 
-// The black pass.  MODE 0/1 (apply): y = S x = D_b x - A_br w at the black
-// voxels, with partials rh.y (MODE 0) or (y.y, y.s) (MODE 1), w from the red
-// pass in mode 0.  MODE 2 (start): the Schur residual r = b_b - D_b x - A_br u
-// with u from the red pass in mode 1 (r_S = b_S - S x, b_S = b_b -
-// A_br D_r^-1 b_r), rh = r, p = v = 0, partials (b.b over the whole grid, r.r);
-// MODE 3 (final): the same, also writing x into the full field xf at the
-// black voxels.
+// The black pass, on the scaled unknowns y = D_b x.  MODE 0/1 (apply):
+// z = S' y = y - A_br w at the black voxels, with partials rh.z (MODE 0) or
+// (z.z, z.s) (MODE 1), w from the red pass in mode 0.  MODE 2 (start): the
+// Schur residual r = b_b - y - A_br u = b_b - D_b x - A_br u with u from a
+// red start pass (r_S = b_S - S x, b_S = b_b - A_br D_r^-1 b_r), rh = r,
+// p = v = 0, partials (b.b over the whole grid, r.r); MODE 3 (final): the
+// same, also writing x = y / d_b into the full field xf at the black
+// voxels.
 template <int MODE, int SYS>
 __global__ __launch_bounds__(EW_BLOCK) void k_tschur_black(
     const double* __restrict__ rows, TGeo g, uint32_t Nh, const double* __restrict__ x0,
@@ -847,9 +869,8 @@ __global__ __launch_bounds__(EW_BLOCK) void k_tschur_black(
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < Nh; m += gridDim.x * blockDim.x) {
         const SPair P = spair(m, g);
-        const double d = srow(rows, Nh, 0, 0, m);
         const double xa = S0 ? x0[m] : 0.0, xb = S1 ? x1[m] : 0.0;
-        double ya = S0 ? d * xa : 0.0, yb = S1 ? d * xb : 0.0;
+        double ya = xa, yb = xb;
         for (int f = 0; f < 6; ++f) {
             uint32_t n;
             if (!tnbr(g, P.i, P.j, P.kb, P.vb, f, n)) continue;
@@ -864,19 +885,20 @@ __global__ __launch_bounds__(EW_BLOCK) void k_tschur_black(
         }
         if (START) {
             const double ra = S0 ? bf0[P.vb] - ya : 0.0, rb = S1 ? bf1[P.vb] - yb : 0.0;
+            const double d = MODE == 3 ? srow(rows, Nh, 0, 0, m) : 1.0;
             if (S0) {
                 y0[m] = ra;
                 rh0[m] = ra;
                 p0[m] = 0.0;
                 v0[m] = 0.0;
-                if (MODE == 3) xf0[P.vb] = xa;
+                if (MODE == 3) xf0[P.vb] = xa / d;
             }
             if (S1) {
                 y1[m] = rb;
                 rh1[m] = rb;
                 p1[m] = 0.0;
                 v1[m] = 0.0;
-                if (MODE == 3) xf1[P.vb] = xb;
+                if (MODE == 3) xf1[P.vb] = xb / d;
             }
             // ||b|| of the full system: the pair (2m, 2m + 1)
             if (S0) {
@@ -967,11 +989,13 @@ __device__ __forceinline__ void tsv_kn(const double* __restrict__ src, uint32_t 
     }
 }
 
-// Red pass.  MODE 0: w = D_r^-1 A_rb p; MODE 1 and 2: w = D_r^-1 (b_r -
-// A_rb x) (the final black pass writes it into the field).
+// Red pass on the scaled unknowns, the modes of k_tschur_red: MODE 0
+// w = R' p; MODE 1 rescales the red rows to R' and 1 / d_r, then as MODE 2;
+// MODE 2 w = D_r^-1 b_r - R' y (the final black pass writes it into the
+// field).
 template <int SYS, int MODE>
 __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
-    const double* __restrict__ rows, TVGeo g, uint32_t Nh, const double* __restrict__ p0,
+    double* __restrict__ rows, TVGeo g, uint32_t Nh, const double* __restrict__ p0,
     const double* __restrict__ p1, double* __restrict__ w0, double* __restrict__ w1,
     const BicgScalars* __restrict__ bs, const double* __restrict__ bf0,
     const double* __restrict__ bf1, double* __restrict__ xf0, double* __restrict__ xf1) {
@@ -986,19 +1010,43 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const int q = 1 - ((i + j) & 1);
         const uint32_t idx = row * g.nhr + m0;
-        d2 o[6], d = d2{1.0, 1.0};
+        const bool hip_ = i + 1 < g.ni, him = i > 0, hjp = j + 1 < g.nj, hjm = j > 0;
+        d2 o[6], inv = zero;
         for (int f = 0; f < 6; ++f) o[f] = zero;
         d2 a = zero, b = zero;
         d2 c0 = zero, c1 = zero;
         if (ok) {
-            d = ld_once(rows + (size_t)1 * rstride(Nh) + idx);
+            if (MODE != 0) inv = *(const d2*)(rows + (size_t)1 * rstride(Nh) + idx);
 #pragma unroll
             for (int f = 0; f < 6; ++f)
                 o[f] = ld_once(rows + ((size_t)(f + 1) * 2 + 1) * rstride(Nh) + idx);
             if (SYS & 1) c0 = *(const d2*)(p0 + idx);
             if (SYS & 2) c1 = *(const d2*)(p1 + idx);
         }
-        const bool hip_ = i + 1 < g.ni, him = i > 0, hjp = j + 1 < g.nj, hjm = j > 0;
+        if (MODE == 1) {
+            // the black neighbours' diagonals (black slot 0), in the order
+            // of the faces; an absent neighbour has coefficient 0
+            const double* db = rows;
+            const d2 dc = ok ? *(const d2*)(db + idx) : zero;
+            d2 dn[6];
+            dn[0] = (ok && hip_) ? *(const d2*)(db + idx + g.plane) : zero;
+            dn[1] = (ok && him) ? *(const d2*)(db + idx - g.plane) : zero;
+            dn[2] = (ok && hjp) ? *(const d2*)(db + idx + g.nhr) : zero;
+            dn[3] = (ok && hjm) ? *(const d2*)(db + idx - g.nhr) : zero;
+            tsv_kn(db, idx, m0, g.nhr, q, ok, dc, dn[5], dn[4]);
+            inv = d2{1.0 / inv.x, 1.0 / inv.y};
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                o[f].x = dn[f].x != 0.0 ? (o[f].x * inv.x) / dn[f].x : 0.0;
+                o[f].y = dn[f].y != 0.0 ? (o[f].y * inv.y) / dn[f].y : 0.0;
+            }
+            if (ok) {
+                *(d2*)(rows + (size_t)1 * rstride(Nh) + idx) = inv;
+#pragma unroll
+                for (int f = 0; f < 6; ++f)
+                    *(d2*)(rows + ((size_t)(f + 1) * 2 + 1) * rstride(Nh) + idx) = o[f];
+            }
+        }
         if (SYS & 1) {
             d2 ip = (ok && hip_) ? *(const d2*)(p0 + idx + g.plane) : zero;
             d2 im = (ok && him) ? *(const d2*)(p0 + idx - g.plane) : zero;
@@ -1029,27 +1077,27 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
         }
         if (!ok) continue;
         if (MODE == 0) {
-            if (SYS & 1) *(d2*)(w0 + idx) = a / d;
-            if (SYS & 2) *(d2*)(w1 + idx) = b / d;
+            if (SYS & 1) *(d2*)(w0 + idx) = a;
+            if (SYS & 2) *(d2*)(w1 + idx) = b;
         } else {
             // the pair's voxels of both colours are the full-layout quad
             // 2 idx .. 2 idx + 3; the red ones are at 2 idx + q, 2 idx + 2 + q
             if (SYS & 1) {
                 const d2 lo = *(const d2*)(bf0 + 2u * idx), hi = *(const d2*)(bf0 + 2u * idx + 2);
                 const d2 br = q ? d2{lo.y, hi.y} : d2{lo.x, hi.x};
-                *(d2*)(w0 + idx) = (br - a) / d;
+                *(d2*)(w0 + idx) = br * inv - a;
             }
             if (SYS & 2) {
                 const d2 lo = *(const d2*)(bf1 + 2u * idx), hi = *(const d2*)(bf1 + 2u * idx + 2);
                 const d2 br = q ? d2{lo.y, hi.y} : d2{lo.x, hi.x};
-                *(d2*)(w1 + idx) = (br - b) / d;
+                *(d2*)(w1 + idx) = br * inv - b;
             }
         }
     }
 }
 
-// Black pass, the modes of k_tschur_black; the final one (MODE 3) writes the
-// full-layout quad of both colours.
+// Black pass on the scaled unknowns, the modes of k_tschur_black; the final
+// one (MODE 3) writes the full-layout quad of both colours.
 template <int MODE, int SYS>
 __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
     const double* __restrict__ rows, TVGeo g, uint32_t Nh, const double* __restrict__ x0,
@@ -1076,11 +1124,11 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const int q = (i + j) & 1;
         const uint32_t idx = row * g.nhr + m0;
-        d2 o[6], d = zero;
+        d2 o[6], d = d2{1.0, 1.0};
         for (int f = 0; f < 6; ++f) o[f] = zero;
         d2 xa = zero, xb = zero, c0 = zero, c1 = zero;
         if (ok) {
-            d = ld_once(rows + idx);
+            if (MODE == 3) d = *(const d2*)(rows + idx);
 #pragma unroll
             for (int f = 0; f < 6; ++f)
                 o[f] = ld_once(rows + ((size_t)(f + 1) * 2) * rstride(Nh) + idx);
@@ -1102,7 +1150,7 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
             d2 jm = (ok && hjm) ? *(const d2*)(wr0 + idx - g.nhr) : zero;
             d2 km, kp;
             tsv_kn(wr0, idx, m0, g.nhr, q, ok, c0, km, kp);
-            ya = d * xa;
+            ya = xa;
             if (START) {
                 ya = ya + o[0] * ip;
                 ya = ya + o[1] * im;
@@ -1126,7 +1174,7 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
             d2 jm = (ok && hjm) ? *(const d2*)(wr1 + idx - g.nhr) : zero;
             d2 km, kp;
             tsv_kn(wr1, idx, m0, g.nhr, q, ok, c1, km, kp);
-            yb = d * xb;
+            yb = xb;
             if (START) {
                 yb = yb + o[0] * ip;
                 yb = yb + o[1] * im;
@@ -1158,8 +1206,9 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
                 *(d2*)(p0 + idx) = zero;
                 *(d2*)(v0 + idx) = zero;
                 if (MODE == 3) {
-                    *(d2*)(xf0 + 2u * idx) = q ? d2{c0.x, xa.x} : d2{xa.x, c0.x};
-                    *(d2*)(xf0 + 2u * idx + 2) = q ? d2{c0.y, xa.y} : d2{xa.y, c0.y};
+                    const d2 xu = xa / d;
+                    *(d2*)(xf0 + 2u * idx) = q ? d2{c0.x, xu.x} : d2{xu.x, c0.x};
+                    *(d2*)(xf0 + 2u * idx + 2) = q ? d2{c0.y, xu.y} : d2{xu.y, c0.y};
                 }
                 // ||b|| of the full system over the quad
                 s0 += lo.x * lo.x + lo.y * lo.y;
@@ -1176,8 +1225,9 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
                 *(d2*)(p1 + idx) = zero;
                 *(d2*)(v1 + idx) = zero;
                 if (MODE == 3) {
-                    *(d2*)(xf1 + 2u * idx) = q ? d2{c1.x, xb.x} : d2{xb.x, c1.x};
-                    *(d2*)(xf1 + 2u * idx + 2) = q ? d2{c1.y, xb.y} : d2{xb.y, c1.y};
+                    const d2 xu = xb / d;
+                    *(d2*)(xf1 + 2u * idx) = q ? d2{c1.x, xu.x} : d2{xu.x, c1.x};
+                    *(d2*)(xf1 + 2u * idx + 2) = q ? d2{c1.y, xu.y} : d2{xu.y, c1.y};
                 }
                 s1 += lo.x * lo.x + lo.y * lo.y;
                 s1 += hi.x * hi.x + hi.y * hi.y;
@@ -1259,7 +1309,8 @@ __global__ void k_bicg_alpha(const double* __restrict__ partials, int n, BicgSca
     }
 }
 
-// s = r - alpha v; sh = s / diag; partial s.s
+// s = r - alpha v; with PRE sh = s / diag; partial s.s
+template <bool PRE>
 __global__ void k_bicg_s(const double* __restrict__ r0_, const double* __restrict__ r1_,
                          const double* __restrict__ v0, const double* __restrict__ v1,
                          double* __restrict__ s0, double* __restrict__ s1, double* __restrict__ sh0,
@@ -1272,17 +1323,17 @@ __global__ void k_bicg_s(const double* __restrict__ r0_, const double* __restric
     const double al0 = bs->alpha[0], al1 = bs->alpha[1];
     double q0 = 0.0, q1 = 0.0;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
-        const double id = invd[v];
+        const double id = PRE ? invd[v] : 1.0;
         if (a0) {
             double sv = r0_[v] - al0 * v0[v];
             s0[v] = sv;
-            sh0[v] = sv * id;
+            if (PRE) sh0[v] = sv * id;
             q0 += sv * sv;
         }
         if (a1) {
             double sv = r1_[v] - al1 * v1[v];
             s1[v] = sv;
-            sh1[v] = sv * id;
+            if (PRE) sh1[v] = sv * id;
             q1 += sv * sv;
         }
     }
@@ -1322,7 +1373,9 @@ __global__ void k_bicg_omega(const double* __restrict__ partials, int n, BicgSca
         }
 }
 
-// x += alpha ph + omega sh; r = s - omega t; partials r.r, r0.r
+// x += alpha ph + omega sh (without PRE: x += alpha p + omega s); r = s -
+// omega t; partials r.r, r0.r
+template <bool PRE>
 __global__ void k_bicg_x(double* __restrict__ x0, double* __restrict__ x1,
                          double* __restrict__ r0_, double* __restrict__ r1_,
                          const double* __restrict__ ph0, const double* __restrict__ ph1,
@@ -1339,15 +1392,17 @@ __global__ void k_bicg_x(double* __restrict__ x0, double* __restrict__ x1,
     double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
         if (a0) {
-            x0[v] += al0 * ph0[v] + om0 * sh0[v];
-            double rv = s0[v] - om0 * t0[v];
+            const double sv = s0[v];
+            x0[v] += al0 * ph0[v] + om0 * (PRE ? sh0[v] : sv);
+            double rv = sv - om0 * t0[v];
             r0_[v] = rv;
             q0 += rv * rv;
             q2 += rh0[v] * rv;
         }
         if (a1) {
-            x1[v] += al1 * ph1[v] + om1 * sh1[v];
-            double rv = s1[v] - om1 * t1[v];
+            const double sv = s1[v];
+            x1[v] += al1 * ph1[v] + om1 * (PRE ? sh1[v] : sv);
+            double rv = sv - om1 * t1[v];
             r1_[v] = rv;
             q1 += rv * rv;
             q3 += rh1[v] * rv;
@@ -1461,7 +1516,6 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     const uint32_t Nh = N / 2;
     const int nbh = ew_blocks(Nh);
     double *xs[2] = {c->txs[0], c->txs[1]}, *wr[2] = {c->twr[0], c->twr[1]};
-    double* invdb = c->tinvdb;
     if (tschur) {
         // transport: conc starts from its linear extrapolation in time
         double* xprev = nullptr;
@@ -1480,7 +1534,7 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         const bool extrap = xprev != nullptr && c->tprev_valid && c->tprev_dt > 0.0;
         hipLaunchKernelGGL((k_adr_rows_split<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
                            c->img, c->qx, c->qy, c->qz, sf, g, N, a, c->arows, x[0], x[1],
-                           c->tb[0], c->tb[1], one ? 1 : 0, invdb, xs[0], xs[1], xprev,
+                           c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], xprev,
                            extrap ? 1 : 0, extrap ? (1.0 / a.inv_dt) / c->tprev_dt : 0.0);
         if (KIND == 0) {
             c->tprev_valid = xprev != nullptr;
@@ -1493,11 +1547,13 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
                            c->tb[1], c->tinvd);
         LS_LAUNCH_CHECK();
     }
-    // the solver's unknowns, length and Jacobi inverse diagonal
+    // the solver's unknowns and length; the Schur form has the Jacobi
+    // scaling inside its operator (unknowns y = D_b x_b), the full system
+    // applies D^-1 to the directions
     double* xv[2] = {tschur ? xs[0] : x[0], tschur ? xs[1] : x[1]};
     const uint32_t Nv = tschur ? Nh : N;
     const int nbv = tschur ? nbh : nb;
-    const double* invd_v = tschur ? invdb : c->tinvd;
+    const double* invd_v = c->tinvd;
     // the vectorised Schur passes: nk % 4 == 0 (16 B aligned pairs per row)
     // and the row count within the grid's y limit
     TVGeo tvg;
@@ -1544,9 +1600,9 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         if (sample) {
             (void)hipEventRecord(c->ev[17], c->stream);
             red_pending = true;
-            // red rows (7 coefficients of the red voxels), p at the black
+            // red rows (6 coefficients of the red voxels), p at the black
             // voxels and w at the red ones, per system iterated
-            c->prof_tred_bytes = (double)c->N * (28.0 + (S == 3 ? 16.0 : 8.0));
+            c->prof_tred_bytes = (double)c->N * (24.0 + (S == 3 ? 16.0 : 8.0));
         }
         if (tvec)
             hipLaunchKernelGGL((k_tschur_black_v<M, S>), tvgrid, dim3(WAVE, TSV_TR), 0, c->stream,
@@ -1649,25 +1705,46 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             sys = (c->h_bicg->done[0] ? 0 : 1) | (c->h_bicg->done[1] ? 0 : 2);
             if (sys == 0) sys = 3;
             sample_red = c->profiling && tschur;
+            // the directions the operator is applied to: D^-1 p and D^-1 s
+            // (full system), or p and s themselves (Schur form)
+            double* const* pd = tschur ? pp : ph;
+            double* const* sd = tschur ? s : sh;
             for (int it = 0; it < batch; ++it) {
-                hipLaunchKernelGGL(k_bicg_p, dim3(nbv), dim3(EW_BLOCK), 0, c->stream, pp[0], pp[1],
-                                   r[0], r[1], vv[0], vv[1], ph[0], ph[1], invd_v, c->bicg, Nv);
-                const int n0 = matvec(std::integral_constant<int, 0>{}, ph[0], ph[1], vv[0],
+                if (tschur)
+                    hipLaunchKernelGGL(k_bicg_p<false>, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
+                                       pp[0], pp[1], r[0], r[1], vv[0], vv[1], nullptr, nullptr,
+                                       nullptr, c->bicg, Nv);
+                else
+                    hipLaunchKernelGGL(k_bicg_p<true>, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
+                                       pp[0], pp[1], r[0], r[1], vv[0], vv[1], ph[0], ph[1],
+                                       invd_v, c->bicg, Nv);
+                const int n0 = matvec(std::integral_constant<int, 0>{}, pd[0], pd[1], vv[0],
                                       vv[1], rh[0], rh[1]);
                 hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1024), 0, c->stream, c->partials,
                                    n0, c->bicg);
-                hipLaunchKernelGGL(k_bicg_s, dim3(nbv), dim3(EW_BLOCK), 0, c->stream, r[0], r[1],
-                                   vv[0], vv[1], s[0], s[1], sh[0], sh[1], invd_v, c->bicg, Nv,
-                                   c->partials);
+                if (tschur)
+                    hipLaunchKernelGGL(k_bicg_s<false>, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
+                                       r[0], r[1], vv[0], vv[1], s[0], s[1], nullptr, nullptr,
+                                       nullptr, c->bicg, Nv, c->partials);
+                else
+                    hipLaunchKernelGGL(k_bicg_s<true>, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
+                                       r[0], r[1], vv[0], vv[1], s[0], s[1], sh[0], sh[1], invd_v,
+                                       c->bicg, Nv, c->partials);
                 hipLaunchKernelGGL(k_bicg_scheck, dim3(1), dim3(1024), 0, c->stream, c->partials,
                                    nbv, c->bicg);
-                const int n1 = matvec(std::integral_constant<int, 1>{}, sh[0], sh[1], t[0], t[1],
+                const int n1 = matvec(std::integral_constant<int, 1>{}, sd[0], sd[1], t[0], t[1],
                                       s[0], s[1]);
                 hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1024), 0, c->stream, c->partials,
                                    n1, c->bicg);
-                hipLaunchKernelGGL(k_bicg_x, dim3(nbv), dim3(EW_BLOCK), 0, c->stream, xv[0], xv[1],
-                                   r[0], r[1], ph[0], ph[1], sh[0], sh[1], s[0], s[1], t[0], t[1],
-                                   rh[0], rh[1], c->bicg, Nv, c->partials);
+                if (tschur)
+                    hipLaunchKernelGGL(k_bicg_x<false>, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
+                                       xv[0], xv[1], r[0], r[1], pp[0], pp[1], nullptr, nullptr,
+                                       s[0], s[1], t[0], t[1], rh[0], rh[1], c->bicg, Nv,
+                                       c->partials);
+                else
+                    hipLaunchKernelGGL(k_bicg_x<true>, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
+                                       xv[0], xv[1], r[0], r[1], ph[0], ph[1], sh[0], sh[1], s[0],
+                                       s[1], t[0], t[1], rh[0], rh[1], c->bicg, Nv, c->partials);
                 hipLaunchKernelGGL(k_bicg_rho, dim3(1), dim3(1024), 0, c->stream, c->partials, nbv,
                                    c->bicg);
             }
@@ -1724,11 +1801,11 @@ __global__ void k_bicg_rearm(BicgScalars* bs) {
 }
 
 // Time one Schur pass of the conc system on the rows and vectors the last
-// transport solve left in the pool: LS_TIME_TRED w = D_r^-1 A_rb p (the
-// red rows' 7 coefficients, p at the black voxels, w at the red ones:
-// 36 B per voxel), LS_TIME_TBLACK y = D_b p - A_br w with the dot of y and
-// a third vector (the black rows' 7 coefficients, p, w, y and the dot's
-// vector: 44 B per voxel).  The launch shapes are bicg_pair's.
+// transport solve left in the pool: LS_TIME_TRED w = R' p (the red rows' 6
+// rescaled coefficients, p at the black voxels, w at the red ones: 32 B per
+// voxel), LS_TIME_TBLACK z = p - A_br w with the dot of z and a third vector
+// (the black rows' 6 coefficients, p, w, z and the dot's vector: 40 B per
+// voxel).  The launch shapes are bicg_pair's.
 int time_transport_pass(ls_ctx* c, int which, int reps, double* ms, double* bytes) {
     if (!c->t_schur || !c->trans_valid) {
         c->err = "no transport system in place (the Schur form's rows are built by a "
@@ -1792,7 +1869,7 @@ int time_transport_pass(ls_ctx* c, int which, int reps, double* ms, double* byte
     float f = 0.0f;
     LS_HIP(hipEventElapsedTime(&f, e0, e1));
     *ms = (double)f / reps;
-    *bytes = (double)c->N * (which == LS_TIME_TRED ? 36.0 : 44.0);
+    *bytes = (double)c->N * (which == LS_TIME_TRED ? 32.0 : 40.0);
     return 0;
 }
 

@@ -131,7 +131,7 @@ int pool_setup(ls_ctx* c) {
     const size_t o_rows = stored ? tc.take(transport_rows_doubles(c) * d) : 0;
     const size_t o_tb0 = tc.take(N * d), o_tb1 = tc.take(N * d);
     const size_t nv = schur ? Nh : N;
-    size_t o_tv[2][8], o_xs[2] = {0, 0}, o_wr[2] = {0, 0}, o_invdb = 0, o_tinvd = 0;
+    size_t o_tv[2][8], o_xs[2] = {0, 0}, o_wr[2] = {0, 0}, o_tinvd = 0;
     for (int f = 0; f < 2; ++f)
         for (int q = 0; q < 8; ++q) o_tv[f][q] = tc.take(nv * d);
     if (schur) {
@@ -139,7 +139,6 @@ int pool_setup(ls_ctx* c) {
             o_xs[f] = tc.take(Nh * d);
             o_wr[f] = tc.take(Nh * d);
         }
-        o_invdb = tc.take(Nh * d);
     } else {
         o_tinvd = tc.take(N * d);
     }
@@ -201,7 +200,6 @@ int pool_setup(ls_ctx* c) {
         c->twr[f] = schur ? (double*)(b + o_wr[f]) : nullptr;
         c->taux[f] = c->P.implicit_sat ? (double*)(b + o_aux[f]) : nullptr;
     }
-    c->tinvdb = schur ? (double*)(b + o_invdb) : nullptr;
     c->tinvd = schur ? nullptr : (double*)(b + o_tinvd);
     return 0;
 }

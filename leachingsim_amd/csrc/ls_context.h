@@ -120,14 +120,14 @@ struct ls_ctx {
     float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr;
     // --- transport (two right-hand sides) ---
     // r, r0, p, v, s, t, ph, sh for conc and acid: N/2 each in the Schur
-    // form (black unknowns), else N
+    // form (black unknowns; ph and sh unused there), else N
     double* tv[2][8] = {};
     double* tb[2] = {};      // right-hand sides (N)
     double* tinvd = nullptr; // Jacobi inverse diagonal (N; not in the Schur form)
-    // Schur form: black unknowns, red values of the last red pass, 1 / D_b
+    // Schur form: the scaled black unknowns y = D_b x_b, red values of the
+    // last red pass
     double* txs[2] = {};
     double* twr[2] = {};
-    double* tinvdb = nullptr;
     double* taux[2] = {};    // implicit saturation: its source field and idle system (N)
     double* arows = nullptr; // stored transport rows (nk even)
     // black half of conc at the last transport solve (N/2, allocated on first
