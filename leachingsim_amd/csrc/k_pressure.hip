@@ -130,6 +130,7 @@ __global__ __launch_bounds__(TK * TJ) void k_stencil(
 // ---------------------------------------------------------------------------
 template <typename T, int W> struct VecOf;
 template <> struct VecOf<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct VecOf<float, 2> { typedef float type __attribute__((ext_vector_type(2))); };
 template <> struct VecOf<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
 
 template <typename T> struct StencilShape;
@@ -279,16 +280,17 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
 //   MODE 4 (reconstruct) y = red ? xc + n : xn               (x_r = r_r + N_rb x_b)
 //   MODE 5 (rhs, packed) as MODE 3, y written as the compacted black vector
 //                        (layout below k_schur_apply's header)
-// Vectorised like k_stencil_vec: 4 floats of one k-row per lane, marching
-// ICH planes along i.
+// Vectorised like k_stencil_vec: W floats of one k-row per lane (4 when
+// nk % 4 == 0, else 2 when nk is even), marching ICH planes along i.
 // ---------------------------------------------------------------------------
-template <int MODE>
+template <int MODE, int W = 4>
 __global__ __launch_bounds__(WAVE * StencilShape<float>::TJV) void k_rb_vec(
     const float* __restrict__ xn, const float* __restrict__ xc, float* __restrict__ y,
     const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
     SGeo g, double* __restrict__ partials, const int* __restrict__ done) {
-    constexpr int W = 4, ICH = StencilShape<float>::ICH, TJV = StencilShape<float>::TJV;
-    typedef VecOf<float, 4>::type V;
+    static_assert(W == 4 || (W == 2 && MODE != 5), "the packed output needs 4 values per lane");
+    constexpr int ICH = StencilShape<float>::ICH, TJV = StencilShape<float>::TJV;
+    typedef typename VecOf<float, W>::type V;
     if (done != nullptr && *done) return;
     __shared__ double lds[TJV];
     const int nbx = gridDim.x, nby = gridDim.y;
@@ -351,7 +353,7 @@ __global__ __launch_bounds__(WAVE * StencilShape<float>::TJV) void k_rb_vec(
                 if (MODE == 2) acc += (double)cen[w] * (double)o;
                 if (MODE == 3 || MODE == 5) acc += (double)o * (double)o;
             }
-            if (MODE == 5) {
+            if constexpr (MODE == 5) {
                 // the lane's two black values, black first element at par
                 typedef float F2 __attribute__((ext_vector_type(2)));
                 const int pb = (i + j) & 1;
@@ -823,6 +825,28 @@ static bool use_vec(const ls_ctx* c) {
            2 * (size_t)vec_blocks<T>((uint32_t)c->N) <= (size_t)MAX_PARTIALS;
 }
 
+// The vectorised elementwise CG kernels take any length (block 0 does the
+// scalar tail); they need their block partials to fit.
+template <typename T>
+static bool ew_vec(uint32_t N) {
+    return 2 * (size_t)vec_blocks<T>(N) <= (size_t)MAX_PARTIALS;
+}
+
+// Width of the red-black passes over full-length FP32 vectors: 4 floats per
+// lane (nk % 4 == 0), 2 (nk % 4 == 2), or 0 for one voxel per thread
+// (k_rb_scalar, odd nk).
+static dim3 rb_grid(const ls_ctx* c, int w) {
+    constexpr int ICH = StencilShape<float>::ICH, TJV = StencilShape<float>::TJV;
+    return dim3((c->nk / w + WAVE - 1) / WAVE, (c->nj + TJV - 1) / TJV, (c->ni + ICH - 1) / ICH);
+}
+
+static int rb_width(const ls_ctx* c) {
+    if (use_vec<float>(c)) return 4;
+    if (c->nk % 2 != 0) return 0;
+    const dim3 g = rb_grid(c, 2);
+    return (size_t)g.x * g.y * g.z <= (size_t)MAX_PARTIALS ? 2 : 0;
+}
+
 // alpha = r.r / p.Ap
 __global__ void k_cg_alpha(const double* __restrict__ partials, int n, CgScalars* cs) {
     if (cs->done) {
@@ -1077,7 +1101,7 @@ template <typename T>
 static int cg_iterations(ls_ctx* c, T* x, T* r, T* p, T* ap, T* const* faces, const double* sdw,
                          int batch) {
     uint32_t N = (uint32_t)c->N;
-    const bool vec = use_vec<T>(c);
+    const bool vec = ew_vec<T>(N);
     const int nbe = vec ? vec_blocks<T>(N) : ew_blocks(N);
     for (int b = 0; b < batch; ++b) {
         const bool sample = c->profiling && b == 0;
@@ -1249,7 +1273,7 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
                            c->cg, trig, maxit, first ? 1 : 0);
         if (!first) {
             // xlive = 0 here: only the direction carries over
-            if (use_vec<float>(c))
+            if (ew_vec<float>(N))
                 hipLaunchKernelGGL(k_cg_direction_vec<float>, dim3(vec_blocks<float>(N)),
                                    dim3(EW_BLOCK), 0, c->stream, c->p32, c->r32, c->x32, c->cg, N);
             else
@@ -1308,16 +1332,26 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
 // block partials (MODE 2 and 3 write them).
 template <int MODE>
 static int launch_rb(ls_ctx* c, const float* xn, const float* xc, float* y, const int* done) {
-    if (MODE != 5 && !use_vec<float>(c)) {
+    const int w = MODE == 5 ? 4 : rb_width(c);
+    if (w == 0) {
         const dim3 grd = stencil_grid(c);
         hipLaunchKernelGGL(k_rb_scalar<(MODE == 5 ? 1 : MODE)>, grd, dim3(TK, TJ), 0, c->stream,
                            xn, xc, y, c->c32[0], c->c32[1], c->c32[2], sgeo(c), c->partials,
                            done);
         return (int)(grd.x * grd.y * grd.z);
     }
-    const dim3 grd = vec_grid<float>(c);
-    hipLaunchKernelGGL(k_rb_vec<MODE>, grd, dim3(WAVE, StencilShape<float>::TJV), 0, c->stream,
-                       xn, xc, y, c->c32[0], c->c32[1], c->c32[2], sgeo(c), c->partials, done);
+    const dim3 grd = rb_grid(c, w);
+    if constexpr (MODE != 5) {
+        if (w == 2) {
+            hipLaunchKernelGGL((k_rb_vec<MODE, 2>), grd, dim3(WAVE, StencilShape<float>::TJV), 0,
+                               c->stream, xn, xc, y, c->c32[0], c->c32[1], c->c32[2], sgeo(c),
+                               c->partials, done);
+            return (int)(grd.x * grd.y * grd.z);
+        }
+    }
+    hipLaunchKernelGGL((k_rb_vec<MODE, 4>), grd, dim3(WAVE, StencilShape<float>::TJV), 0,
+                       c->stream, xn, xc, y, c->c32[0], c->c32[1], c->c32[2], sgeo(c), c->partials,
+                       done);
     return (int)(grd.x * grd.y * grd.z);
 }
 
@@ -1391,7 +1425,7 @@ __global__ void k_set_tol2(CgScalars* cs, double tol2) { cs->tol2 = tol2; }
 static int cg_iterations_rb(ls_ctx* c, bool fused, float* e, float* rs, float* p, float* q,
                             float* w, int batch) {
     const uint32_t Nv = fused ? (uint32_t)(c->N / 2) : (uint32_t)c->N;
-    const bool vec = fused || use_vec<float>(c);
+    const bool vec = ew_vec<float>(Nv);
     const int nbe = vec ? vec_blocks<float>(Nv) : ew_blocks(Nv);
     for (int b = 0; b < batch; ++b) {
         const bool sample = c->profiling && b == 0;
@@ -1433,9 +1467,9 @@ static int cg_iterations_rb(ls_ctx* c, bool fused, float* e, float* rs, float* p
 // r_b + N_br r_r, CG on S runs until the Schur residual has fallen by delta
 // (or its norm reaches the tolerance), then the red correction is
 // e_r = r_r + N_rb e_b, the full correction is added to the FP64 iterate and
-// the FP64 residual recomputed.  Two-pass: k_rb_vec when nk % 4 == 0, else
-// k_rb_scalar; fused (nk % 8 == 0): the inner vectors are compacted
-// (k_schur_apply).
+// the FP64 residual recomputed.  Two-pass: k_rb_vec when nk is even (4 or 2
+// floats per lane), else k_rb_scalar; fused (nk % 8 == 0): the inner vectors
+// are compacted (k_schur_apply).
 static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep, bool fused) {
     const uint32_t N = (uint32_t)c->N;
     const size_t Nv = fused ? c->N / 2 : c->N;
@@ -1485,7 +1519,7 @@ static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep, bool fused) {
         if (first) {
             LS_HIP(hipMemcpyAsync(c->p32, rs, Nv * sizeof(float), hipMemcpyDeviceToDevice,
                                   c->stream));
-        } else if (fused || use_vec<float>(c)) {   // xlive = 0 here: only the direction
+        } else if (ew_vec<float>((uint32_t)Nv)) {   // xlive = 0 here: only the direction
             hipLaunchKernelGGL(k_cg_direction_vec<float>, dim3(vec_blocks<float>((uint32_t)Nv)),
                                dim3(EW_BLOCK), 0, c->stream, c->p32, rs, c->x32, c->cg,
                                (uint32_t)Nv);                  // carries over
@@ -1631,7 +1665,7 @@ int debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot
         c->err = "the fused Schur apply needs nk % " + std::to_string(2 * SCHUR_VW) + " == 0 and nk <= " + std::to_string(SCHUR_MAX_NK);
         return -3;
     }
-    if (form == 2 && (c->nk % 2 != 0 || !use_vec<float>(c))) {
+    if (form == 2 && !use_vec<float>(c)) {
         c->err = "the compacted two-pass form needs nk % 4 == 0 (form 4 takes any nk)";
         return -3;
     }
@@ -1642,8 +1676,9 @@ int debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot
     std::vector<float> full;
     if (form == 4) {
         // full-length vectors, any nk: the gather and apply passes the
-        // solver runs (k_rb_vec for nk % 4 == 0, else k_rb_scalar); the red
-        // entries of p are never read, those of q come back zero
+        // solver runs (k_rb_vec with 4 or 2 floats per lane for nk % 4 == 0
+        // or 2, else k_rb_scalar); the red entries of p are never read,
+        // those of q come back zero
         float* pf = (float*)c->pw[1];
         float* wf = pf + N;
         LS_HIP(hipMemcpyAsync(pf, p, N * sizeof(float), hipMemcpyHostToDevice, c->stream));
@@ -2203,7 +2238,7 @@ static int solve_learned(ls_ctx* c, ls_step_report* rep) {
                 const int nbs = launch_stencil<float>(c, pa, qa, c->c32, &c->fcg->done);
                 hipLaunchKernelGGL(k_lalpha, dim3(1), dim3(1024), 0, c->stream, c->partials, nbs,
                                    c->fcg, c->cg);
-                if (use_vec<float>(c))
+                if (ew_vec<float>(N))
                     hipLaunchKernelGGL((k_cg_update_vec<float, true>), dim3(vec_blocks<float>(N)),
                                        dim3(EW_BLOCK), 0, c->stream, c->x32, c->r32, pa, qa, nullptr,
                                        c->cg, N, c->partials);
@@ -2211,7 +2246,7 @@ static int solve_learned(ls_ctx* c, ls_step_report* rep) {
                     hipLaunchKernelGGL((k_cg_update<float, true>), dim3(nbe), dim3(EW_BLOCK), 0,
                                        c->stream, c->x32, c->r32, pa, qa, nullptr, c->cg, N,
                                        c->partials);
-                const int nbu = use_vec<float>(c) ? vec_blocks<float>(N) : nbe;
+                const int nbu = ew_vec<float>(N) ? vec_blocks<float>(N) : nbe;
                 hipLaunchKernelGGL(k_lcheck, dim3(1), dim3(1024), 0, c->stream, c->partials, nbu,
                                    c->fcg);
                 LS_LAUNCH_CHECK();
