@@ -138,7 +138,10 @@ __device__ __forceinline__ void adr_row(const uint8_t* __restrict__ img, const d
 
 // The explicit (1 - theta) part of the right-hand sides of both systems
 // (src/Simulation.cpp:1221-1222): the neighbours' old values are loaded
-// together before the sum, which runs over the faces in order.
+// together before the sum, which runs over the faces in order.  At theta = 1
+// (fully implicit, the default) every term is 0 x a finite value and leaves
+// the right-hand side as it is, so the callers skip it and its six
+// neighbour loads per system.
 __device__ __forceinline__ void explicit_rhs(const AdrRow& R, uint32_t v,
                                              const double* __restrict__ x0,
                                              const double* __restrict__ x1, double xa, double xb,
@@ -243,7 +246,7 @@ __global__ void k_adr_start(const uint8_t* __restrict__ img, const double* __res
             } else {
                 ba = xa * a.inv_dt;
                 bbv = xb * a.inv_dt;
-                explicit_rhs(R, v, x0, x1, xa, xb, a, ba, bbv);
+                if (a.theta != 1.0) explicit_rhs(R, v, x0, x1, xa, xb, a, ba, bbv);
             }
             b0[v] = ba;
             b1[v] = bbv;
@@ -445,7 +448,7 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows(
         } else {
             ba = xa * a.inv_dt;
             bbv = xb * a.inv_dt;
-            explicit_rhs(R, v, x0, x1, xa, xb, a, ba, bbv);
+            if (a.theta != 1.0) explicit_rhs(R, v, x0, x1, xa, xb, a, ba, bbv);
         }
         b0[v] = ba;
         b1[v] = bbv;
@@ -766,7 +769,7 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
         } else {
             ba = xa * a.inv_dt;
             bbv = xb * a.inv_dt;
-            explicit_rhs(R, v, x0, x1r, xa, xb, a, ba, bbv);
+            if (a.theta != 1.0) explicit_rhs(R, v, x0, x1r, xa, xb, a, ba, bbv);
         }
         b0[v] = ba;
         if (!one) b1[v] = bbv;
@@ -1829,7 +1832,7 @@ int time_transport_pass(ls_ctx* c, int which, int reps, double* ms, double* byte
     const bool tvec = c->nk % 4 == 0 && tvgrid.y <= 65535 &&
                       4 * (size_t)tvgrid.x * tvgrid.y <= 8 * (size_t)MAX_PARTIALS;
     // the iteration's first matvec: p -> (red) w -> (black) v, dot with r0
-    const double* xa = c->tv[0][6];
+    const double* xa = c->tv[0][2];
     double* w = c->twr[0];
     double* y = c->tv[0][3];
     const double* wd = c->tv[0][1];

@@ -2,6 +2,7 @@
 // transfers, the coupled step (src/main.cpp:303-432) and snapshot/rollback.
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <new>
 #include <vector>
 
@@ -131,9 +132,11 @@ int pool_setup(ls_ctx* c) {
     const size_t o_rows = stored ? tc.take(transport_rows_doubles(c) * d) : 0;
     const size_t o_tb0 = tc.take(N * d), o_tb1 = tc.take(N * d);
     const size_t nv = schur ? Nh : N;
-    size_t o_tv[2][8], o_xs[2] = {0, 0}, o_wr[2] = {0, 0}, o_tinvd = 0;
+    // the Schur form keeps its Jacobi scaling inside the operator: no ph, sh
+    const int ntv = schur ? 6 : 8;
+    size_t o_tv[2][8] = {}, o_xs[2] = {0, 0}, o_wr[2] = {0, 0}, o_tinvd = 0;
     for (int f = 0; f < 2; ++f)
-        for (int q = 0; q < 8; ++q) o_tv[f][q] = tc.take(nv * d);
+        for (int q = 0; q < ntv; ++q) o_tv[f][q] = tc.take(nv * d);
     if (schur) {
         for (int f = 0; f < 2; ++f) {
             o_xs[f] = tc.take(Nh * d);
@@ -195,7 +198,7 @@ int pool_setup(ls_ctx* c) {
     c->tb[0] = (double*)(b + o_tb0);
     c->tb[1] = (double*)(b + o_tb1);
     for (int f = 0; f < 2; ++f) {
-        for (int k = 0; k < 8; ++k) c->tv[f][k] = (double*)(b + o_tv[f][k]);
+        for (int k = 0; k < 8; ++k) c->tv[f][k] = k < ntv ? (double*)(b + o_tv[f][k]) : nullptr;
         c->txs[f] = schur ? (double*)(b + o_xs[f]) : nullptr;
         c->twr[f] = schur ? (double*)(b + o_wr[f]) : nullptr;
         c->taux[f] = c->P.implicit_sat ? (double*)(b + o_aux[f]) : nullptr;
@@ -518,6 +521,43 @@ int ls_snapshot(ls_ctx* c) {
     return st;
 }
 
+// The step's own saveState: the flux fields are rewritten in full by the
+// flux phase before anything in the step reads them, so instead of copying
+// them the step swaps each flux field with its backup (the backup then holds
+// the old fluxes; the flux phase fills the other buffer).  A step that stops
+// before its flux phase swaps back and makes the copy after all
+// (step_unswap_fluxes), so the fields and the backup are then exactly what a
+// copying saveState leaves.
+static int step_snapshot(ls_ctx* c) {
+    const size_t N = c->N;
+    int st = 0;
+    st |= copy_d2d(c, c->b_img, c->img, N);
+    st |= copy_d2d(c, c->b_frac, c->frac, N * 4);
+    st |= copy_d2d(c, c->b_inv, c->inv, N * 8);
+    st |= copy_d2d(c, c->b_s, c->s, N * 8);
+    st |= copy_d2d(c, c->b_p, c->p, N * 8);
+    st |= copy_d2d(c, c->b_conc, c->conc, N * 8);
+    st |= copy_d2d(c, c->b_acid, c->acid, N * 8);
+    std::swap(c->qx, c->b_qx);
+    std::swap(c->qy, c->b_qy);
+    std::swap(c->qz, c->b_qz);
+    c->b_acid_zero = c->acid_zero;
+    c->b_frac_clean = c->frac_clean;
+    return st;
+}
+
+static int step_unswap_fluxes(ls_ctx* c) {
+    std::swap(c->qx, c->b_qx);
+    std::swap(c->qy, c->b_qy);
+    std::swap(c->qz, c->b_qz);
+    const size_t N = c->N;
+    int st = 0;
+    st |= copy_d2d(c, c->b_qx, c->qx, N * 8);
+    st |= copy_d2d(c, c->b_qy, c->qy, N * 8);
+    st |= copy_d2d(c, c->b_qz, c->qz, N * 8);
+    return st;
+}
+
 int ls_rollback(ls_ctx* c) {
     if (!c) return -1;
     const size_t N = c->N;
@@ -609,8 +649,13 @@ int ls_step(ls_ctx* c, double t, double dt, ls_step_report* report) {
     reset_prof(c);
     Timer tm{c};
     tm.mark();                                                  // 0
-    int st = ls_snapshot(c);                                    // saveState
+    int st = step_snapshot(c);                                  // saveState
     if (st) return st;
+    bool fluxes_swapped = true;     // until the flux phase has written them
+    auto stop = [&](int code) {     // a fatal error before the flux phase
+        if (fluxes_swapped) (void)step_unswap_fluxes(c);
+        return code;
+    };
     tm.mark();                                                  // 1
     // A. hydrodynamics: one Picard pass with the explicit saturation, three
     // with the implicit one (src/main.cpp:312-370); the phase marks are those
@@ -619,14 +664,15 @@ int ls_step(ls_ctx* c, double t, double dt, ls_step_report* report) {
     const int passes = c->P.implicit_sat ? 3 : 1;
     for (int pass = 0; pass < passes && ok; ++pass) {
         if (pass) tm.n = 1;
-        if ((st = phase_properties(c))) return st;
+        if ((st = phase_properties(c))) return stop(st);
         tm.mark();                                              // 2
         st = solve_pressure(c, t, rep);
-        if (st < 0) return st;
+        if (st < 0) return stop(st);
         tm.mark();                                              // 3
         ok = st == 0;
         if (!ok) break;
-        if ((st = phase_flux(c))) return st;
+        if ((st = phase_flux(c))) return stop(st);
+        fluxes_swapped = false;
         tm.mark();                                              // 4
         if (c->P.implicit_sat) {
             st = phase_saturation_implicit(c, dt, rep);
@@ -637,6 +683,9 @@ int ls_step(ls_ctx* c, double t, double dt, ls_step_report* report) {
         }
         tm.mark();                                              // 5
     }
+    // a pressure solve that failed before any flux phase: fields and backup
+    // as a copying saveState leaves them
+    if (fluxes_swapped && (st = step_unswap_fluxes(c))) return st;
     // an unphysical implicit saturation after the last pass forces a retry
     if (ok && c->P.implicit_sat && !c->sat_valid) {
         c->err = "implicit saturation: unphysical state after the Picard passes";
@@ -754,6 +803,79 @@ int ls_set_profiling(ls_ctx* c, int on) {
     if (!c) return -1;
     c->profiling = on ? 1 : 0;
     return 0;
+}
+
+// Training data of the learned coarse correction (no reference
+// counterpart): the Galerkin operator of the assembled pressure system on
+// the network grid, and one (input, P^T r, e.A e) sample from the current
+// pressure field and a reference solution.
+int ls_debug_learned_galerkin(ls_ctx* c, float* ac_host, size_t count) {
+    if (c) c->trans_valid = false;
+    if (!c || !ac_host) return -1;
+    if (!c->press_assembled) {
+        c->err = "no pressure system assembled";
+        return -1;
+    }
+    int st = learned_setup(c);
+    if (st) return st;
+    const size_t cn = (size_t)c->net_dims[0] * c->net_dims[1] * c->net_dims[2];
+    if (count != 4 * cn) {
+        c->err = "output size must be 4 x the coarse network grid";
+        return -3;
+    }
+    float* ad = nullptr;
+    if (hipMalloc(&ad, 4 * cn * sizeof(float)) != hipSuccess) {
+        c->err = "hipMalloc failed";
+        return -12;
+    }
+    st = learned_galerkin_debug(c, ad);
+    if (!st && hipMemcpyAsync(ac_host, ad, 4 * cn * sizeof(float), hipMemcpyDeviceToHost,
+                              c->stream) != hipSuccess)
+        st = -10;
+    if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = -10;
+    (void)hipFree(ad);
+    return st;
+}
+
+int ls_debug_learned_sample(ls_ctx* c, const double* pstar_host, float* x_host, size_t x_count,
+                            float* rc_host, double* eae) {
+    if (c) c->trans_valid = false;
+    if (!c || !pstar_host || !x_host || !rc_host || !eae) return -1;
+    if (!c->press_assembled) {
+        c->err = "no pressure system assembled";
+        return -1;
+    }
+    int st = learned_setup(c);
+    if (st) return st;
+    const size_t cn = (size_t)c->net_dims[0] * c->net_dims[1] * c->net_dims[2];
+    if (x_count != cn * 16) {
+        c->err = "input size must be 16 x the coarse network grid";
+        return -3;
+    }
+    double* pd = nullptr;
+    float *xd = nullptr, *rcd = nullptr;
+    if (hipMalloc(&pd, c->N * sizeof(double)) != hipSuccess ||
+        hipMalloc(&xd, cn * 16 * sizeof(float)) != hipSuccess ||
+        hipMalloc(&rcd, cn * sizeof(float)) != hipSuccess) {
+        if (pd) (void)hipFree(pd);
+        if (xd) (void)hipFree(xd);
+        c->err = "hipMalloc failed";
+        return -12;
+    }
+    if (hipMemcpyAsync(pd, pstar_host, c->N * sizeof(double), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess)
+        st = -10;
+    if (!st) st = learned_sample_debug(c, pd, xd, rcd, eae);
+    if (!st && (hipMemcpyAsync(x_host, xd, cn * 16 * sizeof(float), hipMemcpyDeviceToHost,
+                               c->stream) != hipSuccess ||
+                hipMemcpyAsync(rc_host, rcd, cn * sizeof(float), hipMemcpyDeviceToHost,
+                               c->stream) != hipSuccess))
+        st = -10;
+    if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = -10;
+    (void)hipFree(pd);
+    (void)hipFree(xd);
+    (void)hipFree(rcd);
+    return st;
 }
 
 int ls_debug_apply_pressure(ls_ctx* c, const double* x, double* y) {

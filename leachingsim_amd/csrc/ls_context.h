@@ -120,7 +120,7 @@ struct ls_ctx {
     float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr;
     // --- transport (two right-hand sides) ---
     // r, r0, p, v, s, t, ph, sh for conc and acid: N/2 each in the Schur
-    // form (black unknowns; ph and sh unused there), else N
+    // form (black unknowns; no ph and sh there: null), else N
     double* tv[2][8] = {};
     double* tb[2] = {};      // right-hand sides (N)
     double* tinvd = nullptr; // Jacobi inverse diagonal (N; not in the Schur form)
@@ -229,6 +229,9 @@ int debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* dot
 int learned_setup(ls_ctx* c);
 void learned_free(ls_ctx* c);
 int learned_pack_debug(ls_ctx* c, const float* r_dev, float* out_dev);
+int learned_galerkin_debug(ls_ctx* c, float* ac_dev);
+int learned_sample_debug(ls_ctx* c, const double* pstar_dev, float* x_dev, float* rc_dev,
+                         double* eae);
 // k_transport.hip
 int phase_transport(ls_ctx* c, double dt, ls_step_report* rep);
 int time_transport_pass(ls_ctx* c, int which, int reps, double* ms, double* bytes);

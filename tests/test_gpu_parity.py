@@ -241,22 +241,42 @@ def test_step_parity_scalar_paths(shape, schur):
     assert ep <= 1e-9 and ec <= 1e-10
 
 
-def test_rollback_restores_state():
+@pytest.mark.parametrize("fail", ["pressure", "transport"])
+def test_rollback_restores_state(fail):
+    """A step that fails in the pressure solve (before its flux phase) or in
+    transport (after it), then restoreState: every field is back bit for bit.
+    The step snapshot swaps the flux fields with their backup instead of
+    copying them; a step that stops before its flux phase leaves the fluxes
+    as they were."""
     img = G.named_geometry("smooth64")[:32, :32, :32].copy()
+    if fail == "transport":
+        img[:, 0:2] = G.AIR          # a conc source, so the transport solve has work
     P = L.default_params(bc_mode=L.BC_ZFACES)
     g = L.GpuSimulation(img.shape, P)
     g.read_labels(img)
     g.init_state()
     g.step(0.0, 1.0)
     before = g.state()
-    P2 = L.default_params(bc_mode=L.BC_ZFACES, press_maxit=1)
+    if fail == "transport":
+        assert np.abs(before[L.F_CONC]).max() > 0
+    if fail == "pressure":
+        P2 = L.default_params(bc_mode=L.BC_ZFACES, press_maxit=1)
+    else:
+        P2 = L.default_params(bc_mode=L.BC_ZFACES, adr_maxit=1, adr_rtol=1e-14)
     g.set_params(P2)
     rep = g.step(1.0, 1.0)
     assert rep.status == 1
+    if fail == "pressure":
+        now = g.state()
+        for f in (L.F_QX, L.F_QY, L.F_QZ):
+            assert np.array_equal(before[f], now[f]), f
     g.restoreState()
     after = g.state()
     for f in before:
         assert np.array_equal(before[f], after[f]), f
+    # the next step runs from the restored state as the first attempt did
+    g.set_params(P)
+    assert g.step(1.0, 1.0).status == 0
 
 
 def test_implicit_saturation_phase_parity():
