@@ -1,15 +1,16 @@
 // k_unet.hip — the 3-level 3D U-Net of the learned pressure preconditioner:
 // BF16 activations (channels-last), BF16 weights, FP32 accumulation on MFMA
-// (v_mfma_f32_16x16x16_bf16), hand-written forward, backward and Adam.
+// (gfx950's 32-deep v_mfma_f32_16x16x32_bf16), hand-written forward,
+// backward and Adam.
 //
 // Convolution as implicit GEMM, D = W . X^T per 16 output channels x 16
-// voxels: the A operand is a 16 (cout) x 16 (cin) weight tile (4 cin per lane,
-// one 8-byte load), the B operand 16 (cin) x 16 voxels (4 consecutive
-// channels of one voxel per lane, one 8-byte load of the channels-last
-// activation), and each lane ends with 4 consecutive output channels of one
-// voxel (one 8-byte store).  A wave computes 64 voxels x all output channels;
-// taps and 16-channel input chunks form the K loop.  Concatenated inputs
-// ([2x upsampled deep, skip]) are read in place from the two tensors.
+// voxels, 32 deep: the A operand is a 16 (cout) x 32 weight tile (8 channels
+// of one tap per lane, one 16-byte load), the B operand 32 x 16 voxels (the
+// same 8 channels of one voxel per lane), and each lane ends with 4
+// consecutive output channels of one voxel (one 8-byte store).  A wave
+// computes 64 voxels x all output channels; pairs of (tap, 16-channel chunk)
+// units form the K loop.  Concatenated inputs ([2x upsampled deep, skip]) are
+// read in place from the two tensors.
 //
 // Backward-data is the same convolution on the ReLU-masked output gradient
 // with the mirrored, transposed weights W'[26 - t][ci][co] = W[t][co][ci].
@@ -53,13 +54,45 @@ __device__ __forceinline__ void vdecode(uint32_t v, const NetLvl& L, int& i, int
 // Convolution forward (also backward-data with mirrored weights)
 //   Y[b][v][co] = act(bias[co] + sum_t sum_ci W[t][co][ci] X[b][v + off(t)][ci])
 //   X = [UPA ? up2(A) : A  (CA channels), B (CB channels)]
+//
+// The reduction runs over units u = t NCK + ck of 16 input channels (tap t,
+// channel chunk ck); unit pairs (2p, 2p + 1) feed one 32-deep MFMA
+// (v_mfma_f32_16x16x32_bf16): lane group h = q >> 1 of the A and B operands
+// takes unit 2p + h, its 8 channels 16 ck + 8 (q & 1) .. + 7 (an odd unit
+// count pads the last pair with zeros).  Every kernel below issues the pairs
+// in this order per output tile, so they agree bit for bit.
 // ---------------------------------------------------------------------------
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf8 load8(const bf16* p) { return *(const bf8*)p; }
+__device__ __forceinline__ bf8 load8(const float* p) {
+    const f4 a = *(const f4*)p, b = *(const f4*)(p + 4);
+    bf8 r;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        r[e] = (bf16)a[e];
+        r[e + 4] = (bf16)b[e];
+    }
+    return r;
+}
+
+__device__ __forceinline__ bf8 zero8() {
+    bf8 r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = (bf16)0.0f;
+    return r;
+}
+
+__device__ __forceinline__ f4 mfma32(bf8 a, bf8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
 template <int CA, int CB, int COUT, bool UPA, bool RELU, typename TIN, typename TOUT>
 __global__ __launch_bounds__(256) void k_conv(const TIN* __restrict__ A, const TIN* __restrict__ B,
                                               const bf16* __restrict__ W,
                                               const float* __restrict__ bias,
                                               TOUT* __restrict__ Y, NetLvl L, NetLvl LA) {
-    constexpr int CIN = CA + CB;
+    constexpr int CIN = CA + CB, NCK = CIN / 16, U = 27 * NCK, NP = (U + 1) / 2;
     constexpr int MT = COUT / 16;
     constexpr int NT = 4;
     const int lane = threadIdx.x & 63;
@@ -68,7 +101,8 @@ __global__ __launch_bounds__(256) void k_conv(const TIN* __restrict__ A, const T
     const uint32_t v0 = (blockIdx.x * 4u + wave) * 64u;
     if (v0 >= L.n) return;  // uniform per wave
     const int col = lane & 15;
-    const int kq = (lane >> 4) * 4;
+    const int q = lane >> 4, h = q >> 1, cq = 8 * (q & 1);
+    const int kq = 4 * q;
     int vi[NT], vj[NT], vk[NT];
 #pragma unroll
     for (int n = 0; n < NT; ++n) vdecode(v0 + n * 16 + col, L, vi[n], vj[n], vk[n]);
@@ -79,37 +113,36 @@ __global__ __launch_bounds__(256) void k_conv(const TIN* __restrict__ A, const T
         for (int n = 0; n < NT; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
     const size_t bA = (size_t)b * (UPA ? LA.n : L.n);
     const size_t bB = (size_t)b * L.n;
-    for (int t = 0; t < 27; ++t) {
+    for (int pr = 0; pr < NP; ++pr) {
+        const int u = 2 * pr + h;
+        const bool uv = u < U;
+        const int t = uv ? u / NCK : 0;
+        const int c0 = 16 * (uv ? u - t * NCK : 0) + cq;     // this lane's first channel
         const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
-        bool inb[NT];
-        uint32_t nA[NT], nB[NT];
+        bf8 wa[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+            wa[m] = uv ? load8(W + ((size_t)t * COUT + m * 16 + col) * CIN + c0) : zero8();
 #pragma unroll
         for (int n = 0; n < NT; ++n) {
             const int ii = vi[n] + di, jj = vj[n] + dj, kk = vk[n] + dk;
-            inb[n] = ii >= 0 && ii < L.ni && jj >= 0 && jj < L.nj && kk >= 0 && kk < L.nk;
-            nB[n] = inb[n] ? ((uint32_t)ii * L.nj + (uint32_t)jj) * L.nk + (uint32_t)kk : 0u;
-            nA[n] = UPA ? (inb[n] ? ((uint32_t)(ii >> 1) * LA.nj + (uint32_t)(jj >> 1)) * LA.nk +
-                                        (uint32_t)(kk >> 1)
-                                  : 0u)
-                        : nB[n];
-        }
-        const bf16* wt = W + (size_t)t * COUT * CIN;
-#pragma unroll
-        for (int c0 = 0; c0 < CIN; c0 += 16) {
-            bf4 wa[MT];
-#pragma unroll
-            for (int m = 0; m < MT; ++m) wa[m] = load4(wt + (size_t)(m * 16 + col) * CIN + c0 + kq);
-#pragma unroll
-            for (int n = 0; n < NT; ++n) {
-                bf4 xb = bf4{(bf16)0.f, (bf16)0.f, (bf16)0.f, (bf16)0.f};
-                if (inb[n]) {
-                    if (c0 < CA) xb = load4(A + (bA + nA[n]) * CA + c0 + kq);
-                    else xb = load4(B + (bB + nB[n]) * CB + (c0 - CA) + kq);
+            const bool inb = uv && ii >= 0 && ii < L.ni && jj >= 0 && jj < L.nj && kk >= 0 &&
+                             kk < L.nk;
+            bf8 xb = zero8();
+            if (inb) {
+                if (c0 < CA) {
+                    const uint32_t na = UPA ? ((uint32_t)(ii >> 1) * LA.nj + (uint32_t)(jj >> 1)) *
+                                                      LA.nk + (uint32_t)(kk >> 1)
+                                            : ((uint32_t)ii * L.nj + (uint32_t)jj) * L.nk +
+                                                  (uint32_t)kk;
+                    xb = load8(A + (bA + na) * CA + c0);
+                } else {
+                    const uint32_t nb = ((uint32_t)ii * L.nj + (uint32_t)jj) * L.nk + (uint32_t)kk;
+                    xb = load8(B + (bB + nb) * CB + (c0 - CA));
                 }
-#pragma unroll
-                for (int m = 0; m < MT; ++m)
-                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[m], xb, acc[m][n], 0, 0, 0);
             }
+#pragma unroll
+            for (int m = 0; m < MT; ++m) acc[m][n] = mfma32(wa[m], xb, acc[m][n]);
         }
     }
 #pragma unroll
@@ -122,7 +155,7 @@ __global__ __launch_bounds__(256) void k_conv(const TIN* __restrict__ A, const T
             f4 o = acc[m][n] + bv;
             if (RELU) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) o[q] = o[q] > 0.f ? o[q] : 0.f;
+                for (int e = 0; e < 4; ++e) o[e] = o[e] > 0.f ? o[e] : 0.f;
             }
             const uint32_t v = v0 + n * 16 + col;
             store4(Y + (bB + v) * COUT + co0, o);
@@ -159,20 +192,6 @@ template <> struct ConvTile<32, 32> { static constexpr int TJ = 2, TK = 32, MS =
 template <> struct ConvTile<32, 64> { static constexpr int TJ = 1, TK = 32, MS = 4; };
 template <> struct ConvTile<48, 16> { static constexpr int TJ = 4, TK = 16, MS = 1; };
 
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ bf8 load8(const bf16* p) { return *(const bf8*)p; }
-__device__ __forceinline__ bf8 load8(const float* p) {
-    const f4 a = *(const f4*)p, b = *(const f4*)(p + 4);
-    bf8 r;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        r[e] = (bf16)a[e];
-        r[e + 4] = (bf16)b[e];
-    }
-    return r;
-}
-
 template <int CA, int CB, int COUT, bool UPA, bool RELU, typename TIN, typename TOUT>
 __global__ __launch_bounds__((64 * ConvTile<CA + CB, COUT>::TJ * ConvTile<CA + CB, COUT>::MS), 2) void k_conv_tiled(
     const TIN* __restrict__ A, const TIN* __restrict__ B, const bf16* __restrict__ W,
@@ -184,24 +203,31 @@ __global__ __launch_bounds__((64 * ConvTile<CA + CB, COUT>::TJ * ConvTile<CA + C
     constexpr int PLANE = (TJ + 2) * NQ * VOX * 4;           // BF16 elements per slot
     constexpr int CHUNKS = (TJ + 2) * VOX * G8;              // 8-channel chunks per plane
     constexpr int PER = (CHUNKS + NTH - 1) / NTH;
+    constexpr int U = 27 * NCK, NP = (U + 1) / 2;           // channel units, MFMA pairs
     constexpr bool WREG = MW == 1 && NCK <= 3;              // weights in registers
     __shared__ bf16 tile[4 * PLANE];
     const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) / MS, mg = (threadIdx.x >> 6) % MS;
-    const int col = lane & 15, q = lane >> 4;
+    const int col = lane & 15, q = lane >> 4, h = q >> 1, cq = 8 * (q & 1);
     const int nkt = L.nk / TK + (L.nk % TK ? 1 : 0);
     const int k0 = (int)(blockIdx.x % (unsigned)nkt) * TK;
     const int j0 = (int)(blockIdx.x / (unsigned)nkt) * TJ;
     const int i0 = blockIdx.y * CT_ICH, i1 = min(i0 + CT_ICH, L.ni);
     const int b = blockIdx.z;
     const size_t offA = (size_t)b * (UPA ? LA.n : L.n), offB = (size_t)b * L.n;
-    bf4 wr[WREG ? 27 * NCK : 1];
+    // the lane's unit of MFMA pair pr: 2 pr + h (tap, channel chunk)
+    auto unit = [&](int pr, int& t, int& c0) {
+        const int u = 2 * pr + h;
+        t = u < U ? u / NCK : -1;
+        c0 = u < U ? 16 * (u - (u / NCK) * NCK) + cq : 0;
+    };
+    bf8 wr[WREG ? NP : 1];
     if constexpr (WREG) {
 #pragma unroll
-        for (int t = 0; t < 27; ++t)
-#pragma unroll
-            for (int ck = 0; ck < NCK; ++ck)
-                wr[t * NCK + ck] =
-                    load4(W + ((size_t)t * COUT + mg * 16 + col) * CIN + 16 * ck + 4 * q);
+        for (int pr = 0; pr < NP; ++pr) {
+            int t, c0;
+            unit(pr, t, c0);
+            wr[pr] = t >= 0 ? load8(W + ((size_t)t * COUT + mg * 16 + col) * CIN + c0) : zero8();
+        }
     }
     // chunk c of a plane: row c / (VOX G8), voxel (c / G8) % VOX, channels
     // 8 g .. 8 g + 7 with g = c % G8 (quads 2 g, 2 g + 1)
@@ -244,13 +270,13 @@ __global__ __launch_bounds__((64 * ConvTile<CA + CB, COUT>::TJ * ConvTile<CA + C
         }
     };
     // this wave's output blocks are mg MW .. mg MW + MW - 1
-    auto wload = [&](int t, bf4 (&wa)[MW][NCK]) {
+    auto wload = [&](int pr, bf8 (&wa)[MW]) {
+        int t, c0;
+        unit(pr, t, c0);
 #pragma unroll
         for (int m = 0; m < MW; ++m)
-#pragma unroll
-            for (int ck = 0; ck < NCK; ++ck)
-                wa[m][ck] = load4(W + ((size_t)t * COUT + (mg * MW + m) * 16 + col) * CIN + 16 * ck +
-                                  4 * q);
+            wa[m] = t >= 0 ? load8(W + ((size_t)t * COUT + (mg * MW + m) * 16 + col) * CIN + c0)
+                           : zero8();
     };
     fetch(i0 - 1);
     publish((i0 - 1) & 3);
@@ -269,43 +295,42 @@ __global__ __launch_bounds__((64 * ConvTile<CA + CB, COUT>::TJ * ConvTile<CA + C
         for (int m = 0; m < MW; ++m)
 #pragma unroll
             for (int n = 0; n < NT; ++n) acc[m][n] = f4{0.f, 0.f, 0.f, 0.f};
-        // one tap: every column, also those past nk in a partial k-tile (their
-        // LDS voxels are zero and their results are not stored; a branch here
-        // would serialise the LDS reads)
-        auto tap = [&](int t, const bf4 (&wa)[MW][NCK]) {
-            const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
+        // one MFMA pair: every column, also those past nk in a partial k-tile
+        // (their LDS voxels are zero and their results are not stored; a
+        // branch here would serialise the LDS reads).  The lane reads the 8
+        // channels of its unit at its tap: two quads, VOX x 4 elements apart.
+        auto pair = [&](int pr, const bf8 (&wa)[MW]) {
+            int t, c0;
+            unit(pr, t, c0);
+            const int tt = t >= 0 ? t : 13;            // a padded unit reads the centre tap
+            const int di = tt / 9 - 1, dj = (tt / 3) % 3 - 1, dk = tt % 3 - 1;
             const bf16* base = tile + ((i + di) & 3) * PLANE +
-                               (((wave + 1 + dj) * NQ + q) * VOX + 1 + dk + col) * 4;
+                               (((wave + 1 + dj) * NQ + c0 / 4) * VOX + 1 + dk + col) * 4;
 #pragma unroll
-            for (int ck = 0; ck < NCK; ++ck)
+            for (int n = 0; n < NT; ++n) {
+                const bf4 lo = *(const bf4*)(base + n * 64);
+                const bf4 hi = *(const bf4*)(base + 4 * VOX + n * 64);
+                bf8 xb = bf8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                if (t < 0) xb = zero8();
 #pragma unroll
-                for (int n = 0; n < NT; ++n) {
-                    const bf4 xb = *(const bf4*)(base + ck * 16 * VOX + n * 64);
-#pragma unroll
-                    for (int m = 0; m < MW; ++m)
-                        acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[m][ck], xb, acc[m][n],
-                                                                            0, 0, 0);
-                }
+                for (int m = 0; m < MW; ++m) acc[m][n] = mfma32(wa[m], xb, acc[m][n]);
+            }
         };
         if constexpr (WREG) {
 #pragma unroll
-            for (int t = 0; t < 27; ++t) {
-                bf4 w1[1][NCK];
-#pragma unroll
-                for (int ck = 0; ck < NCK; ++ck) w1[0][ck] = wr[t * NCK + ck];
-                tap(t, w1);
+            for (int pr = 0; pr < NP; ++pr) {
+                bf8 w1[1] = {wr[pr]};
+                pair(pr, w1);
             }
-        } else {   // weights one tap ahead; the tap loop stays rolled
-            bf4 wa[MW][NCK], wn[MW][NCK];
+        } else {   // weights one pair ahead; the pair loop stays rolled
+            bf8 wa[MW], wn[MW];
             wload(0, wa);
 #pragma unroll 1
-            for (int t = 0; t < 27; ++t) {
-                if (t + 1 < 27) wload(t + 1, wn);
-                tap(t, wa);
+            for (int pr = 0; pr < NP; ++pr) {
+                if (pr + 1 < NP) wload(pr + 1, wn);
+                pair(pr, wa);
 #pragma unroll
-                for (int m = 0; m < MW; ++m)
-#pragma unroll
-                    for (int ck = 0; ck < NCK; ++ck) wa[m][ck] = wn[m][ck];
+                for (int m = 0; m < MW; ++m) wa[m] = wn[m];
             }
         }
         if (j < L.nj) {
@@ -346,15 +371,20 @@ __global__ __launch_bounds__(256) void k_conv16_tiled(const TIN* __restrict__ X,
                                                       TOUT* __restrict__ Y, NetLvl L) {
     __shared__ bf16 tile[4 * CT_PLANE];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int col = lane & 15, q = lane >> 4;
+    const int col = lane & 15, q = lane >> 4, h = q >> 1, cq = 8 * (q & 1);
     const int nkt = L.nk / CT_TK + (L.nk % CT_TK ? 1 : 0);
     const int k0 = (int)(blockIdx.x % (unsigned)nkt) * CT_TK;
     const int j0 = (int)(blockIdx.x / (unsigned)nkt) * CT_TJ;
     const int i0 = blockIdx.y * CT_ICH, i1 = min(i0 + CT_ICH, L.ni);
     const size_t xoff = (size_t)blockIdx.z * L.n;
-    bf4 wr[27];
+    // MFMA pair pr takes taps 2 pr (lane groups 0, 1) and 2 pr + 1 (2, 3);
+    // tap 27 is the zero padding of the last pair
+    bf8 wr[14];
 #pragma unroll
-    for (int t = 0; t < 27; ++t) wr[t] = load4(W + ((size_t)t * 16 + col) * 16 + 4 * q);
+    for (int pr = 0; pr < 14; ++pr) {
+        const int t = 2 * pr + h;
+        wr[pr] = t < 27 ? load8(W + ((size_t)t * 16 + col) * 16 + cq) : zero8();
+    }
     // chunk c of a plane: row c / (2 CT_VOX), voxel (c / 2) % CT_VOX, channels
     // 8 h .. 8 h + 7 with h = c % 2, i.e. quads 2 h and 2 h + 1
     bf8 pre[CT_PER_THREAD];
@@ -404,17 +434,21 @@ __global__ __launch_bounds__(256) void k_conv16_tiled(const TIN* __restrict__ X,
 #pragma unroll
         for (int n = 0; n < 4; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int t = 0; t < 27; ++t) {
+        for (int pr = 0; pr < 14; ++pr) {
+            const int t = 2 * pr + h < 27 ? 2 * pr + h : 13;   // the padding reads the centre
             const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
             const bf16* base = tile + ((i + di) & 3) * CT_PLANE +
-                               (((wave + 1 + dj) * 4 + q) * CT_VOX + 1 + dk + col) * 4;
+                               (((wave + 1 + dj) * 4 + 2 * (q & 1)) * CT_VOX + 1 + dk + col) * 4;
             // all four columns, also those past nk in a partial k-tile (their
             // LDS voxels are zero and their results are not stored): a branch
             // here would serialise the LDS reads
 #pragma unroll
             for (int n = 0; n < 4; ++n) {
-                const bf4 xb = *(const bf4*)(base + n * 64);
-                acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[t], xb, acc[n], 0, 0, 0);
+                const bf4 lo = *(const bf4*)(base + n * 64);
+                const bf4 hi = *(const bf4*)(base + 4 * CT_VOX + n * 64);
+                bf8 xb = bf8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+                if (pr == 13 && h == 1) xb = zero8();
+                acc[n] = mfma32(wr[pr], xb, acc[n]);
             }
         }
         if (j < L.nj) {
@@ -435,8 +469,9 @@ __global__ __launch_bounds__(256) void k_conv16_tiled(const TIN* __restrict__ X,
 
 // ---------------------------------------------------------------------------
 // Weight gradient: part[c][t][co][ci] = sum over the voxels of chunk c of
-// g[v][co] X[v + off(t)][ci]; one wave per (chunk, tap), 64 voxel blocks of
-// 16 along K.  X is the layer input ([up(A), B]), g the FP32 gradient of the
+// g[v][co] X[v + off(t)][ci]; one wave per (chunk, tap), blocks of 32 voxels
+// along K (v_mfma_f32_16x16x32_bf16: lane group q takes voxels 8 q .. 8 q + 7
+// of the block).  X is the layer input ([up(A), B]), g the FP32 gradient of the
 // pre-activation.
 // ---------------------------------------------------------------------------
 template <int CA, int CB, int COUT, bool UPA>
@@ -452,7 +487,8 @@ __global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ A, const
     const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
     const uint32_t c = blockIdx.x * 4u + wave;     // chunk id within the sample
     const int row = lane & 15;
-    const int kq = (lane >> 4) * 4;
+    const int kq = (lane >> 4) * 4;      // output rows
+    const int kv = (lane >> 4) * 8;      // voxels of the block
     f4 acc[MT][CT];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
@@ -463,17 +499,17 @@ __global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ A, const
     const uint32_t vbeg = c * chunk;
     const uint32_t vend = min(vbeg + chunk, L.n);
     if (c < (uint32_t)nchunk) {
-        for (uint32_t vb = vbeg; vb < vend; vb += 16) {
-            // this lane's 4 voxels along K
-            bf4 ga[MT];
-            bf4 xb[CT];
+        for (uint32_t vb = vbeg; vb < vend; vb += 32) {
+            // this lane's 8 voxels along K
+            bf8 ga[MT];
+            bf8 xb[CT];
 #pragma unroll
-            for (int m = 0; m < MT; ++m) ga[m] = bf4{(bf16)0.f, (bf16)0.f, (bf16)0.f, (bf16)0.f};
+            for (int m = 0; m < MT; ++m) ga[m] = zero8();
 #pragma unroll
-            for (int q = 0; q < CT; ++q) xb[q] = bf4{(bf16)0.f, (bf16)0.f, (bf16)0.f, (bf16)0.f};
+            for (int q = 0; q < CT; ++q) xb[q] = zero8();
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const uint32_t v = vb + kq + e;
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t v = vb + kv + e;
                 if (v >= vend) continue;
                 int i, j, k;
                 vdecode(v, L, i, j, k);
@@ -496,7 +532,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ A, const
             for (int m = 0; m < MT; ++m)
 #pragma unroll
                 for (int q = 0; q < CT; ++q)
-                    acc[m][q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ga[m], xb[q], acc[m][q], 0, 0, 0);
+                    acc[m][q] = mfma32(ga[m], xb[q], acc[m][q]);
         }
     }
     // lane holds D[co = m*16 + kq + r][ci = q*16 + row]
