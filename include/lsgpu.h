@@ -398,6 +398,27 @@ int ls_net_time_forward(ls_net* net, int reps, double* ms_per_forward);
  * channels), 0 the general implicit-GEMM kernel everywhere.  Both give the
  * same bits; the switch is for tests and timing. */
 int ls_net_set_conv(ls_net* net, int tiled);
+/* Training on the Galerkin energy loss with the samples held on the device
+ * (no reference counterpart: the training of the learned preconditioner).
+ * With u = P out (the coarse output copied to the solved fine children),
+ * loss = 1 - (u.r)^2 / (u.A u  e.A e) for r = A e, evaluated exactly on the
+ * network's grid from rc = P^T r and the Galerkin operator A_c = P^T A P.
+ * ls_net_store_alloc: `samples` sample slots and `operators` operator slots
+ * (trainable network, batch <= 8).  ls_net_store_operator: A_c as [4][n]
+ * FP32 (diagonal; couplings of voxel q to q + i, q + j, q + k).
+ * ls_net_store_sample: input [n][16] FP32 (kept as BF16), rc [n] FP32 and
+ * e.A e, with the slot of its operator.  ls_net_energy_step: one batch of
+ * stored samples (net batch size) through the forward pass, the loss and
+ * its output gradient on the device; for lr > 0 the backward pass and an
+ * Adam step, for lr <= 0 with grad_w given the backward pass alone, the
+ * weight gradient of the mean loss copied to grad_w (nw floats); losses[b]
+ * is each sample's loss. */
+int ls_net_store_alloc(ls_net* net, int samples, int operators);
+int ls_net_store_operator(ls_net* net, int slot, const float* ac);
+int ls_net_store_sample(ls_net* net, int slot, int op_slot, const float* x, const float* rc,
+                        double eae);
+int ls_net_energy_step(ls_net* net, const int* slots, float lr, double* losses,
+                       float* grad_w);
 
 /* Weights of the learned coarse-level preconditioner used by
  * LS_SOLVER_LEARNED (flat FP32 layout above).  The network runs on the grid
@@ -409,6 +430,17 @@ int ls_network_grid(ls_ctx* ctx, int* dims3);
  * learned preconditioner's restriction (for tests and training data):
  * out = [coarse grid][16] FP32 (values rounded to BF16). */
 int ls_debug_learned_pack(ls_ctx* ctx, const float* r, float* out, size_t out_count);
+/* Training data of the learned coarse correction (no reference counterpart;
+ * used by leachingsim_amd/learn.py).  ls_debug_learned_galerkin: the
+ * Galerkin operator P^T A^ P of the assembled scaled pressure operator on the
+ * network grid, [4][cn] (diagonal; couplings to the +i, +j, +k neighbours),
+ * count = 4 cn.  ls_debug_learned_sample: with the current pressure field p
+ * an iterate and p_star the reference solution (N doubles), the network
+ * input of r = A^ e, e = sqrt(d) (p_star - p) on the solved rows
+ * ([cn][16], x_count = 16 cn), rc = P^T r ([cn]) and e.A^ e. */
+int ls_debug_learned_galerkin(ls_ctx* ctx, float* ac, size_t count);
+int ls_debug_learned_sample(ls_ctx* ctx, const double* p_star, float* x, size_t x_count,
+                            float* rc, double* eae);
 
 #ifdef __cplusplus
 }

@@ -508,8 +508,11 @@ template <int N> struct FVec { typedef float type __attribute__((ext_vector_type
 #define LS_SCHUR_VW 4          // compacted values per lane (2 or 4)
 #endif
 constexpr int SCHUR_VW = LS_SCHUR_VW;
+// Rows wider than one wave's 4 x 64 compacted values (nk > 512) take
+// 1024-thread blocks, so that a block still holds LS_SCHUR_ROWS output rows.
+constexpr int SCHUR_WIDE_THREADS = 1024;
 // Widest row: a block holds WR waves per row and at least 3 rows.
-constexpr int SCHUR_MAX_NK = 2 * SCHUR_VW * WAVE * ((LS_SCHUR_THREADS / WAVE) / 3);
+constexpr int SCHUR_MAX_NK = 2 * SCHUR_VW * WAVE * ((SCHUR_WIDE_THREADS / WAVE) / 3);
 
 // Colour split of the 2 VW consecutive values a of one row whose first
 // element has colour par (0 black): b[x] = a[2x + par], r[x] = a[2x + 1 - par].
@@ -526,8 +529,8 @@ __device__ __forceinline__ void csplit(const typename FVec<2 * VW>::type& a, int
     r = par ? e : o;
 }
 
-template <int ICH, int VW>
-__global__ __launch_bounds__(LS_SCHUR_THREADS) void k_schur_apply(
+template <int ICH, int VW, int NT = LS_SCHUR_THREADS>
+__global__ __launch_bounds__(NT) void k_schur_apply(
     const float* __restrict__ p, float* __restrict__ q, const float* __restrict__ cx,
     const float* __restrict__ cy, const float* __restrict__ cz, RbGeo g,
     double* __restrict__ partials, const int* __restrict__ done) {
@@ -535,7 +538,7 @@ __global__ __launch_bounds__(LS_SCHUR_THREADS) void k_schur_apply(
     typedef typename FVec<2 * VW>::type VF;     // the lane's full-row values (faces)
     if (done != nullptr && *done) return;
     extern __shared__ float wsh[];                   // [3][NS][rowlen]
-    __shared__ double red[LS_SCHUR_THREADS / WAVE];
+    __shared__ double red[NT / WAVE];
     const int lane = threadIdx.x, wr = threadIdx.y, s = threadIdx.z;
     const int NS = blockDim.z, TJ = NS - 2;
     const int rowlen = blockDim.y * (WAVE * VW);
@@ -1359,7 +1362,7 @@ static int launch_rb(ls_ctx* c, const float* xn, const float* xc, float* y, cons
 // with WR waves per row (256 compacted values each); needs nk % 8 == 0 so a
 // lane's 4 compacted values are 16 B aligned.
 struct SchurShape {
-    int wr, tj;
+    int wr, tj, nt;
     dim3 grid, block;
     size_t lds;
 };
@@ -1368,11 +1371,13 @@ static bool schur_shape(ls_ctx* c, SchurShape* sh) {
     if (c->nk % (2 * SCHUR_VW) != 0) return false;
     const int nh = c->nk / 2;
     const int wr = (nh + SCHUR_VW * WAVE - 1) / (SCHUR_VW * WAVE);
-    const int ns_max = (LS_SCHUR_THREADS / WAVE) / wr;
+    const int nt = wr == 1 ? LS_SCHUR_THREADS : SCHUR_WIDE_THREADS;
+    const int ns_max = (nt / WAVE) / wr;
     const int tj = std::min(LS_SCHUR_ROWS, ns_max - 2);
     if (tj < 1) return false;
     sh->wr = wr;
     sh->tj = tj;
+    sh->nt = nt;
     sh->block = dim3(WAVE, wr, tj + 2);
     sh->lds = (size_t)3 * (tj + 2) * wr * SCHUR_VW * WAVE * sizeof(float);
     sh->grid = dim3((c->nj + tj - 1) / tj, (c->ni + SCHUR_ICH - 1) / SCHUR_ICH, 1);
@@ -1403,8 +1408,14 @@ static int schur_form(ls_ctx* c) {
 static int launch_schur(ls_ctx* c, const float* p, float* q, const int* done) {
     SchurShape sh;
     schur_shape(c, &sh);
-    hipLaunchKernelGGL((k_schur_apply<SCHUR_ICH, SCHUR_VW>), sh.grid, sh.block, sh.lds, c->stream, p, q,
-                       c->c32[0], c->c32[1], c->c32[2], rbgeo(c), c->partials, done);
+    if (sh.nt == SCHUR_WIDE_THREADS)
+        hipLaunchKernelGGL((k_schur_apply<SCHUR_ICH, SCHUR_VW, SCHUR_WIDE_THREADS>), sh.grid,
+                           sh.block, sh.lds, c->stream, p, q, c->c32[0], c->c32[1], c->c32[2],
+                           rbgeo(c), c->partials, done);
+    else
+        hipLaunchKernelGGL((k_schur_apply<SCHUR_ICH, SCHUR_VW>), sh.grid, sh.block, sh.lds,
+                           c->stream, p, q, c->c32[0], c->c32[1], c->c32[2], rbgeo(c),
+                           c->partials, done);
     return (int)(sh.grid.x * sh.grid.y);
 }
 
@@ -2128,6 +2139,105 @@ int learned_pack_debug(ls_ctx* c, const float* r_dev, float* out_dev) {
     hipLaunchKernelGGL(k_bf16_to_f32, dim3(ew_blocks(g.cn * 16)), dim3(EW_BLOCK), 0, c->stream,
                        c->net->x, out_dev, g.cn * 16);
     LS_LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Training data of the learned coarse correction, built on the device.
+// ---------------------------------------------------------------------------
+// A_c = P^T A^ P on the network grid, one thread per coarse voxel:
+// ac[q] = solved children - 2 x the scaled faces inside the aggregate;
+// ac[(1 + a) cn + q] = - the faces from the aggregate into its +a neighbour
+// (a stored face is 0 unless both of its voxels are solved rows).
+__global__ void k_lgalerkin(const int8_t* __restrict__ cls, const double* __restrict__ cx,
+                            const double* __restrict__ cy, const double* __restrict__ cz, CGeo g,
+                            float* __restrict__ ac) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < g.cn; q += gridDim.x * blockDim.x) {
+        const uint32_t kq = q % g.ck, jq = (q / g.ck) % g.cj, iq = q / ((uint32_t)g.ck * g.cj);
+        double d = 0.0, o[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int e = 0; e < 2; ++e) {
+                    const int i = 2 * iq + a, j = 2 * jq + b, k = 2 * kq + e;
+                    if (i >= g.ni || j >= g.nj || k >= g.nk) continue;
+                    const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
+                    d += cls[v] == 1 ? 1.0 : 0.0;
+                    const int side[3] = {a, b, e};
+                    const bool nxt[3] = {i + 1 < g.ni, j + 1 < g.nj, k + 1 < g.nk};
+                    const double f[3] = {cx[v], cy[v], cz[v]};
+                    for (int ax = 0; ax < 3; ++ax) {
+                        if (!nxt[ax]) continue;
+                        if (side[ax] == 0) d -= 2.0 * f[ax];
+                        else o[ax] -= f[ax];
+                    }
+                }
+        ac[q] = (float)d;
+        ac[g.cn + q] = (float)o[0];
+        ac[2 * (size_t)g.cn + q] = (float)o[1];
+        ac[3 * (size_t)g.cn + q] = (float)o[2];
+    }
+}
+
+// e = R (sd (p* - p)) in the scaled variables
+__global__ void k_lsample_e(const double* __restrict__ pstar, const double* __restrict__ p,
+                            const double* __restrict__ sd, const int8_t* __restrict__ cls,
+                            double* __restrict__ e, uint32_t N) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x)
+        e[v] = cls[v] == 1 ? sd[v] * (pstar[v] - p[v]) : 0.0;
+}
+
+// r masked to the solved rows (FP32 copy for the restriction), partial e.r
+__global__ void k_lsample_r(const double* __restrict__ e, const double* __restrict__ r,
+                            const int8_t* __restrict__ cls, float* __restrict__ r32, uint32_t N,
+                            double* __restrict__ partials) {
+    __shared__ double lds[EW_BLOCK / WAVE];
+    double acc = 0.0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        const double rv = cls[v] == 1 ? r[v] : 0.0;
+        r32[v] = (float)rv;
+        acc += e[v] * rv;
+    }
+    double a[1] = {acc};
+    block_reduce<1, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a[0];
+}
+
+int learned_galerkin_debug(ls_ctx* c, float* ac_dev) {
+    int st = learned_setup(c);
+    if (st) return st;
+    CGeo g = cgeo(c);
+    hipLaunchKernelGGL(k_lgalerkin, dim3(ew_blocks(g.cn)), dim3(EW_BLOCK), 0, c->stream, c->cls,
+                       c->c64[0], c->c64[1], c->c64[2], g, ac_dev);
+    LS_LAUNCH_CHECK();
+    return 0;
+}
+
+// One sample of the energy loss from the current pressure field p (an
+// iterate) and the reference solution p* (device, unscaled): the network
+// input of r = A^ e, e = R sd (p* - p) (into x_dev, [cn][16]), rc = P^T r
+// (rc_dev) and e.r = e.A^e.  Uses the solver's FP64 vectors as scratch.
+int learned_sample_debug(ls_ctx* c, const double* pstar_dev, float* x_dev, float* rc_dev,
+                         double* eae) {
+    int st = learned_setup(c);
+    if (st) return st;
+    const uint32_t N = (uint32_t)c->N;
+    const int nbe = ew_blocks(N);
+    hipLaunchKernelGGL(k_lsample_e, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, pstar_dev, c->p,
+                       c->sd, c->cls, c->x64, N);
+    launch_stencil<double>(c, c->x64, c->r64, c->c64, nullptr);
+    hipLaunchKernelGGL(k_lsample_r, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->x64, c->r64,
+                       c->cls, c->r32, N, c->partials);
+    LS_LAUNCH_CHECK();
+    std::vector<double> part(nbe);
+    LS_HIP(hipMemcpyAsync(part.data(), c->partials, nbe * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    LS_HIP(hipStreamSynchronize(c->stream));
+    double a = 0.0;
+    for (double x : part) a += x;
+    *eae = a;
+    if ((st = learned_pack_debug(c, c->r32, x_dev))) return st;
+    const CGeo g = cgeo(c);
+    LS_HIP(hipMemcpyAsync(rc_dev, c->rc, g.cn * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
     return 0;
 }
 

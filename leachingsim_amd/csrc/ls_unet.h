@@ -55,6 +55,19 @@ struct UNet {
     double* dpart = nullptr;
     double mask_count = 0.0;
     bool trainable = false;
+    // device-resident training store of the Galerkin energy loss
+    // (ls_net_store_alloc): inputs, restricted residuals P^T r, e.A e per
+    // sample; coarse Galerkin operators P^T A P per operator slot
+    int st_cap = 0, st_nops = 0;
+    __bf16* st_x = nullptr;      // [cap][n][16]
+    float* st_rc = nullptr;      // [cap][n]
+    std::vector<double> st_eae;  // [cap]
+    std::vector<int> st_op;      // operator slot of each sample
+    float* st_ac = nullptr;      // [nops][4][n]: diagonal, couplings to +i, +j, +k
+    float* st_tmp = nullptr;     // [n][16] FP32 upload staging
+    float* st_au = nullptr;      // [batch][n] A_c u
+    double* st_part = nullptr;   // [batch][2][EN_BLOCKS] dot partials
+    double* st_coef = nullptr;   // [batch][4] loss and gradient coefficients
     std::vector<void*> allocations;
     size_t bytes = 0;      // device bytes of the allocations
 };

@@ -142,6 +142,8 @@ def load_library(path: Path | str | None = None) -> C.CDLL:
         "ls_set_network": (i, [vp, C.POINTER(C.c_float), sz]),
         "ls_network_grid": (i, [vp, C.POINTER(C.c_int)]),
         "ls_debug_learned_pack": (i, [vp, vp, vp, sz]),
+        "ls_debug_learned_galerkin": (i, [vp, vp, sz]),
+        "ls_debug_learned_sample": (i, [vp, vp, vp, sz, vp, C.POINTER(d)]),
         "ls_debug_pressure_system": (i, [vp, vp, vp, vp]),
         "ls_debug_pressure_faces": (i, [vp, vp]),
         "ls_debug_schur_apply": (i, [vp, i, vp, vp, C.POINTER(d)]),
@@ -358,6 +360,29 @@ class GpuSimulation:
         out = np.empty(dims + (16,), np.float32)
         self._check(self.lib.ls_debug_learned_pack(self.h, a.ctypes.data, out.ctypes.data, out.size))
         return out
+
+    def learned_galerkin(self):
+        """Galerkin operator P^T A^ P of the assembled scaled pressure operator
+        on the network grid: [4, ci, cj, ck] float32 (diagonal, couplings to
+        the +i, +j, +k neighbours)."""
+        dims = self.network_grid()
+        out = np.empty((4,) + dims, np.float32)
+        self._check(self.lib.ls_debug_learned_galerkin(self.h, out.ctypes.data, out.size))
+        return out
+
+    def learned_sample(self, p_star):
+        """(network input [ci, cj, ck, 16], rc = P^T r [ci, cj, ck], e.A^ e) for
+        e = sqrt(d) (p_star - p) on the solved rows, p the current pressure
+        field, r = A^ e."""
+        ps = np.ascontiguousarray(p_star, dtype=np.float64)
+        assert ps.shape == self.shape
+        dims = self.network_grid()
+        x = np.empty(dims + (16,), np.float32)
+        rc = np.empty(dims, np.float32)
+        eae = C.c_double()
+        self._check(self.lib.ls_debug_learned_sample(self.h, ps.ctypes.data, x.ctypes.data,
+                                                     x.size, rc.ctypes.data, C.byref(eae)))
+        return x, rc, eae.value
 
     def pressure_system(self):
         sd = np.empty(self.shape)

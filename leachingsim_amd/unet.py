@@ -30,6 +30,11 @@ def _lib():
             "ls_net_time_forward": (i, [vp, i, C.POINTER(C.c_double)]),
             "ls_net_set_conv": (i, [vp, i]),
             "ls_net_step": (i, [vp, fp, C.c_float]),
+            "ls_net_store_alloc": (i, [vp, i, i]),
+            "ls_net_store_operator": (i, [vp, i, fp]),
+            "ls_net_store_sample": (i, [vp, i, i, fp, fp, C.c_double]),
+            "ls_net_energy_step": (i, [vp, C.POINTER(C.c_int), C.c_float,
+                                       C.POINTER(C.c_double), fp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -112,6 +117,32 @@ class Net:
         ms = C.c_double()
         self._check(self.lib.ls_net_time_forward(self.h, int(reps), C.byref(ms)))
         return ms.value
+
+    # --- Galerkin energy training with the samples held on the device ---
+    def store_alloc(self, samples, operators):
+        self._check(self.lib.ls_net_store_alloc(self.h, int(samples), int(operators)))
+
+    def store_operator(self, slot, ac):
+        a = np.ascontiguousarray(ac, dtype=np.float32)
+        assert a.shape == (4,) + self.shape
+        self._check(self.lib.ls_net_store_operator(self.h, int(slot), _fp(a)))
+
+    def store_sample(self, slot, op_slot, x, rc, eae):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        rc = np.ascontiguousarray(rc, dtype=np.float32)
+        assert x.shape == self.shape + (S.CIN,) and rc.shape == self.shape
+        self._check(self.lib.ls_net_store_sample(self.h, int(slot), int(op_slot), _fp(x), _fp(rc),
+                                                 float(eae)))
+
+    def energy_step(self, slots, lr, grad=False):
+        """One batch of stored samples: losses (and, with grad and lr = 0, the
+        weight gradient of their mean); lr > 0 takes an Adam step."""
+        sl = (C.c_int * self.batch)(*[int(q) for q in slots])
+        losses = (C.c_double * self.batch)()
+        gw = np.empty(self.nw, np.float32) if grad else None
+        self._check(self.lib.ls_net_energy_step(self.h, sl, float(lr), losses,
+                                                _fp(gw) if grad else None))
+        return np.array(losses[:]), gw
 
     def train_step(self, x, target, mask, lr):
         x = np.ascontiguousarray(x, dtype=np.float32)

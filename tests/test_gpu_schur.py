@@ -87,12 +87,14 @@ def schur_numpy(faces32, p_black, shape):
 
 @pytest.mark.parametrize("shape", [(24, 20, 40), (19, 33, 16), (5, 7, 1024), (34, 30, 512),
                                    (61, 700, 16), (97, 1300, 8),
-                                   (2, 2, 8), (2, 2, 16), (2, 7, 8), (17, 2, 24)])
+                                   (2, 2, 8), (2, 2, 16), (2, 7, 8), (17, 2, 24),
+                                   (4, 7, 1040), (3, 4, 2048), (18, 9, 1536)])
 def test_fused_schur_apply(shape):
     """Ragged tiles: nj not a multiple of the block's rows, ni not a
-    multiple of its planes, rows spanning 2 waves (nk = 1024), grids of
-    thousands of blocks; two planes or two rows (the smallest grids
-    ls_create takes), one lane's worth of k."""
+    multiple of its planes, rows spanning 2 to 4 waves (nk 1024 to 2048,
+    1024-thread blocks of 6, 3 or 2 output rows), grids of thousands of
+    blocks; two planes or two rows (the smallest grids ls_create takes),
+    one lane's worth of k."""
     img = small_mixed_geometry(shape, 3, air_slabs=shape[0] > 4)
     P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED)
     g = L.GpuSimulation(shape, P)
@@ -144,9 +146,10 @@ def test_two_pass_schur_apply_any_nk(shape):
 
 
 def test_fused_schur_rejects_wide_rows():
-    """Rows wider than the fused kernel's block (nk > 1024): the debug
-    entry refuses form 1, the solver runs the two-pass form."""
-    shape = (4, 4, 1040)
+    """Rows wider than the fused kernel's widest block (nk > 2560: 1024
+    threads hold 5 waves per row for 1 output row and 2 halo rows): the
+    debug entry refuses form 1, the solver runs the two-pass form."""
+    shape = (3, 4, 2576)
     img = small_mixed_geometry(shape, 3, air_slabs=False)
     P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED)
     g = L.GpuSimulation(shape, P)
@@ -156,7 +159,7 @@ def test_fused_schur_rejects_wide_rows():
     rep = g.phase(L.PH_PRESSURE, 1500.0)
     assert rep.status == 0
     p = np.ones(img.size // 2, np.float32)
-    with pytest.raises(L.LsError, match="nk <= 1024"):
+    with pytest.raises(L.LsError, match="nk <= 2560"):
         g.schur_apply(p, 1)
     q2, _ = g.schur_apply(p, 2)
     assert np.all(np.isfinite(q2))

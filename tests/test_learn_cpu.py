@@ -51,3 +51,35 @@ def test_energy_loss_gradient_and_bounds():
     # the exact error restricted as coarse correction captures a positive share
     loss_e, _ = learn.energy_loss_grad(learn.restrict_sum(e, (16, 16, 16)) / 8, r, e, faces, mask)
     assert loss_e < 1.0
+
+
+def test_galerkin_operator_is_the_energy_on_the_network_grid():
+    """out.A_c out == u.A^u and out.P^T r == u.r for u = P out, on an even and
+    an odd grid (the coarse grid padded past the fine one)."""
+    for n in (8, 7):
+        rng, mask, faces, e, r = setup(2, n)
+        dims = (8, 8, 8)
+        ac = learn.galerkin_operator(faces, mask, dims)
+        out = rng.standard_normal(dims)
+        u = learn.prolong(out, mask)
+        assert np.isclose((out * learn.coarse_apply(ac, out)).sum(),
+                          (u * learn.apply_op(u, faces)).sum(), rtol=1e-12)
+        rc = learn.restrict_sum(r * mask, dims)
+        assert np.isclose((out * rc).sum(), (u * r).sum(), rtol=1e-12)
+        # the coarse operator is symmetric
+        a = rng.standard_normal(dims)
+        assert np.isclose((a * learn.coarse_apply(ac, out)).sum(),
+                          (out * learn.coarse_apply(ac, a)).sum(), rtol=1e-12)
+
+
+def test_coarse_loss_matches_the_fine_loss():
+    rng, mask, faces, e, r = setup(3)
+    dims = (8, 8, 8)
+    out = rng.standard_normal(dims)
+    ac = learn.galerkin_operator(faces, mask, dims)
+    rc = learn.restrict_sum(r * mask, dims)
+    eae = float((e * r).sum())
+    lf, gf = learn.energy_loss_grad(out, r, e, faces, mask)
+    lc, gc = learn.coarse_loss_grad(out, rc, eae, ac)
+    assert np.isclose(lf, lc, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(gc, gf, rtol=1e-5, atol=1e-7 * np.abs(gf).max())
