@@ -349,7 +349,7 @@ def cpu_baseline(img, label_bytes, dt, crop: int, warmup: int, nsteps: int):
 
 KERNELS = {"stencil64": (64, "k_stencil_vec<double>"), "stencil32": (32, "k_stencil_vec<float>"),
            "rb_gather": (1, "k_rb_vec<1, W>"), "rb_apply": (2, "k_rb_vec<2, W>"),
-           "schur": (3, "k_schur_apply<16, 4>"), "transport_red": (4, "k_tschur_red_v<1, 0>"),
+           "schur": (3, "k_schur_apply<16, 4, T>"), "transport_red": (4, "k_tschur_red_v<1, 0>"),
            "transport_black": (5, "k_tschur_black_v<0, 1>")}
 
 
@@ -435,6 +435,7 @@ def stencil_only(img, label_bytes, kernel, reps, dt=0.5):
     nk = img.shape[2]
     kname = kname.replace("W", "4" if nk % 4 == 0 else "2") if nk % 2 == 0 else \
         kname.replace("k_rb_vec", "k_rb_scalar").replace(", W", "")
+    kname = kname.replace("T>", "512>" if nk <= 512 else "1024>")
     if precision in (L.TIME_TRED, L.TIME_TBLACK):
         P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED,
                              warm_start=L.WARM_PROJECTED)
@@ -621,7 +622,8 @@ def main():
     n, ms = sampled(hard, "prof_schur")
     if n:
         b = hard[-1]["prof_schur_bytes"]
-        kname = "k_schur_apply<16, 4>" if b == img.size * 16 else (
+        nt = 512 if img.shape[2] <= 512 else 1024     # threads per block of the fused apply
+        kname = f"k_schur_apply<16, 4, {nt}>" if b == img.size * 16 else (
             "k_rb_vec<1, 4>" if img.shape[2] % 4 == 0 else
             "k_rb_vec<1, 2>" if img.shape[2] % 2 == 0 else "k_rb_scalar<1>")
         tr = load_traffic() or {}

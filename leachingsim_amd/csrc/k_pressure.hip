@@ -124,9 +124,10 @@ __global__ __launch_bounds__(TK * TJ) void k_stencil(
 // neighbours across lanes come from wave shuffles, the lanes at the wave
 // edges load their single halo value.  y is written with non-temporal
 // stores (streamed once); the faces are read with ordinary loads (the -j
-// face row is re-read by the next row's wave).  Tile shape from a sweep at
-// 512^3 on gfx950 (ICH 2..32, TJ 4..16, load/store flavours): FP32
-// ICH 4 x TJ 8 and FP64 ICH 8 x TJ 4 were fastest.
+// face row is re-read by the next row's wave).  Tile shape from sweeps at
+// 512^3 on gfx950 (ICH 2..32, TJ 4..16, load/store flavours, 2 or 4 doubles
+// per lane): ICH 4 x TJ 8 was fastest in both precisions (FP64: 1.03 ms
+// against 1.07 ms for ICH 8 x TJ 4, 1.10-1.15 ms with 4 doubles per lane).
 // ---------------------------------------------------------------------------
 template <typename T, int W> struct VecOf;
 template <> struct VecOf<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
@@ -135,7 +136,7 @@ template <> struct VecOf<double, 2> { typedef double type __attribute__((ext_vec
 
 template <typename T> struct StencilShape;
 template <> struct StencilShape<float> { static constexpr int W = 4, ICH = 4, TJV = 8; };
-template <> struct StencilShape<double> { static constexpr int W = 2, ICH = 8, TJV = 4; };
+template <> struct StencilShape<double> { static constexpr int W = 2, ICH = 4, TJV = 8; };
 
 // MODE 0: y = A^ x with the block partials of x.y.  MODE 1 (FP64, the
 // residual of k_stencil<double, 1>): y = p2 = b - A^ x with the partials of

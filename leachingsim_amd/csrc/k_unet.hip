@@ -182,6 +182,12 @@ __global__ __launch_bounds__(256) void k_conv(const TIN* __restrict__ A, const T
 // multiples of 16, so a level's extents are multiples of 4.
 // ---------------------------------------------------------------------------
 constexpr int CT_ICH = 16;
+// Weight fragments of a wave's output block stay in registers up to this
+// many 16-channel input chunks per tap (else they are loaded one MFMA pair
+// ahead).
+#ifndef LS_CONV_WREG_NCK
+#define LS_CONV_WREG_NCK 3
+#endif
 
 // Tile of a layer: TJ rows x TK voxels per block, and MS waves per row that
 // split the output channels (each wave computes COUT / MS of them from the
@@ -190,7 +196,13 @@ template <int CIN, int COUT> struct ConvTile;
 template <> struct ConvTile<16, 32> { static constexpr int TJ = 2, TK = 64, MS = 2; };
 template <> struct ConvTile<32, 32> { static constexpr int TJ = 2, TK = 32, MS = 2; };
 template <> struct ConvTile<32, 64> { static constexpr int TJ = 1, TK = 32, MS = 4; };
-template <> struct ConvTile<48, 16> { static constexpr int TJ = 4, TK = 16, MS = 1; };
+#ifndef LS_CT48_TJ
+#define LS_CT48_TJ 4
+#endif
+#ifndef LS_CT48_TK
+#define LS_CT48_TK 16
+#endif
+template <> struct ConvTile<48, 16> { static constexpr int TJ = LS_CT48_TJ, TK = LS_CT48_TK, MS = 1; };
 
 template <int CA, int CB, int COUT, bool UPA, bool RELU, typename TIN, typename TOUT>
 __global__ __launch_bounds__((64 * ConvTile<CA + CB, COUT>::TJ * ConvTile<CA + CB, COUT>::MS), 2) void k_conv_tiled(
@@ -204,7 +216,7 @@ __global__ __launch_bounds__((64 * ConvTile<CA + CB, COUT>::TJ * ConvTile<CA + C
     constexpr int CHUNKS = (TJ + 2) * VOX * G8;              // 8-channel chunks per plane
     constexpr int PER = (CHUNKS + NTH - 1) / NTH;
     constexpr int U = 27 * NCK, NP = (U + 1) / 2;           // channel units, MFMA pairs
-    constexpr bool WREG = MW == 1 && NCK <= 3;              // weights in registers
+    constexpr bool WREG = MW == 1 && NCK <= LS_CONV_WREG_NCK;   // weights in registers
     __shared__ bf16 tile[4 * PLANE];
     const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) / MS, mg = (threadIdx.x >> 6) % MS;
     const int col = lane & 15, q = lane >> 4, h = q >> 1, cq = 8 * (q & 1);
