@@ -250,7 +250,7 @@ int init_state(ls_ctx* c, long long* nsul) {
     c->n_hist = 0;
     c->press_assembled = false;
     c->acid_zero = true;      // k_init: acid = 0, frac in {0, 1}
-    c->tprev_valid = false;
+    c->tprev_n = 0;
     c->frac_clean = true;
     return phase_properties(c);
 }

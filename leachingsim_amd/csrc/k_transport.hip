@@ -704,6 +704,11 @@ __device__ __forceinline__ SPair spair(uint32_t m, const TGeo& g) {
     return s;
 }
 
+// Order of the extrapolated transport start (1 linear, 2 quadratic).
+#ifndef LS_ADR_EXTRAP_ORDER
+#define LS_ADR_EXTRAP_ORDER 1
+#endif
+
 // The 14 colour-split row arrays are TROW_PAD doubles apart beyond their
 // N/2 length, so that arrays streamed together start at different offsets
 // modulo the HBM channel interleave (at 512^3, N/2 doubles are exactly
@@ -727,8 +732,11 @@ __device__ __forceinline__ double srow(const double* __restrict__ rows, uint32_t
 // black voxels also the solver's start y = D_b x.  one: system 1 is known to
 // be 0 with b = 0 (its b and x are not touched).  With xprev (the black part
 // of system 0's field at the previous transport solve) system 0 starts from
-// the linear extrapolation x + ratio (x - xprev), ratio = dt / dt_prev
-// (extrap), and xprev takes the current field for the next solve.
+// its extrapolation in time: order 1, x + ratio (x - xprev) with ratio =
+// dt / dt_prev; order 2 (with xprev2, the field of the solve before), the
+// quadratic through the three, e2 x + e1 xprev + e0 xprev2 (Lagrange weights
+// at the new time).  xprev (and xprev2) then take the current fields for the
+// next solve.
 // Transport (KIND 0) also zeroes the flux at Rock and Precipitate voxels
 // here, as the reference's assembly does (src/Simulation.cpp:1134-1136,
 // 1147-1149): no row reads the flux of such a voxel (a Rock neighbour is an
@@ -740,7 +748,8 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
     const double* __restrict__ sf, TGeo g, uint32_t N, AdrConst a,
     double* __restrict__ rows, const double* __restrict__ x0, const double* __restrict__ x1,
     double* __restrict__ b0, double* __restrict__ b1, int one, double* __restrict__ xs0,
-    double* __restrict__ xs1, double* __restrict__ xprev, int extrap, double ratio) {
+    double* __restrict__ xs1, double* __restrict__ xprev, double* __restrict__ xprev2,
+    int order, double ratio, double e0, double e1, double e2) {
     const uint32_t Nh = N / 2;
     const double* x1r = one ? x0 : x1;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
@@ -783,7 +792,13 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
         if (colour == 0) {
             double xstart = xa;
             if (xprev != nullptr) {
-                if (extrap) xstart = xa + ratio * (xa - xprev[m]);
+                const double xp = xprev[m];
+                if (xprev2 != nullptr) {
+                    const double xq = xprev2[m];
+                    if (order == 2) xstart = e2 * xa + e1 * xp + e0 * xq;
+                    xprev2[m] = xp;
+                }
+                if (order == 1) xstart = xa + ratio * (xa - xp);
                 xprev[m] = xa;
             }
             xs0[m] = R.diag * xstart;
@@ -1520,28 +1535,53 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     const int nbh = ew_blocks(Nh);
     double *xs[2] = {c->txs[0], c->txs[1]}, *wr[2] = {c->twr[0], c->twr[1]};
     if (tschur) {
-        // transport: conc starts from its linear extrapolation in time
-        double* xprev = nullptr;
-        if (KIND == 0 && c->P.adr_extrapolate) {
-            if (c->tprev == nullptr) {
-                if (hipMalloc((void**)&c->tprev, Nh * sizeof(double)) == hipSuccess) {
-                    c->allocations.push_back(c->tprev);
+        // transport: conc starts from its extrapolation in time through the
+        // fields of the last LS_ADR_EXTRAP_ORDER solves
+        double *xprev = nullptr, *xprev2 = nullptr;
+        auto history = [&](double** buf) {
+            if (*buf == nullptr) {
+                if (hipMalloc((void**)buf, Nh * sizeof(double)) == hipSuccess) {
+                    c->allocations.push_back(*buf);
                     c->grid_bytes += Nh * sizeof(double);
-                } else {        // out of memory: start from the old field instead
-                    c->tprev = nullptr;
+                } else {        // out of memory: a lower order instead
+                    *buf = nullptr;
                     (void)hipGetLastError();
                 }
             }
-            xprev = c->tprev;
+            return *buf;
+        };
+        if (KIND == 0 && c->P.adr_extrapolate) {
+            xprev = history(&c->tprev);
+            if (xprev != nullptr && LS_ADR_EXTRAP_ORDER >= 2) xprev2 = history(&c->tprev2);
         }
-        const bool extrap = xprev != nullptr && c->tprev_valid && c->tprev_dt > 0.0;
+        const double h = 1.0 / a.inv_dt;
+        int order = 0;
+        double ratio = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
+        if (xprev != nullptr && c->tprev_n >= 1 && c->tprev_dt > 0.0) {
+            order = 1;
+            ratio = h / c->tprev_dt;
+        }
+        if (xprev2 != nullptr && c->tprev_n >= 2 && c->tprev_dt > 0.0 && c->tprev2_dt > 0.0) {
+            // nodes t2 = 0 (x), t1 = -dt_prev (xprev), t0 = -(dt_prev + dt_prev2) (xprev2),
+            // evaluated at t = h
+            const double t1 = -c->tprev_dt, t0 = -(c->tprev_dt + c->tprev2_dt);
+            order = 2;
+            e2 = (h - t0) * (h - t1) / ((0.0 - t0) * (0.0 - t1));
+            e1 = (h - t0) * (h - 0.0) / ((t1 - t0) * (t1 - 0.0));
+            e0 = (h - t1) * (h - 0.0) / ((t0 - t1) * (t0 - 0.0));
+        }
         hipLaunchKernelGGL((k_adr_rows_split<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
                            c->img, c->qx, c->qy, c->qz, sf, g, N, a, c->arows, x[0], x[1],
-                           c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], xprev,
-                           extrap ? 1 : 0, extrap ? (1.0 / a.inv_dt) / c->tprev_dt : 0.0);
+                           c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], xprev, xprev2, order,
+                           ratio, e0, e1, e2);
         if (KIND == 0) {
-            c->tprev_valid = xprev != nullptr;
-            c->tprev_dt = 1.0 / a.inv_dt;
+            if (xprev == nullptr) {
+                c->tprev_n = 0;
+            } else {
+                c->tprev2_dt = c->tprev_dt;
+                c->tprev_dt = h;
+                c->tprev_n = std::min(c->tprev_n + 1, xprev2 != nullptr ? 2 : 1);
+            }
         }
         LS_LAUNCH_CHECK();
     } else if (stored) {       // rows, right-hand sides and D^-1 once per solve

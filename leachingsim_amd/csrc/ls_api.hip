@@ -384,7 +384,7 @@ int ls_set_params(ls_ctx* c, const ls_params* p) {
     if (int st = validate_params(c, p)) return st;
     const ls_params old = c->P;
     c->P = *p;
-    c->tprev_valid = false;   // the transport's path may change with the settings
+    c->tprev_n = 0;   // the transport's path may change with the settings
     if (int st = pool_setup(c)) {
         // the old layout back (its bytes were freed first)
         const std::string why = c->err;
@@ -443,7 +443,7 @@ int ls_upload(ls_ctx* c, int field, const void* host, size_t elem_bytes) {
     LS_HIP(hipMemcpyAsync(d, host, N * eb, hipMemcpyHostToDevice, c->stream));
     LS_HIP(hipStreamSynchronize(c->stream));
     if (field == LS_F_PRESSURE) c->n_hist = 0;   // history no longer matches p
-    if (field == LS_F_CONC) c->tprev_valid = false;
+    if (field == LS_F_CONC) c->tprev_n = 0;
     if (field == LS_F_ACID) {                    // +0.0 bit patterns only
         const uint64_t* a = (const uint64_t*)host;
         bool z = true;
@@ -575,7 +575,7 @@ int ls_rollback(ls_ctx* c) {
     if (st) return st;
     c->acid_zero = c->b_acid_zero;
     c->frac_clean = c->b_frac_clean;
-    c->tprev_valid = false;   // the last transport solve belonged to the failed attempt
+    c->tprev_n = 0;   // the last transport solve belonged to the failed attempt
     c->n_hist = 0;   // the pressure history no longer belongs to this state
     // rebuild the derived properties (src/Simulation.cpp:2347-2352)
     st = phase_properties(c);

@@ -134,7 +134,11 @@ struct ls_ctx {
     // use), its dt, and whether it belongs to the current state's history
     double* tprev = nullptr;
     double tprev_dt = 0.0;
-    bool tprev_valid = false;
+    // and the one before (N/2, for the second-order start), its dt, and how
+    // many of the two belong to the current state's history (0, 1, 2)
+    double* tprev2 = nullptr;
+    double tprev2_dt = 0.0;
+    int tprev_n = 0;
     // --- reductions ---
     double* partials = nullptr;     // MAX_PARTIALS * 8 values
     CgScalars* cg = nullptr;
