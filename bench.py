@@ -208,6 +208,7 @@ class Run:
                "ms_per_step": 1e3 * self.window_s / self.steps,
                "press_iters_per_step": [r["press_iters"] for r in reps],
                "adr_iters_per_step": [r["adr_iters"] for r in reps],
+               "adr_cycles_per_step": [r["adr_cycles"] for r in reps],
                "warm_relres": [r["press_relres0"] for r in reps],
                "final_relres_max": max(r["press_relres"] for r in reps),
                "fallbacks": sum(r["press_fallback"] for r in reps),
@@ -350,7 +351,9 @@ def cpu_baseline(img, label_bytes, dt, crop: int, warmup: int, nsteps: int):
 KERNELS = {"stencil64": (64, "k_stencil_vec<double>"), "stencil32": (32, "k_stencil_vec<float>"),
            "rb_gather": (1, "k_rb_vec<1, W>"), "rb_apply": (2, "k_rb_vec<2, W>"),
            "schur": (3, "k_schur_apply<16, 4, T>"), "transport_red": (4, "k_tschur_red_v<1, 0>"),
-           "transport_black": (5, "k_tschur_black_v<0, 1>")}
+           "transport_black": (5, "k_tschur_black_v<0, 1>"),
+           "transport_red32": (6, "k_tmix_red<4, 1>"),
+           "transport_black32": (7, "k_tmix_black<0, 4, 1>")}
 
 
 def stencil_rooflines(img, label_bytes, reps=20, rounds=5):
@@ -436,9 +439,12 @@ def stencil_only(img, label_bytes, kernel, reps, dt=0.5):
     kname = kname.replace("W", "4" if nk % 4 == 0 else "2") if nk % 2 == 0 else \
         kname.replace("k_rb_vec", "k_rb_scalar").replace(", W", "")
     kname = kname.replace("T>", "512>" if nk <= 512 else "1024>")
-    if precision in (L.TIME_TRED, L.TIME_TBLACK):
+    if precision in (L.TIME_TRED, L.TIME_TBLACK, L.TIME_TRED32, L.TIME_TBLACK32):
+        # the FP64 passes as the FP64 form runs them, the FP32 ones in the
+        # mixed-precision form
         P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED,
-                             warm_start=L.WARM_PROJECTED)
+                             warm_start=L.WARM_PROJECTED,
+                             adr_mixed=int(precision in (L.TIME_TRED32, L.TIME_TBLACK32)))
         sim = L.GpuSimulation(img.shape, P, label_bytes)
         sim.read_labels(img)
         sim.init_state()
@@ -530,6 +536,10 @@ def main():
     ap.add_argument("--warm-history", type=int, default=None,
                     help="projection basis size of every projected run (default: the "
                          "library's)")
+    ap.add_argument("--adr-mixed", type=int, default=None, choices=[0, 1],
+                    help="transport of the headline and learned runs: 1 FP32 BiCGStab "
+                         "cycles inside FP64 refinement, 0 FP64 BiCGStab (default: the "
+                         "library's, 1); the plain FP64 runs always use 0")
     ap.add_argument("--weights", default=None, help="U-Net weights (float32 file)")
     ap.add_argument("--no-baseline", action="store_true",
                     help="skip the plain FP64 runs (projected and previous-pressure)")
@@ -570,6 +580,8 @@ def main():
     if warm is None:
         warm = L.WARM_LEARNED if head == L.SOLVER_LEARNED else L.WARM_PROJECTED
     extra = {} if args.warm_history is None else {"warm_history": args.warm_history}
+    if args.adr_mixed is not None:
+        extra["adr_mixed"] = args.adr_mixed
     weights = None
     from leachingsim_amd.learn import WEIGHTS, load_weights
     if head == L.SOLVER_LEARNED or warm == L.WARM_LEARNED or not args.no_learned:
@@ -590,14 +602,15 @@ def main():
     if not args.no_baseline:
         # like for like: plain FP64 PCG with the headline's warm start
         bw = warm if warm != L.WARM_LEARNED else L.WARM_PROJECTED
-        base = Run(L.SOLVER_PLAIN64, bw, **extra).run(img, lb, dt, W, K, T, keep_fields=True)
+        base = Run(L.SOLVER_PLAIN64, bw, **{**extra, "adr_mixed": 0}).run(
+            img, lb, dt, W, K, T, keep_fields=True)
         runs[base.name] = base
         errors = field_errors(hr.fields, base.fields)
         hr.fields = base.fields = None
         # the reference's algorithm: plain FP64 PCG from the previous pressure;
         # every step is a full solve, so a short run from step 0 is timed
         npv = max(1, args.previous_steps)
-        prev = Run(L.SOLVER_PLAIN64, L.WARM_PREVIOUS).run(img, lb, dt, 0, npv, npv)
+        prev = Run(L.SOLVER_PLAIN64, L.WARM_PREVIOUS, adr_mixed=0).run(img, lb, dt, 0, npv, npv)
         runs[prev.name] = prev
     learned = None
     if not args.no_learned and head != L.SOLVER_LEARNED:
@@ -614,10 +627,14 @@ def main():
     rl = {}
     n, ms = sampled(reps, "prof_tred")
     if n:
-        sysn = "1" if reps[-1]["prof_tred_bytes"] == img.size * 32 else "3"
-        rl["transport_red_pass"] = kernel_roofline(
-            f"k_tschur_red_v<{sysn}, 0>" if img.shape[2] % 4 == 0 else f"k_tschur_red<{sysn}, 0>",
-            reps[-1]["prof_tred_bytes"], ms, n)
+        # bytes per voxel of the grid: 32 / 40 the FP64 pass for one / two
+        # systems, 16 / 20 the FP32 pass of the mixed-precision form
+        per = reps[-1]["prof_tred_bytes"] / img.size
+        sysn = "1" if per in (32, 16) else "3"
+        kname = (f"k_tmix_red<4, {sysn}>" if per in (16, 20) else
+                 f"k_tschur_red_v<{sysn}, 0>" if img.shape[2] % 4 == 0 else
+                 f"k_tschur_red<{sysn}, 0>")
+        rl["transport_red_pass"] = kernel_roofline(kname, reps[-1]["prof_tred_bytes"], ms, n)
     hard = hr.reports[:min(2, len(hr.reports))]
     n, ms = sampled(hard, "prof_schur")
     if n:
@@ -632,7 +649,8 @@ def main():
             kname, b, ms, n, t.get("bytes_per_launch") if t.get("grid") == list(img.shape)
             else None)
     tr = load_traffic() or {}
-    t = tr.get("transport_red", {})
+    t = tr.get("transport_red32" if rl.get("transport_red_pass", {}).get("kernel", "")
+               .startswith("k_tmix") else "transport_red", {})
     if "transport_red_pass" in rl and t.get("grid") == list(img.shape) and \
             t.get("algorithmic_bytes") == rl["transport_red_pass"]["bytes_per_launch"]:
         rl["transport_red_pass"]["traffic"] = t.get("bytes_per_launch")

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LSGPU_ABI_VERSION 9
+#define LSGPU_ABI_VERSION 10
 
 /* Labels (src/compiler_opts.h:69-82). */
 enum ls_label {
@@ -136,9 +136,9 @@ typedef struct ls_params {
                                 half the iterations of Jacobi-PCG), on vectors
                                 compacted to the black voxels with one fused
                                 pass per operator apply when nk % 8 == 0,
-                                nk <= 1024 and a row of nk / 2 values fills at
-                                least 3/8 of the fused kernel's 256-value wave
-                                rows (so 192 <= nk <= 1024), else as 2; 3: the
+                                nk <= 2560 and a row of nk / 2 values fills at
+                                least 3/8 of the lanes of its waves (256 values
+                                each; so 192 <= nk <= 2560), else as 2; 3: the
                                 fused form wherever nk allows it; 2: the same
                                 on full-length vectors in two passes (any nk,
                                 vectorised when nk % 4 == 0); 0: inner CG on
@@ -149,10 +149,20 @@ typedef struct ls_params {
                                 red values recovered after convergence; needs
                                 nk even; 0: BiCGStab on the full system         */
     int adr_extrapolate;     /* transport on the Schur complement (1): conc starts
-                                from its linear extrapolation in time through the
-                                last two solves, x + (dt / dt_prev) (x - x_prev),
-                                instead of the old field (the reference's start,
+                                from its extrapolation in time through the fields
+                                of the last solves (quadratic through the last
+                                three, linear after the first solve) instead of
+                                the old field (the reference's start,
                                 src/Simulation.cpp:1247); 0: the old field       */
+    int adr_mixed;           /* transport and implicit saturation on the Schur
+                                complement with nk % 8 == 0 (1): BiCGStab in FP32
+                                on FP32 copies of the rows, for a correction of
+                                the FP64 iterate, inside FP64 refinement cycles
+                                whose FP64 residual is the stop test; 0: BiCGStab
+                                in FP64                                          */
+    double adr_mixed_inner_rtol; /* adr_mixed: a cycle's FP32 solve stops at this
+                                reduction of its own residual, or earlier at half
+                                the FP64 tolerance (1e-5)                         */
 } ls_params;
 
 /* Fields that can be uploaded / downloaded.  Element types are fixed:
@@ -243,6 +253,11 @@ typedef struct ls_step_report {
     double prof_tred_ms;
     int prof_tred_n;
     double prof_tred_bytes;
+    /* Mixed-precision transport (adr_mixed): FP64 refinement cycles of the
+     * last transport solve, and 1 if it handed over to FP64 BiCGStab (a
+     * cycle that did not cut the FP64 residual by 16x). */
+    int adr_cycles;
+    int adr_fallback;
 } ls_step_report;
 
 typedef struct ls_ctx ls_ctx;
@@ -325,9 +340,14 @@ int ls_debug_schur_apply(ls_ctx* ctx, int form, const float* p, float* q, double
  * (run ls_step or the transport phase first; the pressure phase reuses those
  * bytes): LS_TIME_TRED the red pass w = R' p, R' = D_r^-1 A_rb D_b^-1 (32 B
  * per voxel of rows, p and w), LS_TIME_TBLACK the black pass z = p - A_br w
- * with its dot (40 B per voxel), both for the conc system alone. */
+ * with its dot (40 B per voxel), both for the conc system alone; the same
+ * two passes of the mixed-precision form on its FP32 rows and vectors
+ * (LS_TIME_TRED32 16 B, LS_TIME_TBLACK32 20 B per voxel; adr_mixed with
+ * nk % 8 == 0). */
 #define LS_TIME_TRED 4
 #define LS_TIME_TBLACK 5
+#define LS_TIME_TRED32 6
+#define LS_TIME_TBLACK32 7
 int ls_time_stencil(ls_ctx* ctx, int precision, int reps, double* ms_per_launch,
                     double* bytes_per_launch);
 /* Copy bytes within device memory on the context's stream, timed with HIP

@@ -464,13 +464,27 @@ constexpr int ADR_TJ = 4, ADR_ICH = 8;
 
 static dim3 adr_vec_grid(const ls_ctx* c);
 
+// The vectorised Schur passes' blocks: TSV_TR waves, each over TSV_RPW rows.
+constexpr int TSV_TR = 4;     // waves (rows in flight) per block
+constexpr int TSV_RPW = 4;    // rows per wave
+
 // Stored rows for nk even when the row kernels' block partials fit; the
 // red-black Schur form on them unless switched off.
-void transport_form(const ls_ctx* c, bool* stored, bool* schur) {
+// With mixed: the Schur form's vectorised passes (nk % 4 == 0, the row count
+// within the grid's y limit) and, where nk % 8 == 0 and adr_mixed is set,
+// FP32 BiCGStab cycles inside FP64 refinement.
+void transport_form(const ls_ctx* c, bool* stored, bool* schur, bool* mixed) {
     const dim3 vg = adr_vec_grid(c);
     const size_t nbm = (size_t)vg.x * vg.y * vg.z;
     *stored = c->nk % 2 == 0 && 4 * nbm <= 8 * (size_t)MAX_PARTIALS;
     *schur = *stored && c->P.adr_schur != 0;
+    if (mixed) {
+        const size_t gx = ((size_t)c->nk / 4 + WAVE - 1) / WAVE;
+        const size_t gy = ((size_t)c->ni * c->nj + TSV_TR * TSV_RPW - 1) / (TSV_TR * TSV_RPW);
+        const bool vec = *schur && c->nk % 4 == 0 && gy <= 65535 &&
+                         4 * gx * gy <= 8 * (size_t)MAX_PARTIALS;
+        *mixed = vec && c->nk % 8 == 0 && c->P.adr_mixed != 0;
+    }
 }
 
 static dim3 adr_vec_grid(const ls_ctx* c) {
@@ -718,9 +732,18 @@ __device__ __forceinline__ SPair spair(uint32_t m, const TGeo& g) {
 // channel).
 constexpr uint32_t TROW_PAD = 544;     // 4352 B
 __device__ __forceinline__ size_t rstride(uint32_t Nh) { return (size_t)Nh + TROW_PAD; }
+// The FP32 copies of the twelve off-diagonal arrays (mixed-precision
+// transport), the same 4352 B apart.
+constexpr uint32_t TROW_PAD32 = 1088;
+__host__ __device__ __forceinline__ size_t rstride32(uint32_t Nh) { return (size_t)Nh + TROW_PAD32; }
 
 // Doubles of the stored rows: 7 N for the full-system layout, 14 (N/2 +
 // TROW_PAD) for the colour-split one.
+// Floats of the FP32 off-diagonal copies of the mixed-precision form.
+size_t transport_rows32_floats(const ls_ctx* c) {
+    return 12 * rstride32((uint32_t)(c->N / 2));
+}
+
 size_t transport_rows_doubles(const ls_ctx* c) {
     return std::max(7 * c->N, 14 * (c->N / 2 + TROW_PAD));
 }
@@ -973,8 +996,6 @@ __global__ __launch_bounds__(EW_BLOCK) void k_tschur_black(
 // run over the faces in the order of the scalar passes (+i, -i, +j, -j, +k,
 // -k), absent neighbours contributing 0 x 0.
 // ---------------------------------------------------------------------------
-constexpr int TSV_TR = 4;     // waves (rows in flight) per block
-constexpr int TSV_RPW = 4;    // rows per wave
 
 typedef double d2 __attribute__((ext_vector_type(2)));
 
@@ -1018,7 +1039,8 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
     double* __restrict__ rows, TVGeo g, uint32_t Nh, const double* __restrict__ p0,
     const double* __restrict__ p1, double* __restrict__ w0, double* __restrict__ w1,
     const BicgScalars* __restrict__ bs, const double* __restrict__ bf0,
-    const double* __restrict__ bf1, double* __restrict__ xf0, double* __restrict__ xf1) {
+    const double* __restrict__ bf1, double* __restrict__ xf0, double* __restrict__ xf1,
+    float* __restrict__ rows32) {
     if (MODE == 0 && bs->all_done) return;
     const uint32_t m0 = 2u * (blockIdx.x * WAVE + threadIdx.x);
     const bool ok = m0 < g.nhr;
@@ -1065,6 +1087,13 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
 #pragma unroll
                 for (int f = 0; f < 6; ++f)
                     *(d2*)(rows + ((size_t)(f + 1) * 2 + 1) * rstride(Nh) + idx) = o[f];
+                if (rows32 != nullptr) {
+                    typedef float f2v __attribute__((ext_vector_type(2)));
+#pragma unroll
+                    for (int f = 0; f < 6; ++f)
+                        *(f2v*)(rows32 + ((size_t)f * 2 + 1) * rstride32(Nh) + idx) =
+                            f2v{(float)o[f].x, (float)o[f].y};
+                }
             }
         }
         if (SYS & 1) {
@@ -1127,7 +1156,7 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
     double* __restrict__ partials, double* __restrict__ rh0, double* __restrict__ rh1,
     double* __restrict__ p0, double* __restrict__ p1, double* __restrict__ v0,
     double* __restrict__ v1, const double* __restrict__ bf0, const double* __restrict__ bf1,
-    double* __restrict__ xf0, double* __restrict__ xf1) {
+    double* __restrict__ xf0, double* __restrict__ xf1, float* __restrict__ rows32) {
     constexpr bool S0 = (SYS & 1) != 0, S1 = (SYS & 2) != 0;
     constexpr bool START = MODE >= 2;
     if (!START && bs->all_done) return;
@@ -1213,18 +1242,29 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
         }
         if (!ok) continue;
         if (START) {
+            if (MODE == 2 && rows32 != nullptr) {
+                typedef float f2v __attribute__((ext_vector_type(2)));
+#pragma unroll
+                for (int f = 0; f < 6; ++f)
+                    *(f2v*)(rows32 + ((size_t)f * 2) * rstride32(Nh) + idx) =
+                        f2v{(float)o[f].x, (float)o[f].y};
+            }
             // the pair's voxels of both colours are the full-layout quad
             // 2 idx .. 2 idx + 3, the black ones at 2 idx + q, 2 idx + 2 + q;
             // the final pass writes the whole quad: x (black) and the red
-            // values u the red pass left in wr at the same compacted index
+            // values u the red pass left in wr at the same compacted index.
+            // rh, p and v (FP64 BiCGStab) are not written when null (the
+            // mixed-precision form lays its FP32 vectors over them)
             if (S0) {
                 const d2 lo = *(const d2*)(bf0 + 2u * idx), hi = *(const d2*)(bf0 + 2u * idx + 2);
                 const d2 bbk = q ? d2{lo.y, hi.y} : d2{lo.x, hi.x};
                 const d2 ra = bbk - ya;
                 *(d2*)(y0 + idx) = ra;
-                *(d2*)(rh0 + idx) = ra;
-                *(d2*)(p0 + idx) = zero;
-                *(d2*)(v0 + idx) = zero;
+                if (rh0 != nullptr) {
+                    *(d2*)(rh0 + idx) = ra;
+                    *(d2*)(p0 + idx) = zero;
+                    *(d2*)(v0 + idx) = zero;
+                }
                 if (MODE == 3) {
                     const d2 xu = xa / d;
                     *(d2*)(xf0 + 2u * idx) = q ? d2{c0.x, xu.x} : d2{xu.x, c0.x};
@@ -1241,9 +1281,11 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
                 const d2 bbk = q ? d2{lo.y, hi.y} : d2{lo.x, hi.x};
                 const d2 rb = bbk - yb;
                 *(d2*)(y1 + idx) = rb;
-                *(d2*)(rh1 + idx) = rb;
-                *(d2*)(p1 + idx) = zero;
-                *(d2*)(v1 + idx) = zero;
+                if (rh1 != nullptr) {
+                    *(d2*)(rh1 + idx) = rb;
+                    *(d2*)(p1 + idx) = zero;
+                    *(d2*)(v1 + idx) = zero;
+                }
                 if (MODE == 3) {
                     const d2 xu = xb / d;
                     *(d2*)(xf1 + 2u * idx) = q ? d2{c1.x, xu.x} : d2{xu.x, c1.x};
@@ -1439,6 +1481,591 @@ __global__ void k_bicg_x(double* __restrict__ x0, double* __restrict__ x1,
     }
 }
 
+// The Schur form's three vector updates on pairs of values, 16 B per lane
+// and array (k_bicg_p, k_bicg_s, k_bicg_x without PRE otherwise): an odd
+// length leaves its last value to thread 0 of block 0.  Buffers start on
+// 16-byte boundaries.
+__device__ __forceinline__ d2 ld2(const double* __restrict__ a, uint32_t u) {
+    return *(const d2*)(a + 2 * (size_t)u);
+}
+__device__ __forceinline__ void st2(double* __restrict__ a, uint32_t u, d2 x) {
+    *(d2*)(a + 2 * (size_t)u) = x;
+}
+
+__global__ __launch_bounds__(EW_BLOCK) void k_bicg_p_v(
+    double* __restrict__ p0, double* __restrict__ p1, const double* __restrict__ r0_,
+    const double* __restrict__ r1_, const double* __restrict__ v0,
+    const double* __restrict__ v1, const BicgScalars* __restrict__ bs, uint32_t N) {
+    if (bs->all_done) return;
+    const bool a0 = !bs->done[0], a1 = !bs->done[1];
+    const double be0 = bs->beta[0], om0 = bs->omega[0], be1 = bs->beta[1], om1 = bs->omega[1];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t u = gid; u < N / 2; u += gridDim.x * blockDim.x) {
+        if (a0) st2(p0, u, ld2(r0_, u) + be0 * (ld2(p0, u) - om0 * ld2(v0, u)));
+        if (a1) st2(p1, u, ld2(r1_, u) + be1 * (ld2(p1, u) - om1 * ld2(v1, u)));
+    }
+    if ((N & 1) && gid == 0) {
+        const uint32_t v = N - 1;
+        if (a0) p0[v] = r0_[v] + be0 * (p0[v] - om0 * v0[v]);
+        if (a1) p1[v] = r1_[v] + be1 * (p1[v] - om1 * v1[v]);
+    }
+}
+
+__global__ __launch_bounds__(EW_BLOCK) void k_bicg_s_v(
+    const double* __restrict__ r0_, const double* __restrict__ r1_,
+    const double* __restrict__ v0, const double* __restrict__ v1, double* __restrict__ s0,
+    double* __restrict__ s1, const BicgScalars* __restrict__ bs, uint32_t N,
+    double* __restrict__ partials) {
+    if (bs->all_done) return;
+    __shared__ double lds[2 * EW_BLOCK / WAVE];
+    const bool a0 = !bs->done[0], a1 = !bs->done[1];
+    const double al0 = bs->alpha[0], al1 = bs->alpha[1];
+    double q0 = 0.0, q1 = 0.0;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t u = gid; u < N / 2; u += gridDim.x * blockDim.x) {
+        if (a0) {
+            const d2 sv = ld2(r0_, u) - al0 * ld2(v0, u);
+            st2(s0, u, sv);
+            q0 += sv.x * sv.x;
+            q0 += sv.y * sv.y;
+        }
+        if (a1) {
+            const d2 sv = ld2(r1_, u) - al1 * ld2(v1, u);
+            st2(s1, u, sv);
+            q1 += sv.x * sv.x;
+            q1 += sv.y * sv.y;
+        }
+    }
+    if ((N & 1) && gid == 0) {
+        const uint32_t v = N - 1;
+        if (a0) {
+            const double sv = r0_[v] - al0 * v0[v];
+            s0[v] = sv;
+            q0 += sv * sv;
+        }
+        if (a1) {
+            const double sv = r1_[v] - al1 * v1[v];
+            s1[v] = sv;
+            q1 += sv * sv;
+        }
+    }
+    double q[2] = {q0, q1};
+    block_reduce<2, RED_SUM, RED_SUM>(q, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = q[0];
+        partials[gridDim.x + blockIdx.x] = q[1];
+    }
+}
+
+// x += alpha p + omega s; r = s - omega t; partials r.r, r0.r
+__global__ __launch_bounds__(EW_BLOCK) void k_bicg_x_v(
+    double* __restrict__ x0, double* __restrict__ x1, double* __restrict__ r0_,
+    double* __restrict__ r1_, const double* __restrict__ p0, const double* __restrict__ p1,
+    const double* __restrict__ s0, const double* __restrict__ s1,
+    const double* __restrict__ t0, const double* __restrict__ t1,
+    const double* __restrict__ rh0, const double* __restrict__ rh1,
+    const BicgScalars* __restrict__ bs, uint32_t N, double* __restrict__ partials) {
+    if (bs->all_done) return;
+    __shared__ double lds[4 * EW_BLOCK / WAVE];
+    const bool a0 = !bs->done[0], a1 = !bs->done[1];
+    const double al0 = bs->alpha[0], al1 = bs->alpha[1], om0 = bs->omega[0], om1 = bs->omega[1];
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t u = gid; u < N / 2; u += gridDim.x * blockDim.x) {
+        if (a0) {
+            const d2 sv = ld2(s0, u);
+            st2(x0, u, ld2(x0, u) + (al0 * ld2(p0, u) + om0 * sv));
+            const d2 rv = sv - om0 * ld2(t0, u);
+            st2(r0_, u, rv);
+            const d2 hv = ld2(rh0, u);
+            q0 += rv.x * rv.x;
+            q0 += rv.y * rv.y;
+            q2 += hv.x * rv.x;
+            q2 += hv.y * rv.y;
+        }
+        if (a1) {
+            const d2 sv = ld2(s1, u);
+            st2(x1, u, ld2(x1, u) + (al1 * ld2(p1, u) + om1 * sv));
+            const d2 rv = sv - om1 * ld2(t1, u);
+            st2(r1_, u, rv);
+            const d2 hv = ld2(rh1, u);
+            q1 += rv.x * rv.x;
+            q1 += rv.y * rv.y;
+            q3 += hv.x * rv.x;
+            q3 += hv.y * rv.y;
+        }
+    }
+    if ((N & 1) && gid == 0) {
+        const uint32_t v = N - 1;
+        if (a0) {
+            const double sv = s0[v];
+            x0[v] += al0 * p0[v] + om0 * sv;
+            const double rv = sv - om0 * t0[v];
+            r0_[v] = rv;
+            q0 += rv * rv;
+            q2 += rh0[v] * rv;
+        }
+        if (a1) {
+            const double sv = s1[v];
+            x1[v] += al1 * p1[v] + om1 * sv;
+            const double rv = sv - om1 * t1[v];
+            r1_[v] = rv;
+            q1 += rv * rv;
+            q3 += rh1[v] * rv;
+        }
+    }
+    double q[4] = {q0, q1, q2, q3};
+    block_reduce<4, RED_SUM, RED_SUM, RED_SUM, RED_SUM>(q, lds);
+    if (threadIdx.x == 0) {
+        const int nb = gridDim.x;
+        partials[blockIdx.x] = q[0];
+        partials[nb + blockIdx.x] = q[1];
+        partials[2 * nb + blockIdx.x] = q[2];
+        partials[3 * nb + blockIdx.x] = q[3];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision transport on the Schur complement: BiCGStab in FP32 on an
+// FP32 copy of the scaled operator S' = I - A_br R' solves for a correction
+// e of the FP64 iterate, inside FP64 refinement cycles.  A cycle takes the
+// FP64 Schur residual of the start pass, scaled by a power of two to unit
+// size, solves S' e = r to the FP64 tolerance (with a margin) or to a fixed
+// reduction of its own start, adds scale * e to y in FP64, and recomputes
+// the FP64 residual with the start pass; that pass's stop test is the
+// solve's.  FP32 copies of the off-diagonal coefficients (12 arrays of
+// N/2 + TROW_PAD32 floats: the red rows of R' and the black rows of A_br)
+// are written by the first start pass.  Vectors are FP32, dot products are
+// accumulated in FP64.
+// ---------------------------------------------------------------------------
+template <int VW> struct FVec;
+template <> struct FVec<2> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct FVec<4> { typedef float type __attribute__((ext_vector_type(4))); };
+
+// The other colour's values at the k neighbours of the lane's VW values
+// (km at -k, kp at +k); q is the own k parity (as tsv_kn).
+template <int VW, typename V = typename FVec<VW>::type>
+__device__ __forceinline__ void tmx_kn(const float* __restrict__ src, uint32_t idx, uint32_t m0,
+                                       uint32_t nhr, int q, bool ok, V c, V& km, V& kp) {
+    const int lane = threadIdx.x;
+    float prev = __shfl_up(c[VW - 1], 1, WAVE);
+    float next = __shfl_down(c[0], 1, WAVE);
+    if (lane == 0) prev = (ok && q == 0 && m0 >= 1) ? src[idx - 1] : 0.0f;
+    if (lane == WAVE - 1) next = (ok && q == 1 && m0 + VW < nhr) ? src[idx + VW] : 0.0f;
+    if (q == 0) {
+        km[0] = m0 >= 1 ? prev : 0.0f;
+#pragma unroll
+        for (int e = 1; e < VW; ++e) km[e] = c[e - 1];
+        kp = c;
+    } else {
+        km = c;
+#pragma unroll
+        for (int e = 0; e + 1 < VW; ++e) kp[e] = c[e + 1];
+        kp[VW - 1] = m0 + VW < nhr ? next : 0.0f;
+    }
+}
+
+template <int VW, typename V = typename FVec<VW>::type>
+__device__ __forceinline__ V tmx_ld(const float* __restrict__ a, size_t i) {
+    return *(const V*)(a + i);
+}
+
+// The six neighbours of the lane's values in src, o-weighted: sum_f o_f n_f.
+template <int VW, typename V = typename FVec<VW>::type>
+__device__ __forceinline__ V tmx_gather(const float* __restrict__ src, const V (&o)[6], V c,
+                                        uint32_t idx, uint32_t m0, const TVGeo& g, int q, bool ok,
+                                        bool hip_, bool him, bool hjp, bool hjm) {
+    const V zero = (V)(0.0f);
+    const V ip = (ok && hip_) ? tmx_ld<VW>(src, idx + g.plane) : zero;
+    const V im = (ok && him) ? tmx_ld<VW>(src, idx - g.plane) : zero;
+    const V jp = (ok && hjp) ? tmx_ld<VW>(src, idx + g.nhr) : zero;
+    const V jm = (ok && hjm) ? tmx_ld<VW>(src, idx - g.nhr) : zero;
+    V km, kp;
+    tmx_kn<VW>(src, idx, m0, g.nhr, q, ok, c, km, kp);
+    V a = o[0] * ip;
+    a = a + o[1] * im;
+    a = a + o[2] * jp;
+    a = a + o[3] * jm;
+    a = a + o[4] * kp;
+    a = a + o[5] * km;
+    return a;
+}
+
+// Red pass: w = R' p on the FP32 rows.
+template <int VW, int SYS>
+__global__ __launch_bounds__(WAVE * TSV_TR) void k_tmix_red(
+    const float* __restrict__ rows, TVGeo g, uint32_t Nh, const float* __restrict__ p0,
+    const float* __restrict__ p1, float* __restrict__ w0, float* __restrict__ w1,
+    const BicgScalars* __restrict__ bs) {
+    typedef typename FVec<VW>::type V;
+    if (bs->all_done) return;
+    const uint32_t m0 = VW * (blockIdx.x * WAVE + threadIdx.x);
+    const bool ok = m0 < g.nhr;
+    const V zero = (V)(0.0f);
+    const size_t rs = rstride32(Nh);
+#pragma unroll
+    for (int q8 = 0; q8 < TSV_RPW; ++q8) {
+        const uint32_t row = (blockIdx.y * TSV_RPW + q8) * TSV_TR + threadIdx.y;
+        if (row >= g.nrows) continue;                   // uniform per wave
+        const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
+        const int q = 1 - ((i + j) & 1);
+        const uint32_t idx = row * g.nhr + m0;
+        const bool hip_ = i + 1 < g.ni, him = i > 0, hjp = j + 1 < g.nj, hjm = j > 0;
+        V o[6];
+        V c0 = zero, c1 = zero;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            o[f] = ok ? __builtin_nontemporal_load((const V*)(rows + ((size_t)f * 2 + 1) * rs + idx))
+                      : zero;
+        if (ok) {
+            if (SYS & 1) c0 = tmx_ld<VW>(p0, idx);
+            if (SYS & 2) c1 = tmx_ld<VW>(p1, idx);
+        }
+        V a = zero, b = zero;
+        if (SYS & 1) a = tmx_gather<VW>(p0, o, c0, idx, m0, g, q, ok, hip_, him, hjp, hjm);
+        if (SYS & 2) b = tmx_gather<VW>(p1, o, c1, idx, m0, g, q, ok, hip_, him, hjp, hjm);
+        if (!ok) continue;
+        if (SYS & 1) *(V*)(w0 + idx) = a;
+        if (SYS & 2) *(V*)(w1 + idx) = b;
+    }
+}
+
+// Black pass: z = p - A_br w on the FP32 rows, with partials (FP64) of
+// rh.z (MODE 0) or (z.z, z.s) (MODE 1), the vector of the dot in d0 / d1.
+template <int MODE, int VW, int SYS>
+__global__ __launch_bounds__(WAVE * TSV_TR) void k_tmix_black(
+    const float* __restrict__ rows, TVGeo g, uint32_t Nh, const float* __restrict__ x0,
+    const float* __restrict__ x1, const float* __restrict__ wr0, const float* __restrict__ wr1,
+    float* __restrict__ y0, float* __restrict__ y1, const float* __restrict__ d0,
+    const float* __restrict__ d1, const BicgScalars* __restrict__ bs,
+    double* __restrict__ partials) {
+    typedef typename FVec<VW>::type V;
+    constexpr bool S0 = (SYS & 1) != 0, S1 = (SYS & 2) != 0;
+    if (bs->all_done) return;
+    __shared__ double lds[4 * TSV_TR];
+    const bool a0 = !bs->done[0], a1 = !bs->done[1];
+    const uint32_t m0 = VW * (blockIdx.x * WAVE + threadIdx.x);
+    const bool ok = m0 < g.nhr;
+    const V zero = (V)(0.0f);
+    const size_t rs = rstride32(Nh);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int q8 = 0; q8 < TSV_RPW; ++q8) {
+        const uint32_t row = (blockIdx.y * TSV_RPW + q8) * TSV_TR + threadIdx.y;
+        if (row >= g.nrows) continue;                   // uniform per wave
+        const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
+        const int q = (i + j) & 1;
+        const uint32_t idx = row * g.nhr + m0;
+        const bool hip_ = i + 1 < g.ni, him = i > 0, hjp = j + 1 < g.nj, hjm = j > 0;
+        V o[6];
+        V xa = zero, xb = zero, c0 = zero, c1 = zero;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            o[f] = ok ? __builtin_nontemporal_load((const V*)(rows + ((size_t)f * 2) * rs + idx))
+                      : zero;
+        if (ok) {
+            if (S0) {
+                xa = tmx_ld<VW>(x0, idx);
+                c0 = tmx_ld<VW>(wr0, idx);
+            }
+            if (S1) {
+                xb = tmx_ld<VW>(x1, idx);
+                c1 = tmx_ld<VW>(wr1, idx);
+            }
+        }
+        V ya = zero, yb = zero;
+        if (S0) ya = xa - tmx_gather<VW>(wr0, o, c0, idx, m0, g, q, ok, hip_, him, hjp, hjm);
+        if (S1) yb = xb - tmx_gather<VW>(wr1, o, c1, idx, m0, g, q, ok, hip_, him, hjp, hjm);
+        if (!ok) continue;
+        if (S0 && a0) *(V*)(y0 + idx) = ya;
+        if (S1 && a1) *(V*)(y1 + idx) = yb;
+        if (S0) {
+            const V dv = tmx_ld<VW>(d0, idx);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                if (MODE == 0) {
+                    s0 += (double)dv[e] * (double)ya[e];
+                } else {
+                    s0 += (double)ya[e] * (double)ya[e];
+                    s2 += (double)ya[e] * (double)dv[e];
+                }
+            }
+        }
+        if (S1) {
+            const V dv = tmx_ld<VW>(d1, idx);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                if (MODE == 0) {
+                    s1 += (double)dv[e] * (double)yb[e];
+                } else {
+                    s1 += (double)yb[e] * (double)yb[e];
+                    s3 += (double)yb[e] * (double)dv[e];
+                }
+            }
+        }
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    s3 = wave_sum(s3);
+    if (threadIdx.x == 0) {
+        lds[threadIdx.y] = s0;
+        lds[TSV_TR + threadIdx.y] = s1;
+        lds[2 * TSV_TR + threadIdx.y] = s2;
+        lds[3 * TSV_TR + threadIdx.y] = s3;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const int nb = gridDim.x * gridDim.y;
+        const int b = blockIdx.x + gridDim.x * blockIdx.y;
+        for (int a = 0; a < 4; ++a) {
+            double t = 0.0;
+            for (int w = 0; w < TSV_TR; ++w) t += lds[a * TSV_TR + w];
+            partials[a * nb + b] = t;
+        }
+    }
+}
+
+// The scale of a cycle's residual: the power of two at or below its norm.
+__device__ __forceinline__ double tmix_scale(double rr) {
+    return ldexp(1.0, ilogb(sqrt(rr)));
+}
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f4 ldf4(const float* __restrict__ a, uint32_t u) {
+    return *(const f4*)(a + 4 * (size_t)u);
+}
+__device__ __forceinline__ void stf4(float* __restrict__ a, uint32_t u, f4 x) {
+    *(f4*)(a + 4 * (size_t)u) = x;
+}
+__device__ __forceinline__ double dot4(f4 a, f4 b) {
+    double s = (double)a.x * (double)b.x;
+    s += (double)a.y * (double)b.y;
+    s += (double)a.z * (double)b.z;
+    s += (double)a.w * (double)b.w;
+    return s;
+}
+
+// The FP32 vectors of one system, passed by value: r, rh, p, v, s, t, e, w.
+struct TMixVecs {
+    float* v[8];
+};
+
+// Start of a cycle for the systems still iterating in FP64: r = r64 / scale
+// (FP32), rh = r, p = v = e = 0; partials r.r.  N % 4 == 0.
+__global__ __launch_bounds__(EW_BLOCK) void k_tmix_start(
+    const double* __restrict__ g0, const double* __restrict__ g1, TMixVecs V0, TMixVecs V1,
+    const BicgScalars* __restrict__ bs64, uint32_t N,
+    double* __restrict__ partials) {
+    __shared__ double lds[2 * EW_BLOCK / WAVE];
+    const bool a0 = !bs64->done[0], a1 = !bs64->done[1];
+    const double i0 = a0 ? 1.0 / tmix_scale(bs64->rr[0]) : 0.0;
+    const double i1 = a1 ? 1.0 / tmix_scale(bs64->rr[1]) : 0.0;
+    double q0 = 0.0, q1 = 0.0;
+    const f4 z = (f4)(0.0f);
+    float *__restrict__ r0 = V0.v[0], *__restrict__ h0 = V0.v[1], *__restrict__ p0 = V0.v[2],
+          *__restrict__ w0 = V0.v[3], *__restrict__ e0 = V0.v[6];
+    float *__restrict__ r1 = V1.v[0], *__restrict__ h1 = V1.v[1], *__restrict__ p1 = V1.v[2],
+          *__restrict__ w1 = V1.v[3], *__restrict__ e1 = V1.v[6];
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < N / 4;
+         u += gridDim.x * blockDim.x) {
+        if (a0) {
+            const d2 lo = ld2(g0, 2 * u), hi = ld2(g0, 2 * u + 1);
+            const f4 r = f4{(float)(lo.x * i0), (float)(lo.y * i0), (float)(hi.x * i0),
+                            (float)(hi.y * i0)};
+            stf4(r0, u, r);
+            stf4(h0, u, r);
+            stf4(p0, u, z);
+            stf4(w0, u, z);
+            stf4(e0, u, z);
+            q0 += dot4(r, r);
+        }
+        if (a1) {
+            const d2 lo = ld2(g1, 2 * u), hi = ld2(g1, 2 * u + 1);
+            const f4 r = f4{(float)(lo.x * i1), (float)(lo.y * i1), (float)(hi.x * i1),
+                            (float)(hi.y * i1)};
+            stf4(r1, u, r);
+            stf4(h1, u, r);
+            stf4(p1, u, z);
+            stf4(w1, u, z);
+            stf4(e1, u, z);
+            q1 += dot4(r, r);
+        }
+    }
+    double q[2] = {q0, q1};
+    block_reduce<2, RED_SUM, RED_SUM>(q, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = q[0];
+        partials[gridDim.x + blockIdx.x] = q[1];
+    }
+}
+
+// The cycle's FP32 scalars: r0 = r, p = r on the first iteration; stop at
+// the FP64 tolerance over the scale with a margin of 2, or at a reduction
+// by kappa, whichever is reached first; the iteration count and cap carry
+// over from the FP64 scalars.
+__global__ void k_tmix_init(const double* __restrict__ partials, int n, BicgScalars* bs,
+                            const BicgScalars* __restrict__ bs64, double kappa) {
+    __shared__ double lds[1024 / WAVE];
+    __shared__ double tot[2];
+    sum_partials(partials, n, 2, tot, lds);
+    if (threadIdx.x == 0) {
+        bs->maxit = bs64->maxit;
+        int all = 1;
+        for (int f = 0; f < 2; ++f) {
+            bs->iter[f] = bs64->iter[f];
+            bs->s_conv[f] = 0;
+            bs->alpha[f] = 1.0;
+            bs->omega[f] = 1.0;
+            bs->beta[f] = 0.0;
+            if (bs64->done[f]) {
+                bs->done[f] = 1;
+                bs->converged[f] = bs64->converged[f];
+                bs->rr[f] = 0.0;
+                continue;
+            }
+            const double rr = tot[f];
+            const double sc = tmix_scale(bs64->rr[f]);
+            const double abs2 = 0.25 * bs64->tol2[f] / (sc * sc);
+            bs->rr[f] = rr;
+            bs->rho[f] = rr;
+            bs->r0r[f] = rr;
+            bs->bnorm2[f] = rr;
+            bs->tol2[f] = fmax(kappa * kappa * rr, abs2);
+            const int ok = !(rr > bs->tol2[f]);
+            bs->done[f] = ok;
+            bs->converged[f] = ok;
+            all &= ok;
+        }
+        bs->all_done = all;
+    }
+}
+
+// p = r + beta (p - omega v), FP32.
+__global__ __launch_bounds__(EW_BLOCK) void k_tmix_p(TMixVecs V0, TMixVecs V1,
+                                                     const BicgScalars* __restrict__ bs,
+                                                     uint32_t N) {
+    if (bs->all_done) return;
+    const bool a0 = !bs->done[0], a1 = !bs->done[1];
+    const float be0 = (float)bs->beta[0], om0 = (float)bs->omega[0];
+    const float be1 = (float)bs->beta[1], om1 = (float)bs->omega[1];
+    const float *__restrict__ r0 = V0.v[0], *__restrict__ w0 = V0.v[3];
+    const float *__restrict__ r1 = V1.v[0], *__restrict__ w1 = V1.v[3];
+    float *__restrict__ p0 = V0.v[2], *__restrict__ p1 = V1.v[2];
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < N / 4;
+         u += gridDim.x * blockDim.x) {
+        if (a0) stf4(p0, u, ldf4(r0, u) + be0 * (ldf4(p0, u) - om0 * ldf4(w0, u)));
+        if (a1) stf4(p1, u, ldf4(r1, u) + be1 * (ldf4(p1, u) - om1 * ldf4(w1, u)));
+    }
+}
+
+// s = r - alpha v, FP32; partials s.s
+__global__ __launch_bounds__(EW_BLOCK) void k_tmix_s(TMixVecs V0, TMixVecs V1,
+                                                     const BicgScalars* __restrict__ bs,
+                                                     uint32_t N, double* __restrict__ partials) {
+    if (bs->all_done) return;
+    __shared__ double lds[2 * EW_BLOCK / WAVE];
+    const bool a0 = !bs->done[0], a1 = !bs->done[1];
+    const float al0 = (float)bs->alpha[0], al1 = (float)bs->alpha[1];
+    const float *__restrict__ r0 = V0.v[0], *__restrict__ w0 = V0.v[3];
+    const float *__restrict__ r1 = V1.v[0], *__restrict__ w1 = V1.v[3];
+    float *__restrict__ s0 = V0.v[4], *__restrict__ s1 = V1.v[4];
+    double q0 = 0.0, q1 = 0.0;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < N / 4;
+         u += gridDim.x * blockDim.x) {
+        if (a0) {
+            const f4 sv = ldf4(r0, u) - al0 * ldf4(w0, u);
+            stf4(s0, u, sv);
+            q0 += dot4(sv, sv);
+        }
+        if (a1) {
+            const f4 sv = ldf4(r1, u) - al1 * ldf4(w1, u);
+            stf4(s1, u, sv);
+            q1 += dot4(sv, sv);
+        }
+    }
+    double q[2] = {q0, q1};
+    block_reduce<2, RED_SUM, RED_SUM>(q, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = q[0];
+        partials[gridDim.x + blockIdx.x] = q[1];
+    }
+}
+
+// e += alpha p + omega s; r = s - omega t, FP32; partials r.r, rh.r
+__global__ __launch_bounds__(EW_BLOCK) void k_tmix_x(TMixVecs V0, TMixVecs V1,
+                                                     const BicgScalars* __restrict__ bs,
+                                                     uint32_t N, double* __restrict__ partials) {
+    if (bs->all_done) return;
+    __shared__ double lds[4 * EW_BLOCK / WAVE];
+    const bool a0 = !bs->done[0], a1 = !bs->done[1];
+    const float al0 = (float)bs->alpha[0], al1 = (float)bs->alpha[1];
+    const float om0 = (float)bs->omega[0], om1 = (float)bs->omega[1];
+    const float *__restrict__ h0 = V0.v[1], *__restrict__ p0 = V0.v[2],
+                *__restrict__ s0 = V0.v[4], *__restrict__ t0 = V0.v[5];
+    const float *__restrict__ h1 = V1.v[1], *__restrict__ p1 = V1.v[2],
+                *__restrict__ s1 = V1.v[4], *__restrict__ t1 = V1.v[5];
+    float *__restrict__ r0 = V0.v[0], *__restrict__ e0 = V0.v[6];
+    float *__restrict__ r1 = V1.v[0], *__restrict__ e1 = V1.v[6];
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < N / 4;
+         u += gridDim.x * blockDim.x) {
+        if (a0) {
+            const f4 sv = ldf4(s0, u);
+            stf4(e0, u, ldf4(e0, u) + (al0 * ldf4(p0, u) + om0 * sv));
+            const f4 rv = sv - om0 * ldf4(t0, u);
+            stf4(r0, u, rv);
+            q0 += dot4(rv, rv);
+            q2 += dot4(ldf4(h0, u), rv);
+        }
+        if (a1) {
+            const f4 sv = ldf4(s1, u);
+            stf4(e1, u, ldf4(e1, u) + (al1 * ldf4(p1, u) + om1 * sv));
+            const f4 rv = sv - om1 * ldf4(t1, u);
+            stf4(r1, u, rv);
+            q1 += dot4(rv, rv);
+            q3 += dot4(ldf4(h1, u), rv);
+        }
+    }
+    double q[4] = {q0, q1, q2, q3};
+    block_reduce<4, RED_SUM, RED_SUM, RED_SUM, RED_SUM>(q, lds);
+    if (threadIdx.x == 0) {
+        const int nb = gridDim.x;
+        partials[blockIdx.x] = q[0];
+        partials[nb + blockIdx.x] = q[1];
+        partials[2 * nb + blockIdx.x] = q[2];
+        partials[3 * nb + blockIdx.x] = q[3];
+    }
+}
+
+// End of a cycle: y += scale e (FP64) for the systems iterated; the FP64
+// scalars take the iteration count.
+__global__ __launch_bounds__(EW_BLOCK) void k_tmix_update(
+    double* __restrict__ y0, double* __restrict__ y1, const float* __restrict__ e0,
+    const float* __restrict__ e1, BicgScalars* __restrict__ bs64,
+    const BicgScalars* __restrict__ bs, uint32_t N) {
+    const bool a0 = !bs64->done[0], a1 = !bs64->done[1];
+    const double sc0 = a0 ? tmix_scale(bs64->rr[0]) : 0.0;
+    const double sc1 = a1 ? tmix_scale(bs64->rr[1]) : 0.0;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < N / 4;
+         u += gridDim.x * blockDim.x) {
+        if (a0) {
+            const f4 e = ldf4(e0, u);
+            st2(y0, 2 * u, ld2(y0, 2 * u) + sc0 * d2{(double)e.x, (double)e.y});
+            st2(y0, 2 * u + 1, ld2(y0, 2 * u + 1) + sc0 * d2{(double)e.z, (double)e.w});
+        }
+        if (a1) {
+            const f4 e = ldf4(e1, u);
+            st2(y1, 2 * u, ld2(y1, 2 * u) + sc1 * d2{(double)e.x, (double)e.y});
+            st2(y1, 2 * u + 1, ld2(y1, 2 * u + 1) + sc1 * d2{(double)e.z, (double)e.w});
+        }
+    }
+    // every block reads bs64->rr / done above before any write here: the
+    // counts are the only fields written, and no block reads them
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int f = 0; f < 2; ++f) bs64->iter[f] = bs->iter[f];
+}
+
 __global__ void k_bicg_rho(const double* __restrict__ partials, int n, BicgScalars* bs) {
     if (bs->all_done) return;
     __shared__ double lds[1024 / WAVE];
@@ -1496,6 +2123,12 @@ __global__ void k_bicg_zero_rhs(BicgScalars* bs) {
     bs->all_done = bs->done[0] && bs->done[1];
 }
 
+// Refinement cycles of the mixed-precision Schur form before FP64 BiCGStab
+// takes over.
+#ifndef LS_ADR_MAX_CYCLES
+#define LS_ADR_MAX_CYCLES 30
+#endif
+
 // Lock-step BiCGStab of the two systems x0, x1 (KIND 0: conc, acid; KIND 1:
 // the implicit saturation in x0, x1 idle).  Returns 0 converged, 1 not.
 template <int KIND>
@@ -1516,6 +2149,9 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     const int batch = 4;
     int restarts = 0;
     bool first = true;
+    c->t_cycles = 0;
+    c->t_fallback = 0;
+    c->t_rows32_valid = false;
     // stored rows (nk even) and the Schur form, as the pool was cut for:
     // the solver then runs on the black unknowns (N/2 each)
     const dim3 vg = adr_vec_grid(c);
@@ -1612,6 +2248,13 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     const bool tvec = tschur && c->nk % 4 == 0 && tvgrid.y <= 65535 &&
                       4 * (size_t)tvgrid.x * tvgrid.y <= 8 * (size_t)MAX_PARTIALS;
     const int nbs_schur = tvec ? (int)(tvgrid.x * tvgrid.y) : nbh;
+    // mixed precision (the pool was cut for it): FP32 cycles, 4 values per
+    // lane in the passes, for conc alone (acid known zero: its vectors hold
+    // the FP32 rows)
+    bool mixed = tvec && one && c->t_mixed && c->trows32 != nullptr;
+    double* nulls[2] = {nullptr, nullptr};
+    const dim3 tmgrid((tvg.nhr / 4 + WAVE - 1) / WAVE, tvgrid.y);
+    const int nb_mix = (int)(tmgrid.x * tmgrid.y);
     // systems still iterating, as the host last saw them (a system that
     // converges inside a batch is masked on the device)
     int sys = 3;
@@ -1634,10 +2277,16 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         }
         // black mode M: 0, 1 apply; 2 start; 3 final start (fields written)
         constexpr int RM = M == 2 ? 1 : (M == 3 ? 2 : 0);
+        // the mixed form: the first start writes the FP32 rows, and no
+        // start writes the FP64 rh, p, v its FP32 vectors lie over
+        float* const r32 = (mixed && M == 2) ? c->trows32 : nullptr;
+        double* const* rhs = mixed ? nulls : rh;
+        double* const* pps = mixed ? nulls : pp;
+        double* const* vvs = mixed ? nulls : vv;
         if (tvec)
             hipLaunchKernelGGL((k_tschur_red_v<S, RM>), tvgrid, dim3(WAVE, TSV_TR), 0, c->stream,
                                c->arows, tvg, Nh, xa, xb, wr[0], wr[1], c->bicg, c->tb[0],
-                               c->tb[1], x[0], x[1]);
+                               c->tb[1], x[0], x[1], r32);
         else
             hipLaunchKernelGGL((k_tschur_red<S, RM>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream,
                                c->arows, g, Nh, xa, xb, wr[0], wr[1], c->bicg, c->tb[0], c->tb[1],
@@ -1652,8 +2301,8 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         if (tvec)
             hipLaunchKernelGGL((k_tschur_black_v<M, S>), tvgrid, dim3(WAVE, TSV_TR), 0, c->stream,
                                c->arows, tvg, Nh, xa, xb, wr[0], wr[1], ya, yb, wa, wb, c->bicg,
-                               c->partials, rh[0], rh[1], pp[0], pp[1], vv[0], vv[1], c->tb[0],
-                               c->tb[1], x[0], x[1]);
+                               c->partials, rhs[0], rhs[1], pps[0], pps[1], vvs[0], vvs[1],
+                               c->tb[0], c->tb[1], x[0], x[1], r32);
         else
             hipLaunchKernelGGL((k_tschur_black<M, S>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream,
                                c->arows, g, Nh, xa, xb, wr[0], wr[1], ya, yb, wa, wb, c->bicg,
@@ -1685,6 +2334,34 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
                                c->bicg, c->partials);
         }
         return stored ? nbm : nb;
+    };
+    // the mixed form's FP32 apply z = S' p for conc (red pass into w, then
+    // the black pass with the partials of its mode)
+    auto mix_apply = [&](auto MC, const float* pa, const float* pb, float* za, float* zb,
+                         const float* da, const float* db) {
+        constexpr int M = decltype(MC)::value;
+        auto go = [&](auto SC) {
+            constexpr int S = decltype(SC)::value;
+            const bool sample = sample_red;
+            if (sample) {
+                (void)hipEventRecord(c->ev[16], c->stream);
+                sample_red = false;
+            }
+            hipLaunchKernelGGL((k_tmix_red<4, S>), tmgrid, dim3(WAVE, TSV_TR), 0, c->stream,
+                               c->trows32, tvg, Nh, pa, pb, c->tf[0][7], nullptr, c->bicg32);
+            if (sample) {
+                (void)hipEventRecord(c->ev[17], c->stream);
+                red_pending = true;
+                // FP32 red rows (6 coefficients at N/2), p at the black and w
+                // at the red voxels per system iterated
+                c->prof_tred_bytes = (double)c->N * (12.0 + (S == 3 ? 8.0 : 4.0));
+            }
+            hipLaunchKernelGGL((k_tmix_black<M, 4, S>), tmgrid, dim3(WAVE, TSV_TR), 0, c->stream,
+                               c->trows32, tvg, Nh, pa, pb, c->tf[0][7], nullptr, za, zb, da,
+                               db, c->bicg32, c->partials);
+        };
+        go(std::integral_constant<int, 1>{});     // conc alone
+        return nb_mix;
     };
     // r = b - A x, rh = r, p = v = 0, partials (b.b, r.r); in the Schur form
     // every start after the first also writes the iterate into the fields
@@ -1746,6 +2423,103 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             LS_HIP(hipStreamSynchronize(c->stream));
         }
         first = false;
+        if (mixed) {
+            c->t_rows32_valid = true;
+            // FP32 cycles on the FP64 residual of the last start pass; the
+            // start pass after each cycle writes the fields and is the stop
+            // test.  A cycle that does not cut an unconverged system's FP64
+            // residual by 16x hands the solve to FP64 BiCGStab.
+            const double kappa = c->P.adr_mixed_inner_rtol;
+            // system 1 is idle (known zero, done at every start): its
+            // vectors are never touched, the kernels get conc's again
+            TMixVecs V0, V1;
+            for (int k = 0; k < 8; ++k) {
+                V0.v[k] = c->tf[0][k];
+                V1.v[k] = c->tf[0][k];
+            }
+            int cycles = 0;
+            bool wrote = false, stalled = false;
+            while (!c->h_bicg->all_done) {
+                const BicgScalars h0 = *c->h_bicg;
+                hipLaunchKernelGGL(k_tmix_start, dim3(nbh), dim3(EW_BLOCK), 0, c->stream, r[0],
+                                   r[1], V0, V1, c->bicg, Nh, c->partials);
+                hipLaunchKernelGGL(k_tmix_init, dim3(1), dim3(1024), 0, c->stream, c->partials,
+                                   nbh, c->bicg32, c->bicg, kappa);
+                LS_LAUNCH_CHECK();
+                c->h_bicg32->all_done = 0;
+                c->h_bicg32->done[0] = h0.done[0];
+                c->h_bicg32->done[1] = h0.done[1];
+                while (!c->h_bicg32->all_done) {
+                    sample_red = c->profiling;
+                    for (int it = 0; it < batch; ++it) {
+                        hipLaunchKernelGGL(k_tmix_p, dim3(nbh), dim3(EW_BLOCK), 0, c->stream, V0,
+                                           V1, c->bicg32, Nh);
+                        // v = S' p, partials rh.v
+                        const int n0 = mix_apply(std::integral_constant<int, 0>{}, V0.v[2],
+                                                 V1.v[2], V0.v[3], V1.v[3], V0.v[1], V1.v[1]);
+                        hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1024), 0, c->stream,
+                                           c->partials, n0, c->bicg32);
+                        hipLaunchKernelGGL(k_tmix_s, dim3(nbh), dim3(EW_BLOCK), 0, c->stream, V0,
+                                           V1, c->bicg32, Nh, c->partials);
+                        hipLaunchKernelGGL(k_bicg_scheck, dim3(1), dim3(1024), 0, c->stream,
+                                           c->partials, nbh, c->bicg32);
+                        // t = S' s, partials t.t, t.s
+                        const int n1 = mix_apply(std::integral_constant<int, 1>{}, V0.v[4],
+                                                 V1.v[4], V0.v[5], V1.v[5], V0.v[4], V1.v[4]);
+                        hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1024), 0, c->stream,
+                                           c->partials, n1, c->bicg32);
+                        hipLaunchKernelGGL(k_tmix_x, dim3(nbh), dim3(EW_BLOCK), 0, c->stream, V0,
+                                           V1, c->bicg32, Nh, c->partials);
+                        hipLaunchKernelGGL(k_bicg_rho, dim3(1), dim3(1024), 0, c->stream,
+                                           c->partials, nbh, c->bicg32);
+                    }
+                    LS_LAUNCH_CHECK();
+                    LS_HIP(hipMemcpyAsync(c->h_bicg32, c->bicg32, sizeof(BicgScalars),
+                                          hipMemcpyDeviceToHost, c->stream));
+                    LS_HIP(hipStreamSynchronize(c->stream));
+                    if (red_pending) {
+                        float ms = 0.0f;
+                        LS_HIP(hipEventElapsedTime(&ms, c->ev[16], c->ev[17]));
+                        c->prof_ms[3] += ms;
+                        c->prof_n[3] += 1;
+                        red_pending = false;
+                    }
+                }
+                hipLaunchKernelGGL(k_tmix_update, dim3(nbh), dim3(EW_BLOCK), 0, c->stream, xs[0],
+                                   xs[1], V0.v[6], V1.v[6], c->bicg, c->bicg32, Nh);
+                const int nc = start(false);
+                hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(1024), 0, c->stream, c->partials,
+                                   nc, c->bicg, c->P.adr_rtol, c->P.adr_maxit, 0);
+                LS_LAUNCH_CHECK();
+                LS_HIP(hipMemcpyAsync(c->h_bicg, c->bicg, sizeof(BicgScalars),
+                                      hipMemcpyDeviceToHost, c->stream));
+                LS_HIP(hipStreamSynchronize(c->stream));
+                wrote = true;
+                c->t_cycles += 1;
+                const BicgScalars& h = *c->h_bicg;
+                if (h.all_done) break;
+                if (h.iter[0] >= c->P.adr_maxit || h.iter[1] >= c->P.adr_maxit) break;
+                stalled = ++cycles >= LS_ADR_MAX_CYCLES;
+                for (int f = 0; f < 2; ++f)
+                    if (!h.done[f] && !(h.rr[f] < 0.0625 * h0.rr[f])) stalled = true;
+                if (stalled) break;
+            }
+            if (!stalled) {
+                if (!wrote) {       // converged at the first start: write the fields
+                    const int nc = start(false);
+                    hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(1024), 0, c->stream,
+                                       c->partials, nc, c->bicg, c->P.adr_rtol, c->P.adr_maxit, 0);
+                    LS_LAUNCH_CHECK();
+                    LS_HIP(hipMemcpyAsync(c->h_bicg, c->bicg, sizeof(BicgScalars),
+                                          hipMemcpyDeviceToHost, c->stream));
+                    LS_HIP(hipStreamSynchronize(c->stream));
+                }
+                break;
+            }
+            mixed = false;        // FP64 BiCGStab from the current iterate
+            c->t_fallback = 1;
+            continue;
+        }
         while (!c->h_bicg->all_done) {
             sys = (c->h_bicg->done[0] ? 0 : 1) | (c->h_bicg->done[1] ? 0 : 2);
             if (sys == 0) sys = 3;
@@ -1756,9 +2530,8 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             double* const* sd = tschur ? s : sh;
             for (int it = 0; it < batch; ++it) {
                 if (tschur)
-                    hipLaunchKernelGGL(k_bicg_p<false>, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
-                                       pp[0], pp[1], r[0], r[1], vv[0], vv[1], nullptr, nullptr,
-                                       nullptr, c->bicg, Nv);
+                    hipLaunchKernelGGL(k_bicg_p_v, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
+                                       pp[0], pp[1], r[0], r[1], vv[0], vv[1], c->bicg, Nv);
                 else
                     hipLaunchKernelGGL(k_bicg_p<true>, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
                                        pp[0], pp[1], r[0], r[1], vv[0], vv[1], ph[0], ph[1],
@@ -1768,9 +2541,9 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
                 hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1024), 0, c->stream, c->partials,
                                    n0, c->bicg);
                 if (tschur)
-                    hipLaunchKernelGGL(k_bicg_s<false>, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
-                                       r[0], r[1], vv[0], vv[1], s[0], s[1], nullptr, nullptr,
-                                       nullptr, c->bicg, Nv, c->partials);
+                    hipLaunchKernelGGL(k_bicg_s_v, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
+                                       r[0], r[1], vv[0], vv[1], s[0], s[1], c->bicg, Nv,
+                                       c->partials);
                 else
                     hipLaunchKernelGGL(k_bicg_s<true>, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
                                        r[0], r[1], vv[0], vv[1], s[0], s[1], sh[0], sh[1], invd_v,
@@ -1782,10 +2555,9 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
                 hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1024), 0, c->stream, c->partials,
                                    n1, c->bicg);
                 if (tschur)
-                    hipLaunchKernelGGL(k_bicg_x<false>, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
-                                       xv[0], xv[1], r[0], r[1], pp[0], pp[1], nullptr, nullptr,
-                                       s[0], s[1], t[0], t[1], rh[0], rh[1], c->bicg, Nv,
-                                       c->partials);
+                    hipLaunchKernelGGL(k_bicg_x_v, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
+                                       xv[0], xv[1], r[0], r[1], pp[0], pp[1], s[0], s[1], t[0],
+                                       t[1], rh[0], rh[1], c->bicg, Nv, c->partials);
                 else
                     hipLaunchKernelGGL(k_bicg_x<true>, dim3(nbv), dim3(EW_BLOCK), 0, c->stream,
                                        xv[0], xv[1], r[0], r[1], ph[0], ph[1], sh[0], sh[1], s[0],
@@ -1879,12 +2651,29 @@ int time_transport_pass(ls_ctx* c, int which, int reps, double* ms, double* byte
     double* y = c->tv[0][3];
     const double* wd = c->tv[0][1];
     hipLaunchKernelGGL(k_bicg_rearm, dim3(1), dim3(1), 0, c->stream, c->bicg);
+    const bool f32 = which == LS_TIME_TRED32 || which == LS_TIME_TBLACK32;
+    if (f32 && !c->t_rows32_valid) {
+        c->err = "the FP32 transport passes need a transport solve in the mixed-precision form "
+                 "(adr_mixed, nk % 8 == 0, acid zero)";
+        return -3;
+    }
+    const dim3 tmgrid((tvg.nhr / 4 + WAVE - 1) / WAVE, tvgrid.y);
+    if (f32) hipLaunchKernelGGL(k_bicg_rearm, dim3(1), dim3(1), 0, c->stream, c->bicg32);
     auto launch = [&]() {
-        if (which == LS_TIME_TRED) {
+        if (which == LS_TIME_TRED32) {
+            hipLaunchKernelGGL((k_tmix_red<4, 1>), tmgrid, dim3(WAVE, TSV_TR), 0, c->stream,
+                               c->trows32, tvg, Nh, c->tf[0][2], nullptr, c->tf[0][7], nullptr,
+                               c->bicg32);
+        } else if (which == LS_TIME_TBLACK32) {
+            hipLaunchKernelGGL((k_tmix_black<0, 4, 1>), tmgrid, dim3(WAVE, TSV_TR), 0, c->stream,
+                               c->trows32, tvg, Nh, c->tf[0][2], nullptr, c->tf[0][7], nullptr,
+                               c->tf[0][3], nullptr, c->tf[0][1], nullptr, c->bicg32,
+                               c->partials);
+        } else if (which == LS_TIME_TRED) {
             if (tvec)
                 hipLaunchKernelGGL((k_tschur_red_v<1, 0>), tvgrid, dim3(WAVE, TSV_TR), 0,
                                    c->stream, c->arows, tvg, Nh, xa, nullptr, w, nullptr, c->bicg,
-                                   c->tb[0], c->tb[1], nullptr, nullptr);
+                                   c->tb[0], c->tb[1], nullptr, nullptr, nullptr);
             else
                 hipLaunchKernelGGL((k_tschur_red<1, 0>), dim3(nbh), dim3(EW_BLOCK), 0, c->stream,
                                    c->arows, g, Nh, xa, nullptr, w, nullptr, c->bicg, c->tb[0],
@@ -1895,7 +2684,7 @@ int time_transport_pass(ls_ctx* c, int which, int reps, double* ms, double* byte
                                    c->stream, c->arows, tvg, Nh, xa, nullptr, w, nullptr, y,
                                    nullptr, wd, nullptr, c->bicg, c->partials, nullptr, nullptr,
                                    nullptr, nullptr, nullptr, nullptr, c->tb[0], c->tb[1],
-                                   nullptr, nullptr);
+                                   nullptr, nullptr, nullptr);
             else
                 hipLaunchKernelGGL((k_tschur_black<0, 1>), dim3(nbh), dim3(EW_BLOCK), 0,
                                    c->stream, c->arows, g, Nh, xa, nullptr, w, nullptr, y,
@@ -1914,7 +2703,8 @@ int time_transport_pass(ls_ctx* c, int which, int reps, double* ms, double* byte
     float f = 0.0f;
     LS_HIP(hipEventElapsedTime(&f, e0, e1));
     *ms = (double)f / reps;
-    *bytes = (double)c->N * (which == LS_TIME_TRED ? 32.0 : 40.0);
+    *bytes = (double)c->N * (which == LS_TIME_TRED ? 32.0 : which == LS_TIME_TBLACK ? 40.0 :
+                             which == LS_TIME_TRED32 ? 16.0 : 20.0);
     return 0;
 }
 
@@ -1946,6 +2736,8 @@ int phase_transport(ls_ctx* c, double dt, ls_step_report* rep) {
         rep->adr_relres = relres;
         rep->adr_iters_conc = sys_iters[0];
         rep->adr_iters_acid = sys_iters[1];
+        rep->adr_cycles = c->t_cycles;
+        rep->adr_fallback = c->t_fallback;
     }
     if (st) c->err = "transport solve did not converge";
     return st;

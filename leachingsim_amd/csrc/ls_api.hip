@@ -110,10 +110,11 @@ namespace ls {
 // buffers over the same bytes.  Cut again (and reallocated) when the
 // parameters change the layout; the pool holds nothing between phases.
 int pool_setup(ls_ctx* c) {
-    bool stored = false, schur = false;
-    transport_form(c, &stored, &schur);
+    bool stored = false, schur = false, mixed = false;
+    transport_form(c, &stored, &schur, &mixed);
     const int npw = pool_pw_count(c->P);
-    const int key = (stored ? 1 : 0) | (schur ? 2 : 0) | (c->P.implicit_sat ? 4 : 0) | (npw << 3);
+    const int key = (stored ? 1 : 0) | (schur ? 2 : 0) | (c->P.implicit_sat ? 4 : 0) |
+                    (mixed ? 8 : 0) | (npw << 4);
     if (key == c->pool_key && c->pool) return 0;
     const size_t N = c->N, Nh = N / 2, d = sizeof(double);
     // pressure phase
@@ -135,8 +136,21 @@ int pool_setup(ls_ctx* c) {
     // the Schur form keeps its Jacobi scaling inside the operator: no ph, sh
     const int ntv = schur ? 6 : 8;
     size_t o_tv[2][8] = {}, o_xs[2] = {0, 0}, o_wr[2] = {0, 0}, o_tinvd = 0;
-    for (int f = 0; f < 2; ++f)
-        for (int q = 0; q < ntv; ++q) o_tv[f][q] = tc.take(nv * d);
+    // the mixed form lays two FP32 vectors over each of conc's rh, p, v, s,
+    // the second LS_ALLOC_STAGGER bytes past the first one's end
+    const size_t vpad = mixed ? 2 * LS_ALLOC_STAGGER : 0;
+    for (int q = 0; q < ntv; ++q) o_tv[0][q] = tc.take(nv * d + vpad);
+    // the Schur form cuts acid's six vectors as one block the size of the
+    // twelve FP32 off-diagonal arrays (each vector two arrays' stride apart):
+    // the mixed form keeps those arrays there while acid is known zero
+    size_t o_s1 = 0;
+    if (schur) {
+        o_s1 = tc.take(transport_rows32_floats(c) * sizeof(float));
+        const size_t vstride = 2 * (transport_rows32_floats(c) / 12) * sizeof(float);
+        for (int q = 0; q < ntv; ++q) o_tv[1][q] = o_s1 + q * vstride;
+    } else {
+        for (int q = 0; q < ntv; ++q) o_tv[1][q] = tc.take(nv * d);
+    }
     if (schur) {
         for (int f = 0; f < 2; ++f) {
             o_xs[f] = tc.take(Nh * d);
@@ -204,6 +218,12 @@ int pool_setup(ls_ctx* c) {
         c->taux[f] = c->P.implicit_sat ? (double*)(b + o_aux[f]) : nullptr;
     }
     c->tinvd = schur ? nullptr : (double*)(b + o_tinvd);
+    c->t_mixed = mixed;
+    c->trows32 = mixed ? (float*)(b + o_s1) : nullptr;
+    for (int k = 0; k < 8; ++k)
+        c->tf[0][k] = mixed ? (float*)((char*)c->tv[0][1 + k / 2] +
+                                       (k & 1) * (Nh * sizeof(float) + LS_ALLOC_STAGGER))
+                            : nullptr;
     return 0;
 }
 
@@ -244,6 +264,8 @@ void ls_default_params(ls_params* p) {
     p->mixed_schur = 1;
     p->adr_schur = 1;
     p->adr_extrapolate = 1;
+    p->adr_mixed = 1;
+    p->adr_mixed_inner_rtol = 1e-5;
     p->mixed_inner_rtol = 0.1;
 }
 
@@ -274,8 +296,10 @@ static int validate_params(ls_ctx* c, const ls_params* p) {
         return -3;
     }
     if (p->mixed_schur < 0 || p->mixed_schur > 3 || p->adr_schur < 0 || p->adr_schur > 1 ||
-        p->adr_extrapolate < 0 || p->adr_extrapolate > 1) {
-        c->err = "invalid mixed_schur (0..3), adr_schur (0..1) or adr_extrapolate (0..1)";
+        p->adr_extrapolate < 0 || p->adr_extrapolate > 1 || p->adr_mixed < 0 ||
+        p->adr_mixed > 1 || !(p->adr_mixed_inner_rtol > 0.0 && p->adr_mixed_inner_rtol < 1.0)) {
+        c->err = "invalid mixed_schur (0..3), adr_schur (0..1), adr_extrapolate (0..1), "
+                 "adr_mixed (0..1) or adr_mixed_inner_rtol (0 < x < 1)";
         return -3;
     }
     return 0;
@@ -321,6 +345,7 @@ int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes, const ls_pa
     st |= dalloc(c, &c->partials, (size_t)MAX_PARTIALS * 8);
     st |= dalloc(c, &c->cg, 1);
     st |= dalloc(c, &c->bicg, 1);
+    st |= dalloc(c, &c->bicg32, 1);
     st |= dalloc(c, &c->ss, 1);
     st |= dalloc(c, &c->fcg, 1);
     if (st || (st = pool_setup(c))) {
@@ -329,6 +354,7 @@ int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes, const ls_pa
     }
     if (hipHostMalloc((void**)&c->h_cg, sizeof(CgScalars)) != hipSuccess ||
         hipHostMalloc((void**)&c->h_bicg, sizeof(BicgScalars)) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_bicg32, sizeof(BicgScalars)) != hipSuccess ||
         hipHostMalloc((void**)&c->h_ss, sizeof(StepScalars)) != hipSuccess ||
         hipHostMalloc((void**)&c->h_fcg, sizeof(FcgScalars)) != hipSuccess) {
         c->err = "hipHostMalloc failed";
@@ -342,6 +368,7 @@ int ls_create(ls_ctx** out, int ni, int nj, int nk, int label_bytes, const ls_pa
         (void)hipMemsetAsync(f, 0, N * sizeof(double), c->stream);
     (void)hipMemsetAsync(c->cg, 0, sizeof(CgScalars), c->stream);
     (void)hipMemsetAsync(c->bicg, 0, sizeof(BicgScalars), c->stream);
+    (void)hipMemsetAsync(c->bicg32, 0, sizeof(BicgScalars), c->stream);
     (void)hipMemsetAsync(c->ss, 0, sizeof(StepScalars), c->stream);
     (void)hipMemsetAsync(c->fcg, 0, sizeof(FcgScalars), c->stream);
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -361,6 +388,7 @@ void ls_destroy(ls_ctx* c) {
     if (c->pool) (void)hipFree(c->pool);
     if (c->h_cg) (void)hipHostFree(c->h_cg);
     if (c->h_bicg) (void)hipHostFree(c->h_bicg);
+    if (c->h_bicg32) (void)hipHostFree(c->h_bicg32);
     if (c->h_ss) (void)hipHostFree(c->h_ss);
     if (c->h_fcg) (void)hipHostFree(c->h_fcg);
     if (c->h_wsum) (void)hipHostFree(c->h_wsum);
@@ -903,12 +931,14 @@ int ls_debug_schur_apply(ls_ctx* c, int form, const float* p, float* q, double* 
 
 int ls_time_stencil(ls_ctx* c, int precision, int reps, double* ms, double* bytes) {
     if (!c || !ms || !bytes || reps < 1) return -1;
-    if (precision == LS_TIME_TRED || precision == LS_TIME_TBLACK)
+    if (precision == LS_TIME_TRED || precision == LS_TIME_TBLACK ||
+        precision == LS_TIME_TRED32 || precision == LS_TIME_TBLACK32)
         return time_transport_pass(c, precision, reps, ms, bytes);
     if (precision != 32 && precision != 64 && precision != LS_TIME_RB_GATHER &&
         precision != LS_TIME_RB_APPLY && precision != LS_TIME_SCHUR) {
         c->err = "ls_time_stencil: precision must be 64, 32, LS_TIME_RB_GATHER, LS_TIME_RB_APPLY, "
-                 "LS_TIME_SCHUR, LS_TIME_TRED or LS_TIME_TBLACK";
+                 "LS_TIME_SCHUR, LS_TIME_TRED, LS_TIME_TBLACK, LS_TIME_TRED32 or "
+                 "LS_TIME_TBLACK32";
         return -3;
     }
     c->trans_valid = false;     // pressure scratch of the pool written

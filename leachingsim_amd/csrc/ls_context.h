@@ -128,6 +128,17 @@ struct ls_ctx {
     // last red pass
     double* txs[2] = {};
     double* twr[2] = {};
+    // mixed-precision Schur form (conc alone, acid known zero): FP32 copies
+    // of the off-diagonal rows over acid's vectors, and conc's FP32 cycle
+    // vectors r, rh, p, v, s, t, e, w laid two to an FP64 vector over its
+    // rh, p, v, s (tf[1] stays null)
+    bool t_mixed = false;
+    float* trows32 = nullptr;
+    float* tf[2][8] = {};
+    BicgScalars* bicg32 = nullptr;
+    BicgScalars* h_bicg32 = nullptr;
+    int t_cycles = 0, t_fallback = 0;   // of the last mixed-precision solve
+    bool t_rows32_valid = false;        // the FP32 rows belong to the last solve
     double* taux[2] = {};    // implicit saturation: its source field and idle system (N)
     double* arows = nullptr; // stored transport rows (nk even)
     // black half of conc at the last transport solve (N/2, allocated on first
@@ -209,8 +220,9 @@ namespace ls {
 int pool_setup(ls_ctx* c);
 // k_transport.hip: the transport form the parameters and grid select, and
 // the doubles its stored rows take
-void transport_form(const ls_ctx* c, bool* stored, bool* schur);
+void transport_form(const ls_ctx* c, bool* stored, bool* schur, bool* mixed = nullptr);
 size_t transport_rows_doubles(const ls_ctx* c);
+size_t transport_rows32_floats(const ls_ctx* c);
 // k_physics.hip
 int init_state(ls_ctx* c, long long* nsul);
 int phase_properties(ls_ctx* c);

@@ -23,6 +23,7 @@ SOLVER_PLAIN64, SOLVER_MIXED, SOLVER_LEARNED = 0, 1, 2
 BC_REFERENCE, BC_ZFACES = 0, 1
 WARM_PREVIOUS, WARM_PROJECTED, WARM_LEARNED = 0, 1, 2
 TIME_RB_GATHER, TIME_RB_APPLY, TIME_SCHUR, TIME_TRED, TIME_TBLACK = 1, 2, 3, 4, 5
+TIME_TRED32, TIME_TBLACK32 = 6, 7
 WARM_MAX = 8
 
 (F_IMG, F_FRAC, F_INVENTORY, F_SATURATION, F_PRESSURE, F_CONC, F_ACID, F_QX, F_QY, F_QZ,
@@ -49,7 +50,8 @@ class Params(C.Structure):
         ("adr_rtol", C.c_double), ("adr_maxit", C.c_int), ("warm_history", C.c_int),
         ("mixed_inner_rtol", C.c_double), ("learned_race", C.c_double),
         ("implicit_sat", C.c_int), ("mixed_schur", C.c_int), ("adr_schur", C.c_int),
-        ("adr_extrapolate", C.c_int),
+        ("adr_extrapolate", C.c_int), ("adr_mixed", C.c_int),
+        ("adr_mixed_inner_rtol", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -81,7 +83,7 @@ class StepReport(C.Structure):
         ("sat_raw_max", C.c_double), ("prof_schur_ms", C.c_double), ("prof_schur_n", C.c_int),
         ("prof_schur_bytes", C.c_double), ("adr_iters_conc", C.c_int),
         ("adr_iters_acid", C.c_int), ("prof_tred_ms", C.c_double), ("prof_tred_n", C.c_int),
-        ("prof_tred_bytes", C.c_double),
+        ("prof_tred_bytes", C.c_double), ("adr_cycles", C.c_int), ("adr_fallback", C.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -97,7 +99,8 @@ def default_params(**overrides) -> Params:
                solver=SOLVER_PLAIN64, warm_start=WARM_PREVIOUS, press_rtol=1e-8,
                press_maxit=10000, adr_rtol=1e-8, adr_maxit=10000, warm_history=1,
                mixed_inner_rtol=0.1, learned_race=1.0, implicit_sat=0,
-               mixed_schur=1, adr_schur=1, adr_extrapolate=1)
+               mixed_schur=1, adr_schur=1, adr_extrapolate=1, adr_mixed=1,
+               adr_mixed_inner_rtol=1e-5)
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)
@@ -339,7 +342,8 @@ class GpuSimulation:
         """ms per launch and algorithmic bytes per launch of one kernel:
         precision 64 / 32 (the stencil), TIME_RB_GATHER, TIME_RB_APPLY,
         TIME_SCHUR (pressure), TIME_TRED, TIME_TBLACK (the transport's
-        Schur passes on the rows of the last transport solve)."""
+        Schur passes on the rows of the last transport solve), TIME_TRED32,
+        TIME_TBLACK32 (the same passes of the mixed-precision form)."""
         ms, by = C.c_double(), C.c_double()
         self._check(self.lib.ls_time_stencil(self.h, precision, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value

@@ -145,6 +145,56 @@ def test_transport_parity(rng, theta, nk, acid_zero, schur):
         assert np.array_equal(g.download(f), o.download(f))
 
 
+@pytest.mark.parametrize("acid_zero", [True, False])
+@pytest.mark.parametrize("inner", [1e-5, 0.9])
+def test_transport_mixed_precision_parity(rng, acid_zero, inner):
+    """nk % 8 == 0 with acid known zero: the mixed-precision Schur form, FP32
+    BiCGStab cycles on FP32 copies of the rows inside FP64 refinement, solved
+    to 1e-13 against the oracle (several cycles).  Inner rtol 0.9 makes every
+    cycle short: the solve either converges in many cycles or hands over to
+    FP64 BiCGStab, and must reach the same field either way.  With acid
+    present both systems take FP64 BiCGStab (no cycles)."""
+    shape = (11, 10, 16)
+    img = small_mixed_geometry(shape, 13)
+    P = L.default_params(adr_rtol=1e-13, adr_mixed_inner_rtol=inner)
+    g, o = both(shape, P, img)
+    q = [rng.uniform(-0.3, 0.3, size=shape) for _ in range(3)]
+    conc = rng.uniform(0.0, 1.0, size=shape)
+    acid = np.zeros(shape) if acid_zero else rng.uniform(0.0, 1.0, size=shape)
+    for x in (g, o):
+        for f, a in zip((L.F_QX, L.F_QY, L.F_QZ), q):
+            x.upload(f, a)
+        x.upload(L.F_CONC, conc)
+        x.upload(L.F_ACID, acid)
+    rg = g.phase(L.PH_TRANSPORT, 0.0, 0.7)
+    ro = o.phase(L.PH_TRANSPORT, 0.0, 0.7)
+    assert rg.status == 0 and ro.status == 0
+    assert rg.adr_relres <= 1e-13
+    assert (rg.adr_cycles >= 1) if acid_zero else (rg.adr_cycles == 0)
+    print(f"inner {inner} acid_zero {acid_zero}: {rg.adr_iters} iterations, "
+          f"{rg.adr_cycles} cycles, fallback {rg.adr_fallback}")
+    if inner < 1e-3 or not acid_zero:
+        assert rg.adr_fallback == 0
+    assert relerr(g.download(L.F_CONC), o.download(L.F_CONC)) <= 1e-11
+    if acid_zero:
+        assert np.array_equal(g.download(L.F_ACID), np.zeros(shape))
+    else:
+        assert relerr(g.download(L.F_ACID), o.download(L.F_ACID)) <= 1e-11
+    # the FP64 form on the same input agrees with the mixed one
+    P64 = L.default_params(adr_rtol=1e-13, adr_mixed=0)
+    h = L.GpuSimulation(shape, P64)
+    h.read_labels(img)
+    h.init_state()
+    for f, a in zip((L.F_QX, L.F_QY, L.F_QZ), q):
+        h.upload(f, a)
+    h.upload(L.F_CONC, conc)
+    h.upload(L.F_ACID, acid)
+    rh = h.phase(L.PH_TRANSPORT, 0.0, 0.7)
+    assert rh.status == 0 and rh.adr_cycles == 0
+    assert relerr(g.download(L.F_CONC), h.download(L.F_CONC)) <= 1e-11
+    h.close()
+
+
 @pytest.mark.parametrize("nsub_scale", [1.0, 30.0])
 def test_leaching_general_path_bit_exact(rng, nsub_scale):
     shape = (9, 8, 10)
