@@ -540,6 +540,9 @@ def main():
                     help="transport of the headline and learned runs: 1 FP32 BiCGStab "
                          "cycles inside FP64 refinement, 0 FP64 BiCGStab (default: the "
                          "library's, 1); the plain FP64 runs always use 0")
+    ap.add_argument("--adr-mixed-inner-rtol", type=float, default=None,
+                    help="reduction of a mixed-precision transport cycle's FP32 solve "
+                         "(default: the library's, 1e-6)")
     ap.add_argument("--weights", default=None, help="U-Net weights (float32 file)")
     ap.add_argument("--no-baseline", action="store_true",
                     help="skip the plain FP64 runs (projected and previous-pressure)")
@@ -582,6 +585,8 @@ def main():
     extra = {} if args.warm_history is None else {"warm_history": args.warm_history}
     if args.adr_mixed is not None:
         extra["adr_mixed"] = args.adr_mixed
+    if args.adr_mixed_inner_rtol is not None:
+        extra["adr_mixed_inner_rtol"] = args.adr_mixed_inner_rtol
     weights = None
     from leachingsim_amd.learn import WEIGHTS, load_weights
     if head == L.SOLVER_LEARNED or warm == L.WARM_LEARNED or not args.no_learned:

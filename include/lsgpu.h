@@ -162,7 +162,7 @@ typedef struct ls_params {
                                 in FP64                                          */
     double adr_mixed_inner_rtol; /* adr_mixed: a cycle's FP32 solve stops at this
                                 reduction of its own residual, or earlier at half
-                                the FP64 tolerance (1e-5)                         */
+                                the FP64 tolerance (1e-6)                         */
 } ls_params;
 
 /* Fields that can be uploaded / downloaded.  Element types are fixed:

@@ -1061,6 +1061,14 @@ static int nblocks_stencil(const ls_ctx* c) {
 
 // r = b^ - A^ x^ (FP64), p = r; fills cs (rr, wrr, bnorm2, tol2, done).
 static int cg64_residual(ls_ctx* c, double rtol, int maxit) {
+    if (c->resid_ready > 0) {
+        // the projected start left r64 (and p64) and the partials in place
+        hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(1024), 0, c->stream, c->partials,
+                           c->resid_ready, c->cg, rtol, maxit);
+        c->resid_ready = 0;
+        LS_LAUNCH_CHECK();
+        return 0;
+    }
     SGeo g = sgeo(c);
     int nb;
     if (use_vec<double>(c)) {
@@ -2508,6 +2516,38 @@ __global__ void k_wcombine(WBasis B, WCoef C, int m, double* __restrict__ x, uin
     }
 }
 
+// The one-vector projected start and its residual in one pass: x = cc w and
+// r = b - cc (A^ w), with A^ w from the projection's stencil pass (p = r as
+// well when p is given), and the partials of the residual kernel
+// (k_stencil_vec MODE 1): r.r, |sd r|^2, |sd b|^2.  w may be x itself.
+__global__ __launch_bounds__(EW_BLOCK) void k_wfinish(const double* w,
+                                                      const double* __restrict__ aw,
+                                                      const double* __restrict__ b,
+                                                      const double* __restrict__ sd, double cc,
+                                                      double* x, double* __restrict__ r,
+                                                      double* __restrict__ p, uint32_t N,
+                                                      double* __restrict__ partials) {
+    __shared__ double lds[3 * EW_BLOCK / WAVE];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+        const double wv = w[v], bv = b[v], sv = sd[v];
+        const double rv = bv - cc * aw[v];
+        x[v] = cc * wv;
+        r[v] = rv;
+        if (p != nullptr) p[v] = rv;
+        const double sr = sv * rv, sb = sv * bv;
+        a0 += rv * rv;
+        a1 += sr * sr;
+        a2 += sb * sb;
+    }
+    double a[3] = {a0, a1, a2};
+    block_reduce<3, RED_SUM, RED_SUM, RED_SUM>(a, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = a[0];
+        partials[gridDim.x + blockIdx.x] = a[1];
+        partials[2 * gridDim.x + blockIdx.x] = a[2];
+    }
+}
 __global__ void k_wscale_inplace(double* __restrict__ a, double* __restrict__ b, double s,
                                  uint32_t N) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
@@ -2605,12 +2645,20 @@ static int warm_accumulate(ls_ctx* c, const WarmSet& S, int from) {
     return 0;
 }
 
+// A solution of the history ring; a one-slot ring keeps none of its own:
+// its solution is the pressure field, which only the pressure phase writes
+// (an upload, a rollback or a new initial state empties the history).
+static const double* hist_vec(const ls_ctx* c, int slot) {
+    return c->hist_ring == 1 ? c->p : c->hist[slot];
+}
+
 // x64 <- projected warm start (the assembled system is in place and x64
 // holds the previous-pressure start).  Work vectors: the pressure scratch
 // (basis vector a in pw[2a], its image A^ w in pw[2a + 1]).
 static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
     rep->warm_basis = 0;
     rep->warm_net = 0;
+    c->resid_ready = 0;
     if (c->P.warm_start == LS_WARM_PREVIOUS || c->n_hist == 0) return 0;
     int st = warm_alloc(c);
     if (st) return st;
@@ -2637,10 +2685,14 @@ static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
     // its energy and projection from one stencil pass with both dots
     if (m == 1 && c->P.warm_start == LS_WARM_PROJECTED && use_vec<double>(c)) {
         const int slot = (c->hist_head + c->hist_ring - 1) % c->hist_ring;
-        double* w = c->pw[0];
+        // with a one-solution history the solution is p itself, and w = sd p
+        // is the previous-pressure start k_press_rhs_scale left in x64
+        const bool wp = c->hist_ring == 1;
+        double* w = wp ? c->x64 : c->pw[0];
         double* aw = c->pw[1];
-        hipLaunchKernelGGL(k_wscale, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->hist[slot],
-                           nullptr, c->sd, w, N);
+        if (!wp)
+            hipLaunchKernelGGL(k_wscale, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->hist[slot],
+                               nullptr, c->sd, w, N);
         const dim3 grd = vec_grid<double>(c);
         const int nbv = (int)(grd.x * grd.y * grd.z);
         hipLaunchKernelGGL((k_stencil_vec<double, 2>), grd, dim3(WAVE, StencilShape<double>::TJV),
@@ -2653,13 +2705,24 @@ static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
         LS_HIP(hipStreamSynchronize(c->stream));
         const double e = c->h_wsum[0], gb = c->h_wsum[1];
         if (!(e > 0.0) || !std::isfinite(e) || !std::isfinite(gb)) return 0;
-        WBasis B{};
-        WCoef C{};
-        B.v[0] = w;
-        C.c[0] = gb / e;
-        hipLaunchKernelGGL(k_wcombine, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, B, C, 1, c->x64,
-                           N, 0);
-        LS_LAUNCH_CHECK();
+        if (c->P.solver != LS_SOLVER_LEARNED) {
+            // x = (w.b^ / w.A^w) w and its residual b^ - x.A^w in one pass;
+            // the solver's first residual call takes them
+            hipLaunchKernelGGL(k_wfinish, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, w, aw,
+                               c->bhat, c->sd, gb / e, c->x64, c->r64,
+                               c->P.solver == LS_SOLVER_PLAIN64 ? c->p64 : nullptr, N,
+                               c->partials);
+            LS_LAUNCH_CHECK();
+            c->resid_ready = nbe;
+        } else {
+            WBasis B{};
+            WCoef C{};
+            B.v[0] = w;
+            C.c[0] = gb / e;
+            hipLaunchKernelGGL(k_wcombine, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, B, C, 1,
+                               c->x64, N, 0);
+            LS_LAUNCH_CHECK();
+        }
         rep->warm_basis = 1;
         return 0;
     }
@@ -2671,8 +2734,8 @@ static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
         double* w = c->pw[2 * slot_used];
         double* aw = c->pw[2 * slot_used + 1];
         hipLaunchKernelGGL(k_wscale, dim3(nbe), dim3(EW_BLOCK), 0, c->stream,
-                           a == 0 ? c->hist[slot] : c->hist[newer],
-                           a == 0 ? nullptr : c->hist[slot], c->sd, w, N);
+                           a == 0 ? hist_vec(c, slot) : hist_vec(c, newer),
+                           a == 0 ? nullptr : hist_vec(c, slot), c->sd, w, N);
         LS_LAUNCH_CHECK();
         bool added = false;
         if ((st = warm_add(c, S, w, aw, &added))) return st;
@@ -2731,7 +2794,7 @@ static int warm_push(ls_ctx* c, double ptop, bool replace) {
     // and frees the slots beyond it
     const int ring = std::max(1, std::min(c->P.warm_history, LS_WARM_MAX));
     if (ring != c->hist_ring) {
-        for (int q = ring; q < LS_WARM_MAX; ++q)
+        for (int q = ring == 1 ? 0 : ring; q < LS_WARM_MAX; ++q)
             if (c->hist[q]) {
                 LS_HIP(hipStreamSynchronize(c->stream));
                 for (size_t a = 0; a < c->allocations.size(); ++a)
@@ -2747,6 +2810,12 @@ static int warm_push(ls_ctx* c, double ptop, bool replace) {
         c->n_hist = 0;
         c->hist_head = 0;
         replace = false;
+    }
+    if (ring == 1) {            // the solution is p itself: nothing to copy
+        c->hist_ptop[0] = ptop;
+        c->hist_head = 0;
+        c->n_hist = 1;
+        return 0;
     }
     if (replace && c->n_hist > 0) {
         const int newest = (c->hist_head + c->hist_ring - 1) % c->hist_ring;

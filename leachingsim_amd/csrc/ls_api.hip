@@ -265,7 +265,7 @@ void ls_default_params(ls_params* p) {
     p->adr_schur = 1;
     p->adr_extrapolate = 1;
     p->adr_mixed = 1;
-    p->adr_mixed_inner_rtol = 1e-5;
+    p->adr_mixed_inner_rtol = 1e-6;
     p->mixed_inner_rtol = 0.1;
 }
 
@@ -565,7 +565,8 @@ static int step_snapshot(ls_ctx* c) {
     st |= copy_d2d(c, c->b_s, c->s, N * 8);
     st |= copy_d2d(c, c->b_p, c->p, N * 8);
     st |= copy_d2d(c, c->b_conc, c->conc, N * 8);
-    st |= copy_d2d(c, c->b_acid, c->acid, N * 8);
+    // acid known zero is restored by zeroing it (b_acid_zero below)
+    if (!c->acid_zero) st |= copy_d2d(c, c->b_acid, c->acid, N * 8);
     std::swap(c->qx, c->b_qx);
     std::swap(c->qy, c->b_qy);
     std::swap(c->qz, c->b_qz);
@@ -596,7 +597,10 @@ int ls_rollback(ls_ctx* c) {
     st |= copy_d2d(c, c->s, c->b_s, N * 8);
     st |= copy_d2d(c, c->p, c->b_p, N * 8);
     st |= copy_d2d(c, c->conc, c->b_conc, N * 8);
-    st |= copy_d2d(c, c->acid, c->b_acid, N * 8);
+    if (c->b_acid_zero)
+        LS_HIP(hipMemsetAsync(c->acid, 0, N * 8, c->stream));
+    else
+        st |= copy_d2d(c, c->acid, c->b_acid, N * 8);
     st |= copy_d2d(c, c->qx, c->b_qx, N * 8);
     st |= copy_d2d(c, c->qy, c->b_qy, N * 8);
     st |= copy_d2d(c, c->qz, c->b_qz, N * 8);

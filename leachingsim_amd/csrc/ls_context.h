@@ -190,6 +190,8 @@ struct ls_ctx {
     // warm-start history: the last n_hist pressure solutions (physical p,
     // FP64), newest at hist[(hist_head + hist_ring - 1) % hist_ring]
     double* hist[LS_WARM_MAX] = {};
+    // block partials of a residual the projected start computed (0: none)
+    int resid_ready = 0;
     double hist_ptop[LS_WARM_MAX] = {};   // effective top pressure of each
     int n_hist = 0;
     int hist_head = 0;

@@ -100,7 +100,7 @@ def default_params(**overrides) -> Params:
                press_maxit=10000, adr_rtol=1e-8, adr_maxit=10000, warm_history=1,
                mixed_inner_rtol=0.1, learned_race=1.0, implicit_sat=0,
                mixed_schur=1, adr_schur=1, adr_extrapolate=1, adr_mixed=1,
-               adr_mixed_inner_rtol=1e-5)
+               adr_mixed_inner_rtol=1e-6)
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise KeyError(k)
