@@ -65,11 +65,14 @@ struct AdrRow {
     uint8_t type;
 };
 
-__device__ __forceinline__ void adr_row(const uint8_t* __restrict__ img, const double* __restrict__ qx,
-                                        const double* __restrict__ qy, const double* __restrict__ qz,
-                                        const TGeo& g, int i, int j, int k, uint32_t v,
-                                        const AdrConst& a, AdrRow& R) {
-    const uint8_t tv = img[v];
+// The row of one voxel from its label, its neighbours' labels (has[f]: the
+// neighbour exists) and, for a fluid voxel, its face fluxes, which getq
+// loads (qp the +faces, qm the -faces) only for such a voxel.  R.nb is the
+// caller's.
+template <typename QF>
+__device__ __forceinline__ void adr_row_core(uint8_t tv, const uint8_t (&nt)[6],
+                                             const bool (&has)[6], QF getq, const AdrConst& a,
+                                             AdrRow& R) {
     R.type = tv;
     R.mask = 0;
     R.explicit_mask = 0;
@@ -77,19 +80,6 @@ __device__ __forceinline__ void adr_row(const uint8_t* __restrict__ img, const d
         R.diag = 1.0;
         return;
     }
-    // The neighbours' labels (and, for a fluid voxel, the face fluxes) are
-    // all loaded before any decision on them, so the loads overlap; a missing
-    // neighbour points at v itself and is not used.
-    bool has[6];
-    uint32_t nb[6];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        has[f] = tnbr(g, i, j, k, v, f, nb[f]);
-        if (!has[f]) nb[f] = v;
-    }
-    uint8_t nt[6];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) nt[f] = img[nb[f]];
     R.diag = a.inv_dt;
     if (tv == ROCK) {          // 2. Rock: 0-D Warren-Root bin
 #pragma unroll
@@ -97,7 +87,6 @@ __device__ __forceinline__ void adr_row(const uint8_t* __restrict__ img, const d
             if (has[f] && nt[f] == PORE) {
                 R.diag += a.alpha;
                 R.off[f] = -a.alpha;
-                R.nb[f] = nb[f];
                 R.mask |= 1 << f;
             }
         }
@@ -105,14 +94,13 @@ __device__ __forceinline__ void adr_row(const uint8_t* __restrict__ img, const d
     }
     // 3. Pore / Air / Sulphide: active fluid network.  Outward face flux:
     // +face q[v], -face -q[v - s] (faces +i, -i, +j, -j, +k, -k)
-    const double qp[3] = {qx[v], qy[v], qz[v]};
-    const double qm[3] = {qx[nb[1]], qy[nb[3]], qz[nb[5]]};
+    double qp[3], qm[3];
+    getq(qp, qm);
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
         if (!has[f]) continue;
         const double q_face = (f & 1) ? -qm[f >> 1] : qp[f >> 1];
         const uint8_t neigh_type = nt[f];
-        R.nb[f] = nb[f];
         R.mask |= 1 << f;
         if (neigh_type == ROCK) {
             R.diag += a.alpha;
@@ -134,6 +122,42 @@ __device__ __forceinline__ void adr_row(const uint8_t* __restrict__ img, const d
         R.cn[f] = coeff_neighbor;
         R.explicit_mask |= 1 << f;
     }
+}
+
+__device__ __forceinline__ void adr_row(const uint8_t* __restrict__ img, const double* __restrict__ qx,
+                                        const double* __restrict__ qy, const double* __restrict__ qz,
+                                        const TGeo& g, int i, int j, int k, uint32_t v,
+                                        const AdrConst& a, AdrRow& R) {
+    const uint8_t tv = img[v];
+    if (tv == PREC) {
+        R.type = tv;
+        R.mask = 0;
+        R.explicit_mask = 0;
+        R.diag = 1.0;
+        return;
+    }
+    // The neighbours' labels (and, for a fluid voxel, the face fluxes) are
+    // all loaded before any decision on them, so the loads overlap; a missing
+    // neighbour points at v itself and is not used.
+    bool has[6];
+    uint32_t nb[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        has[f] = tnbr(g, i, j, k, v, f, nb[f]);
+        if (!has[f]) nb[f] = v;
+    }
+    uint8_t nt[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) nt[f] = img[nb[f]];
+    adr_row_core(tv, nt, has,
+                 [&](double (&qp)[3], double (&qm)[3]) {
+                     qp[0] = qx[v]; qp[1] = qy[v]; qp[2] = qz[v];
+                     qm[0] = qx[nb[1]]; qm[1] = qy[nb[3]]; qm[2] = qz[nb[5]];
+                 },
+                 a, R);
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+        if (R.mask & (1 << f)) R.nb[f] = nb[f];
 }
 
 // The explicit (1 - theta) part of the right-hand sides of both systems
@@ -1027,6 +1051,176 @@ __device__ __forceinline__ void tsv_kn(const double* __restrict__ src, uint32_t 
     } else {
         km = d2{c.x, c.y};
         kp = d2{c.y, m0 + 2 < nhr ? next : 0.0};
+    }
+}
+
+// The conc system's rows (KIND 0 of k_adr_rows_split) with a voxel pair per
+// lane: the pair (2m, 2m + 1) of a row (i, j) holds one black and one red
+// voxel (the black one at 2m + par, par = (i + j) & 1), so labels are read
+// as 2-byte pairs and fluxes and fields as 16-byte pairs, the k neighbours
+// come from the neighbouring lanes, and each colour-split row array is
+// written by all lanes of a wave at consecutive m.  Rows are computed by
+// adr_row_core, as in k_adr_rows_split: the same bits.  A block is 64 lanes
+// x TSV_TR waves, each wave walks TSV_RPW rows (i, j); nk even.
+__device__ __forceinline__ uint32_t ld_lab2(const uint8_t* __restrict__ img, uint32_t v) {
+    return (uint32_t)(*(const uint16_t*)(img + v));
+}
+
+template <bool EXPL>
+__global__ __launch_bounds__(WAVE * TSV_TR) void k_adr_rows_pair(
+    const uint8_t* __restrict__ img, double* qx, double* qy, double* qz, TVGeo g, uint32_t N,
+    AdrConst a, double* __restrict__ rows, const double* __restrict__ x0,
+    const double* __restrict__ x1, double* __restrict__ b0, double* __restrict__ b1, int one,
+    double* __restrict__ xs0, double* __restrict__ xs1, double* __restrict__ xprev,
+    double* __restrict__ xprev2, int order, double ratio, double e0c, double e1c, double e2c) {
+    const uint32_t Nh = N / 2;
+    const double* x1r = one ? x0 : x1;
+    const int lane = threadIdx.x;
+    const uint32_t m = blockIdx.x * WAVE + lane;
+    const bool ok = m < g.nhr;
+    const uint32_t nk = 2 * g.nhr;
+    const uint32_t sj = nk, si = nk * (uint32_t)g.nj;
+    const d2 zero = d2{0.0, 0.0};
+    constexpr bool expl = EXPL;     // theta != 1: the explicit part of the old field
+#pragma unroll 1
+    for (int q8 = 0; q8 < TSV_RPW; ++q8) {
+        const uint32_t row = (blockIdx.y * TSV_RPW + q8) * TSV_TR + threadIdx.y;
+        if (row >= g.nrows) continue;                   // uniform per wave
+        const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
+        const int par = (i + j) & 1;
+        const uint32_t v0 = row * nk + 2 * m;
+        const uint32_t mg = row * g.nhr + m;            // the pair's compacted index
+        const bool hip_ = i + 1 < g.ni, him = i > 0, hjp = j + 1 < g.nj, hjm = j > 0;
+        // every load of the pair first
+        const uint32_t lc = ok ? ld_lab2(img, v0) : 0u;
+        const uint32_t lip = (ok && hip_) ? ld_lab2(img, v0 + si) : 0u;
+        const uint32_t lim = (ok && him) ? ld_lab2(img, v0 - si) : 0u;
+        const uint32_t ljp = (ok && hjp) ? ld_lab2(img, v0 + sj) : 0u;
+        const uint32_t ljm = (ok && hjm) ? ld_lab2(img, v0 - sj) : 0u;
+        const d2 qxp = ok ? *(const d2*)(qx + v0) : zero;
+        const d2 qyp = ok ? *(const d2*)(qy + v0) : zero;
+        const d2 qzp = ok ? *(const d2*)(qz + v0) : zero;
+        const d2 qxm = (ok && him) ? *(const d2*)(qx + v0 - si) : zero;
+        const d2 qym = (ok && hjm) ? *(const d2*)(qy + v0 - sj) : zero;
+        const d2 xa2 = ok ? *(const d2*)(x0 + v0) : zero;
+        const d2 xb2 = ok ? *(const d2*)(x1r + v0) : zero;
+        // the k neighbours outside the pair: voxel 2m - 1 (the left lane's
+        // second) and 2m + 2 (the right lane's first); the wave's edge lanes
+        // load theirs
+        uint32_t kl = (uint32_t)__shfl_up((int)lc, 1, WAVE) >> 8;
+        uint32_t kr = (uint32_t)__shfl_down((int)lc, 1, WAVE) & 0xffu;
+        double qzl = __shfl_up(qzp.y, 1, WAVE);
+        if (lane == 0) {
+            kl = (ok && m >= 1) ? (uint32_t)img[v0 - 1] : 0u;
+            qzl = (ok && m >= 1) ? qz[v0 - 1] : 0.0;
+        }
+        if (lane == WAVE - 1) kr = (ok && m + 1 < g.nhr) ? (uint32_t)img[v0 + 2] : 0u;
+        // the old field at the neighbours (the explicit part, theta != 1)
+        d2 nip = zero, nim = zero, njp = zero, njm = zero, mip = zero, mim = zero, mjp = zero,
+           mjm = zero;
+        double nkl = 0.0, nkr = 0.0, mkl = 0.0, mkr = 0.0;
+        if (expl) {
+            if (ok && hip_) { nip = *(const d2*)(x0 + v0 + si); mip = *(const d2*)(x1r + v0 + si); }
+            if (ok && him) { nim = *(const d2*)(x0 + v0 - si); mim = *(const d2*)(x1r + v0 - si); }
+            if (ok && hjp) { njp = *(const d2*)(x0 + v0 + sj); mjp = *(const d2*)(x1r + v0 + sj); }
+            if (ok && hjm) { njm = *(const d2*)(x0 + v0 - sj); mjm = *(const d2*)(x1r + v0 - sj); }
+            nkl = __shfl_up(xa2.y, 1, WAVE);
+            mkl = __shfl_up(xb2.y, 1, WAVE);
+            nkr = __shfl_down(xa2.x, 1, WAVE);
+            mkr = __shfl_down(xb2.x, 1, WAVE);
+            if (lane == 0) {
+                nkl = (ok && m >= 1) ? x0[v0 - 1] : 0.0;
+                mkl = (ok && m >= 1) ? x1r[v0 - 1] : 0.0;
+            }
+            if (lane == WAVE - 1) {
+                nkr = (ok && m + 1 < g.nhr) ? x0[v0 + 2] : 0.0;
+                mkr = (ok && m + 1 < g.nhr) ? x1r[v0 + 2] : 0.0;
+            }
+        }
+        if (!ok) continue;
+        double bav[2], bbv[2];
+        bool zq[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int k = 2 * (int)m + e;
+            const uint8_t tv = (uint8_t)(e ? (lc >> 8) : (lc & 0xffu));
+            const bool has[6] = {hip_, him, hjp, hjm, k + 1 < (int)nk, k > 0};
+            const uint8_t nt[6] = {(uint8_t)(e ? (lip >> 8) : (lip & 0xffu)),
+                                   (uint8_t)(e ? (lim >> 8) : (lim & 0xffu)),
+                                   (uint8_t)(e ? (ljp >> 8) : (ljp & 0xffu)),
+                                   (uint8_t)(e ? (ljm >> 8) : (ljm & 0xffu)),
+                                   (uint8_t)(e ? kr : (lc >> 8)),
+                                   (uint8_t)(e ? (lc & 0xffu) : kl)};
+            AdrRow R;
+            adr_row_core(tv, nt, has,
+                         [&](double (&qp)[3], double (&qm)[3]) {
+                             qp[0] = e ? qxp.y : qxp.x;
+                             qp[1] = e ? qyp.y : qyp.x;
+                             qp[2] = e ? qzp.y : qzp.x;
+                             qm[0] = e ? qxm.y : qxm.x;
+                             qm[1] = e ? qym.y : qym.x;
+                             qm[2] = e ? qzp.x : qzl;
+                         },
+                         a, R);
+            zq[e] = R.type == PREC || R.type == ROCK;
+            const double xa = e ? xa2.y : xa2.x, xb = e ? xb2.y : xb2.x;
+            double ba, bb;
+            if (R.type == PREC) {
+                ba = xa;
+                bb = xb;
+            } else {
+                ba = xa * a.inv_dt;
+                bb = xb * a.inv_dt;
+                if (expl) {
+                    const double n0[6] = {e ? nip.y : nip.x, e ? nim.y : nim.x,
+                                          e ? njp.y : njp.x, e ? njm.y : njm.x,
+                                          e ? nkr : xa2.y, e ? xa2.x : nkl};
+                    const double n1[6] = {e ? mip.y : mip.x, e ? mim.y : mim.x,
+                                          e ? mjp.y : mjp.x, e ? mjm.y : mjm.x,
+                                          e ? mkr : xb2.y, e ? xb2.x : mkl};
+#pragma unroll
+                    for (int f = 0; f < 6; ++f) {
+                        if (!(R.explicit_mask & (1 << f))) continue;
+                        double ea = -(R.cs[f] * xa + R.cn[f] * n0[f]);
+                        double eb = -(R.cs[f] * xb + R.cn[f] * n1[f]);
+                        ba += (1.0 - a.theta) * ea;
+                        bb += (1.0 - a.theta) * eb;
+                    }
+                }
+            }
+            bav[e] = ba;
+            bbv[e] = bb;
+            const int colour = (par + e) & 1;
+            rows[((size_t)0 * 2 + colour) * rstride(Nh) + mg] = R.diag;
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+                rows[((size_t)(f + 1) * 2 + colour) * rstride(Nh) + mg] =
+                    (R.mask & (1 << f)) ? R.off[f] : 0.0;
+            if (colour == 0) {
+                double xstart = xa;
+                if (xprev != nullptr) {
+                    const double xp = xprev[mg];
+                    if (xprev2 != nullptr) {
+                        const double xq = xprev2[mg];
+                        if (order == 2) xstart = e2c * xa + e1c * xp + e0c * xq;
+                        xprev2[mg] = xp;
+                    }
+                    if (order == 1) xstart = xa + ratio * (xa - xp);
+                    xprev[mg] = xa;
+                }
+                xs0[mg] = R.diag * xstart;
+                if (!one) xs1[mg] = R.diag * xb;
+            }
+        }
+        *(d2*)(b0 + v0) = d2{bav[0], bav[1]};
+        if (!one) *(d2*)(b1 + v0) = d2{bbv[0], bbv[1]};
+        // the flux at Rock and Precipitate voxels is zeroed, as in
+        // k_adr_rows_split (no row uses it)
+        if (zq[0] || zq[1]) {
+            *(d2*)(qx + v0) = d2{zq[0] ? 0.0 : qxp.x, zq[1] ? 0.0 : qxp.y};
+            *(d2*)(qy + v0) = d2{zq[0] ? 0.0 : qyp.x, zq[1] ? 0.0 : qyp.y};
+            *(d2*)(qz + v0) = d2{zq[0] ? 0.0 : qzp.x, zq[1] ? 0.0 : qzp.y};
+        }
     }
 }
 
@@ -2208,10 +2402,30 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             e1 = (h - t0) * (h - 0.0) / ((t1 - t0) * (t1 - 0.0));
             e0 = (h - t1) * (h - 0.0) / ((t0 - t1) * (t0 - 0.0));
         }
-        hipLaunchKernelGGL((k_adr_rows_split<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
-                           c->img, c->qx, c->qy, c->qz, sf, g, N, a, c->arows, x[0], x[1],
-                           c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], xprev, xprev2, order,
-                           ratio, e0, e1, e2);
+        // conc's rows with a voxel pair per lane where the rows fit the grid
+        TVGeo pg;
+        pg.nhr = (uint32_t)(c->nk / 2);
+        pg.nrows = (uint32_t)c->ni * (uint32_t)c->nj;
+        pg.plane = (uint32_t)c->nj * pg.nhr;
+        pg.ni = c->ni;
+        pg.nj = c->nj;
+        const dim3 pgrid((pg.nhr + WAVE - 1) / WAVE,
+                         (pg.nrows + TSV_TR * TSV_RPW - 1) / (TSV_TR * TSV_RPW));
+        if (KIND == 0 && pgrid.y <= 65535 && a.theta == 1.0)
+            hipLaunchKernelGGL(k_adr_rows_pair<false>, pgrid, dim3(WAVE, TSV_TR), 0, c->stream,
+                               c->img, c->qx, c->qy, c->qz, pg, N, a, c->arows, x[0], x[1],
+                               c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], xprev, xprev2,
+                               order, ratio, e0, e1, e2);
+        else if (KIND == 0 && pgrid.y <= 65535)
+            hipLaunchKernelGGL(k_adr_rows_pair<true>, pgrid, dim3(WAVE, TSV_TR), 0, c->stream,
+                               c->img, c->qx, c->qy, c->qz, pg, N, a, c->arows, x[0], x[1],
+                               c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], xprev, xprev2,
+                               order, ratio, e0, e1, e2);
+        else
+            hipLaunchKernelGGL((k_adr_rows_split<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
+                               c->img, c->qx, c->qy, c->qz, sf, g, N, a, c->arows, x[0], x[1],
+                               c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], xprev, xprev2,
+                               order, ratio, e0, e1, e2);
         if (KIND == 0) {
             if (xprev == nullptr) {
                 c->tprev_n = 0;
