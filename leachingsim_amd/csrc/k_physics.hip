@@ -284,6 +284,47 @@ __device__ __forceinline__ double face_cond(uint8_t tv, uint8_t tn, double pv, d
     return 0.0;
 }
 
+// One voxel of k_press_diag: its class, Dirichlet value and sqrt(d) from its
+// label, its neighbours' labels and permeabilities (ok[f]: the neighbour
+// exists) and its own permeability and pressure.
+struct PressDiag {
+    int8_t cls;
+    double pD, sd;
+};
+__device__ __forceinline__ PressDiag press_diag_core(uint8_t tv, const uint8_t (&tn)[6],
+                                                     const bool (&ok)[6], double pv,
+                                                     const double (&kn)[6], double pvv, int i,
+                                                     int ni, int zfaces, double ptop) {
+    PressDiag o;
+    if (tv == PORE) {
+        double diagonal_term = 0.0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            if (!ok[f]) continue;
+            const double K_neighbor = (tn[f] == AIR) ? 1.0e-9 : kn[f];
+            diagonal_term -= pore_link(pv, K_neighbor, viscosity(tv));
+        }
+        // the virtual inlet / outlet planes of the z-face boundary
+        if (zfaces && i == 0) diagonal_term -= pore_link(pv, 1.0e-9, viscosity(tv));
+        if (zfaces && i == ni - 1) diagonal_term -= pore_link(pv, 1.0e-9, viscosity(tv));
+        const int8_t cl = (fabs(diagonal_term) < 1e-16) ? 2 : 1;
+        o.cls = cl;
+        o.pD = (cl == 2) ? pvv : 0.0;  // floored rows: lagged value
+        o.sd = (cl == 1) ? sqrt(-diagonal_term) : 0.0;
+    } else {
+        o.cls = 0;
+        o.sd = 0.0;
+        double val = 0.0;
+        if (tv == AIR) {  // surface Air: any in-grid neighbour is not Air (:696-711)
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+                if (ok[f] && tn[f] != AIR) val = ptop;
+        }
+        o.pD = val;
+    }
+    return o;
+}
+
 __global__ void k_press_diag(const uint8_t* __restrict__ img, const double* __restrict__ perm,
                              const double* __restrict__ p,
                              int8_t* __restrict__ cls, double* __restrict__ pD,
@@ -302,36 +343,45 @@ __global__ void k_press_diag(const uint8_t* __restrict__ img, const double* __re
     uint8_t tn[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) tn[f] = img[n[f]];
+    double kn[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double pv = 0.0;
     if (tv == PORE) {
-        double kn[6];
 #pragma unroll
         for (int f = 0; f < 6; ++f) kn[f] = perm[n[f]];
-        const double pv = perm[v];
-        double diagonal_term = 0.0;
-#pragma unroll
-        for (int f = 0; f < 6; ++f) {
-            if (!ok[f]) continue;
-            const double K_neighbor = (tn[f] == AIR) ? 1.0e-9 : kn[f];
-            diagonal_term -= pore_link(pv, K_neighbor, viscosity(tv));
-        }
-        // the virtual inlet / outlet planes of the z-face boundary
-        if (zfaces && i == 0) diagonal_term -= pore_link(pv, 1.0e-9, viscosity(tv));
-        if (zfaces && i == g.ni - 1) diagonal_term -= pore_link(pv, 1.0e-9, viscosity(tv));
-        int8_t cl = (fabs(diagonal_term) < 1e-16) ? 2 : 1;
-        cls[v] = cl;
-        pD[v] = (cl == 2) ? p[v] : 0.0;  // floored rows: lagged value
-        sd[v] = (cl == 1) ? sqrt(-diagonal_term) : 0.0;
-    } else {
-        cls[v] = 0;
-        sd[v] = 0.0;
-        double val = 0.0;
-        if (tv == AIR) {  // surface Air: any in-grid neighbour is not Air (:696-711)
-#pragma unroll
-            for (int f = 0; f < 6; ++f)
-                if (ok[f] && tn[f] != AIR) val = ptop;
-        }
-        pD[v] = val;
+        pv = perm[v];
     }
+    const PressDiag o = press_diag_core(tv, tn, ok, pv, kn, tv == PORE ? p[v] : 0.0, i, g.ni,
+                                        zfaces, ptop);
+    cls[v] = o.cls;
+    pD[v] = o.pD;
+    sd[v] = o.sd;
+}
+
+// One solved voxel of k_press_rhs_scale: the scaled right-hand side and the
+// scaled +faces (sc), from the neighbours' labels, classes, permeabilities,
+// Dirichlet values and sqrt(d).
+__device__ __forceinline__ double press_rhs_core(const uint8_t (&tn)[6], const int8_t (&cn)[6],
+                                                 const bool (&ok)[6], double pv, double sv,
+                                                 const double (&kn)[6], const double (&dn)[6],
+                                                 const double (&sn)[6], int i, int zfaces,
+                                                 double ptop, double evap_dx, double (&sc)[3]) {
+    double source_term = 0.0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+        if (ok[f] && tn[f] == AIR) source_term -= evap_dx;
+    double rhs = -source_term;
+    sc[0] = sc[1] = sc[2] = 0.0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        if (!ok[f]) continue;
+        const int8_t cl = cn[f];
+        if (cl == 1 && (f & 1)) continue;       // a solved -face: the neighbour's +face
+        const double fc = face_cond(PORE, tn[f], pv, kn[f]);
+        if (cl != 1) rhs += fc * dn[f];
+        else sc[f >> 1] = fc / (sv * sn[f]);
+    }
+    if (zfaces && i == 0) rhs += pore_link(pv, 1.0e-9, viscosity(PORE)) * ptop;
+    return rhs / sv;
 }
 
 // Right-hand side (Air neighbours' evaporation source, Dirichlet neighbours,
@@ -376,26 +426,224 @@ __global__ void k_press_rhs_scale(const uint8_t* __restrict__ img, const double*
     const double pv = perm[v];
     const double sv = sd[v];
     const double pvv = p[v];
-    double source_term = 0.0;
-#pragma unroll
-    for (int f = 0; f < 6; ++f)
-        if (ok[f] && tn[f] == AIR) source_term -= evap_dx;
-    double rhs = -source_term;
-    double sc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        if (!ok[f]) continue;
-        const int8_t cl = cn[f];
-        if (cl == 1 && (f & 1)) continue;       // a solved -face: the neighbour's +face
-        const double fc = face_cond(PORE, tn[f], pv, kn[f]);
-        if (cl != 1) rhs += fc * dn[f];
-        else sc[f >> 1] = fc / (sv * sn[f]);
-    }
-    if (zfaces && i == 0) rhs += pore_link(pv, 1.0e-9, viscosity(PORE)) * ptop;
-    bhat[v] = rhs / sv;
+    double sc[3];
+    bhat[v] = press_rhs_core(tn, cn, ok, pv, sv, kn, dn, sn, i, zfaces, ptop, evap_dx, sc);
     x[v] = sv * pvv;  // warm start from the previous pressure, scaled variables
     hx[v] = sc[0]; hy[v] = sc[1]; hz[v] = sc[2];
     fx[v] = (float)sc[0]; fy[v] = (float)sc[1]; fz[v] = (float)sc[2];
+}
+
+// The same two passes with a voxel pair (k = 2m, 2m + 1) per lane, nk even:
+// labels and classes as 2-byte pairs, doubles as 16-byte pairs, the k
+// neighbours outside the pair from the neighbouring lanes (the wave's edge
+// lanes load theirs), and the per-voxel arithmetic of press_diag_core /
+// press_rhs_core (the same bits).  A block is 64 lanes x PP_TR waves, each
+// wave walks PP_RPW rows (i, j).
+constexpr int PP_TR = 4, PP_RPW = 4;
+typedef double pd2 __attribute__((ext_vector_type(2)));
+typedef float pf2 __attribute__((ext_vector_type(2)));
+
+struct PairGeo {
+    uint32_t nhr, nrows;   // pairs per row (nk / 2), rows (ni nj)
+    uint32_t si, sj;
+    int ni, nj;
+};
+
+__device__ __forceinline__ uint32_t ld_b2(const void* __restrict__ a, uint32_t v) {
+    return (uint32_t)(*(const uint16_t*)((const uint8_t*)a + v));
+}
+__device__ __forceinline__ pd2 ld_d2(const double* __restrict__ a, uint32_t v) {
+    return *(const pd2*)(a + v);
+}
+// the k neighbours outside the pair: the left lane's second value and the
+// right lane's first (the edge lanes load them, m the lane's pair index)
+template <typename T>
+__device__ __forceinline__ void k_out(T lo, T hi, const T* __restrict__ a, uint32_t v0,
+                                      uint32_t m, uint32_t nhr, bool ok, T& left, T& right) {
+    const int lane = threadIdx.x;
+    left = __shfl_up(hi, 1, WAVE);
+    right = __shfl_down(lo, 1, WAVE);
+    if (lane == 0) left = (ok && m >= 1) ? a[v0 - 1] : T(0);
+    if (lane == WAVE - 1) right = (ok && m + 1 < nhr) ? a[v0 + 2] : T(0);
+}
+
+__global__ __launch_bounds__(WAVE * PP_TR) void k_press_diag_pair(
+    const uint8_t* __restrict__ img, const double* __restrict__ perm,
+    const double* __restrict__ p, int8_t* __restrict__ cls, double* __restrict__ pD,
+    double* __restrict__ sd, PairGeo g, int zfaces, double ptop) {
+    const uint32_t m = blockIdx.x * WAVE + threadIdx.x;
+    const bool ok_ = m < g.nhr;
+    const int nk = 2 * (int)g.nhr;
+    const pd2 z2 = pd2{0.0, 0.0};
+#pragma unroll 1
+    for (int q8 = 0; q8 < PP_RPW; ++q8) {
+        const uint32_t row = (blockIdx.y * PP_RPW + q8) * PP_TR + threadIdx.y;
+        if (row >= g.nrows) continue;                   // uniform per wave
+        const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
+        const uint32_t v0 = row * (uint32_t)nk + 2 * m;
+        const bool hip_ = i + 1 < g.ni, him = i > 0, hjp = j + 1 < g.nj, hjm = j > 0;
+        const uint32_t lc = ok_ ? ld_b2(img, v0) : 0u;
+        const uint32_t lip = (ok_ && hip_) ? ld_b2(img, v0 + g.si) : 0u;
+        const uint32_t lim = (ok_ && him) ? ld_b2(img, v0 - g.si) : 0u;
+        const uint32_t ljp = (ok_ && hjp) ? ld_b2(img, v0 + g.sj) : 0u;
+        const uint32_t ljm = (ok_ && hjm) ? ld_b2(img, v0 - g.sj) : 0u;
+        const pd2 kc = ok_ ? ld_d2(perm, v0) : z2;
+        const pd2 kip = (ok_ && hip_) ? ld_d2(perm, v0 + g.si) : z2;
+        const pd2 kim = (ok_ && him) ? ld_d2(perm, v0 - g.si) : z2;
+        const pd2 kjp = (ok_ && hjp) ? ld_d2(perm, v0 + g.sj) : z2;
+        const pd2 kjm = (ok_ && hjm) ? ld_d2(perm, v0 - g.sj) : z2;
+        const pd2 pc = ok_ ? ld_d2(p, v0) : z2;
+        int l_left, l_right;
+        k_out<int>((int)(lc & 0xffu), (int)(lc >> 8), (const int*)nullptr, 0u, m, g.nhr, false,
+                   l_left, l_right);
+        uint32_t kl = (uint32_t)l_left, kr = (uint32_t)l_right;
+        if (threadIdx.x == 0) kl = (ok_ && m >= 1) ? (uint32_t)img[v0 - 1] : 0u;
+        if (threadIdx.x == WAVE - 1) kr = (ok_ && m + 1 < g.nhr) ? (uint32_t)img[v0 + 2] : 0u;
+        double kkl, kkr;
+        k_out<double>(kc.x, kc.y, perm, v0, m, g.nhr, ok_, kkl, kkr);
+        if (!ok_) continue;
+        PressDiag o[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int k = 2 * (int)m + e;
+            const bool okf[6] = {hip_, him, hjp, hjm, k + 1 < nk, k > 0};
+            const uint8_t tn[6] = {(uint8_t)(e ? (lip >> 8) : (lip & 0xffu)),
+                                   (uint8_t)(e ? (lim >> 8) : (lim & 0xffu)),
+                                   (uint8_t)(e ? (ljp >> 8) : (ljp & 0xffu)),
+                                   (uint8_t)(e ? (ljm >> 8) : (ljm & 0xffu)),
+                                   (uint8_t)(e ? kr : (lc >> 8)),
+                                   (uint8_t)(e ? (lc & 0xffu) : kl)};
+            const double kn[6] = {e ? kip.y : kip.x, e ? kim.y : kim.x, e ? kjp.y : kjp.x,
+                                  e ? kjm.y : kjm.x, e ? kkr : kc.y, e ? kc.x : kkl};
+            const uint8_t tv = (uint8_t)(e ? (lc >> 8) : (lc & 0xffu));
+            o[e] = press_diag_core(tv, tn, okf, e ? kc.y : kc.x, kn, e ? pc.y : pc.x, i, g.ni,
+                                   zfaces, ptop);
+        }
+        *(uint16_t*)(cls + v0) = (uint16_t)((uint8_t)o[0].cls | ((uint32_t)(uint8_t)o[1].cls << 8));
+        *(pd2*)(pD + v0) = pd2{o[0].pD, o[1].pD};
+        *(pd2*)(sd + v0) = pd2{o[0].sd, o[1].sd};
+    }
+}
+
+__global__ __launch_bounds__(WAVE * PP_TR) void k_press_rhs_scale_pair(
+    const uint8_t* __restrict__ img, const double* __restrict__ perm,
+    const int8_t* __restrict__ cls, const double* __restrict__ pD,
+    const double* __restrict__ p, const double* __restrict__ sd, double* __restrict__ bhat,
+    double* __restrict__ x, double* __restrict__ hx, double* __restrict__ hy,
+    double* __restrict__ hz, float* __restrict__ fx, float* __restrict__ fy,
+    float* __restrict__ fz, PairGeo g, int zfaces, double ptop, double evap_dx) {
+    const uint32_t m = blockIdx.x * WAVE + threadIdx.x;
+    const bool ok_ = m < g.nhr;
+    const int nk = 2 * (int)g.nhr;
+    const pd2 z2 = pd2{0.0, 0.0};
+#pragma unroll 1
+    for (int q8 = 0; q8 < PP_RPW; ++q8) {
+        const uint32_t row = (blockIdx.y * PP_RPW + q8) * PP_TR + threadIdx.y;
+        if (row >= g.nrows) continue;                   // uniform per wave
+        const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
+        const uint32_t v0 = row * (uint32_t)nk + 2 * m;
+        const bool hip_ = i + 1 < g.ni, him = i > 0, hjp = j + 1 < g.nj, hjm = j > 0;
+        const uint32_t cc = ok_ ? ld_b2(cls, v0) : 0u;
+        // the pair's solved voxels need their neighbours; a pair with none
+        // writes zeros
+        const bool any = ok_ && (((cc & 0xffu) == 1u) || ((cc >> 8) == 1u));
+        uint32_t lip = 0, lim = 0, ljp = 0, ljm = 0, lc = 0;
+        uint32_t cip = 0, cim = 0, cjp = 0, cjm = 0;
+        pd2 kc = z2, kip = z2, kim = z2, kjp = z2, kjm = z2;
+        pd2 dc = z2, dip = z2, dim = z2, djp = z2, djm = z2;
+        pd2 sc2 = z2, sip = z2, sim = z2, sjp = z2, sjm = z2, pc = z2;
+        // the pair's own values serve the neighbouring lanes' k faces too
+        if (ok_) {
+            lc = ld_b2(img, v0);
+            kc = ld_d2(perm, v0);
+            dc = ld_d2(pD, v0);
+            sc2 = ld_d2(sd, v0);
+        }
+        if (any) {
+            pc = ld_d2(p, v0);
+            if (hip_) {
+                lip = ld_b2(img, v0 + g.si); cip = ld_b2(cls, v0 + g.si);
+                kip = ld_d2(perm, v0 + g.si); dip = ld_d2(pD, v0 + g.si); sip = ld_d2(sd, v0 + g.si);
+            }
+            if (him) {
+                lim = ld_b2(img, v0 - g.si); cim = ld_b2(cls, v0 - g.si);
+                kim = ld_d2(perm, v0 - g.si); dim = ld_d2(pD, v0 - g.si); sim = ld_d2(sd, v0 - g.si);
+            }
+            if (hjp) {
+                ljp = ld_b2(img, v0 + g.sj); cjp = ld_b2(cls, v0 + g.sj);
+                kjp = ld_d2(perm, v0 + g.sj); djp = ld_d2(pD, v0 + g.sj); sjp = ld_d2(sd, v0 + g.sj);
+            }
+            if (hjm) {
+                ljm = ld_b2(img, v0 - g.sj); cjm = ld_b2(cls, v0 - g.sj);
+                kjm = ld_d2(perm, v0 - g.sj); djm = ld_d2(pD, v0 - g.sj); sjm = ld_d2(sd, v0 - g.sj);
+            }
+        }
+        // k neighbours outside the pair (every lane takes part in the
+        // shuffles; the edge lanes load theirs when their pair needs them)
+        int t_l, t_r, c_l, c_r;
+        k_out<int>((int)(lc & 0xffu), (int)(lc >> 8), (const int*)nullptr, 0u, m, g.nhr, false,
+                   t_l, t_r);
+        k_out<int>((int)(cc & 0xffu), (int)(cc >> 8), (const int*)nullptr, 0u, m, g.nhr, false,
+                   c_l, c_r);
+        double k_l, k_r, d_l, d_r, s_l, s_r;
+        k_out<double>(kc.x, kc.y, perm, v0, m, g.nhr, any, k_l, k_r);
+        k_out<double>(dc.x, dc.y, pD, v0, m, g.nhr, any, d_l, d_r);
+        k_out<double>(sc2.x, sc2.y, sd, v0, m, g.nhr, any, s_l, s_r);
+        if (threadIdx.x == 0) {
+            t_l = (any && m >= 1) ? (int)img[v0 - 1] : 0;
+            c_l = (any && m >= 1) ? (int)(uint8_t)cls[v0 - 1] : 0;
+        }
+        if (threadIdx.x == WAVE - 1) {
+            t_r = (any && m + 1 < g.nhr) ? (int)img[v0 + 2] : 0;
+            c_r = (any && m + 1 < g.nhr) ? (int)(uint8_t)cls[v0 + 2] : 0;
+        }
+        if (!ok_) continue;
+        double bh[2] = {0.0, 0.0}, xs[2] = {0.0, 0.0}, f0[2] = {0.0, 0.0}, f1[2] = {0.0, 0.0},
+               f2[2] = {0.0, 0.0};
+        if (any) {
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const uint32_t ce = e ? (cc >> 8) : (cc & 0xffu);
+                if (ce != 1u) continue;
+                const int k = 2 * (int)m + e;
+                const bool okf[6] = {hip_, him, hjp, hjm, k + 1 < nk, k > 0};
+                const uint8_t tn[6] = {(uint8_t)(e ? (lip >> 8) : (lip & 0xffu)),
+                                       (uint8_t)(e ? (lim >> 8) : (lim & 0xffu)),
+                                       (uint8_t)(e ? (ljp >> 8) : (ljp & 0xffu)),
+                                       (uint8_t)(e ? (ljm >> 8) : (ljm & 0xffu)),
+                                       (uint8_t)(e ? (uint32_t)t_r : (lc >> 8)),
+                                       (uint8_t)(e ? (lc & 0xffu) : (uint32_t)t_l)};
+                const int8_t cn[6] = {(int8_t)(e ? (cip >> 8) : (cip & 0xffu)),
+                                      (int8_t)(e ? (cim >> 8) : (cim & 0xffu)),
+                                      (int8_t)(e ? (cjp >> 8) : (cjp & 0xffu)),
+                                      (int8_t)(e ? (cjm >> 8) : (cjm & 0xffu)),
+                                      (int8_t)(e ? (uint32_t)c_r : (cc >> 8)),
+                                      (int8_t)(e ? (cc & 0xffu) : (uint32_t)c_l)};
+                const double kn[6] = {e ? kip.y : kip.x, e ? kim.y : kim.x, e ? kjp.y : kjp.x,
+                                      e ? kjm.y : kjm.x, e ? k_r : kc.y, e ? kc.x : k_l};
+                const double dn[6] = {e ? dip.y : dip.x, e ? dim.y : dim.x, e ? djp.y : djp.x,
+                                      e ? djm.y : djm.x, e ? d_r : dc.y, e ? dc.x : d_l};
+                const double sn[6] = {e ? sip.y : sip.x, e ? sim.y : sim.x, e ? sjp.y : sjp.x,
+                                      e ? sjm.y : sjm.x, e ? s_r : sc2.y, e ? sc2.x : s_l};
+                const double sv = e ? sc2.y : sc2.x;
+                double scf[3];
+                bh[e] = press_rhs_core(tn, cn, okf, e ? kc.y : kc.x, sv, kn, dn, sn, i, zfaces,
+                                       ptop, evap_dx, scf);
+                xs[e] = sv * (e ? pc.y : pc.x);
+                f0[e] = scf[0];
+                f1[e] = scf[1];
+                f2[e] = scf[2];
+            }
+        }
+        *(pd2*)(bhat + v0) = pd2{bh[0], bh[1]};
+        *(pd2*)(x + v0) = pd2{xs[0], xs[1]};
+        *(pd2*)(hx + v0) = pd2{f0[0], f0[1]};
+        *(pd2*)(hy + v0) = pd2{f1[0], f1[1]};
+        *(pd2*)(hz + v0) = pd2{f2[0], f2[1]};
+        *(pf2*)(fx + v0) = pf2{(float)f0[0], (float)f0[1]};
+        *(pf2*)(fy + v0) = pf2{(float)f1[0], (float)f1[1]};
+        *(pf2*)(fz + v0) = pf2{(float)f2[0], (float)f2[1]};
+    }
 }
 
 __global__ void k_count_floored(const int8_t* __restrict__ cls, uint32_t N, double* partials) {
@@ -421,12 +669,31 @@ int assemble_pressure(ls_ctx* c, double t) {
     const double ptop = effective_top_pressure(c->P.top_pressure, t);
     const int zf = c->P.bc_mode == LS_BC_ZFACES;
     dim3 blk(TK, TJ), grd((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, c->ni);
-    hipLaunchKernelGGL(k_press_diag, grd, blk, 0, c->stream, c->img, c->perm, c->p, c->cls,
-                       c->pD, c->sd, g, zf, ptop);
+    PairGeo pg;
+    pg.nhr = (uint32_t)(c->nk / 2);
+    pg.nrows = (uint32_t)c->ni * (uint32_t)c->nj;
+    pg.si = g.si;
+    pg.sj = g.sj;
+    pg.ni = c->ni;
+    pg.nj = c->nj;
+    const dim3 pgrid((pg.nhr + WAVE - 1) / WAVE, (pg.nrows + PP_TR * PP_RPW - 1) / (PP_TR * PP_RPW));
+    const bool pair = c->nk % 2 == 0 && pgrid.y <= 65535;
+    if (pair)
+        hipLaunchKernelGGL(k_press_diag_pair, pgrid, dim3(WAVE, PP_TR), 0, c->stream, c->img,
+                           c->perm, c->p, c->cls, c->pD, c->sd, pg, zf, ptop);
+    else
+        hipLaunchKernelGGL(k_press_diag, grd, blk, 0, c->stream, c->img, c->perm, c->p, c->cls,
+                           c->pD, c->sd, g, zf, ptop);
     LS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_press_rhs_scale, grd, blk, 0, c->stream, c->img, c->perm, c->cls, c->pD,
-                       c->p, c->sd, c->bhat, c->x64, c->c64[0], c->c64[1], c->c64[2], c->c32[0],
-                       c->c32[1], c->c32[2], g, zf, ptop, c->P.evap_flux * c->P.dx);
+    if (pair)
+        hipLaunchKernelGGL(k_press_rhs_scale_pair, pgrid, dim3(WAVE, PP_TR), 0, c->stream, c->img,
+                           c->perm, c->cls, c->pD, c->p, c->sd, c->bhat, c->x64, c->c64[0],
+                           c->c64[1], c->c64[2], c->c32[0], c->c32[1], c->c32[2], pg, zf, ptop,
+                           c->P.evap_flux * c->P.dx);
+    else
+        hipLaunchKernelGGL(k_press_rhs_scale, grd, blk, 0, c->stream, c->img, c->perm, c->cls,
+                           c->pD, c->p, c->sd, c->bhat, c->x64, c->c64[0], c->c64[1], c->c64[2],
+                           c->c32[0], c->c32[1], c->c32[2], g, zf, ptop, c->P.evap_flux * c->P.dx);
     LS_LAUNCH_CHECK();
     uint32_t N = (uint32_t)c->N;
     int nb = ew_blocks(N);

@@ -40,8 +40,12 @@ def test_properties_parity(rng):
 
 
 @pytest.mark.parametrize("bc", [L.BC_REFERENCE, L.BC_ZFACES])
-def test_pressure_operator_parity(rng, bc):
-    shape = (12, 13, 14)
+@pytest.mark.parametrize("shape", [(12, 13, 14), (12, 13, 15), (9, 7, 130)])
+def test_pressure_operator_parity(rng, bc, shape):
+    """The assembled operator against the oracle's: nk even takes the
+    voxel-pair assembly (k_press_diag_pair, k_press_rhs_scale_pair; nk = 130
+    spans three waves per row, so pairs meet at the waves' edges), nk odd
+    one voxel per thread.  Row classes bit-exact, A x to 1e-13."""
     img = small_mixed_geometry(shape, 3)
     P = L.default_params(bc_mode=bc, press_rtol=1e-12)
     g, o = both(shape, P, img)
@@ -49,6 +53,7 @@ def test_pressure_operator_parity(rng, bc):
         x.phase(L.PH_PROPERTIES)
         x.phase(L.PH_PRESSURE, 100.0)
     cls = o.pressure_classes()
+    assert np.array_equal(g.pressure_system()[2], cls)
     xv = rng.standard_normal(shape)
     xv[cls != 1] = 0.0
     yg, yo = g.apply_pressure(xv), o.apply_pressure(xv)
