@@ -171,6 +171,11 @@ int phase_leaching(ls_ctx* c, double dt, ls_step_report* rep) {
         }
         return 0;
     }
+    // the sweep writes frac: the step's snapshot of it first
+    if (c->bfrac_stale) {
+        if (int st = copy_d2d(c, c->b_frac, c->frac, c->N * 4)) return st;
+        c->bfrac_stale = false;
+    }
     LGeo g;
     g.ni = c->ni; g.nj = c->nj; g.nk = c->nk;
     g.sj = (uint32_t)c->nk;

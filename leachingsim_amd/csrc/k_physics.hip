@@ -741,6 +741,13 @@ __global__ void k_press_final(const uint8_t* __restrict__ img, const double* __r
 }
 
 int finalize_pressure(ls_ctx* c, double ptop) {
+    // after the step's snapshot p still holds the snapshot's values: the new
+    // field goes into the backup's buffer and the two swap (k_press_final
+    // writes every voxel and reads no p)
+    if (c->bp_stale) {
+        std::swap(c->p, c->b_p);
+        c->bp_stale = false;
+    }
     Geo g = geo_of(c);
     dim3 blk(TK, TJ), grd((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, c->ni);
     hipLaunchKernelGGL(k_press_final, grd, blk, 0, c->stream, c->img, c->perm, c->cls, c->pD,

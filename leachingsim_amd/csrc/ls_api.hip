@@ -36,10 +36,7 @@ int dalloc(ls_ctx* c, T** p, size_t n) {
     return 0;
 }
 
-int copy_d2d(ls_ctx* c, void* dst, const void* src, size_t bytes) {
-    LS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
-    return 0;
-}
+
 
 struct Timer {
     ls_ctx* c;
@@ -105,6 +102,21 @@ int pool_pw_count(const ls_params& P) {
 }  // namespace
 
 namespace ls {
+
+int copy_d2d(ls_ctx* c, void* dst, const void* src, size_t bytes) {
+    LS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+// The snapshot's p and frac that the step left in the fields themselves,
+// copied into the backups before anything else writes the fields.
+int materialize_backups(ls_ctx* c) {
+    int st = 0;
+    if (c->bp_stale) st |= copy_d2d(c, c->b_p, c->p, c->N * 8);
+    if (c->bfrac_stale) st |= copy_d2d(c, c->b_frac, c->frac, c->N * 4);
+    c->bp_stale = c->bfrac_stale = false;
+    return st;
+}
 
 // The phase-scoped pool: the pressure phase's and the transport phase's
 // buffers over the same bytes.  Cut again (and reallocated) when the
@@ -443,6 +455,7 @@ static void* field_ptr(ls_ctx* c, int field, size_t* eb) {
 int ls_upload(ls_ctx* c, int field, const void* host, size_t elem_bytes) {
     if (!c || !host) return -1;
     const size_t N = c->N;
+    if (int st = materialize_backups(c)) return st;
     if (field == LS_F_IMG) {
         if (elem_bytes != (size_t)c->label_bytes) {
             c->err = "label element size does not match the context";
@@ -524,6 +537,7 @@ int ls_download(ls_ctx* c, int field, void* host, size_t elem_bytes) {
 
 int ls_init_state(ls_ctx* c, long long* nsul) {
     if (!c) return -1;
+    if (int st = materialize_backups(c)) return st;
     int st = init_state(c, nsul);
     if (st) return st;
     LS_HIP(hipStreamSynchronize(c->stream));
@@ -546,6 +560,7 @@ int ls_snapshot(ls_ctx* c) {
     st |= copy_d2d(c, c->b_qz, c->qz, N * 8);
     c->b_acid_zero = c->acid_zero;
     c->b_frac_clean = c->frac_clean;
+    c->bp_stale = c->bfrac_stale = false;
     return st;
 }
 
@@ -560,10 +575,13 @@ static int step_snapshot(ls_ctx* c) {
     const size_t N = c->N;
     int st = 0;
     st |= copy_d2d(c, c->b_img, c->img, N);
-    st |= copy_d2d(c, c->b_frac, c->frac, N * 4);
+    // frac only changes in a leaching sweep, which acid known zero and a
+    // clean frac skip; p is swapped with its backup by the pressure phase
+    c->bfrac_stale = c->acid_zero && c->frac_clean;
+    if (!c->bfrac_stale) st |= copy_d2d(c, c->b_frac, c->frac, N * 4);
     st |= copy_d2d(c, c->b_inv, c->inv, N * 8);
     st |= copy_d2d(c, c->b_s, c->s, N * 8);
-    st |= copy_d2d(c, c->b_p, c->p, N * 8);
+    c->bp_stale = true;
     st |= copy_d2d(c, c->b_conc, c->conc, N * 8);
     // acid known zero is restored by zeroing it (b_acid_zero below)
     if (!c->acid_zero) st |= copy_d2d(c, c->b_acid, c->acid, N * 8);
@@ -592,10 +610,10 @@ int ls_rollback(ls_ctx* c) {
     const size_t N = c->N;
     int st = 0;
     st |= copy_d2d(c, c->img, c->b_img, N);
-    st |= copy_d2d(c, c->frac, c->b_frac, N * 4);
+    if (!c->bfrac_stale) st |= copy_d2d(c, c->frac, c->b_frac, N * 4);
     st |= copy_d2d(c, c->inv, c->b_inv, N * 8);
     st |= copy_d2d(c, c->s, c->b_s, N * 8);
-    st |= copy_d2d(c, c->p, c->b_p, N * 8);
+    if (!c->bp_stale) st |= copy_d2d(c, c->p, c->b_p, N * 8);
     st |= copy_d2d(c, c->conc, c->b_conc, N * 8);
     if (c->b_acid_zero)
         LS_HIP(hipMemsetAsync(c->acid, 0, N * 8, c->stream));

@@ -184,6 +184,13 @@ struct ls_ctx {
     // With both, the leaching sweep cannot change anything.
     bool acid_zero = false, frac_clean = false;
     bool b_acid_zero = false, b_frac_clean = false;   // at the last snapshot
+    // The step's snapshot copies neither p nor (while acid is known zero and
+    // frac clean, so that leaching cannot change it) frac: the backup is
+    // stale and the field itself still holds the snapshot's values.  The
+    // pressure phase then writes the new p into the backup's buffer and swaps
+    // the two; the leaching sweep copies frac before it writes; any other
+    // write of the state first copies the field (materialize_backups).
+    bool bp_stale = false, bfrac_stale = false;
     double p_top_prev = 0.0;
     bool have_prev = false;
     bool sat_valid = true;     // physics check of the last implicit saturation solve
@@ -224,6 +231,8 @@ int pool_setup(ls_ctx* c);
 // the doubles its stored rows take
 void transport_form(const ls_ctx* c, bool* stored, bool* schur, bool* mixed = nullptr);
 size_t transport_rows_doubles(const ls_ctx* c);
+int copy_d2d(ls_ctx* c, void* dst, const void* src, size_t bytes);
+int materialize_backups(ls_ctx* c);
 size_t transport_rows32_floats(const ls_ctx* c);
 // k_physics.hip
 int init_state(ls_ctx* c, long long* nsul);

@@ -364,6 +364,39 @@ def test_rollback_restores_state(fail):
     assert g.step(1.0, 1.0).status == 0
 
 
+def test_rollback_restores_the_step_start_after_uploads():
+    """The step's snapshot leaves p (and frac, while leaching cannot run) in
+    the fields and swaps p with its backup in the pressure phase.  After a
+    successful step restoreState returns to the step's start, twice in a
+    row; an upload between the step and restoreState does not leak into the
+    restored state."""
+    img = G.named_geometry("smooth64")[:24, :24, :32].copy()
+    img[:, 0:2] = G.AIR
+    P = L.default_params(bc_mode=L.BC_ZFACES, solver=L.SOLVER_MIXED,
+                         warm_start=L.WARM_PROJECTED)
+    g = L.GpuSimulation(img.shape, P)
+    g.read_labels(img)
+    g.init_state()
+    g.step(0.0, 1.0)
+    start = g.state()
+    assert g.step(1.0, 1.0).status == 0
+    assert not np.array_equal(g.download(L.F_PRESSURE), start[L.F_PRESSURE])
+    for _ in range(2):
+        g.restoreState()
+        now = g.state()
+        for f in start:
+            assert np.array_equal(start[f], now[f]), f
+    # a step, then uploads of p and frac, then restoreState
+    assert g.step(1.0, 1.0).status == 0
+    g.upload(L.F_PRESSURE, np.full(img.shape, 7.0))
+    g.upload(L.F_FRAC, np.full(img.shape, 0.5, np.float32))
+    g.restoreState()
+    now = g.state()
+    for f in start:
+        assert np.array_equal(start[f], now[f]), f
+    assert g.step(1.0, 1.0).status == 0
+
+
 def test_implicit_saturation_phase_parity():
     """setupSaturationEqns + solveSaturation on the GPU (the transport's
     BiCGStab on the saturation rows) against the oracle, on a random state."""
