@@ -449,6 +449,26 @@ struct PairGeo {
     int ni, nj;
 };
 
+static PairGeo pair_geo(const ls_ctx* c) {
+    PairGeo pg;
+    pg.nhr = (uint32_t)(c->nk / 2);
+    pg.nrows = (uint32_t)c->ni * (uint32_t)c->nj;
+    pg.sj = (uint32_t)c->nk;
+    pg.si = (uint32_t)c->nk * (uint32_t)c->nj;
+    pg.ni = c->ni;
+    pg.nj = c->nj;
+    return pg;
+}
+static dim3 pair_grid(const PairGeo& pg) {
+    return dim3((pg.nhr + WAVE - 1) / WAVE, (pg.nrows + PP_TR * PP_RPW - 1) / (PP_TR * PP_RPW));
+}
+// the voxel-pair kernels: nk even, the rows within the grid's y limit and
+// one block partial each within the partials array
+static bool use_pairs(const ls_ctx* c) {
+    const dim3 gr = pair_grid(pair_geo(c));
+    return c->nk % 2 == 0 && gr.y <= 65535 && (size_t)gr.x * gr.y <= (size_t)MAX_PARTIALS;
+}
+
 __device__ __forceinline__ uint32_t ld_b2(const void* __restrict__ a, uint32_t v) {
     return (uint32_t)(*(const uint16_t*)((const uint8_t*)a + v));
 }
@@ -669,15 +689,9 @@ int assemble_pressure(ls_ctx* c, double t) {
     const double ptop = effective_top_pressure(c->P.top_pressure, t);
     const int zf = c->P.bc_mode == LS_BC_ZFACES;
     dim3 blk(TK, TJ), grd((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, c->ni);
-    PairGeo pg;
-    pg.nhr = (uint32_t)(c->nk / 2);
-    pg.nrows = (uint32_t)c->ni * (uint32_t)c->nj;
-    pg.si = g.si;
-    pg.sj = g.sj;
-    pg.ni = c->ni;
-    pg.nj = c->nj;
-    const dim3 pgrid((pg.nhr + WAVE - 1) / WAVE, (pg.nrows + PP_TR * PP_RPW - 1) / (PP_TR * PP_RPW));
-    const bool pair = c->nk % 2 == 0 && pgrid.y <= 65535;
+    const PairGeo pg = pair_geo(c);
+    const dim3 pgrid = pair_grid(pg);
+    const bool pair = use_pairs(c);
     if (pair)
         hipLaunchKernelGGL(k_press_diag_pair, pgrid, dim3(WAVE, PP_TR), 0, c->stream, c->img,
                            c->perm, c->p, c->cls, c->pD, c->sd, pg, zf, ptop);
@@ -770,6 +784,31 @@ __device__ __forceinline__ double grad1(double fm, double fc, double fp, int pos
     return (fp - fm) / (2.0 * dx);
 }
 
+// One voxel's Darcy flux from p and Pc at the voxel and its neighbours (the
+// voxel's own values where a neighbour is missing) and its permeability.
+__device__ __forceinline__ void flux_core(uint8_t tv, double pv, double cv, double kv,
+                                          const double (&pn)[6], const double (&cn)[6], int i,
+                                          int j, int k, const Geo& g, double dx, double mcg,
+                                          double& qxo, double& qyo, double& qzo) {
+    double gradP_x = grad1(pn[1], pv, pn[0], i, g.ni, dx);
+    double gradP_y = grad1(pn[3], pv, pn[2], j, g.nj, dx);
+    double gradP_z = grad1(pn[5], pv, pn[4], k, g.nk, dx);
+    double gradPc_x = grad1(cn[1], cv, cn[0], i, g.ni, dx);
+    double gradPc_y = grad1(cn[3], cv, cn[2], j, g.nj, dx);
+    double gradPc_z = grad1(cn[5], cv, cn[4], k, g.nk, dx);
+    gradPc_x = dmax(-mcg, dmin(mcg, gradPc_x));
+    gradPc_y = dmax(-mcg, dmin(mcg, gradPc_y));
+    gradPc_z = dmax(-mcg, dmin(mcg, gradPc_z));
+    double transmissivity = kv / viscosity(tv);
+    const double rho_g = 1000.0 * 9.81;
+    double liq_grad_x = gradP_x - gradPc_x;
+    double liq_grad_y = gradP_y - gradPc_y;
+    double liq_grad_z = gradP_z - gradPc_z;
+    qxo = -transmissivity * (liq_grad_x - rho_g);
+    qyo = -transmissivity * liq_grad_y;
+    qzo = -transmissivity * liq_grad_z;
+}
+
 __global__ void k_flux(const uint8_t* __restrict__ img, const double* __restrict__ p,
                        const double* __restrict__ pcap, const double* __restrict__ perm,
                        double* __restrict__ qx, double* __restrict__ qy, double* __restrict__ qz,
@@ -789,32 +828,79 @@ __global__ void k_flux(const uint8_t* __restrict__ img, const double* __restrict
         pn[f] = p[n[f]];
         cn[f] = pcap[n[f]];
     }
-    const double pv = p[v], cv = pcap[v], kv = perm[v];
-    const uint8_t tv = img[v];
-    double gradP_x = grad1(pn[1], pv, pn[0], i, g.ni, dx);
-    double gradP_y = grad1(pn[3], pv, pn[2], j, g.nj, dx);
-    double gradP_z = grad1(pn[5], pv, pn[4], k, g.nk, dx);
-    double gradPc_x = grad1(cn[1], cv, cn[0], i, g.ni, dx);
-    double gradPc_y = grad1(cn[3], cv, cn[2], j, g.nj, dx);
-    double gradPc_z = grad1(cn[5], cv, cn[4], k, g.nk, dx);
-    gradPc_x = dmax(-mcg, dmin(mcg, gradPc_x));
-    gradPc_y = dmax(-mcg, dmin(mcg, gradPc_y));
-    gradPc_z = dmax(-mcg, dmin(mcg, gradPc_z));
-    double transmissivity = kv / viscosity(tv);
-    const double rho_g = 1000.0 * 9.81;
-    double liq_grad_x = gradP_x - gradPc_x;
-    double liq_grad_y = gradP_y - gradPc_y;
-    double liq_grad_z = gradP_z - gradPc_z;
-    qx[v] = -transmissivity * (liq_grad_x - rho_g);
-    qy[v] = -transmissivity * liq_grad_y;
-    qz[v] = -transmissivity * liq_grad_z;
+    double a, b, c;
+    flux_core(img[v], p[v], pcap[v], perm[v], pn, cn, i, j, k, g, dx, mcg, a, b, c);
+    qx[v] = a;
+    qy[v] = b;
+    qz[v] = c;
+}
+
+// The same with a voxel pair per lane (nk even; rows (i, j) per wave as
+// k_press_diag_pair): p, Pc and K as 16-byte pairs, the k neighbours outside
+// the pair from the neighbouring lanes, the fluxes written as pairs.
+__global__ __launch_bounds__(WAVE * PP_TR) void k_flux_pair(
+    const uint8_t* __restrict__ img, const double* __restrict__ p,
+    const double* __restrict__ pcap, const double* __restrict__ perm, double* __restrict__ qx,
+    double* __restrict__ qy, double* __restrict__ qz, PairGeo pg, Geo g, double dx, double mcg) {
+    const uint32_t m = blockIdx.x * WAVE + threadIdx.x;
+    const bool ok_ = m < pg.nhr;
+    const pd2 z2 = pd2{0.0, 0.0};
+#pragma unroll 1
+    for (int q8 = 0; q8 < PP_RPW; ++q8) {
+        const uint32_t row = (blockIdx.y * PP_RPW + q8) * PP_TR + threadIdx.y;
+        if (row >= pg.nrows) continue;                  // uniform per wave
+        const int i = (int)(row / (uint32_t)pg.nj), j = (int)(row - (uint32_t)i * pg.nj);
+        const uint32_t v0 = row * (uint32_t)g.nk + 2 * m;
+        const bool hip_ = i + 1 < g.ni, him = i > 0, hjp = j + 1 < g.nj, hjm = j > 0;
+        const pd2 pc = ok_ ? ld_d2(p, v0) : z2, cc = ok_ ? ld_d2(pcap, v0) : z2;
+        const pd2 kc = ok_ ? ld_d2(perm, v0) : z2;
+        const uint32_t lc = ok_ ? ld_b2(img, v0) : 0u;
+        // a missing neighbour takes the voxel's own values (as k_flux)
+        const pd2 pip = (ok_ && hip_) ? ld_d2(p, v0 + g.si) : pc;
+        const pd2 pim = (ok_ && him) ? ld_d2(p, v0 - g.si) : pc;
+        const pd2 pjp = (ok_ && hjp) ? ld_d2(p, v0 + g.sj) : pc;
+        const pd2 pjm = (ok_ && hjm) ? ld_d2(p, v0 - g.sj) : pc;
+        const pd2 cip = (ok_ && hip_) ? ld_d2(pcap, v0 + g.si) : cc;
+        const pd2 cim = (ok_ && him) ? ld_d2(pcap, v0 - g.si) : cc;
+        const pd2 cjp = (ok_ && hjp) ? ld_d2(pcap, v0 + g.sj) : cc;
+        const pd2 cjm = (ok_ && hjm) ? ld_d2(pcap, v0 - g.sj) : cc;
+        double p_l, p_r, c_l, c_r;
+        k_out<double>(pc.x, pc.y, p, v0, m, pg.nhr, ok_, p_l, p_r);
+        k_out<double>(cc.x, cc.y, pcap, v0, m, pg.nhr, ok_, c_l, c_r);
+        if (!ok_) continue;
+        const int k0 = 2 * (int)m;
+        double qo[3][2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int k = k0 + e;
+            const double pv = e ? pc.y : pc.x, cv = e ? cc.y : cc.x;
+            const double pkp = e ? (k + 1 < g.nk ? p_r : pv) : pc.y;
+            const double pkm = e ? pc.x : (k > 0 ? p_l : pv);
+            const double ckp = e ? (k + 1 < g.nk ? c_r : cv) : cc.y;
+            const double ckm = e ? cc.x : (k > 0 ? c_l : cv);
+            const double pn[6] = {e ? pip.y : pip.x, e ? pim.y : pim.x, e ? pjp.y : pjp.x,
+                                  e ? pjm.y : pjm.x, pkp, pkm};
+            const double cn[6] = {e ? cip.y : cip.x, e ? cim.y : cim.x, e ? cjp.y : cjp.x,
+                                  e ? cjm.y : cjm.x, ckp, ckm};
+            flux_core((uint8_t)(e ? (lc >> 8) : (lc & 0xffu)), pv, cv, e ? kc.y : kc.x, pn, cn, i,
+                      j, k, g, dx, mcg, qo[0][e], qo[1][e], qo[2][e]);
+        }
+        *(pd2*)(qx + v0) = pd2{qo[0][0], qo[0][1]};
+        *(pd2*)(qy + v0) = pd2{qo[1][0], qo[1][1]};
+        *(pd2*)(qz + v0) = pd2{qo[2][0], qo[2][1]};
+    }
 }
 
 int phase_flux(ls_ctx* c) {
     Geo g = geo_of(c);
     dim3 blk(TK, TJ), grd((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, c->ni);
-    hipLaunchKernelGGL(k_flux, grd, blk, 0, c->stream, c->img, c->p, c->pc, c->perm, c->qx, c->qy,
-                       c->qz, g, c->P.dx, c->P.max_cap_grad);
+    if (use_pairs(c))
+        hipLaunchKernelGGL(k_flux_pair, pair_grid(pair_geo(c)), dim3(WAVE, PP_TR), 0, c->stream,
+                           c->img, c->p, c->pc, c->perm, c->qx, c->qy, c->qz, pair_geo(c), g,
+                           c->P.dx, c->P.max_cap_grad);
+    else
+        hipLaunchKernelGGL(k_flux, grd, blk, 0, c->stream, c->img, c->p, c->pc, c->perm, c->qx,
+                           c->qy, c->qz, g, c->P.dx, c->P.max_cap_grad);
     LS_LAUNCH_CHECK();
     return 0;
 }
@@ -828,27 +914,19 @@ __device__ __forceinline__ bool sat_inactive(uint8_t t) {
     return t == AIR || t == ROCK || t == SUL || t == PREC;
 }
 
-__device__ __forceinline__ void sat_rates(const uint8_t* img, const double* qx, const double* qy,
-                                          const double* qz, const Geo& g, int i, int j, int k,
-                                          uint32_t v, double dx, double evap, double& div_q,
-                                          double& evap_sink) {
-    uint32_t n[6];
-    bool ok[6];
-    nbr6(g, i, j, k, v, n, ok);
-    // every load first (neighbour indices outside the grid are v itself)
-    const double qxv = qx[v], qyv = qy[v], qzv = qz[v];
-    const double qn0 = qx[n[0]], qn1 = qx[n[1]], qn2 = qy[n[2]], qn3 = qy[n[3]];
-    const double qn4 = qz[n[4]], qn5 = qz[n[5]];
-    const uint8_t tv = img[v];
-    uint8_t tn[6];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) tn[f] = img[n[f]];
-    double q_face_xp = ok[0] ? (qxv + qn0) / 2.0 : 0.0;
-    double q_face_xm = ok[1] ? (qxv + qn1) / 2.0 : 0.0;
-    double q_face_yp = ok[2] ? (qyv + qn2) / 2.0 : 0.0;
-    double q_face_ym = ok[3] ? (qyv + qn3) / 2.0 : 0.0;
-    double q_face_zp = ok[4] ? (qzv + qn4) / 2.0 : 0.0;
-    double q_face_zm = ok[5] ? (qzv + qn5) / 2.0 : 0.0;
+// One voxel's div q (cell fluxes averaged to its faces) and evaporation
+// sink, from its own fluxes, the neighbours' fluxes along each axis (qn: +x,
+// -x, +y, -y, +z, -z) and labels; ok[f]: the neighbour exists.
+__device__ __forceinline__ void sat_rates_core(uint8_t tv, const uint8_t (&tn)[6],
+                                               const bool (&ok)[6], double qxv, double qyv,
+                                               double qzv, const double (&qn)[6], double dx,
+                                               double evap, double& div_q, double& evap_sink) {
+    double q_face_xp = ok[0] ? (qxv + qn[0]) / 2.0 : 0.0;
+    double q_face_xm = ok[1] ? (qxv + qn[1]) / 2.0 : 0.0;
+    double q_face_yp = ok[2] ? (qyv + qn[2]) / 2.0 : 0.0;
+    double q_face_ym = ok[3] ? (qyv + qn[3]) / 2.0 : 0.0;
+    double q_face_zp = ok[4] ? (qzv + qn[4]) / 2.0 : 0.0;
+    double q_face_zm = ok[5] ? (qzv + qn[5]) / 2.0 : 0.0;
     div_q = ((q_face_xp - q_face_xm) + (q_face_yp - q_face_ym) + (q_face_zp - q_face_zm)) / dx;
     evap_sink = 0.0;
     if (tv == PORE) {
@@ -858,6 +936,77 @@ __device__ __forceinline__ void sat_rates(const uint8_t* img, const double* qx, 
             if (ok[f] && tn[f] == AIR) air_faces++;
         evap_sink = (air_faces * evap) / dx;
     }
+}
+
+__device__ __forceinline__ void sat_rates(const uint8_t* img, const double* qx, const double* qy,
+                                          const double* qz, const Geo& g, int i, int j, int k,
+                                          uint32_t v, double dx, double evap, double& div_q,
+                                          double& evap_sink) {
+    uint32_t n[6];
+    bool ok[6];
+    nbr6(g, i, j, k, v, n, ok);
+    // every load first (neighbour indices outside the grid are v itself)
+    const double qxv = qx[v], qyv = qy[v], qzv = qz[v];
+    const double qn[6] = {qx[n[0]], qx[n[1]], qy[n[2]], qy[n[3]], qz[n[4]], qz[n[5]]};
+    const uint8_t tv = img[v];
+    uint8_t tn[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) tn[f] = img[n[f]];
+    sat_rates_core(tv, tn, ok, qxv, qyv, qzv, qn, dx, evap, div_q, evap_sink);
+}
+
+// A voxel pair's rates (voxel-pair layout of k_press_diag_pair): fluxes as
+// 16-byte pairs, labels as 2-byte pairs, the k neighbours outside the pair
+// from the neighbouring lanes.  Every lane of the wave must call it.
+struct SatPair {
+    double div_q[2], evap_sink[2];
+    uint32_t lc;           // the pair's labels
+};
+__device__ __forceinline__ SatPair sat_rates_pair(const uint8_t* __restrict__ img,
+                                                  const double* __restrict__ qx,
+                                                  const double* __restrict__ qy,
+                                                  const double* __restrict__ qz,
+                                                  const PairGeo& pg, int nk, uint32_t m,
+                                                  bool ok_, int i, int j, uint32_t v0, double dx,
+                                                  double evap) {
+    const pd2 z2 = pd2{0.0, 0.0};
+    const bool hip_ = i + 1 < pg.ni, him = i > 0, hjp = j + 1 < pg.nj, hjm = j > 0;
+    const pd2 qxc = ok_ ? ld_d2(qx, v0) : z2, qyc = ok_ ? ld_d2(qy, v0) : z2;
+    const pd2 qzc = ok_ ? ld_d2(qz, v0) : z2;
+    const pd2 qxp = (ok_ && hip_) ? ld_d2(qx, v0 + pg.si) : z2;
+    const pd2 qxm = (ok_ && him) ? ld_d2(qx, v0 - pg.si) : z2;
+    const pd2 qyp = (ok_ && hjp) ? ld_d2(qy, v0 + pg.sj) : z2;
+    const pd2 qym = (ok_ && hjm) ? ld_d2(qy, v0 - pg.sj) : z2;
+    SatPair o;
+    o.lc = ok_ ? ld_b2(img, v0) : 0u;
+    const uint32_t lip = (ok_ && hip_) ? ld_b2(img, v0 + pg.si) : 0u;
+    const uint32_t lim = (ok_ && him) ? ld_b2(img, v0 - pg.si) : 0u;
+    const uint32_t ljp = (ok_ && hjp) ? ld_b2(img, v0 + pg.sj) : 0u;
+    const uint32_t ljm = (ok_ && hjm) ? ld_b2(img, v0 - pg.sj) : 0u;
+    double qz_l, qz_r;
+    k_out<double>(qzc.x, qzc.y, qz, v0, m, pg.nhr, ok_, qz_l, qz_r);
+    int t_l, t_r;
+    k_out<int>((int)(o.lc & 0xffu), (int)(o.lc >> 8), (const int*)nullptr, 0u, m, pg.nhr, false,
+               t_l, t_r);
+    if (threadIdx.x == 0) t_l = (ok_ && m >= 1) ? (int)img[v0 - 1] : 0;
+    if (threadIdx.x == WAVE - 1) t_r = (ok_ && m + 1 < pg.nhr) ? (int)img[v0 + 2] : 0;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int k = 2 * (int)m + e;
+        const bool okf[6] = {hip_, him, hjp, hjm, k + 1 < nk, k > 0};
+        const uint8_t tn[6] = {(uint8_t)(e ? (lip >> 8) : (lip & 0xffu)),
+                               (uint8_t)(e ? (lim >> 8) : (lim & 0xffu)),
+                               (uint8_t)(e ? (ljp >> 8) : (ljp & 0xffu)),
+                               (uint8_t)(e ? (ljm >> 8) : (ljm & 0xffu)),
+                               (uint8_t)(e ? (uint32_t)t_r : (o.lc >> 8)),
+                               (uint8_t)(e ? (o.lc & 0xffu) : (uint32_t)t_l)};
+        const double qn[6] = {e ? qxp.y : qxp.x, e ? qxm.y : qxm.x, e ? qyp.y : qyp.x,
+                              e ? qym.y : qym.x, e ? qz_r : qzc.y, e ? qzc.x : qz_l};
+        sat_rates_core((uint8_t)(e ? (o.lc >> 8) : (o.lc & 0xffu)), tn, okf, e ? qxc.y : qxc.x,
+                       e ? qyc.y : qyc.x, e ? qzc.y : qzc.x, qn, dx, evap, o.div_q[e],
+                       o.evap_sink[e]);
+    }
+    return o;
 }
 
 __global__ void k_sat_max(const uint8_t* __restrict__ img, const double* __restrict__ qx,
@@ -925,7 +1074,89 @@ __global__ void k_sat_update(const uint8_t* __restrict__ img, const double* __re
     s[v] = sv;
 }
 
+__global__ __launch_bounds__(WAVE * PP_TR) void k_sat_max_pair(
+    const uint8_t* __restrict__ img, const double* __restrict__ qx,
+    const double* __restrict__ qy, const double* __restrict__ qz, PairGeo pg, int nk,
+    double dx, double evap, double* partials) {
+    __shared__ double lds[PP_TR];
+    const uint32_t m = blockIdx.x * WAVE + threadIdx.x;
+    const bool ok_ = m < pg.nhr;
+    double mx = 0.0;
+#pragma unroll 1
+    for (int q8 = 0; q8 < PP_RPW; ++q8) {
+        const uint32_t row = (blockIdx.y * PP_RPW + q8) * PP_TR + threadIdx.y;
+        if (row >= pg.nrows) continue;                  // uniform per wave
+        const int i = (int)(row / (uint32_t)pg.nj), j = (int)(row - (uint32_t)i * pg.nj);
+        const uint32_t v0 = row * (uint32_t)nk + 2 * m;
+        const SatPair r = sat_rates_pair(img, qx, qy, qz, pg, nk, m, ok_, i, j, v0, dx, evap);
+        if (!ok_) continue;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            if (sat_inactive((uint8_t)(e ? (r.lc >> 8) : (r.lc & 0xffu)))) continue;
+            const double total_rate = fabs(r.div_q[e]) + r.evap_sink[e];
+            if (total_rate > mx) mx = total_rate;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, WAVE));
+    if (threadIdx.x == 0) lds[threadIdx.y] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double a = lds[0];
+        for (int w = 1; w < PP_TR; ++w) a = fmax(a, lds[w]);
+        partials[blockIdx.x + gridDim.x * blockIdx.y] = a;
+    }
+}
+
+__global__ __launch_bounds__(WAVE * PP_TR) void k_sat_update_pair(
+    const uint8_t* __restrict__ img, const double* __restrict__ qx,
+    const double* __restrict__ qy, const double* __restrict__ qz, double* __restrict__ s,
+    PairGeo pg, int nk, double dx, double evap, const StepScalars* __restrict__ ss) {
+    const uint32_t m = blockIdx.x * WAVE + threadIdx.x;
+    const bool ok_ = m < pg.nhr;
+    const int nsub = ss->sat_nsub;
+    const double dt_sub_over_phi = ss->sat_dt_sub_over_phi;
+#pragma unroll 1
+    for (int q8 = 0; q8 < PP_RPW; ++q8) {
+        const uint32_t row = (blockIdx.y * PP_RPW + q8) * PP_TR + threadIdx.y;
+        if (row >= pg.nrows) continue;                  // uniform per wave
+        const int i = (int)(row / (uint32_t)pg.nj), j = (int)(row - (uint32_t)i * pg.nj);
+        const uint32_t v0 = row * (uint32_t)nk + 2 * m;
+        const SatPair r = sat_rates_pair(img, qx, qy, qz, pg, nk, m, ok_, i, j, v0, dx, evap);
+        if (!ok_) continue;
+        const bool act0 = !sat_inactive((uint8_t)(r.lc & 0xffu));
+        const bool act1 = !sat_inactive((uint8_t)(r.lc >> 8));
+        if (!act0 && !act1) continue;                   // inactive voxels keep their value
+        pd2 sv = ld_d2(s, v0);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            if (!(e ? act1 : act0)) continue;
+            double x = e ? sv.y : sv.x;
+            for (int step = 0; step < nsub; ++step) {
+                double new_sat = x - dt_sub_over_phi * (r.div_q[e] + r.evap_sink[e]);
+                x = dmax(0.0, dmin(1.0, new_sat));
+            }
+            if (e) sv.y = x; else sv.x = x;
+        }
+        *(pd2*)(s + v0) = sv;
+    }
+}
+
 int phase_saturation(ls_ctx* c, double dt, ls_step_report* rep) {
+    if (use_pairs(c)) {
+        const PairGeo pg = pair_geo(c);
+        const dim3 pgrid = pair_grid(pg);
+        const int nbp = (int)(pgrid.x * pgrid.y);
+        hipLaunchKernelGGL(k_sat_max_pair, pgrid, dim3(WAVE, PP_TR), 0, c->stream, c->img, c->qx,
+                           c->qy, c->qz, pg, c->nk, c->P.dx, c->P.evap_flux, c->partials);
+        hipLaunchKernelGGL(k_sat_finalize, dim3(1), dim3(1024), 0, c->stream, c->partials, nbp,
+                           dt, c->P.porosity, c->ss);
+        hipLaunchKernelGGL(k_sat_update_pair, pgrid, dim3(WAVE, PP_TR), 0, c->stream, c->img,
+                           c->qx, c->qy, c->qz, c->s, pg, c->nk, c->P.dx, c->P.evap_flux, c->ss);
+        LS_LAUNCH_CHECK();
+        (void)rep;
+        return 0;
+    }
     Geo g = geo_of(c);
     uint32_t N = (uint32_t)c->N;
     int nb = ew_blocks(N);

@@ -77,8 +77,10 @@ def test_pressure_solution_parity(bc, solver):
     assert relerr(pg, po) <= 1e-9
 
 
-def test_flux_saturation_parity(rng):
-    shape = (10, 12, 9)
+@pytest.mark.parametrize("shape", [(10, 12, 9), (10, 12, 10), (7, 5, 130)])
+def test_flux_saturation_parity(rng, shape):
+    """nk odd: one voxel per thread; nk even: the voxel-pair flux and
+    saturation kernels (nk = 130: pairs at the waves' edges)."""
     img = small_mixed_geometry(shape, 9)
     g, o = both(shape, L.default_params(evap_flux=1e-7), img)
     p = rng.uniform(0.0, 1e4, size=shape)
