@@ -1835,6 +1835,22 @@ __global__ __launch_bounds__(EW_BLOCK) void k_bicg_x_v(
 template <int VW> struct FVec;
 template <> struct FVec<2> { typedef float type __attribute__((ext_vector_type(2))); };
 template <> struct FVec<4> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct FVec<8> { typedef float type __attribute__((ext_vector_type(8))); };
+
+// Blocks of the FP32 passes: TMX_TR waves, each over TMX_RPW rows; TMX_VW
+// values per lane (nk % (2 TMX_VW) == 0; 4 otherwise).  At 512^3 the red
+// pass took 404-405 us with 8 x 2 or 8 x 4 or 4 x 2, 410 us with 4 x 4,
+// 436 us with 2 x 8, and 476 us with 8 values per lane.
+#ifndef LS_TMIX_TR
+#define LS_TMIX_TR 8
+#endif
+#ifndef LS_TMIX_RPW
+#define LS_TMIX_RPW 2
+#endif
+#ifndef LS_TMIX_VW
+#define LS_TMIX_VW 4
+#endif
+constexpr int TMX_TR = LS_TMIX_TR, TMX_RPW = LS_TMIX_RPW, TMX_VW = LS_TMIX_VW;
 
 // The other colour's values at the k neighbours of the lane's VW values
 // (km at -k, kp at +k); q is the own k parity (as tsv_kn).
@@ -1887,7 +1903,7 @@ __device__ __forceinline__ V tmx_gather(const float* __restrict__ src, const V (
 
 // Red pass: w = R' p on the FP32 rows.
 template <int VW, int SYS>
-__global__ __launch_bounds__(WAVE * TSV_TR) void k_tmix_red(
+__global__ __launch_bounds__(WAVE * TMX_TR) void k_tmix_red(
     const float* __restrict__ rows, TVGeo g, uint32_t Nh, const float* __restrict__ p0,
     const float* __restrict__ p1, float* __restrict__ w0, float* __restrict__ w1,
     const BicgScalars* __restrict__ bs) {
@@ -1898,8 +1914,8 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tmix_red(
     const V zero = (V)(0.0f);
     const size_t rs = rstride32(Nh);
 #pragma unroll
-    for (int q8 = 0; q8 < TSV_RPW; ++q8) {
-        const uint32_t row = (blockIdx.y * TSV_RPW + q8) * TSV_TR + threadIdx.y;
+    for (int q8 = 0; q8 < TMX_RPW; ++q8) {
+        const uint32_t row = (blockIdx.y * TMX_RPW + q8) * TMX_TR + threadIdx.y;
         if (row >= g.nrows) continue;                   // uniform per wave
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const int q = 1 - ((i + j) & 1);
@@ -1927,7 +1943,7 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tmix_red(
 // Black pass: z = p - A_br w on the FP32 rows, with partials (FP64) of
 // rh.z (MODE 0) or (z.z, z.s) (MODE 1), the vector of the dot in d0 / d1.
 template <int MODE, int VW, int SYS>
-__global__ __launch_bounds__(WAVE * TSV_TR) void k_tmix_black(
+__global__ __launch_bounds__(WAVE * TMX_TR) void k_tmix_black(
     const float* __restrict__ rows, TVGeo g, uint32_t Nh, const float* __restrict__ x0,
     const float* __restrict__ x1, const float* __restrict__ wr0, const float* __restrict__ wr1,
     float* __restrict__ y0, float* __restrict__ y1, const float* __restrict__ d0,
@@ -1936,7 +1952,7 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tmix_black(
     typedef typename FVec<VW>::type V;
     constexpr bool S0 = (SYS & 1) != 0, S1 = (SYS & 2) != 0;
     if (bs->all_done) return;
-    __shared__ double lds[4 * TSV_TR];
+    __shared__ double lds[4 * TMX_TR];
     const bool a0 = !bs->done[0], a1 = !bs->done[1];
     const uint32_t m0 = VW * (blockIdx.x * WAVE + threadIdx.x);
     const bool ok = m0 < g.nhr;
@@ -1944,8 +1960,8 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tmix_black(
     const size_t rs = rstride32(Nh);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
-    for (int q8 = 0; q8 < TSV_RPW; ++q8) {
-        const uint32_t row = (blockIdx.y * TSV_RPW + q8) * TSV_TR + threadIdx.y;
+    for (int q8 = 0; q8 < TMX_RPW; ++q8) {
+        const uint32_t row = (blockIdx.y * TMX_RPW + q8) * TMX_TR + threadIdx.y;
         if (row >= g.nrows) continue;                   // uniform per wave
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const int q = (i + j) & 1;
@@ -2004,9 +2020,9 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tmix_black(
     s3 = wave_sum(s3);
     if (threadIdx.x == 0) {
         lds[threadIdx.y] = s0;
-        lds[TSV_TR + threadIdx.y] = s1;
-        lds[2 * TSV_TR + threadIdx.y] = s2;
-        lds[3 * TSV_TR + threadIdx.y] = s3;
+        lds[TMX_TR + threadIdx.y] = s1;
+        lds[2 * TMX_TR + threadIdx.y] = s2;
+        lds[3 * TMX_TR + threadIdx.y] = s3;
     }
     __syncthreads();
     if (threadIdx.x == 0 && threadIdx.y == 0) {
@@ -2014,7 +2030,7 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tmix_black(
         const int b = blockIdx.x + gridDim.x * blockIdx.y;
         for (int a = 0; a < 4; ++a) {
             double t = 0.0;
-            for (int w = 0; w < TSV_TR; ++w) t += lds[a * TSV_TR + w];
+            for (int w = 0; w < TMX_TR; ++w) t += lds[a * TMX_TR + w];
             partials[a * nb + b] = t;
         }
     }
@@ -2465,10 +2481,13 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     // mixed precision (the pool was cut for it): FP32 cycles, 4 values per
     // lane in the passes, for conc alone (acid known zero: its vectors hold
     // the FP32 rows)
-    bool mixed = tvec && one && c->t_mixed && c->trows32 != nullptr;
-    double* nulls[2] = {nullptr, nullptr};
-    const dim3 tmgrid((tvg.nhr / 4 + WAVE - 1) / WAVE, tvgrid.y);
+    const bool tm_wide = TMX_VW != 4 && c->nk % (2 * TMX_VW) == 0;
+    const dim3 tmgrid((tvg.nhr / (tm_wide ? TMX_VW : 4) + WAVE - 1) / WAVE,
+                      (tvg.nrows + TMX_TR * TMX_RPW - 1) / (TMX_TR * TMX_RPW));
     const int nb_mix = (int)(tmgrid.x * tmgrid.y);
+    bool mixed = tvec && one && c->t_mixed && c->trows32 != nullptr && tmgrid.y <= 65535 &&
+                 4 * (size_t)nb_mix <= 8 * (size_t)MAX_PARTIALS;
+    double* nulls[2] = {nullptr, nullptr};
     // systems still iterating, as the host last saw them (a system that
     // converges inside a batch is masked on the device)
     int sys = 3;
@@ -2561,8 +2580,13 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
                 (void)hipEventRecord(c->ev[16], c->stream);
                 sample_red = false;
             }
-            hipLaunchKernelGGL((k_tmix_red<4, S>), tmgrid, dim3(WAVE, TSV_TR), 0, c->stream,
-                               c->trows32, tvg, Nh, pa, pb, c->tf[0][7], nullptr, c->bicg32);
+            if (tm_wide)
+                hipLaunchKernelGGL((k_tmix_red<TMX_VW, S>), tmgrid, dim3(WAVE, TMX_TR), 0,
+                                   c->stream, c->trows32, tvg, Nh, pa, pb, c->tf[0][7], nullptr,
+                                   c->bicg32);
+            else
+                hipLaunchKernelGGL((k_tmix_red<4, S>), tmgrid, dim3(WAVE, TMX_TR), 0, c->stream,
+                                   c->trows32, tvg, Nh, pa, pb, c->tf[0][7], nullptr, c->bicg32);
             if (sample) {
                 (void)hipEventRecord(c->ev[17], c->stream);
                 red_pending = true;
@@ -2570,9 +2594,14 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
                 // at the red voxels per system iterated
                 c->prof_tred_bytes = (double)c->N * (12.0 + (S == 3 ? 8.0 : 4.0));
             }
-            hipLaunchKernelGGL((k_tmix_black<M, 4, S>), tmgrid, dim3(WAVE, TSV_TR), 0, c->stream,
-                               c->trows32, tvg, Nh, pa, pb, c->tf[0][7], nullptr, za, zb, da,
-                               db, c->bicg32, c->partials);
+            if (tm_wide)
+                hipLaunchKernelGGL((k_tmix_black<M, TMX_VW, S>), tmgrid, dim3(WAVE, TMX_TR), 0,
+                                   c->stream, c->trows32, tvg, Nh, pa, pb, c->tf[0][7], nullptr,
+                                   za, zb, da, db, c->bicg32, c->partials);
+            else
+                hipLaunchKernelGGL((k_tmix_black<M, 4, S>), tmgrid, dim3(WAVE, TMX_TR), 0,
+                                   c->stream, c->trows32, tvg, Nh, pa, pb, c->tf[0][7], nullptr,
+                                   za, zb, da, db, c->bicg32, c->partials);
         };
         go(std::integral_constant<int, 1>{});     // conc alone
         return nb_mix;
@@ -2871,15 +2900,20 @@ int time_transport_pass(ls_ctx* c, int which, int reps, double* ms, double* byte
                  "(adr_mixed, nk % 8 == 0, acid zero)";
         return -3;
     }
-    const dim3 tmgrid((tvg.nhr / 4 + WAVE - 1) / WAVE, tvgrid.y);
+    const dim3 tmgrid((tvg.nhr / 4 + WAVE - 1) / WAVE,
+                      (tvg.nrows + TMX_TR * TMX_RPW - 1) / (TMX_TR * TMX_RPW));
+    if (f32 && tmgrid.y > 65535) {
+        c->err = "the FP32 transport passes' grid exceeds the row limit";
+        return -3;
+    }
     if (f32) hipLaunchKernelGGL(k_bicg_rearm, dim3(1), dim3(1), 0, c->stream, c->bicg32);
     auto launch = [&]() {
         if (which == LS_TIME_TRED32) {
-            hipLaunchKernelGGL((k_tmix_red<4, 1>), tmgrid, dim3(WAVE, TSV_TR), 0, c->stream,
+            hipLaunchKernelGGL((k_tmix_red<4, 1>), tmgrid, dim3(WAVE, TMX_TR), 0, c->stream,
                                c->trows32, tvg, Nh, c->tf[0][2], nullptr, c->tf[0][7], nullptr,
                                c->bicg32);
         } else if (which == LS_TIME_TBLACK32) {
-            hipLaunchKernelGGL((k_tmix_black<0, 4, 1>), tmgrid, dim3(WAVE, TSV_TR), 0, c->stream,
+            hipLaunchKernelGGL((k_tmix_black<0, 4, 1>), tmgrid, dim3(WAVE, TMX_TR), 0, c->stream,
                                c->trows32, tvg, Nh, c->tf[0][2], nullptr, c->tf[0][7], nullptr,
                                c->tf[0][3], nullptr, c->tf[0][1], nullptr, c->bicg32,
                                c->partials);
