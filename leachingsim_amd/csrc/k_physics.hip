@@ -460,13 +460,13 @@ static PairGeo pair_geo(const ls_ctx* c) {
     return pg;
 }
 static dim3 pair_grid(const PairGeo& pg) {
-    return dim3((pg.nhr + WAVE - 1) / WAVE, (pg.nrows + PP_TR * PP_RPW - 1) / (PP_TR * PP_RPW));
+    return row_grid((pg.nhr + WAVE - 1) / WAVE, (pg.nrows + PP_TR * PP_RPW - 1) / (PP_TR * PP_RPW));
 }
-// the voxel-pair kernels: nk even, the rows within the grid's y limit and
-// one block partial each within the partials array
+// the voxel-pair kernels: nk even and one block partial each within the
+// partials array
 static bool use_pairs(const ls_ctx* c) {
     const dim3 gr = pair_grid(pair_geo(c));
-    return c->nk % 2 == 0 && gr.y <= 65535 && (size_t)gr.x * gr.y <= (size_t)MAX_PARTIALS;
+    return c->nk % 2 == 0 && (size_t)gr.x * gr.y * gr.z <= (size_t)MAX_PARTIALS;
 }
 
 __device__ __forceinline__ uint32_t ld_b2(const void* __restrict__ a, uint32_t v) {
@@ -497,7 +497,7 @@ __global__ __launch_bounds__(WAVE * PP_TR) void k_press_diag_pair(
     const pd2 z2 = pd2{0.0, 0.0};
 #pragma unroll 1
     for (int q8 = 0; q8 < PP_RPW; ++q8) {
-        const uint32_t row = (blockIdx.y * PP_RPW + q8) * PP_TR + threadIdx.y;
+        const uint32_t row = (row_block() * PP_RPW + q8) * PP_TR + threadIdx.y;
         if (row >= g.nrows) continue;                   // uniform per wave
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const uint32_t v0 = row * (uint32_t)nk + 2 * m;
@@ -558,7 +558,7 @@ __global__ __launch_bounds__(WAVE * PP_TR) void k_press_rhs_scale_pair(
     const pd2 z2 = pd2{0.0, 0.0};
 #pragma unroll 1
     for (int q8 = 0; q8 < PP_RPW; ++q8) {
-        const uint32_t row = (blockIdx.y * PP_RPW + q8) * PP_TR + threadIdx.y;
+        const uint32_t row = (row_block() * PP_RPW + q8) * PP_TR + threadIdx.y;
         if (row >= g.nrows) continue;                   // uniform per wave
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const uint32_t v0 = row * (uint32_t)nk + 2 * m;
@@ -847,7 +847,7 @@ __global__ __launch_bounds__(WAVE * PP_TR) void k_flux_pair(
     const pd2 z2 = pd2{0.0, 0.0};
 #pragma unroll 1
     for (int q8 = 0; q8 < PP_RPW; ++q8) {
-        const uint32_t row = (blockIdx.y * PP_RPW + q8) * PP_TR + threadIdx.y;
+        const uint32_t row = (row_block() * PP_RPW + q8) * PP_TR + threadIdx.y;
         if (row >= pg.nrows) continue;                  // uniform per wave
         const int i = (int)(row / (uint32_t)pg.nj), j = (int)(row - (uint32_t)i * pg.nj);
         const uint32_t v0 = row * (uint32_t)g.nk + 2 * m;
@@ -1084,7 +1084,7 @@ __global__ __launch_bounds__(WAVE * PP_TR) void k_sat_max_pair(
     double mx = 0.0;
 #pragma unroll 1
     for (int q8 = 0; q8 < PP_RPW; ++q8) {
-        const uint32_t row = (blockIdx.y * PP_RPW + q8) * PP_TR + threadIdx.y;
+        const uint32_t row = (row_block() * PP_RPW + q8) * PP_TR + threadIdx.y;
         if (row >= pg.nrows) continue;                  // uniform per wave
         const int i = (int)(row / (uint32_t)pg.nj), j = (int)(row - (uint32_t)i * pg.nj);
         const uint32_t v0 = row * (uint32_t)nk + 2 * m;
@@ -1104,7 +1104,7 @@ __global__ __launch_bounds__(WAVE * PP_TR) void k_sat_max_pair(
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         double a = lds[0];
         for (int w = 1; w < PP_TR; ++w) a = fmax(a, lds[w]);
-        partials[blockIdx.x + gridDim.x * blockIdx.y] = a;
+        partials[block_linear()] = a;
     }
 }
 
@@ -1118,7 +1118,7 @@ __global__ __launch_bounds__(WAVE * PP_TR) void k_sat_update_pair(
     const double dt_sub_over_phi = ss->sat_dt_sub_over_phi;
 #pragma unroll 1
     for (int q8 = 0; q8 < PP_RPW; ++q8) {
-        const uint32_t row = (blockIdx.y * PP_RPW + q8) * PP_TR + threadIdx.y;
+        const uint32_t row = (row_block() * PP_RPW + q8) * PP_TR + threadIdx.y;
         if (row >= pg.nrows) continue;                  // uniform per wave
         const int i = (int)(row / (uint32_t)pg.nj), j = (int)(row - (uint32_t)i * pg.nj);
         const uint32_t v0 = row * (uint32_t)nk + 2 * m;
@@ -1146,7 +1146,7 @@ int phase_saturation(ls_ctx* c, double dt, ls_step_report* rep) {
     if (use_pairs(c)) {
         const PairGeo pg = pair_geo(c);
         const dim3 pgrid = pair_grid(pg);
-        const int nbp = (int)(pgrid.x * pgrid.y);
+        const int nbp = (int)(pgrid.x * pgrid.y * pgrid.z);
         hipLaunchKernelGGL(k_sat_max_pair, pgrid, dim3(WAVE, PP_TR), 0, c->stream, c->img, c->qx,
                            c->qy, c->qz, pg, c->nk, c->P.dx, c->P.evap_flux, c->partials);
         hipLaunchKernelGGL(k_sat_finalize, dim3(1), dim3(1024), 0, c->stream, c->partials, nbp,

@@ -505,8 +505,7 @@ void transport_form(const ls_ctx* c, bool* stored, bool* schur, bool* mixed) {
     if (mixed) {
         const size_t gx = ((size_t)c->nk / 4 + WAVE - 1) / WAVE;
         const size_t gy = ((size_t)c->ni * c->nj + TSV_TR * TSV_RPW - 1) / (TSV_TR * TSV_RPW);
-        const bool vec = *schur && c->nk % 4 == 0 && gy <= 65535 &&
-                         4 * gx * gy <= 8 * (size_t)MAX_PARTIALS;
+        const bool vec = *schur && c->nk % 4 == 0 && 4 * gx * gy <= 8 * (size_t)MAX_PARTIALS;
         *mixed = vec && c->nk % 8 == 0 && c->P.adr_mixed != 0;
     }
 }
@@ -1084,7 +1083,7 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_adr_rows_pair(
     constexpr bool expl = EXPL;     // theta != 1: the explicit part of the old field
 #pragma unroll 1
     for (int q8 = 0; q8 < TSV_RPW; ++q8) {
-        const uint32_t row = (blockIdx.y * TSV_RPW + q8) * TSV_TR + threadIdx.y;
+        const uint32_t row = (row_block() * TSV_RPW + q8) * TSV_TR + threadIdx.y;
         if (row >= g.nrows) continue;                   // uniform per wave
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const int par = (i + j) & 1;
@@ -1241,7 +1240,7 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
     const d2 zero = d2{0.0, 0.0};
 #pragma unroll
     for (int q8 = 0; q8 < TSV_RPW; ++q8) {
-        const uint32_t row = (blockIdx.y * TSV_RPW + q8) * TSV_TR + threadIdx.y;
+        const uint32_t row = (row_block() * TSV_RPW + q8) * TSV_TR + threadIdx.y;
         if (row >= g.nrows) continue;                   // uniform per wave
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const int q = 1 - ((i + j) & 1);
@@ -1362,7 +1361,7 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
     for (int q8 = 0; q8 < TSV_RPW; ++q8) {
-        const uint32_t row = (blockIdx.y * TSV_RPW + q8) * TSV_TR + threadIdx.y;
+        const uint32_t row = (row_block() * TSV_RPW + q8) * TSV_TR + threadIdx.y;
         if (row >= g.nrows) continue;                   // uniform per wave
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const int q = (i + j) & 1;
@@ -1535,8 +1534,8 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
     }
     __syncthreads();
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const int nb = gridDim.x * gridDim.y;
-        const int b = blockIdx.x + gridDim.x * blockIdx.y;
+        const int nb = block_count();
+        const int b = block_linear();
         for (int a = 0; a < 4; ++a) {
             double t = 0.0;
             for (int w = 0; w < TSV_TR; ++w) t += lds[a * TSV_TR + w];
@@ -1915,7 +1914,7 @@ __global__ __launch_bounds__(WAVE * TMX_TR) void k_tmix_red(
     const size_t rs = rstride32(Nh);
 #pragma unroll
     for (int q8 = 0; q8 < TMX_RPW; ++q8) {
-        const uint32_t row = (blockIdx.y * TMX_RPW + q8) * TMX_TR + threadIdx.y;
+        const uint32_t row = (row_block() * TMX_RPW + q8) * TMX_TR + threadIdx.y;
         if (row >= g.nrows) continue;                   // uniform per wave
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const int q = 1 - ((i + j) & 1);
@@ -1961,7 +1960,7 @@ __global__ __launch_bounds__(WAVE * TMX_TR) void k_tmix_black(
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
     for (int q8 = 0; q8 < TMX_RPW; ++q8) {
-        const uint32_t row = (blockIdx.y * TMX_RPW + q8) * TMX_TR + threadIdx.y;
+        const uint32_t row = (row_block() * TMX_RPW + q8) * TMX_TR + threadIdx.y;
         if (row >= g.nrows) continue;                   // uniform per wave
         const int i = (int)(row / (uint32_t)g.nj), j = (int)(row - (uint32_t)i * g.nj);
         const int q = (i + j) & 1;
@@ -2026,8 +2025,8 @@ __global__ __launch_bounds__(WAVE * TMX_TR) void k_tmix_black(
     }
     __syncthreads();
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const int nb = gridDim.x * gridDim.y;
-        const int b = blockIdx.x + gridDim.x * blockIdx.y;
+        const int nb = block_count();
+        const int b = block_linear();
         for (int a = 0; a < 4; ++a) {
             double t = 0.0;
             for (int w = 0; w < TMX_TR; ++w) t += lds[a * TMX_TR + w];
@@ -2425,14 +2424,14 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         pg.plane = (uint32_t)c->nj * pg.nhr;
         pg.ni = c->ni;
         pg.nj = c->nj;
-        const dim3 pgrid((pg.nhr + WAVE - 1) / WAVE,
-                         (pg.nrows + TSV_TR * TSV_RPW - 1) / (TSV_TR * TSV_RPW));
-        if (KIND == 0 && pgrid.y <= 65535 && a.theta == 1.0)
+        const dim3 pgrid = row_grid((pg.nhr + WAVE - 1) / WAVE,
+                                    (pg.nrows + TSV_TR * TSV_RPW - 1) / (TSV_TR * TSV_RPW));
+        if (KIND == 0 && a.theta == 1.0)
             hipLaunchKernelGGL(k_adr_rows_pair<false>, pgrid, dim3(WAVE, TSV_TR), 0, c->stream,
                                c->img, c->qx, c->qy, c->qz, pg, N, a, c->arows, x[0], x[1],
                                c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], xprev, xprev2,
                                order, ratio, e0, e1, e2);
-        else if (KIND == 0 && pgrid.y <= 65535)
+        else if (KIND == 0)
             hipLaunchKernelGGL(k_adr_rows_pair<true>, pgrid, dim3(WAVE, TSV_TR), 0, c->stream,
                                c->img, c->qx, c->qy, c->qz, pg, N, a, c->arows, x[0], x[1],
                                c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], xprev, xprev2,
@@ -2473,19 +2472,19 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     tvg.plane = (uint32_t)c->nj * tvg.nhr;
     tvg.ni = c->ni;
     tvg.nj = c->nj;
-    const dim3 tvgrid((tvg.nhr / 2 + WAVE - 1) / WAVE,
-                      (tvg.nrows + TSV_TR * TSV_RPW - 1) / (TSV_TR * TSV_RPW));
-    const bool tvec = tschur && c->nk % 4 == 0 && tvgrid.y <= 65535 &&
-                      4 * (size_t)tvgrid.x * tvgrid.y <= 8 * (size_t)MAX_PARTIALS;
-    const int nbs_schur = tvec ? (int)(tvgrid.x * tvgrid.y) : nbh;
+    const dim3 tvgrid = row_grid((tvg.nhr / 2 + WAVE - 1) / WAVE,
+                                 (tvg.nrows + TSV_TR * TSV_RPW - 1) / (TSV_TR * TSV_RPW));
+    const size_t nb_tv = (size_t)tvgrid.x * tvgrid.y * tvgrid.z;
+    const bool tvec = tschur && c->nk % 4 == 0 && 4 * nb_tv <= 8 * (size_t)MAX_PARTIALS;
+    const int nbs_schur = tvec ? (int)nb_tv : nbh;
     // mixed precision (the pool was cut for it): FP32 cycles, 4 values per
     // lane in the passes, for conc alone (acid known zero: its vectors hold
     // the FP32 rows)
     const bool tm_wide = TMX_VW != 4 && c->nk % (2 * TMX_VW) == 0;
-    const dim3 tmgrid((tvg.nhr / (tm_wide ? TMX_VW : 4) + WAVE - 1) / WAVE,
-                      (tvg.nrows + TMX_TR * TMX_RPW - 1) / (TMX_TR * TMX_RPW));
-    const int nb_mix = (int)(tmgrid.x * tmgrid.y);
-    bool mixed = tvec && one && c->t_mixed && c->trows32 != nullptr && tmgrid.y <= 65535 &&
+    const dim3 tmgrid = row_grid((tvg.nhr / (tm_wide ? TMX_VW : 4) + WAVE - 1) / WAVE,
+                                 (tvg.nrows + TMX_TR * TMX_RPW - 1) / (TMX_TR * TMX_RPW));
+    const int nb_mix = (int)((size_t)tmgrid.x * tmgrid.y * tmgrid.z);
+    bool mixed = tvec && one && c->t_mixed && c->trows32 != nullptr &&
                  4 * (size_t)nb_mix <= 8 * (size_t)MAX_PARTIALS;
     double* nulls[2] = {nullptr, nullptr};
     // systems still iterating, as the host last saw them (a system that
@@ -2884,10 +2883,10 @@ int time_transport_pass(ls_ctx* c, int which, int reps, double* ms, double* byte
     tvg.plane = (uint32_t)c->nj * tvg.nhr;
     tvg.ni = c->ni;
     tvg.nj = c->nj;
-    const dim3 tvgrid((tvg.nhr / 2 + WAVE - 1) / WAVE,
-                      (tvg.nrows + TSV_TR * TSV_RPW - 1) / (TSV_TR * TSV_RPW));
-    const bool tvec = c->nk % 4 == 0 && tvgrid.y <= 65535 &&
-                      4 * (size_t)tvgrid.x * tvgrid.y <= 8 * (size_t)MAX_PARTIALS;
+    const dim3 tvgrid = row_grid((tvg.nhr / 2 + WAVE - 1) / WAVE,
+                                 (tvg.nrows + TSV_TR * TSV_RPW - 1) / (TSV_TR * TSV_RPW));
+    const bool tvec = c->nk % 4 == 0 &&
+                      4 * (size_t)tvgrid.x * tvgrid.y * tvgrid.z <= 8 * (size_t)MAX_PARTIALS;
     // the iteration's first matvec: p -> (red) w -> (black) v, dot with r0
     const double* xa = c->tv[0][2];
     double* w = c->twr[0];
@@ -2900,12 +2899,8 @@ int time_transport_pass(ls_ctx* c, int which, int reps, double* ms, double* byte
                  "(adr_mixed, nk % 8 == 0, acid zero)";
         return -3;
     }
-    const dim3 tmgrid((tvg.nhr / 4 + WAVE - 1) / WAVE,
-                      (tvg.nrows + TMX_TR * TMX_RPW - 1) / (TMX_TR * TMX_RPW));
-    if (f32 && tmgrid.y > 65535) {
-        c->err = "the FP32 transport passes' grid exceeds the row limit";
-        return -3;
-    }
+    const dim3 tmgrid = row_grid((tvg.nhr / 4 + WAVE - 1) / WAVE,
+                                 (tvg.nrows + TMX_TR * TMX_RPW - 1) / (TMX_TR * TMX_RPW));
     if (f32) hipLaunchKernelGGL(k_bicg_rearm, dim3(1), dim3(1), 0, c->stream, c->bicg32);
     auto launch = [&]() {
         if (which == LS_TIME_TRED32) {

@@ -100,6 +100,22 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double* lds) {
     }
 }
 
+// Row-block kernels (a block walks rows (i, j) of the grid) fold their row
+// blocks over grid y and z, so a grid of more than 65535 row blocks (1024 x
+// 1024 rows and more) still launches: the block's row block, its linear
+// index and the block count.
+__device__ __forceinline__ uint32_t row_block() { return blockIdx.y + gridDim.y * blockIdx.z; }
+__device__ __forceinline__ int block_linear() {
+    return (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+}
+__device__ __forceinline__ int block_count() { return (int)(gridDim.x * gridDim.y * gridDim.z); }
+// The grid of gx x rowblocks blocks, the row blocks folded over y and z.
+inline dim3 row_grid(uint32_t gx, uint32_t rowblocks) {
+    uint32_t gz = 1;
+    while ((rowblocks + gz - 1) / gz > 65535) gz *= 2;
+    return dim3(gx, (rowblocks + gz - 1) / gz, gz);
+}
+
 // Number of blocks for grid-stride elementwise/reduction kernels: enough to
 // fill 256 CUs several times over, bounded so partial arrays stay small.
 inline int ew_blocks(size_t n) {
