@@ -1201,10 +1201,72 @@ __global__ void k_surface(const uint8_t* __restrict__ img, uint8_t* __restrict__
     img_out[v] = t;
 }
 
+// The same with a voxel pair per lane (nk even; the layout of
+// k_press_diag_pair): a Pore voxel with s > 0 and an Air neighbour becomes
+// Precipitate with conc = 1.
+__global__ __launch_bounds__(WAVE * PP_TR) void k_surface_pair(
+    const uint8_t* __restrict__ img, uint8_t* __restrict__ img_out,
+    const double* __restrict__ s, double* __restrict__ conc, PairGeo pg, int nk) {
+    const uint32_t m = blockIdx.x * WAVE + threadIdx.x;
+    const bool ok_ = m < pg.nhr;
+#pragma unroll 1
+    for (int q8 = 0; q8 < PP_RPW; ++q8) {
+        const uint32_t row = (row_block() * PP_RPW + q8) * PP_TR + threadIdx.y;
+        if (row >= pg.nrows) continue;                  // uniform per wave
+        const int i = (int)(row / (uint32_t)pg.nj), j = (int)(row - (uint32_t)i * pg.nj);
+        const uint32_t v0 = row * (uint32_t)nk + 2 * m;
+        const uint32_t lc = ok_ ? ld_b2(img, v0) : 0u;
+        const pd2 sc = ok_ ? ld_d2(s, v0) : pd2{0.0, 0.0};
+        const bool c0 = (lc & 0xffu) == PORE && sc.x > 0.0;
+        const bool c1 = (lc >> 8) == PORE && sc.y > 0.0;
+        const bool need = c0 || c1;
+        const bool hip_ = i + 1 < pg.ni, him = i > 0, hjp = j + 1 < pg.nj, hjm = j > 0;
+        uint32_t lip = 0, lim = 0, ljp = 0, ljm = 0;
+        if (need) {
+            if (hip_) lip = ld_b2(img, v0 + pg.si);
+            if (him) lim = ld_b2(img, v0 - pg.si);
+            if (hjp) ljp = ld_b2(img, v0 + pg.sj);
+            if (hjm) ljm = ld_b2(img, v0 - pg.sj);
+        }
+        int t_l, t_r;
+        k_out<int>((int)(lc & 0xffu), (int)(lc >> 8), (const int*)nullptr, 0u, m, pg.nhr, false,
+                   t_l, t_r);
+        if (threadIdx.x == 0) t_l = (need && m >= 1) ? (int)img[v0 - 1] : 0;
+        if (threadIdx.x == WAVE - 1) t_r = (need && m + 1 < pg.nhr) ? (int)img[v0 + 2] : 0;
+        if (!ok_) continue;
+        uint32_t out = lc;
+        if (need) {
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                if (!(e ? c1 : c0)) continue;
+                const int k = 2 * (int)m + e;
+                const bool okf[6] = {hip_, him, hjp, hjm, k + 1 < nk, k > 0};
+                const uint32_t tn[6] = {e ? (lip >> 8) : (lip & 0xffu), e ? (lim >> 8) : (lim & 0xffu),
+                                        e ? (ljp >> 8) : (ljp & 0xffu), e ? (ljm >> 8) : (ljm & 0xffu),
+                                        e ? (uint32_t)t_r : (lc >> 8),
+                                        e ? (lc & 0xffu) : (uint32_t)t_l};
+                bool surface = false;
+#pragma unroll
+                for (int f = 0; f < 6; ++f)
+                    if (okf[f] && tn[f] == AIR) surface = true;
+                if (surface) {
+                    conc[v0 + e] = 1.0;
+                    out = e ? ((out & 0xffu) | ((uint32_t)PREC << 8)) : ((out & 0xff00u) | PREC);
+                }
+            }
+        }
+        *(uint16_t*)(img_out + v0) = (uint16_t)out;
+    }
+}
+
 int phase_surface(ls_ctx* c) {
     Geo g = geo_of(c);
     dim3 blk(TK, TJ), grd((c->nk + TK - 1) / TK, (c->nj + TJ - 1) / TJ, c->ni);
-    hipLaunchKernelGGL(k_surface, grd, blk, 0, c->stream, c->img, c->img2, c->s, c->conc, g);
+    if (use_pairs(c))
+        hipLaunchKernelGGL(k_surface_pair, pair_grid(pair_geo(c)), dim3(WAVE, PP_TR), 0,
+                           c->stream, c->img, c->img2, c->s, c->conc, pair_geo(c), c->nk);
+    else
+        hipLaunchKernelGGL(k_surface, grd, blk, 0, c->stream, c->img, c->img2, c->s, c->conc, g);
     LS_LAUNCH_CHECK();
     uint8_t* tmp = c->img;
     c->img = c->img2;
@@ -1280,7 +1342,7 @@ __global__ void k_summary(const double* __restrict__ p, const double* __restrict
     double pmn = 1.0 / 0.0, pmx = -1.0 / 0.0, cmn = 1.0 / 0.0, cmx = -1.0 / 0.0;
     double amn = 1.0 / 0.0, amx = -1.0 / 0.0, sc = 0.0, sa = 0.0, sw = 0.0;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
-        double a = p[v], b = conc[v], d = acid[v];
+        double a = p[v], b = conc[v], d = acid != nullptr ? acid[v] : 0.0;
         pmn = fmin(pmn, a); pmx = fmax(pmx, a);
         cmn = fmin(cmn, b); cmx = fmax(cmx, b);
         amn = fmin(amn, d); amx = fmax(amx, d);
@@ -1330,8 +1392,9 @@ int summarize(ls_ctx* c, ls_step_report* rep) {
     uint32_t N = (uint32_t)c->N;
     int nb = ew_blocks(N);
     if (nb * 9 > MAX_PARTIALS * 8) nb = MAX_PARTIALS * 8 / 9;
-    hipLaunchKernelGGL(k_summary, dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->p, c->conc, c->acid,
-                       c->s, N, c->partials);
+    // acid known zero (+0.0 everywhere) reads as zeros without its loads
+    hipLaunchKernelGGL(k_summary, dim3(nb), dim3(EW_BLOCK), 0, c->stream, c->p, c->conc,
+                       c->acid_zero ? nullptr : c->acid, c->s, N, c->partials);
     LS_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_summary_finalize, dim3(1), dim3(1024), 0, c->stream, c->partials, nb,
                        c->ss);

@@ -101,8 +101,10 @@ def test_flux_saturation_parity(rng, shape):
     assert np.array_equal(g.download(L.F_SATURATION), o.download(L.F_SATURATION))
 
 
-def test_surface_precipitation_bit_exact(rng):
-    shape = (8, 9, 10)
+@pytest.mark.parametrize("shape", [(8, 9, 10), (8, 9, 11), (5, 6, 130)])
+def test_surface_precipitation_bit_exact(rng, shape):
+    """nk even: the voxel-pair surface kernel (nk = 130: pairs at the waves'
+    edges); nk odd: one voxel per thread."""
     img = small_mixed_geometry(shape, 11)
     P = L.default_params()
     g, o = both(shape, P, img)
