@@ -150,8 +150,8 @@ typedef struct ls_params {
                                 nk even; 0: BiCGStab on the full system         */
     int adr_extrapolate;     /* transport on the Schur complement (1): conc starts
                                 from its extrapolation in time through the fields
-                                of the last solves (quadratic through the last
-                                three, linear after the first solve) instead of
+                                of the last solves (cubic through the last four,
+                                lower orders after fewer solves) instead of
                                 the old field (the reference's start,
                                 src/Simulation.cpp:1247); 0: the old field       */
     int adr_mixed;           /* transport and implicit saturation on the Schur

@@ -35,6 +35,33 @@ struct AdrConst {
     int dx_pow2;           // dx is a power of two: q / dx == q * (1 / dx)
 };
 
+// The extrapolated transport start: y = w[0] x + sum_k w[k + 1] h_k over the
+// first n history fields h_0 (newest) .. h_2, Lagrange weights at the new time.
+struct Extrap {
+    int n;
+    double w[4];
+};
+
+// One black voxel's start from its current field and the history, which
+// then shifts by one (h_2 <- h_1 <- h_0 <- x) where it is held.
+__device__ __forceinline__ double extrap_start(double xa, uint32_t m, double* __restrict__ h0,
+                                               double* __restrict__ h1, double* __restrict__ h2,
+                                               const Extrap& ex) {
+    const double v0 = h0 != nullptr ? h0[m] : 0.0;
+    const double v1 = h1 != nullptr ? h1[m] : 0.0;
+    const double v2 = h2 != nullptr ? h2[m] : 0.0;
+    double xs = xa;
+    if (ex.n >= 1) {
+        xs = ex.w[0] * xa + ex.w[1] * v0;
+        if (ex.n >= 2) xs += ex.w[2] * v1;
+        if (ex.n >= 3) xs += ex.w[3] * v2;
+    }
+    if (h2 != nullptr) h2[m] = v1;
+    if (h1 != nullptr) h1[m] = v0;
+    if (h0 != nullptr) h0[m] = xa;
+    return xs;
+}
+
 __device__ __forceinline__ void tdecode(uint32_t v, const TGeo& g, int& i, int& j, int& k) {
     uint32_t ij = v / (uint32_t)g.nk;
     k = (int)(v - ij * (uint32_t)g.nk);
@@ -741,11 +768,14 @@ __device__ __forceinline__ SPair spair(uint32_t m, const TGeo& g) {
     return s;
 }
 
-// Order of the extrapolated transport start (1 linear, 2 quadratic).  On the
-// 512^3 fractured workload the quadratic start took BiCGStab from 5 to 4
-// iterations per step from step 13 on (steps 5-24: 69.3 -> 65.7 ms per step).
+// Order of the extrapolated transport start (1 linear, 2 quadratic, 3 cubic;
+// one history field of N/2 doubles each).  On the 512^3 fractured workload
+// the quadratic start took FP64 BiCGStab from 5 to 4 iterations per step
+// from step 13 on (steps 5-24: 69.3 -> 65.7 ms per step), and the cubic one
+// the mixed-precision form's FP32 cycle from 5 to 4 iterations from step 11
+// on (steps 5-24: 50.0 -> 47.2 ms per step).
 #ifndef LS_ADR_EXTRAP_ORDER
-#define LS_ADR_EXTRAP_ORDER 2
+#define LS_ADR_EXTRAP_ORDER 3
 #endif
 
 // The 14 colour-split row arrays are TROW_PAD doubles apart beyond their
@@ -778,13 +808,10 @@ __device__ __forceinline__ double srow(const double* __restrict__ rows, uint32_t
 
 // The stored rows, colour-split, with b (full layout) as k_adr_rows; at the
 // black voxels also the solver's start y = D_b x.  one: system 1 is known to
-// be 0 with b = 0 (its b and x are not touched).  With xprev (the black part
-// of system 0's field at the previous transport solve) system 0 starts from
-// its extrapolation in time: order 1, x + ratio (x - xprev) with ratio =
-// dt / dt_prev; order 2 (with xprev2, the field of the solve before), the
-// quadratic through the three, e2 x + e1 xprev + e0 xprev2 (Lagrange weights
-// at the new time).  xprev (and xprev2) then take the current fields for the
-// next solve.
+// be 0 with b = 0 (its b and x are not touched).  With a history (the black
+// part of system 0's field at the last transport solves, hist0 the newest)
+// system 0 starts from its extrapolation in time through the current field
+// and ex.n history fields (extrap_start), and the history shifts by one.
 // Transport (KIND 0) also zeroes the flux at Rock and Precipitate voxels
 // here, as the reference's assembly does (src/Simulation.cpp:1134-1136,
 // 1147-1149): no row reads the flux of such a voxel (a Rock neighbour is an
@@ -796,8 +823,8 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
     const double* __restrict__ sf, TGeo g, uint32_t N, AdrConst a,
     double* __restrict__ rows, const double* __restrict__ x0, const double* __restrict__ x1,
     double* __restrict__ b0, double* __restrict__ b1, int one, double* __restrict__ xs0,
-    double* __restrict__ xs1, double* __restrict__ xprev, double* __restrict__ xprev2,
-    int order, double ratio, double e0, double e1, double e2) {
+    double* __restrict__ xs1, double* __restrict__ hist0, double* __restrict__ hist1,
+    double* __restrict__ hist2, Extrap ex) {
     const uint32_t Nh = N / 2;
     const double* x1r = one ? x0 : x1;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
@@ -838,17 +865,7 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
             rows[((size_t)(f + 1) * 2 + colour) * rstride(Nh) + m] =
                 (R.mask & (1 << f)) ? R.off[f] : 0.0;
         if (colour == 0) {
-            double xstart = xa;
-            if (xprev != nullptr) {
-                const double xp = xprev[m];
-                if (xprev2 != nullptr) {
-                    const double xq = xprev2[m];
-                    if (order == 2) xstart = e2 * xa + e1 * xp + e0 * xq;
-                    xprev2[m] = xp;
-                }
-                if (order == 1) xstart = xa + ratio * (xa - xp);
-                xprev[m] = xa;
-            }
+            const double xstart = extrap_start(xa, m, hist0, hist1, hist2, ex);
             xs0[m] = R.diag * xstart;
             if (!one) xs1[m] = R.diag * xb;
         }
@@ -1070,8 +1087,8 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_adr_rows_pair(
     const uint8_t* __restrict__ img, double* qx, double* qy, double* qz, TVGeo g, uint32_t N,
     AdrConst a, double* __restrict__ rows, const double* __restrict__ x0,
     const double* __restrict__ x1, double* __restrict__ b0, double* __restrict__ b1, int one,
-    double* __restrict__ xs0, double* __restrict__ xs1, double* __restrict__ xprev,
-    double* __restrict__ xprev2, int order, double ratio, double e0c, double e1c, double e2c) {
+    double* __restrict__ xs0, double* __restrict__ xs1, double* __restrict__ hist0,
+    double* __restrict__ hist1, double* __restrict__ hist2, Extrap ex) {
     const uint32_t Nh = N / 2;
     const double* x1r = one ? x0 : x1;
     const int lane = threadIdx.x;
@@ -1196,17 +1213,7 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_adr_rows_pair(
                 rows[((size_t)(f + 1) * 2 + colour) * rstride(Nh) + mg] =
                     (R.mask & (1 << f)) ? R.off[f] : 0.0;
             if (colour == 0) {
-                double xstart = xa;
-                if (xprev != nullptr) {
-                    const double xp = xprev[mg];
-                    if (xprev2 != nullptr) {
-                        const double xq = xprev2[mg];
-                        if (order == 2) xstart = e2c * xa + e1c * xp + e0c * xq;
-                        xprev2[mg] = xp;
-                    }
-                    if (order == 1) xstart = xa + ratio * (xa - xp);
-                    xprev[mg] = xa;
-                }
+                const double xstart = extrap_start(xa, mg, hist0, hist1, hist2, ex);
                 xs0[mg] = R.diag * xstart;
                 if (!one) xs1[mg] = R.diag * xb;
             }
@@ -2384,7 +2391,7 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
     if (tschur) {
         // transport: conc starts from its extrapolation in time through the
         // fields of the last LS_ADR_EXTRAP_ORDER solves
-        double *xprev = nullptr, *xprev2 = nullptr;
+        double* hist[3] = {nullptr, nullptr, nullptr};
         auto history = [&](double** buf) {
             if (*buf == nullptr) {
                 if (hipMalloc((void**)buf, Nh * sizeof(double)) == hipSuccess) {
@@ -2397,25 +2404,28 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             }
             return *buf;
         };
-        if (KIND == 0 && c->P.adr_extrapolate) {
-            xprev = history(&c->tprev);
-            if (xprev != nullptr && LS_ADR_EXTRAP_ORDER >= 2) xprev2 = history(&c->tprev2);
-        }
+        int held = 0;           // history fields held (the extrapolation's order)
+        if (KIND == 0 && c->P.adr_extrapolate)
+            while (held < LS_ADR_EXTRAP_ORDER && held < 3 && history(&c->thist[held]) != nullptr) {
+                hist[held] = c->thist[held];
+                ++held;
+            }
         const double h = 1.0 / a.inv_dt;
-        int order = 0;
-        double ratio = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
-        if (xprev != nullptr && c->tprev_n >= 1 && c->tprev_dt > 0.0) {
-            order = 1;
-            ratio = h / c->tprev_dt;
-        }
-        if (xprev2 != nullptr && c->tprev_n >= 2 && c->tprev_dt > 0.0 && c->tprev2_dt > 0.0) {
-            // nodes t2 = 0 (x), t1 = -dt_prev (xprev), t0 = -(dt_prev + dt_prev2) (xprev2),
-            // evaluated at t = h
-            const double t1 = -c->tprev_dt, t0 = -(c->tprev_dt + c->tprev2_dt);
-            order = 2;
-            e2 = (h - t0) * (h - t1) / ((0.0 - t0) * (0.0 - t1));
-            e1 = (h - t0) * (h - 0.0) / ((t1 - t0) * (t1 - 0.0));
-            e0 = (h - t1) * (h - 0.0) / ((t0 - t1) * (t0 - 0.0));
+        // Lagrange weights at t = h through the current field (t = 0) and the
+        // usable history fields (t = -(dt_0 + ... + dt_k))
+        Extrap ex{};
+        ex.n = std::min(held, c->tprev_n);
+        for (int k = 0; k < ex.n; ++k)
+            if (!(c->thist_dt[k] > 0.0)) ex.n = k;
+        {
+            double tau[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int k = 1; k <= ex.n; ++k) tau[k] = tau[k - 1] - c->thist_dt[k - 1];
+            for (int q = 0; q <= ex.n; ++q) {
+                double w = 1.0;
+                for (int l = 0; l <= ex.n; ++l)
+                    if (l != q) w *= (h - tau[l]) / (tau[q] - tau[l]);
+                ex.w[q] = w;
+            }
         }
         // conc's rows with a voxel pair per lane where the rows fit the grid
         TVGeo pg;
@@ -2429,25 +2439,26 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         if (KIND == 0 && a.theta == 1.0)
             hipLaunchKernelGGL(k_adr_rows_pair<false>, pgrid, dim3(WAVE, TSV_TR), 0, c->stream,
                                c->img, c->qx, c->qy, c->qz, pg, N, a, c->arows, x[0], x[1],
-                               c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], xprev, xprev2,
-                               order, ratio, e0, e1, e2);
+                               c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], hist[0], hist[1],
+                               hist[2], ex);
         else if (KIND == 0)
             hipLaunchKernelGGL(k_adr_rows_pair<true>, pgrid, dim3(WAVE, TSV_TR), 0, c->stream,
                                c->img, c->qx, c->qy, c->qz, pg, N, a, c->arows, x[0], x[1],
-                               c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], xprev, xprev2,
-                               order, ratio, e0, e1, e2);
+                               c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], hist[0], hist[1],
+                               hist[2], ex);
         else
             hipLaunchKernelGGL((k_adr_rows_split<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
                                c->img, c->qx, c->qy, c->qz, sf, g, N, a, c->arows, x[0], x[1],
-                               c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], xprev, xprev2,
-                               order, ratio, e0, e1, e2);
+                               c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], hist[0], hist[1],
+                               hist[2], ex);
         if (KIND == 0) {
-            if (xprev == nullptr) {
+            if (held == 0) {
                 c->tprev_n = 0;
             } else {
-                c->tprev2_dt = c->tprev_dt;
-                c->tprev_dt = h;
-                c->tprev_n = std::min(c->tprev_n + 1, xprev2 != nullptr ? 2 : 1);
+                c->thist_dt[2] = c->thist_dt[1];
+                c->thist_dt[1] = c->thist_dt[0];
+                c->thist_dt[0] = h;
+                c->tprev_n = std::min(c->tprev_n + 1, held);
             }
         }
         LS_LAUNCH_CHECK();

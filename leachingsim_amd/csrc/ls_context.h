@@ -141,14 +141,12 @@ struct ls_ctx {
     bool t_rows32_valid = false;        // the FP32 rows belong to the last solve
     double* taux[2] = {};    // implicit saturation: its source field and idle system (N)
     double* arows = nullptr; // stored transport rows (nk even)
-    // black half of conc at the last transport solve (N/2, allocated on first
-    // use), its dt, and whether it belongs to the current state's history
-    double* tprev = nullptr;
-    double tprev_dt = 0.0;
-    // and the one before (N/2, for the second-order start), its dt, and how
-    // many of the two belong to the current state's history (0, 1, 2)
-    double* tprev2 = nullptr;
-    double tprev2_dt = 0.0;
+    // the black half of conc at the last transport solves, newest first (N/2
+    // each, allocated on first use), the time from each to the next newer
+    // one (thist_dt[0]: to the current field), and how many belong to the
+    // current state's history
+    double* thist[3] = {nullptr, nullptr, nullptr};
+    double thist_dt[3] = {0.0, 0.0, 0.0};
     int tprev_n = 0;
     // --- reductions ---
     double* partials = nullptr;     // MAX_PARTIALS * 8 values
