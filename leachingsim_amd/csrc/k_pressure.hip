@@ -133,6 +133,7 @@ template <typename T, int W> struct VecOf;
 template <> struct VecOf<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
 template <> struct VecOf<float, 2> { typedef float type __attribute__((ext_vector_type(2))); };
 template <> struct VecOf<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct VecOf<double, 4> { typedef double type __attribute__((ext_vector_type(4))); };
 
 template <typename T> struct StencilShape;
 template <> struct StencilShape<float> { static constexpr int W = 4, ICH = 4, TJV = 8; };
@@ -281,6 +282,8 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
 //   MODE 4 (reconstruct) y = red ? xc + n : xn               (x_r = r_r + N_rb x_b)
 //   MODE 5 (rhs, packed) as MODE 3, y written as the compacted black vector
 //                        (layout below k_schur_apply's header)
+//   MODE 6 (reconstruct and add) as MODE 4, then xacc += scale * y (FP64)
+//                        instead of storing y
 // Vectorised like k_stencil_vec: W floats of one k-row per lane (4 when
 // nk % 4 == 0, else 2 when nk is even), marching ICH planes along i.
 // ---------------------------------------------------------------------------
@@ -288,7 +291,8 @@ template <int MODE, int W = 4>
 __global__ __launch_bounds__(WAVE * StencilShape<float>::TJV) void k_rb_vec(
     const float* __restrict__ xn, const float* __restrict__ xc, float* __restrict__ y,
     const float* __restrict__ cx, const float* __restrict__ cy, const float* __restrict__ cz,
-    SGeo g, double* __restrict__ partials, const int* __restrict__ done) {
+    SGeo g, double* __restrict__ partials, const int* __restrict__ done,
+    double* __restrict__ xacc = nullptr, double scale = 0.0) {
     static_assert(W == 4 || (W == 2 && MODE != 5), "the packed output needs 4 values per lane");
     constexpr int ICH = StencilShape<float>::ICH, TJV = StencilShape<float>::TJV;
     typedef typename VecOf<float, W>::type V;
@@ -360,6 +364,15 @@ __global__ __launch_bounds__(WAVE * StencilShape<float>::TJV) void k_rb_vec(
                 const int pb = (i + j) & 1;
                 const F2 o2 = pb ? F2{out[1], out[3]} : F2{out[0], out[2]};
                 if (ok) *(F2*)(y + (v >> 1)) = o2;
+            } else if constexpr (MODE == 6) {
+                // as k_refine_add: x += scale * (double) y
+                typedef typename VecOf<double, W>::type VD;
+                if (ok) {
+                    VD xv = *(const VD*)(xacc + v);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) xv[w] += scale * (double)out[w];
+                    *(VD*)(xacc + v) = xv;
+                }
             } else if (ok) {
                 __builtin_nontemporal_store(out, (V*)(y + v));
             }
@@ -1342,6 +1355,11 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
 
 // Launch of one red-black pass over the FP32 faces; returns the number of
 // block partials (MODE 2 and 3 write them).
+// x64 += scale * (red reconstruction of e): k_rb_vec mode 6 in one pass, or
+// mode 4 into the scratch w and k_refine_add (one voxel per thread).
+static void launch_reconstruct_add(ls_ctx* c, const float* e, const float* r, float* w,
+                                   double scale);
+
 template <int MODE>
 static int launch_rb(ls_ctx* c, const float* xn, const float* xc, float* y, const int* done) {
     const int w = MODE == 5 ? 4 : rb_width(c);
@@ -1365,6 +1383,26 @@ static int launch_rb(ls_ctx* c, const float* xn, const float* xc, float* y, cons
                        c->stream, xn, xc, y, c->c32[0], c->c32[1], c->c32[2], sgeo(c), c->partials,
                        done);
     return (int)(grd.x * grd.y * grd.z);
+}
+
+static void launch_reconstruct_add(ls_ctx* c, const float* e, const float* r, float* w,
+                                   double scale) {
+    const int wd = rb_width(c);
+    if (wd == 0) {
+        launch_rb<4>(c, e, r, w, nullptr);
+        hipLaunchKernelGGL(k_refine_add, dim3(ew_blocks((uint32_t)c->N)), dim3(EW_BLOCK), 0,
+                           c->stream, c->x64, w, (uint32_t)c->N, scale);
+        return;
+    }
+    const dim3 grd = rb_grid(c, wd);
+    if (wd == 2)
+        hipLaunchKernelGGL((k_rb_vec<6, 2>), grd, dim3(WAVE, StencilShape<float>::TJV), 0,
+                           c->stream, e, r, nullptr, c->c32[0], c->c32[1], c->c32[2], sgeo(c),
+                           c->partials, nullptr, c->x64, scale);
+    else
+        hipLaunchKernelGGL((k_rb_vec<6, 4>), grd, dim3(WAVE, StencilShape<float>::TJV), 0,
+                           c->stream, e, r, nullptr, c->c32[0], c->c32[1], c->c32[2], sgeo(c),
+                           c->partials, nullptr, c->x64, scale);
 }
 
 // Fused Schur apply on compacted vectors: block shape 64 x WR x (TJ + 2)
@@ -1580,12 +1618,10 @@ static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep, bool fused) {
         // (compacted e: expanded into ap32 first, q being dead here)
         if (fused) {
             launch_unpack(c, c->x32, c->ap32);
-            launch_rb<4>(c, c->ap32, c->r32, w, nullptr);
+            launch_reconstruct_add(c, c->ap32, c->r32, w, scale);
         } else {
-            launch_rb<4>(c, c->x32, c->r32, w, nullptr);
+            launch_reconstruct_add(c, c->x32, c->r32, w, scale);
         }
-        hipLaunchKernelGGL(k_refine_add, dim3(nbe), dim3(EW_BLOCK), 0, c->stream, c->x64, w, N,
-                           scale);
         LS_LAUNCH_CHECK();
         rep->press_outer += 1;
         if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
