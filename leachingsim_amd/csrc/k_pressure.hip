@@ -90,7 +90,7 @@ __global__ __launch_bounds__(TK * TJ) void k_stencil(
                 const double bv = b[v];
                 const double r = bv - (double)ax;
                 y[v] = (T)r;
-                p2[v] = (T)r;
+                if (p2 != nullptr) p2[v] = (T)r;
                 const double s = sd[v];
                 acc0 += r * r;
                 acc1 += (s * r) * (s * r);
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(WAVE * StencilShape<T>::TJV) void k_stencil_vec(
             }
             if (ok) {
                 __builtin_nontemporal_store(ax, (V*)(y + v));
-                if (MODE == 1) __builtin_nontemporal_store(ax, (V*)(p2 + v));
+                if (MODE == 1 && p2 != nullptr) __builtin_nontemporal_store(ax, (V*)(p2 + v));
             }
             xm = xc;
             xc = xp;
@@ -1073,7 +1073,10 @@ static int nblocks_stencil(const ls_ctx* c) {
 }
 
 // r = b^ - A^ x^ (FP64), p = r; fills cs (rr, wrr, bnorm2, tol2, done).
-static int cg64_residual(ls_ctx* c, double rtol, int maxit) {
+// The FP64 residual r64 = b^ - A^ x64 and the CG scalars of its norms;
+// with_p also p64 = r64 (FP64 CG's first direction; the mixed solvers start
+// their FP32 iterations from r64 alone).
+static int cg64_residual(ls_ctx* c, double rtol, int maxit, bool with_p = true) {
     if (c->resid_ready > 0) {
         // the projected start left r64 (and p64) and the partials in place
         hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(1024), 0, c->stream, c->partials,
@@ -1090,13 +1093,13 @@ static int cg64_residual(ls_ctx* c, double rtol, int maxit) {
         hipLaunchKernelGGL((k_stencil_vec<double, 1>), grd,
                            dim3(WAVE, StencilShape<double>::TJV), 0, c->stream, c->x64, c->r64,
                            c->c64[0], c->c64[1], c->c64[2], g, c->partials, nullptr, c->bhat,
-                           c->p64, c->sd);
+                           with_p ? c->p64 : nullptr, c->sd);
     } else {
         dim3 grd = stencil_grid(c), blk(TK, TJ);
         nb = nblocks_stencil(c);
         hipLaunchKernelGGL((k_stencil<double, 1>), grd, blk, 0, c->stream, c->x64, c->r64,
                            c->c64[0], c->c64[1], c->c64[2], g, c->partials, nullptr, c->bhat,
-                           c->p64, c->sd);
+                           with_p ? c->p64 : nullptr, c->sd);
     }
     LS_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(1024), 0, c->stream, c->partials, nb, c->cg, rtol,
@@ -1252,7 +1255,7 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
     InnerBatch ib(batch_size(c));
     const double delta = c->P.mixed_inner_rtol > 0 ? c->P.mixed_inner_rtol : 0.1;
     int st;
-    if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+    if ((st = cg64_residual(c, c->P.press_rtol, 1, false))) return st;
     if ((st = poll_cg(c))) return st;
     const double bnorm2 = c->h_cg->bnorm2;
     const double tol2 = c->h_cg->tol2;
@@ -1325,7 +1328,7 @@ static int solve_mixed(ls_ctx* c, ls_step_report* rep) {
         LS_LAUNCH_CHECK();
         rep->press_outer += 1;
         // FP64 residual replacement (keeps the device iteration counter)
-        if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+        if ((st = cg64_residual(c, c->P.press_rtol, 1, false))) return st;
         if ((st = poll_cg(c))) return st;
         rr64 = c->h_cg->rr;
         wrr = c->h_cg->wrr;
@@ -1537,7 +1540,7 @@ static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep, bool fused) {
     float* rs = (float*)c->pw[0];       // Schur residual (FP32, in the pressure scratch)
     float* w = (float*)c->pw[1];
     int st;
-    if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+    if ((st = cg64_residual(c, c->P.press_rtol, 1, false))) return st;
     if ((st = poll_cg(c))) return st;
     const double bnorm2 = c->h_cg->bnorm2;
     const double tol2 = c->h_cg->tol2;
@@ -1624,7 +1627,7 @@ static int solve_mixed_rb(ls_ctx* c, ls_step_report* rep, bool fused) {
         }
         LS_LAUNCH_CHECK();
         rep->press_outer += 1;
-        if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+        if ((st = cg64_residual(c, c->P.press_rtol, 1, false))) return st;
         if ((st = poll_cg(c))) return st;
         rr64 = c->h_cg->rr;
         wrr = c->h_cg->wrr;
@@ -2345,7 +2348,7 @@ static int solve_learned(ls_ctx* c, ls_step_report* rep) {
     float* w = (float*)c->pw[3];
     float* sw = w + N;
     double* xbackup = c->pw[4];
-    if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+    if ((st = cg64_residual(c, c->P.press_rtol, 1, false))) return st;
     if ((st = poll_cg(c))) return st;
     const double bnorm2 = c->h_cg->bnorm2;
     const double tol2 = c->h_cg->tol2;
@@ -2434,7 +2437,7 @@ static int solve_learned(ls_ctx* c, ls_step_report* rep) {
                            scale);
         LS_LAUNCH_CHECK();
         rep->press_outer += 1;
-        if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+        if ((st = cg64_residual(c, c->P.press_rtol, 1, false))) return st;
         if ((st = poll_cg(c))) return st;
         rr64 = c->h_cg->rr;
         wrr = c->h_cg->wrr;
@@ -2784,7 +2787,7 @@ static int warm_start(ls_ctx* c, double ptop, ls_step_report* rep) {
     // network correction of the projected residual, in FP32 like the
     // preconditioner: r = b^ - A^ x0, e = P U(R r), then e joins the basis
     if ((st = learned_setup(c))) return st;
-    if ((st = cg64_residual(c, c->P.press_rtol, 1))) return st;
+    if ((st = cg64_residual(c, c->P.press_rtol, 1, false))) return st;
     if ((st = poll_cg(c))) return st;
     if (c->h_cg->converged || !(c->h_cg->rr > 0.0)) return 0;
     const double scale = std::ldexp(1.0, std::ilogb(std::sqrt(c->h_cg->rr)));
