@@ -126,7 +126,7 @@ __device__ __forceinline__ double permeability(uint8_t type, double krel, double
             K_intrinsic = pc.k_rock;
         } else {
             double log_k_eff = (p_frac * pc.log_k_rock) + ((1.0 - p_frac) * pc.log_k_pore);
-            K_intrinsic = pow(10.0, log_k_eff);
+            K_intrinsic = exp10(log_k_eff);
         }
     } else if (type == SUL) {
         double f = fr;
@@ -136,7 +136,7 @@ __device__ __forceinline__ double permeability(uint8_t type, double krel, double
             K_intrinsic = pc.k_pore;
         } else {
             double log_k_eff = (f * pc.log_k_rock) + ((1.0 - f) * pc.log_k_pore);
-            K_intrinsic = pow(10.0, log_k_eff);
+            K_intrinsic = exp10(log_k_eff);
         }
     }
     double K_rel = dmax(krel, 1e-15);

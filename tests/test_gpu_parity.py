@@ -154,9 +154,10 @@ def test_transport_parity(rng, theta, nk, acid_zero, schur):
         assert np.array_equal(g.download(f), o.download(f))
 
 
+@pytest.mark.parametrize("theta", [1.0, 0.5])
 @pytest.mark.parametrize("acid_zero", [True, False])
 @pytest.mark.parametrize("inner", [1e-5, 0.9])
-def test_transport_mixed_precision_parity(rng, acid_zero, inner):
+def test_transport_mixed_precision_parity(rng, acid_zero, inner, theta):
     """nk % 8 == 0 with acid known zero: the mixed-precision Schur form, FP32
     BiCGStab cycles on FP32 copies of the rows inside FP64 refinement, solved
     to 1e-13 against the oracle (several cycles).  Inner rtol 0.9 makes every
@@ -165,7 +166,7 @@ def test_transport_mixed_precision_parity(rng, acid_zero, inner):
     present both systems take FP64 BiCGStab (no cycles)."""
     shape = (11, 10, 16)
     img = small_mixed_geometry(shape, 13)
-    P = L.default_params(adr_rtol=1e-13, adr_mixed_inner_rtol=inner)
+    P = L.default_params(adr_rtol=1e-13, adr_mixed_inner_rtol=inner, theta=theta)
     g, o = both(shape, P, img)
     q = [rng.uniform(-0.3, 0.3, size=shape) for _ in range(3)]
     conc = rng.uniform(0.0, 1.0, size=shape)
@@ -190,7 +191,7 @@ def test_transport_mixed_precision_parity(rng, acid_zero, inner):
     else:
         assert relerr(g.download(L.F_ACID), o.download(L.F_ACID)) <= 1e-11
     # the FP64 form on the same input agrees with the mixed one
-    P64 = L.default_params(adr_rtol=1e-13, adr_mixed=0)
+    P64 = L.default_params(adr_rtol=1e-13, adr_mixed=0, theta=theta)
     h = L.GpuSimulation(shape, P64)
     h.read_labels(img)
     h.init_state()
