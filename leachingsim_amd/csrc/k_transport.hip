@@ -42,23 +42,20 @@ struct Extrap {
     double w[4];
 };
 
-// One black voxel's start from its current field and the history, which
-// then shifts by one (h_2 <- h_1 <- h_0 <- x) where it is held.
-__device__ __forceinline__ double extrap_start(double xa, uint32_t m, double* __restrict__ h0,
-                                               double* __restrict__ h1, double* __restrict__ h2,
+// One black voxel's start from its current field and the first ex.n
+// history fields; the current field then goes into hw (the buffer the host
+// makes the newest history field, the oldest one's, read before it is
+// overwritten).
+__device__ __forceinline__ double extrap_start(double xa, uint32_t m, const double* h0,
+                                               const double* h1, const double* h2, double* hw,
                                                const Extrap& ex) {
-    const double v0 = h0 != nullptr ? h0[m] : 0.0;
-    const double v1 = h1 != nullptr ? h1[m] : 0.0;
-    const double v2 = h2 != nullptr ? h2[m] : 0.0;
     double xs = xa;
     if (ex.n >= 1) {
-        xs = ex.w[0] * xa + ex.w[1] * v0;
-        if (ex.n >= 2) xs += ex.w[2] * v1;
-        if (ex.n >= 3) xs += ex.w[3] * v2;
+        xs = ex.w[0] * xa + ex.w[1] * h0[m];
+        if (ex.n >= 2) xs += ex.w[2] * h1[m];
+        if (ex.n >= 3) xs += ex.w[3] * h2[m];
     }
-    if (h2 != nullptr) h2[m] = v1;
-    if (h1 != nullptr) h1[m] = v0;
-    if (h0 != nullptr) h0[m] = xa;
+    if (hw != nullptr) hw[m] = xa;
     return xs;
 }
 
@@ -823,8 +820,8 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
     const double* __restrict__ sf, TGeo g, uint32_t N, AdrConst a,
     double* __restrict__ rows, const double* __restrict__ x0, const double* __restrict__ x1,
     double* __restrict__ b0, double* __restrict__ b1, int one, double* __restrict__ xs0,
-    double* __restrict__ xs1, double* __restrict__ hist0, double* __restrict__ hist1,
-    double* __restrict__ hist2, Extrap ex) {
+    double* __restrict__ xs1, const double* hist0,
+    const double* hist1, const double* hist2, double* hw, Extrap ex) {
     const uint32_t Nh = N / 2;
     const double* x1r = one ? x0 : x1;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
@@ -865,7 +862,7 @@ __global__ __launch_bounds__(EW_BLOCK) void k_adr_rows_split(
             rows[((size_t)(f + 1) * 2 + colour) * rstride(Nh) + m] =
                 (R.mask & (1 << f)) ? R.off[f] : 0.0;
         if (colour == 0) {
-            const double xstart = extrap_start(xa, m, hist0, hist1, hist2, ex);
+            const double xstart = extrap_start(xa, m, hist0, hist1, hist2, hw, ex);
             xs0[m] = R.diag * xstart;
             if (!one) xs1[m] = R.diag * xb;
         }
@@ -1087,8 +1084,8 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_adr_rows_pair(
     const uint8_t* __restrict__ img, double* qx, double* qy, double* qz, TVGeo g, uint32_t N,
     AdrConst a, double* __restrict__ rows, const double* __restrict__ x0,
     const double* __restrict__ x1, double* __restrict__ b0, double* __restrict__ b1, int one,
-    double* __restrict__ xs0, double* __restrict__ xs1, double* __restrict__ hist0,
-    double* __restrict__ hist1, double* __restrict__ hist2, Extrap ex) {
+    double* __restrict__ xs0, double* __restrict__ xs1, const double* hist0,
+    const double* hist1, const double* hist2, double* hw, Extrap ex) {
     const uint32_t Nh = N / 2;
     const double* x1r = one ? x0 : x1;
     const int lane = threadIdx.x;
@@ -1213,7 +1210,7 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_adr_rows_pair(
                 rows[((size_t)(f + 1) * 2 + colour) * rstride(Nh) + mg] =
                     (R.mask & (1 << f)) ? R.off[f] : 0.0;
             if (colour == 0) {
-                const double xstart = extrap_start(xa, mg, hist0, hist1, hist2, ex);
+                const double xstart = extrap_start(xa, mg, hist0, hist1, hist2, hw, ex);
                 xs0[mg] = R.diag * xstart;
                 if (!one) xs1[mg] = R.diag * xb;
             }
@@ -2404,12 +2401,15 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             }
             return *buf;
         };
-        int held = 0;           // history fields held (the extrapolation's order)
+        int held = 0;           // history buffers held (the extrapolation's order)
         if (KIND == 0 && c->P.adr_extrapolate)
             while (held < LS_ADR_EXTRAP_ORDER && held < 3 && history(&c->thist[held]) != nullptr) {
                 hist[held] = c->thist[held];
                 ++held;
             }
+        // the current field goes into the oldest buffer, which then becomes
+        // the newest
+        double* const hw = held > 0 ? hist[held - 1] : nullptr;
         const double h = 1.0 / a.inv_dt;
         // Lagrange weights at t = h through the current field (t = 0) and the
         // usable history fields (t = -(dt_0 + ... + dt_k))
@@ -2440,21 +2440,23 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
             hipLaunchKernelGGL(k_adr_rows_pair<false>, pgrid, dim3(WAVE, TSV_TR), 0, c->stream,
                                c->img, c->qx, c->qy, c->qz, pg, N, a, c->arows, x[0], x[1],
                                c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], hist[0], hist[1],
-                               hist[2], ex);
+                               hist[2], hw, ex);
         else if (KIND == 0)
             hipLaunchKernelGGL(k_adr_rows_pair<true>, pgrid, dim3(WAVE, TSV_TR), 0, c->stream,
                                c->img, c->qx, c->qy, c->qz, pg, N, a, c->arows, x[0], x[1],
                                c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], hist[0], hist[1],
-                               hist[2], ex);
+                               hist[2], hw, ex);
         else
             hipLaunchKernelGGL((k_adr_rows_split<KIND>), dim3(nb), dim3(EW_BLOCK), 0, c->stream,
                                c->img, c->qx, c->qy, c->qz, sf, g, N, a, c->arows, x[0], x[1],
                                c->tb[0], c->tb[1], one ? 1 : 0, xs[0], xs[1], hist[0], hist[1],
-                               hist[2], ex);
+                               hist[2], hw, ex);
         if (KIND == 0) {
             if (held == 0) {
                 c->tprev_n = 0;
             } else {
+                for (int k = held - 1; k > 0; --k) c->thist[k] = c->thist[k - 1];
+                c->thist[0] = hw;
                 c->thist_dt[2] = c->thist_dt[1];
                 c->thist_dt[1] = c->thist_dt[0];
                 c->thist_dt[0] = h;
