@@ -1230,7 +1230,9 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_adr_rows_pair(
 // Red pass on the scaled unknowns, the modes of k_tschur_red: MODE 0
 // w = R' p; MODE 1 rescales the red rows to R' and 1 / d_r, then as MODE 2;
 // MODE 2 w = D_r^-1 b_r - R' y (the final black pass writes it into the
-// field).
+// field).  The mixed-precision form keeps the FP64 red rows unscaled: MODE 3
+// is MODE 1 that writes only the FP32 copies of R', MODE 4 is MODE 2 on the
+// unscaled rows, rescaled in registers as MODE 1 does (the same bits).
 template <int SYS, int MODE>
 __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
     double* __restrict__ rows, TVGeo g, uint32_t Nh, const double* __restrict__ p0,
@@ -1262,7 +1264,7 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
             if (SYS & 1) c0 = *(const d2*)(p0 + idx);
             if (SYS & 2) c1 = *(const d2*)(p1 + idx);
         }
-        if (MODE == 1) {
+        if (MODE == 1 || MODE == 3 || MODE == 4) {
             // the black neighbours' diagonals (black slot 0), in the order
             // of the faces; an absent neighbour has coefficient 0
             const double* db = rows;
@@ -1279,11 +1281,13 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
                 o[f].x = dn[f].x != 0.0 ? (o[f].x * inv.x) / dn[f].x : 0.0;
                 o[f].y = dn[f].y != 0.0 ? (o[f].y * inv.y) / dn[f].y : 0.0;
             }
-            if (ok) {
+            if (ok && MODE == 1) {
                 *(d2*)(rows + (size_t)1 * rstride(Nh) + idx) = inv;
 #pragma unroll
                 for (int f = 0; f < 6; ++f)
                     *(d2*)(rows + ((size_t)(f + 1) * 2 + 1) * rstride(Nh) + idx) = o[f];
+            }
+            if (ok && (MODE == 1 || MODE == 3)) {
                 if (rows32 != nullptr) {
                     typedef float f2v __attribute__((ext_vector_type(2)));
 #pragma unroll
@@ -2528,7 +2532,12 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
         double* const* rhs = mixed ? nulls : rh;
         double* const* pps = mixed ? nulls : pp;
         double* const* vvs = mixed ? nulls : vv;
-        if (tvec)
+        if (tvec && mixed && RM != 0)
+            // unscaled FP64 red rows, rescaled in registers
+            hipLaunchKernelGGL((k_tschur_red_v<S, RM == 1 ? 3 : 4>), tvgrid, dim3(WAVE, TSV_TR),
+                               0, c->stream, c->arows, tvg, Nh, xa, xb, wr[0], wr[1], c->bicg,
+                               c->tb[0], c->tb[1], x[0], x[1], r32);
+        else if (tvec)
             hipLaunchKernelGGL((k_tschur_red_v<S, RM>), tvgrid, dim3(WAVE, TSV_TR), 0, c->stream,
                                c->arows, tvg, Nh, xa, xb, wr[0], wr[1], c->bicg, c->tb[0],
                                c->tb[1], x[0], x[1], r32);
@@ -2771,7 +2780,13 @@ static int bicg_pair(ls_ctx* c, double* x0_, double* x1_, const double* sf, cons
                 }
                 break;
             }
-            mixed = false;        // FP64 BiCGStab from the current iterate
+            // FP64 BiCGStab from the current iterate, on the red rows rescaled
+            // to R' in place (the mixed form left them unscaled)
+            hipLaunchKernelGGL((k_tschur_red_v<1, 1>), tvgrid, dim3(WAVE, TSV_TR), 0, c->stream,
+                               c->arows, tvg, Nh, xs[0], xs[1], wr[0], wr[1], c->bicg, c->tb[0],
+                               c->tb[1], x[0], x[1], nullptr);
+            LS_LAUNCH_CHECK();
+            mixed = false;
             c->t_fallback = 1;
             continue;
         }
