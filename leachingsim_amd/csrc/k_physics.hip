@@ -540,8 +540,8 @@ __global__ __launch_bounds__(WAVE * PP_TR) void k_press_diag_pair(
                                    zfaces, ptop);
         }
         *(uint16_t*)(cls + v0) = (uint16_t)((uint8_t)o[0].cls | ((uint32_t)(uint8_t)o[1].cls << 8));
-        *(pd2*)(pD + v0) = pd2{o[0].pD, o[1].pD};
-        *(pd2*)(sd + v0) = pd2{o[0].sd, o[1].sd};
+        __builtin_nontemporal_store(pd2{o[0].pD, o[1].pD}, (pd2*)(pD + v0));
+        __builtin_nontemporal_store(pd2{o[0].sd, o[1].sd}, (pd2*)(sd + v0));
     }
 }
 
@@ -655,14 +655,15 @@ __global__ __launch_bounds__(WAVE * PP_TR) void k_press_rhs_scale_pair(
                 f2[e] = scf[2];
             }
         }
-        *(pd2*)(bhat + v0) = pd2{bh[0], bh[1]};
-        *(pd2*)(x + v0) = pd2{xs[0], xs[1]};
-        *(pd2*)(hx + v0) = pd2{f0[0], f0[1]};
-        *(pd2*)(hy + v0) = pd2{f1[0], f1[1]};
-        *(pd2*)(hz + v0) = pd2{f2[0], f2[1]};
-        *(pf2*)(fx + v0) = pf2{(float)f0[0], (float)f0[1]};
-        *(pf2*)(fy + v0) = pf2{(float)f1[0], (float)f1[1]};
-        *(pf2*)(fz + v0) = pf2{(float)f2[0], (float)f2[1]};
+        // written for later kernels only: nontemporal stores
+        __builtin_nontemporal_store(pd2{bh[0], bh[1]}, (pd2*)(bhat + v0));
+        __builtin_nontemporal_store(pd2{xs[0], xs[1]}, (pd2*)(x + v0));
+        __builtin_nontemporal_store(pd2{f0[0], f0[1]}, (pd2*)(hx + v0));
+        __builtin_nontemporal_store(pd2{f1[0], f1[1]}, (pd2*)(hy + v0));
+        __builtin_nontemporal_store(pd2{f2[0], f2[1]}, (pd2*)(hz + v0));
+        __builtin_nontemporal_store(pf2{(float)f0[0], (float)f0[1]}, (pf2*)(fx + v0));
+        __builtin_nontemporal_store(pf2{(float)f1[0], (float)f1[1]}, (pf2*)(fy + v0));
+        __builtin_nontemporal_store(pf2{(float)f2[0], (float)f2[1]}, (pf2*)(fz + v0));
     }
 }
 
@@ -885,9 +886,9 @@ __global__ __launch_bounds__(WAVE * PP_TR) void k_flux_pair(
             flux_core((uint8_t)(e ? (lc >> 8) : (lc & 0xffu)), pv, cv, e ? kc.y : kc.x, pn, cn, i,
                       j, k, g, dx, mcg, qo[0][e], qo[1][e], qo[2][e]);
         }
-        *(pd2*)(qx + v0) = pd2{qo[0][0], qo[0][1]};
-        *(pd2*)(qy + v0) = pd2{qo[1][0], qo[1][1]};
-        *(pd2*)(qz + v0) = pd2{qo[2][0], qo[2][1]};
+        __builtin_nontemporal_store(pd2{qo[0][0], qo[0][1]}, (pd2*)(qx + v0));
+        __builtin_nontemporal_store(pd2{qo[1][0], qo[1][1]}, (pd2*)(qy + v0));
+        __builtin_nontemporal_store(pd2{qo[2][0], qo[2][1]}, (pd2*)(qz + v0));
     }
 }
 
