@@ -165,8 +165,8 @@ __global__ void k_props(const uint8_t* __restrict__ img, const double* __restric
         uint8_t t = img[v];
         double sv = s[v];
         double kr = rel_perm(t, sv, pc);
-        pcap[v] = cap_pressure(t, sv, pc);
-        perm[v] = permeability(t, kr, inv[v], frac[v], pc);
+        __builtin_nontemporal_store(cap_pressure(t, sv, pc), pcap + v);
+        __builtin_nontemporal_store(permeability(t, kr, inv[v], frac[v], pc), perm + v);
     }
 }
 

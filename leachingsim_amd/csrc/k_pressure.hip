@@ -2571,9 +2571,9 @@ __global__ __launch_bounds__(EW_BLOCK) void k_wfinish(const double* w,
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
         const double wv = w[v], bv = b[v], sv = sd[v];
         const double rv = bv - cc * aw[v];
-        x[v] = cc * wv;
-        r[v] = rv;
-        if (p != nullptr) p[v] = rv;
+        __builtin_nontemporal_store(cc * wv, x + v);
+        __builtin_nontemporal_store(rv, r + v);
+        if (p != nullptr) __builtin_nontemporal_store(rv, p + v);
         const double sr = sv * rv, sb = sv * bv;
         a0 += rv * rv;
         a1 += sr * sr;

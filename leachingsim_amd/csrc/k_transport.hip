@@ -1296,8 +1296,9 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_red_v(
                     typedef float f2v __attribute__((ext_vector_type(2)));
 #pragma unroll
                     for (int f = 0; f < 6; ++f)
-                        *(f2v*)(rows32 + ((size_t)f * 2 + 1) * rstride32(Nh) + idx) =
-                            f2v{(float)o[f].x, (float)o[f].y};
+                        __builtin_nontemporal_store(
+                            f2v{(float)o[f].x, (float)o[f].y},
+                            (f2v*)(rows32 + ((size_t)f * 2 + 1) * rstride32(Nh) + idx));
                 }
             }
         }
@@ -1451,8 +1452,8 @@ __global__ __launch_bounds__(WAVE * TSV_TR) void k_tschur_black_v(
                 typedef float f2v __attribute__((ext_vector_type(2)));
 #pragma unroll
                 for (int f = 0; f < 6; ++f)
-                    *(f2v*)(rows32 + ((size_t)f * 2) * rstride32(Nh) + idx) =
-                        f2v{(float)o[f].x, (float)o[f].y};
+                    __builtin_nontemporal_store(f2v{(float)o[f].x, (float)o[f].y},
+                                                (f2v*)(rows32 + ((size_t)f * 2) * rstride32(Nh) + idx));
             }
             // the pair's voxels of both colours are the full-layout quad
             // 2 idx .. 2 idx + 3, the black ones at 2 idx + q, 2 idx + 2 + q;
