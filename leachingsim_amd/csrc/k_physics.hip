@@ -737,7 +737,7 @@ __global__ void k_press_final(const uint8_t* __restrict__ img, const double* __r
     const uint32_t v = (uint32_t)i * g.si + (uint32_t)j * g.sj + (uint32_t)k;
     const int8_t cl = cls[v];
     if (cl == 1) {
-        p[v] = x[v] / sd[v];
+        __builtin_nontemporal_store(x[v] / sd[v], p + v);
     } else if (cl == 2) {
         const double pv = perm[v];
         double source_term = 0.0, sum = 0.0;
@@ -749,9 +749,9 @@ __global__ void k_press_final(const uint8_t* __restrict__ img, const double* __r
             sum += face_cond(PORE, img[n], pv, perm[n]) * pn;
         }
         if (zfaces && i == 0) sum += pore_link(pv, 1.0e-9, viscosity(PORE)) * ptop;
-        p[v] = (source_term - sum) / 1e-16;
+        __builtin_nontemporal_store((source_term - sum) / 1e-16, p + v);
     } else {
-        p[v] = pD[v];
+        __builtin_nontemporal_store(pD[v], p + v);
     }
 }
 
